@@ -1,0 +1,365 @@
+// extern "C" entry points of libbfmhip.so (declared in include/bfmx.h).
+
+#include <cstring>
+#include <exception>
+#include <new>
+
+#include "internal.h"
+
+namespace bfmx {
+
+namespace {
+thread_local char g_err[1024] = "";
+}
+
+void set_error(const char* fmt, ...) {
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(g_err, sizeof g_err, fmt, ap);
+	va_end(ap);
+}
+
+namespace {
+
+int check(bfmx_system* s) {
+	if (!s) {
+		set_error("null system");
+		return BFMX_E_ARG;
+	}
+	hipError_t const e = hipSetDevice(s->device);
+	if (e != hipSuccess) {
+		set_error("hipSetDevice(%d): %s", s->device, hipGetErrorString(e));
+		return BFMX_E_HIP;
+	}
+	return BFMX_OK;
+}
+
+void release(bfmx_system* s) {
+	release_solver(s);
+	dfree(s->d_coords);
+	dfree(s->d_elem);
+	dfree(s->d_slots);
+	dfree(s->d_elemE);
+	dfree(s->d_slice_ptr);
+	dfree(s->d_row_len);
+	dfree(s->d_diag_blk);
+	dfree(s->d_col);
+	dfree(s->d_val);
+	dfree(s->d_b);
+	dfree(s->d_bsolve);
+	dfree(s->d_fixed);
+	dfree(s->d_vfirst);
+	dfree(s->d_flags);
+	dfree(s->d_list);
+	dfree(s->d_rows);
+	dfree(s->d_edge_nodes);
+	dfree(s->d_edge_bnd);
+	dfree(s->d_inc_ptr);
+	dfree(s->d_inc);
+	if (s->stream) {
+		(void) hipStreamDestroy(s->stream);
+		s->stream = nullptr;
+	}
+}
+
+// Exceptions (std::bad_alloc from host vectors) must not cross the C ABI.
+template <typename F>
+int guarded(F&& f) {
+	try {
+		return f();
+	} catch (const std::bad_alloc&) {
+		set_error("host allocation failed");
+		return BFMX_E_NOMEM;
+	} catch (const std::exception& ex) {
+		set_error("%s", ex.what());
+		return BFMX_E_ARG;
+	}
+}
+
+}  // namespace
+}  // namespace bfmx
+
+using namespace bfmx;
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+const char* bfmx_last_error(void) {
+	return g_err;
+}
+
+int bfmx_device_count(int* count) {
+	if (!count) {
+		set_error("null count");
+		return BFMX_E_ARG;
+	}
+	int n = 0;
+	hipError_t const e = hipGetDeviceCount(&n);
+	if (e == hipErrorNoDevice || e == hipErrorInsufficientDriver) {
+		*count = 0;
+		return BFMX_OK;
+	}
+	if (e != hipSuccess) {
+		*count = 0;
+		set_error("hipGetDeviceCount: %s", hipGetErrorString(e));
+		return BFMX_E_HIP;
+	}
+	*count = n;
+	return BFMX_OK;
+}
+
+int bfmx_system_create(bfmx_system_t** out, int kind, size_t n_nodes, const double* coords, size_t n_elems,
+                       const size_t* elems, int renumber) {
+	return guarded([&]() -> int {
+		if (!out || !coords || (n_elems && !elems)) {
+			set_error("null argument");
+			return BFMX_E_ARG;
+		}
+		*out = nullptr;
+		if (kind != 3 && kind != 4) {
+			set_error("element kind %d not supported (3 = P1 triangles, 4 = Q1 quads)", kind);
+			return BFMX_E_UNSUPPORTED;
+		}
+		int ndev = 0;
+		if (bfmx_device_count(&ndev) != BFMX_OK || ndev == 0) {
+			set_error("no GPU visible to HIP");
+			return BFMX_E_NODEV;
+		}
+		auto* s = new bfmx_system();
+		BFMX_HIP(hipGetDevice(&s->device));
+		s->kind = kind;
+		s->n_nodes = static_cast<int64_t>(n_nodes);
+		s->n_elems = static_cast<int64_t>(n_elems);
+		hipError_t const e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+		if (e != hipSuccess) {
+			set_error("hipStreamCreate: %s", hipGetErrorString(e));
+			delete s;
+			return BFMX_E_HIP;
+		}
+		int rc = build_topology(s, coords, elems, renumber);
+		if (rc == BFMX_OK) {
+			rc = compute_slots(s);
+		}
+		if (rc == BFMX_OK) {
+			hipError_t const e2 = hipStreamSynchronize(s->stream);
+			if (e2 != hipSuccess) {
+				set_error("topology upload: %s", hipGetErrorString(e2));
+				rc = BFMX_E_HIP;
+			}
+		}
+		if (rc != BFMX_OK) {
+			release(s);
+			delete s;
+			return rc;
+		}
+		*out = s;
+		return BFMX_OK;
+	});
+}
+
+int bfmx_system_destroy(bfmx_system_t* s) {
+	if (!s) {
+		return BFMX_OK;
+	}
+	(void) hipSetDevice(s->device);
+	if (s->stream) {
+		(void) hipStreamSynchronize(s->stream);
+	}
+	release(s);
+	delete s;
+	return BFMX_OK;
+}
+
+int bfmx_system_set_edges(bfmx_system_t* s, size_t n_edges, const size_t* edge_nodes, const uint8_t* edge_boundary) {
+	BFMX_CHECK(check(s));
+	if (n_edges && !edge_nodes) {
+		set_error("null edge list");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return set_edges(s, n_edges, edge_nodes, edge_boundary); });
+}
+
+int bfmx_system_assemble(bfmx_system_t* s, int stress, double E, double nu, double rho, const double* elem_E,
+                         const double* node_force, double gx, double gy) {
+	BFMX_CHECK(check(s));
+	return guarded([&]() -> int {
+		BFMX_CHECK(assemble(s, stress, E, nu, rho, elem_E, node_force, gx, gy));
+		BFMX_HIP(hipStreamSynchronize(s->stream));
+		return BFMX_OK;
+	});
+}
+
+int bfmx_system_apply_condition(bfmx_system_t* s, int kind, double value, const uint8_t* node_flags) {
+	BFMX_CHECK(check(s));
+	if (!node_flags) {
+		set_error("null node flags");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return apply_condition(s, kind, value, node_flags); });
+}
+
+int bfmx_system_solve(bfmx_system_t* s, const double* b_in, double* u_out, int precond, int precision, double rtol,
+                      size_t max_iter, bfmx_stats_t* stats) {
+	BFMX_CHECK(check(s));
+	if (!s->assembled) {
+		set_error("solve before assemble");
+		return BFMX_E_ARG;
+	}
+	return guarded([&]() -> int {
+		BFMX_CHECK(finalize_conditions(s));
+		int const rc = solve(s, b_in, precond, precision, rtol, max_iter, stats);
+		if (u_out && (rc == BFMX_OK || rc == BFMX_E_MAXITER)) {
+			int const rc2 = bfmx_system_get_u(s, u_out);
+			if (rc2 != BFMX_OK) {
+				return rc2;
+			}
+		}
+		return rc;
+	});
+}
+
+int bfmx_system_info(bfmx_system_t* s, size_t* n_nodes, size_t* nnzb, size_t* nnzb_padded, size_t* n_colors) {
+	if (!s) {
+		set_error("null system");
+		return BFMX_E_ARG;
+	}
+	if (n_nodes) {
+		*n_nodes = static_cast<size_t>(s->n_nodes);
+	}
+	if (nnzb) {
+		*nnzb = static_cast<size_t>(s->nnzb);
+	}
+	if (nnzb_padded) {
+		*nnzb_padded = static_cast<size_t>(s->nnzb_pad);
+	}
+	if (n_colors) {
+		*n_colors = static_cast<size_t>(s->n_colors);
+	}
+	return BFMX_OK;
+}
+
+int bfmx_system_node_perm(bfmx_system_t* s, int64_t* perm) {
+	if (!s || !perm) {
+		set_error("null argument");
+		return BFMX_E_ARG;
+	}
+	for (int64_t i = 0; i < s->n_nodes; i++) {
+		perm[i] = s->perm[i];
+	}
+	return BFMX_OK;
+}
+
+int bfmx_system_get_bsr(bfmx_system_t* s, int64_t* rowptr, int32_t* col, double* val) {
+	BFMX_CHECK(check(s));
+	if (!s->assembled) {
+		set_error("matrix not assembled");
+		return BFMX_E_ARG;
+	}
+	return guarded([&]() -> int {
+		BFMX_CHECK(finalize_conditions(s));
+		int64_t const n = s->n_nodes;
+		if (rowptr) {
+			for (int64_t i = 0; i <= n; i++) {
+				rowptr[i] = s->adj_ptr[i];
+			}
+		}
+		if (col) {
+			std::memcpy(col, s->adj.data(), s->adj.size() * sizeof(int32_t));
+		}
+		if (val) {
+			std::vector<double> v(4 * s->nnzb_pad);
+			BFMX_HIP(hipMemcpy(v.data(), s->d_val, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+			for (int64_t i = 0; i < n; i++) {
+				int const r = static_cast<int>(i & 63);
+				int64_t const base = s->slice_ptr[i >> 6] + r;
+				for (int64_t k = 0; k < s->row_len[i]; k++) {
+					int64_t const blk = base + k * kSlice;
+					const double* top = v.data() + 4 * blk - 2 * r;
+					double* o = val + 4 * (s->adj_ptr[i] + k);
+					o[0] = top[0];
+					o[1] = top[1];
+					o[2] = top[2 * kSlice];
+					o[3] = top[2 * kSlice + 1];
+				}
+			}
+		}
+		return BFMX_OK;
+	});
+}
+
+int bfmx_system_get_rhs(bfmx_system_t* s, double* b) {
+	BFMX_CHECK(check(s));
+	if (!b || !s->assembled) {
+		set_error(!b ? "null output" : "not assembled");
+		return BFMX_E_ARG;
+	}
+	return guarded([&]() -> int {
+		BFMX_CHECK(finalize_conditions(s));
+		int64_t const n = s->n_nodes;
+		std::vector<double> v(2 * n);
+		BFMX_HIP(hipMemcpy(v.data(), s->d_b, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+		for (int64_t i = 0; i < n; i++) {
+			b[2 * i] = v[2 * s->perm[i]];
+			b[2 * i + 1] = v[2 * s->perm[i] + 1];
+		}
+		return BFMX_OK;
+	});
+}
+
+int bfmx_system_get_u(bfmx_system_t* s, double* u) {
+	BFMX_CHECK(check(s));
+	if (!u || !s->has_u) {
+		set_error(!u ? "null output" : "no solution available");
+		return BFMX_E_ARG;
+	}
+	return guarded([&]() -> int {
+		int64_t const n = s->n_nodes;
+		std::vector<double> v(2 * n);
+		BFMX_HIP(hipMemcpy(v.data(), s->d_x, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+		for (int64_t i = 0; i < n; i++) {
+			u[2 * i] = v[2 * s->perm[i]];
+			u[2 * i + 1] = v[2 * s->perm[i] + 1];
+		}
+		return BFMX_OK;
+	});
+}
+
+int bfmx_system_get_entry(bfmx_system_t* s, size_t i, size_t j, double* out) {
+	BFMX_CHECK(check(s));
+	if (!out || !s->assembled) {
+		set_error(!out ? "null output" : "not assembled");
+		return BFMX_E_ARG;
+	}
+	size_t const m = 2 * static_cast<size_t>(s->n_nodes);
+	if (i >= m || j >= m) {
+		set_error("entry (%zu, %zu) outside a %zu x %zu matrix", i, j, m, m);
+		return BFMX_E_ARG;
+	}
+	return guarded([&]() -> int {
+		BFMX_CHECK(finalize_conditions(s));
+		int32_t const ri = s->perm[i / 2], cj = s->perm[j / 2];
+		*out = 0;
+		for (int64_t k = 0; k < s->row_len[ri]; k++) {
+			if (s->adj[s->adj_ptr[ri] + k] == cj) {
+				int const r = ri & 63;
+				int64_t const blk = s->slice_ptr[ri >> 6] + r + k * kSlice;
+				int64_t const off = 4 * blk - 2 * r + (i % 2) * 2 * kSlice + (j % 2);
+				BFMX_HIP(hipMemcpy(out, s->d_val + off, sizeof(double), hipMemcpyDeviceToHost));
+				break;
+			}
+		}
+		return BFMX_OK;
+	});
+}
+
+int bfmx_system_time_spmv(bfmx_system_t* s, int precision, int reps, double* avg_ms, double* bytes) {
+	BFMX_CHECK(check(s));
+	if (!avg_ms || reps <= 0 || !s->assembled) {
+		set_error("bad arguments to time_spmv");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return time_spmv(s, precision, reps, avg_ms, bytes); });
+}
+
+}  // extern "C"
+#pragma GCC visibility pop

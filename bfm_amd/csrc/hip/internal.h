@@ -1,0 +1,205 @@
+// Internal declarations of libbfmhip.so (not installed, not part of the ABI).
+//
+// Device data layout of one planar system (all in the renumbered node order):
+//   coords   double[2 n]            node positions
+//   matrix   sliced ELL of 2x2 FP64 blocks, one slice = 64 consecutive block
+//            rows = one wavefront. Slice s holds K_s = max row length block
+//            columns; block k of row r (r = row & 63) is entry
+//            slice_ptr[s] + 64 k + r of col[], and its values sit at
+//            val[4 (slice_ptr[s] + 64 k)] + {2r, 2r+1} (top row a00 a01) and
+//            + {128 + 2r, 128 + 2r + 1} (bottom row a10 a11), so one lane's
+//            16-byte loads of a block row are contiguous across the wavefront.
+//            Padding blocks repeat the row's own column with zero values.
+//   elements int32 node ids in colour order; a colour's elements share no
+//            node, so one lane per element adds into K without atomics and
+//            the sum order is fixed (deterministic).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "bfmx.h"
+
+namespace bfmx {
+
+constexpr int kSlice = 64;     // block rows per sliced-ELL slice (one wavefront)
+constexpr int kBlock = 256;    // threads per workgroup for the streaming kernels
+constexpr int kChunk = 16;     // CG iterations per captured graph
+
+void set_error(const char* fmt, ...);
+
+#define BFMX_HIP(expr)                                                                              \
+	do {                                                                                            \
+		hipError_t e_ = (expr);                                                                     \
+		if (e_ != hipSuccess) {                                                                     \
+			::bfmx::set_error("%s:%d %s: %s", __FILE__, __LINE__, #expr, hipGetErrorString(e_)); \
+			return BFMX_E_HIP;                                                                      \
+		}                                                                                           \
+	} while (0)
+
+#define BFMX_CHECK(expr)          \
+	do {                          \
+		int rc_ = (expr);         \
+		if (rc_ != BFMX_OK) {     \
+			return rc_;           \
+		}                         \
+	} while (0)
+
+// Device-resident scalars of one CG run. Written by the last workgroup of a
+// kernel, read by the next kernel (kernel boundaries order the accesses).
+struct alignas(128) Scalars {
+	double rz;       // r . z
+	double alpha;
+	double beta;
+	double rr;       // r . r (recursive residual)
+	double pq;       // p . A p
+	double tol2;     // (rtol ||b||)^2
+	double bb;       // b . b
+	double rr_true;  // ||b - A x||^2 from the residual kernel
+	double rz_prev;  // r . z of the previous iteration
+	int64_t iter;
+	int32_t done;    // 0 running, 1 converged (true residual), 2 breakdown
+	int32_t conv;    // recursive residual met the tolerance (FP64 path)
+	uint32_t cnt[4]; // last-workgroup counters, reset by the last workgroup
+};
+
+struct Graph {
+	hipGraphExec_t exec = nullptr;
+	int precond = -1;
+	int precision = -1;
+	double rtol2 = -1;
+};
+
+}  // namespace bfmx
+
+struct bfmx_system {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	int kind = 0;
+	int64_t n_nodes = 0;
+	int64_t n_elems = 0;
+
+	// renumbering: perm[old] = new, iperm[new] = old
+	std::vector<int32_t> perm, iperm;
+	std::vector<double> coords;      // renumbered, host copy
+	std::vector<int64_t> adj_ptr;    // node adjacency (renumbered, sorted, incl. self)
+	std::vector<int32_t> adj;
+
+	// sliced ELL
+	int64_t n_slices = 0, nnzb = 0, nnzb_pad = 0;
+	std::vector<int32_t> slice_ptr;  // n_slices + 1, in blocks
+	std::vector<int32_t> row_len;    // per block row
+	std::vector<int32_t> diag_blk;   // block index of each row's diagonal block
+
+	// elements in colour order; elem_order[pos] = original element index
+	int n_colors = 0;
+	std::vector<int64_t> color_ptr;
+	std::vector<int32_t> elem_order;
+
+	// edges (renumbered) and node -> incident edge list in edge order
+	int64_t n_edges = 0;
+	std::vector<int32_t> edge_nodes;
+	std::vector<uint8_t> edge_bnd;
+	std::vector<int64_t> inc_ptr;
+	std::vector<int32_t> inc;
+
+	// device buffers
+	double* d_coords = nullptr;
+	int32_t* d_elem = nullptr;       // kind per element, colour order
+	int32_t* d_slots = nullptr;      // kind*kind block indices per element
+	double* d_elemE = nullptr;
+	int32_t* d_slice_ptr = nullptr;
+	int32_t* d_row_len = nullptr;
+	int32_t* d_diag_blk = nullptr;
+	int32_t* d_col = nullptr;
+	double* d_val = nullptr;         // 4 * nnzb_pad
+	float* d_val32 = nullptr;        // FP32 copy for mixed precision (on demand)
+	double* d_b = nullptr;           // assembled + condition-applied load
+	double* d_bsolve = nullptr;      // right-hand side of the current solve
+	uint8_t* d_fixed = nullptr;      // per DOF: constrained
+	double* d_vfirst = nullptr;      // per DOF: value of the first constraint (lift)
+	uint8_t* d_flags = nullptr;      // per node scratch: condition flags
+	int32_t* d_list = nullptr;       // per node scratch: node lists
+	int32_t* d_rows = nullptr;       // rows touched by Dirichlet elimination
+	int32_t* d_edge_nodes = nullptr;
+	uint8_t* d_edge_bnd = nullptr;
+	int64_t* d_inc_ptr = nullptr;
+	int32_t* d_inc = nullptr;
+
+	// solver workspace
+	double *d_x = nullptr, *d_r = nullptr, *d_z = nullptr, *d_q = nullptr;
+	double* d_p[2] = {nullptr, nullptr};
+	double* d_dinv = nullptr;        // 4 per node (block inverse) or 2 per node (point)
+	float* d_xs[2] = {nullptr, nullptr};   // FP32 search directions (mixed)
+	float* d_zs = nullptr;                 // FP32 copy of z (mixed)
+	double* d_partial = nullptr;     // per-workgroup partial sums (3 per workgroup)
+	bfmx::Scalars* d_scal = nullptr;
+	bfmx::Scalars* h_scal = nullptr; // pinned host mirror ring (4 entries)
+	hipEvent_t ev[6] = {};
+	bfmx::Graph graph;
+
+	bool assembled = false;
+	bool finalized = false;
+	int stress = 0;
+	std::vector<uint8_t> dirichlet_nodes;  // union of Dirichlet-flagged nodes (renumbered)
+	int64_t last_iters = 0;
+	bool has_u = false;
+};
+
+namespace bfmx {
+
+// prep.hip (host)
+int build_topology(bfmx_system* s, const double* coords, const size_t* elems, int renumber);
+int set_edges(bfmx_system* s, size_t n_edges, const size_t* edge_nodes, const uint8_t* edge_boundary);
+// assemble.hip
+int compute_slots(bfmx_system* s);
+int assemble(bfmx_system* s, int stress, double E, double nu, double rho, const double* elemE,
+             const double* node_force, double gx, double gy);
+int apply_condition(bfmx_system* s, int kind, double value, const uint8_t* flags);
+int finalize_conditions(bfmx_system* s);
+// pcg.hip
+int solve(bfmx_system* s, const double* b_in, int precond, int precision, double rtol, size_t max_iter,
+          bfmx_stats_t* stats);
+int time_spmv(bfmx_system* s, int precision, int reps, double* avg_ms, double* bytes);
+void release_solver(bfmx_system* s);
+
+template <typename T>
+int dalloc(T** p, size_t n) {
+	if (n == 0) {
+		n = 1;
+	}
+	hipError_t e = hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+	if (e != hipSuccess) {
+		*p = nullptr;
+		set_error("hipMalloc of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+		return e == hipErrorOutOfMemory ? BFMX_E_NOMEM : BFMX_E_HIP;
+	}
+	return BFMX_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+	if (p) {
+		(void) hipFree(p);
+		p = nullptr;
+	}
+}
+
+// Workgroup index remapped so that each XCD (workgroups are dealt round-robin
+// over the 8 XCDs) processes one contiguous range of rows; neighbouring rows
+// then share the XCD's L2 for the gathered vector entries.
+__device__ __forceinline__ int xcd_remap(int wg, int nwg) {
+	constexpr int kXcd = 8;
+	int const per = nwg / kXcd;
+	int const rem = nwg % kXcd;
+	int const xcd = wg % kXcd;
+	int const local = wg / kXcd;
+	return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+}
+
+}  // namespace bfmx

@@ -1,0 +1,759 @@
+// Preconditioned conjugate gradients on the device.
+//
+// One CG iteration is two kernels:
+//   k_cg_spmv    p_new = z + beta p_old and q = A p_new for its rows, gathering
+//                z and p_old at the block columns (so the direction update
+//                costs no separate pass), plus the partial sums of p_new . q;
+//                the last workgroup to finish reduces them and sets
+//                alpha = (r . z) / (p . q), or flags a breakdown.
+//   k_cg_update  x += alpha p, r -= alpha q, z = M^-1 r and the partial sums of
+//                r . z and r . r; the last workgroup sets beta, counts the
+//                iteration and raises the convergence flag.
+// Scalars live in device memory and every kernel returns at once when the
+// flag is up, so a captured graph of kChunk iterations can be replayed
+// without the host looking at each iteration. All reductions are in FP64
+// and in a fixed order (deterministic).
+//
+// k_residual recomputes r = b - A x with the FP64 matrix. It starts a solve
+// (mode 0: zero directions, beta = 0) and replaces the recursive residual by
+// the true one (mode 1, keeping the directions): after the recursive residual
+// meets the tolerance (FP64 path), and at the end of every graph chunk
+// (mixed path, whose iteration uses the FP32 matrix and FP32 directions while
+// x, r, z, the dots and this check stay FP64).
+//
+// Preconditioners: point Jacobi (1 / diag) or 2x2 block Jacobi.
+
+#include <chrono>
+#include <cmath>
+
+#include "internal.h"
+
+namespace bfmx {
+
+namespace {
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef float flt2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ dbl2 ld2(const double* p) {
+	return *reinterpret_cast<const dbl2*>(p);
+}
+__device__ __forceinline__ void st2(double* p, dbl2 v) {
+	*reinterpret_cast<dbl2*>(p) = v;
+}
+__device__ __forceinline__ dbl2 ld2_stream(const double* p) {
+	return __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(p));
+}
+__device__ __forceinline__ flt2 ld2f_stream(const float* p) {
+	return __builtin_nontemporal_load(reinterpret_cast<const flt2*>(p));
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		v += __shfl_xor(v, off, 64);
+	}
+	return v;
+}
+
+// Sums NV values over the workgroup; the result is valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[kBlock / 64]) {
+#pragma unroll
+	for (int k = 0; k < NV; k++) {
+		v[k] = wave_sum(v[k]);
+	}
+	int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0) {
+#pragma unroll
+		for (int k = 0; k < NV; k++) {
+			lds[k][w] = v[k];
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+#pragma unroll
+		for (int k = 0; k < NV; k++) {
+			double s = 0;
+#pragma unroll
+			for (int i = 0; i < kBlock / 64; i++) {
+				s += lds[k][i];
+			}
+			v[k] = s;
+		}
+	}
+}
+
+__device__ __forceinline__ double load_coherent(const double* p) {
+	unsigned long long const u =
+	    __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	return __longlong_as_double(static_cast<long long>(u));
+}
+
+// Writes this workgroup's NV partial sums, and returns true in the last
+// workgroup to finish, with tot[] holding the grand totals (thread 0 only),
+// summed in workgroup-index order.
+template <int NV>
+__device__ bool last_block_reduce(double (&v)[NV], double* partial, unsigned int* counter, double (&tot)[NV]) {
+	__shared__ double lds[NV][kBlock / 64];
+	__shared__ bool is_last;
+	int const nwg = gridDim.x;
+	block_sum<NV>(v, lds);
+	if (threadIdx.x == 0) {
+#pragma unroll
+		for (int k = 0; k < NV; k++) {
+			partial[k * nwg + blockIdx.x] = v[k];
+		}
+		__threadfence();
+		unsigned int const prev = atomicAdd(counter, 1u);
+		is_last = prev == static_cast<unsigned int>(nwg - 1);
+	}
+	__syncthreads();
+	if (!is_last) {
+		return false;
+	}
+	__threadfence();
+	double acc[NV];
+#pragma unroll
+	for (int k = 0; k < NV; k++) {
+		acc[k] = 0;
+		for (int i = threadIdx.x; i < nwg; i += kBlock) {
+			acc[k] += load_coherent(&partial[k * nwg + i]);
+		}
+	}
+	__syncthreads();
+	block_sum<NV>(acc, lds);
+	if (threadIdx.x == 0) {
+#pragma unroll
+		for (int k = 0; k < NV; k++) {
+			tot[k] = acc[k];
+		}
+		*counter = 0;
+	}
+	return true;
+}
+
+// z = M^-1 r for one node.
+template <int PRE>
+__device__ __forceinline__ dbl2 apply_precond(const double* __restrict__ dinv, int i, dbl2 r) {
+	if (PRE == BFMX_PRECOND_JACOBI) {
+		dbl2 const d = ld2(dinv + 2 * i);
+		return dbl2{d.x * r.x, d.y * r.y};
+	} else {
+		dbl2 const d0 = ld2(dinv + 4 * i), d1 = ld2(dinv + 4 * i + 2);
+		return dbl2{d0.x * r.x + d0.y * r.y, d1.x * r.x + d1.y * r.y};
+	}
+}
+
+// y = A x for block row `row` with the FP64 matrix; ldx(c) returns x at node c.
+template <typename LoadX>
+__device__ __forceinline__ dbl2 row_spmv64(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ col,
+                                           const double* __restrict__ val, int row, LoadX ldx) {
+	int const s = row >> 6, r = row & 63;
+	int32_t const beg = slice_ptr[s], end = slice_ptr[s + 1];
+	const double* v = val + 4 * static_cast<int64_t>(beg) + 2 * r;
+	double y0 = 0, y1 = 0;
+	for (int32_t blk = beg + r; blk < end; blk += kSlice, v += 4 * kSlice) {
+		int32_t const c = __builtin_nontemporal_load(col + blk);
+		dbl2 const t = ld2_stream(v), u = ld2_stream(v + 2 * kSlice);
+		dbl2 const xv = ldx(c);
+		y0 += t.x * xv.x + t.y * xv.y;
+		y1 += u.x * xv.x + u.y * xv.y;
+	}
+	return dbl2{y0, y1};
+}
+
+// Same with the FP32 copy of the matrix (value layout identical, 4-byte entries).
+template <typename LoadX>
+__device__ __forceinline__ dbl2 row_spmv32(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ col,
+                                           const float* __restrict__ val, int row, LoadX ldx) {
+	int const s = row >> 6, r = row & 63;
+	int32_t const beg = slice_ptr[s], end = slice_ptr[s + 1];
+	const float* v = val + 4 * static_cast<int64_t>(beg) + 2 * r;
+	float y0 = 0, y1 = 0;
+	for (int32_t blk = beg + r; blk < end; blk += kSlice, v += 4 * kSlice) {
+		int32_t const c = __builtin_nontemporal_load(col + blk);
+		flt2 const t = ld2f_stream(v), u = ld2f_stream(v + 2 * kSlice);
+		flt2 const xv = ldx(c);
+		y0 += t.x * xv.x + t.y * xv.y;
+		y1 += u.x * xv.x + u.y * xv.y;
+	}
+	return dbl2{y0, y1};
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_spmv64(int nb, const int32_t* __restrict__ slice_ptr,
+                                                      const int32_t* __restrict__ col,
+                                                      const double* __restrict__ val, const double* __restrict__ z,
+                                                      const double* __restrict__ pold, double* __restrict__ pnew,
+                                                      double* __restrict__ q, double* __restrict__ partial,
+                                                      Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	double const beta = S->beta;
+	int const row = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	double v[1] = {0};
+	if (row < nb) {
+		dbl2 const y = row_spmv64(slice_ptr, col, val, row, [&](int c) {
+			return ld2(z + 2 * c) + beta * ld2(pold + 2 * c);
+		});
+		dbl2 const pn = ld2(z + 2 * row) + beta * ld2(pold + 2 * row);
+		st2(pnew + 2 * row, pn);
+		st2(q + 2 * row, y);
+		v[0] = pn.x * y.x + pn.y * y.y;
+	}
+	double tot[1];
+	if (last_block_reduce<1>(v, partial, &S->cnt[0], tot) && threadIdx.x == 0) {
+		double const pq = tot[0];
+		S->pq = pq;
+		if (!(pq > 0) || !isfinite(pq)) {
+			S->done = 2;
+		} else {
+			S->alpha = S->rz / pq;
+		}
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg_spmv32(int nb, const int32_t* __restrict__ slice_ptr,
+                                                      const int32_t* __restrict__ col,
+                                                      const float* __restrict__ val, const float* __restrict__ z,
+                                                      const float* __restrict__ pold, float* __restrict__ pnew,
+                                                      double* __restrict__ q, double* __restrict__ partial,
+                                                      Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	float const beta = static_cast<float>(S->beta);
+	int const row = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	double v[1] = {0};
+	if (row < nb) {
+		auto ldx = [&](int c) {
+			return *reinterpret_cast<const flt2*>(z + 2 * c) + beta * *reinterpret_cast<const flt2*>(pold + 2 * c);
+		};
+		dbl2 const y = row_spmv32(slice_ptr, col, val, row, ldx);
+		flt2 const pn = ldx(row);
+		*reinterpret_cast<flt2*>(pnew + 2 * row) = pn;
+		st2(q + 2 * row, y);
+		v[0] = static_cast<double>(pn.x) * y.x + static_cast<double>(pn.y) * y.y;
+	}
+	double tot[1];
+	if (last_block_reduce<1>(v, partial, &S->cnt[0], tot) && threadIdx.x == 0) {
+		double const pq = tot[0];
+		S->pq = pq;
+		if (!(pq > 0) || !isfinite(pq)) {
+			S->done = 2;
+		} else {
+			S->alpha = S->rz / pq;
+		}
+	}
+}
+
+// MIXED: p is FP32 (p32), z is also written as FP32 for the next gather, and
+// the recursive residual never ends the run (the FP64 check does).
+template <int PRE, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_cg_update(int nb, const double* __restrict__ p,
+                                                      const float* __restrict__ p32, const double* __restrict__ q,
+                                                      const double* __restrict__ dinv, double* __restrict__ x,
+                                                      double* __restrict__ r, double* __restrict__ z,
+                                                      float* __restrict__ z32, double* __restrict__ partial,
+                                                      Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	double const alpha = S->alpha;
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	double v[2] = {0, 0};
+	if (i < nb) {
+		dbl2 pv;
+		if (MIXED) {
+			flt2 const pf = *reinterpret_cast<const flt2*>(p32 + 2 * i);
+			pv = dbl2{pf.x, pf.y};
+		} else {
+			pv = ld2(p + 2 * i);
+		}
+		dbl2 const xv = ld2(x + 2 * i) + alpha * pv;
+		dbl2 const rv = ld2(r + 2 * i) - alpha * ld2(q + 2 * i);
+		dbl2 const zv = apply_precond<PRE>(dinv, i, rv);
+		st2(x + 2 * i, xv);
+		st2(r + 2 * i, rv);
+		if (MIXED) {
+			*reinterpret_cast<flt2*>(z32 + 2 * i) = flt2{static_cast<float>(zv.x), static_cast<float>(zv.y)};
+		} else {
+			st2(z + 2 * i, zv);
+		}
+		v[0] = rv.x * zv.x + rv.y * zv.y;
+		v[1] = rv.x * rv.x + rv.y * rv.y;
+	}
+	double tot[2];
+	if (last_block_reduce<2>(v, partial, &S->cnt[1], tot) && threadIdx.x == 0) {
+		double const rz = tot[0];
+		S->beta = rz / S->rz;
+		S->rz_prev = S->rz;
+		S->rz = rz;
+		S->rr = tot[1];
+		S->iter += 1;
+		if (!MIXED && tot[1] <= S->tol2) {
+			S->conv = 1;
+		}
+	}
+}
+
+// r = b - A x (FP64), z = M^-1 r. mode 0 starts a run (directions zeroed,
+// beta = 0); mode 1 replaces the recursive residual and keeps the directions
+// (beta = r'.z' / previous r.z). Sets done = 1 when ||r|| <= rtol ||b||, else 0.
+template <int PRE, bool MIXED>
+__global__ __launch_bounds__(kBlock) void k_residual(int nb, int mode, double rtol2,
+                                                     const int32_t* __restrict__ slice_ptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const double* __restrict__ val, const double* __restrict__ b,
+                                                     const double* __restrict__ dinv, const double* __restrict__ x,
+                                                     double* __restrict__ r, double* __restrict__ z,
+                                                     float* __restrict__ z32, double* __restrict__ p0,
+                                                     double* __restrict__ p1, float* __restrict__ ps0,
+                                                     float* __restrict__ ps1, double* __restrict__ partial,
+                                                     Scalars* __restrict__ S) {
+	if (S->done == 2) {
+		return;
+	}
+	int const row = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	double v[3] = {0, 0, 0};
+	if (row < nb) {
+		dbl2 const ax = row_spmv64(slice_ptr, col, val, row, [&](int c) { return ld2(x + 2 * c); });
+		dbl2 const bv = ld2(b + 2 * row);
+		dbl2 const rv = bv - ax;
+		dbl2 const zv = apply_precond<PRE>(dinv, row, rv);
+		st2(r + 2 * row, rv);
+		if (MIXED) {
+			*reinterpret_cast<flt2*>(z32 + 2 * row) = flt2{static_cast<float>(zv.x), static_cast<float>(zv.y)};
+		} else {
+			st2(z + 2 * row, zv);
+		}
+		if (mode == 0) {
+			if (MIXED) {
+				*reinterpret_cast<flt2*>(ps0 + 2 * row) = flt2{0.f, 0.f};
+				*reinterpret_cast<flt2*>(ps1 + 2 * row) = flt2{0.f, 0.f};
+			} else {
+				st2(p0 + 2 * row, dbl2{0, 0});
+				st2(p1 + 2 * row, dbl2{0, 0});
+			}
+		}
+		v[0] = rv.x * zv.x + rv.y * zv.y;
+		v[1] = rv.x * rv.x + rv.y * rv.y;
+		v[2] = bv.x * bv.x + bv.y * bv.y;
+	}
+	double tot[3];
+	if (last_block_reduce<3>(v, partial, &S->cnt[2], tot) && threadIdx.x == 0) {
+		S->bb = tot[2];
+		S->tol2 = rtol2 * tot[2];
+		S->rr = tot[1];
+		S->rr_true = tot[1];
+		if (mode == 0) {
+			S->beta = 0;
+			S->iter = 0;
+		} else {
+			S->beta = S->rz_prev != 0 ? tot[0] / S->rz_prev : 0;
+		}
+		S->rz = tot[0];
+		S->conv = 0;
+		S->done = tot[1] <= S->tol2 ? 1 : 0;
+	}
+}
+
+// Point or 2x2 block Jacobi from the diagonal blocks.
+__global__ __launch_bounds__(kBlock) void k_precond(int nb, int kind, const int32_t* __restrict__ diag_blk,
+                                                    const double* __restrict__ val, double* __restrict__ dinv) {
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= nb) {
+		return;
+	}
+	int const r = i & 63;
+	const double* top = val + 4 * static_cast<int64_t>(diag_blk[i]) - 2 * r;
+	double const a00 = top[0], a01 = top[1], a10 = top[2 * kSlice], a11 = top[2 * kSlice + 1];
+	if (kind == BFMX_PRECOND_JACOBI) {
+		dinv[2 * i] = 1.0 / a00;
+		dinv[2 * i + 1] = 1.0 / a11;
+	} else {
+		double const det = a00 * a11 - a01 * a10;
+		dinv[4 * i + 0] = a11 / det;
+		dinv[4 * i + 1] = -a01 / det;
+		dinv[4 * i + 2] = -a10 / det;
+		dinv[4 * i + 3] = a00 / det;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_to_f32(int64_t n, const double* __restrict__ a, float* __restrict__ b) {
+	int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+	if (i < n) {
+		b[i] = static_cast<float>(a[i]);
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill(int64_t n, double v, double* __restrict__ a) {
+	int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+	if (i < n) {
+		a[i] = v;
+	}
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill32(int64_t n, float v, float* __restrict__ a) {
+	int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+	if (i < n) {
+		a[i] = v;
+	}
+}
+
+int grid(int64_t n) {
+	return static_cast<int>((n + kBlock - 1) / kBlock);
+}
+
+int ensure_workspace(bfmx_system* s) {
+	int64_t const n = s->n_nodes;
+	if (s->d_x) {
+		return BFMX_OK;
+	}
+	BFMX_CHECK(dalloc(&s->d_x, 2 * n));
+	BFMX_CHECK(dalloc(&s->d_r, 2 * n));
+	BFMX_CHECK(dalloc(&s->d_z, 2 * n));
+	BFMX_CHECK(dalloc(&s->d_q, 2 * n));
+	BFMX_CHECK(dalloc(&s->d_p[0], 2 * n));
+	BFMX_CHECK(dalloc(&s->d_p[1], 2 * n));
+	BFMX_CHECK(dalloc(&s->d_dinv, 4 * n));
+	BFMX_CHECK(dalloc(&s->d_partial, 3 * static_cast<size_t>(grid(n))));
+	BFMX_CHECK(dalloc(&s->d_scal, 1));
+	BFMX_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_scal), 4 * sizeof(Scalars), hipHostMallocDefault));
+	for (auto& e : s->ev) {
+		BFMX_HIP(hipEventCreate(&e));
+	}
+	BFMX_HIP(hipMemsetAsync(s->d_scal, 0, sizeof(Scalars), s->stream));
+	BFMX_HIP(hipMemsetAsync(s->d_p[0], 0, 2 * n * sizeof(double), s->stream));
+	BFMX_HIP(hipMemsetAsync(s->d_p[1], 0, 2 * n * sizeof(double), s->stream));
+	return BFMX_OK;
+}
+
+int ensure_mixed(bfmx_system* s) {
+	int64_t const n = s->n_nodes;
+	if (!s->d_xs[0]) {
+		BFMX_CHECK(dalloc(&s->d_xs[0], 2 * n));
+		BFMX_CHECK(dalloc(&s->d_xs[1], 2 * n));
+		BFMX_CHECK(dalloc(&s->d_zs, 2 * n));
+	}
+	if (!s->d_val32) {
+		BFMX_CHECK(dalloc(&s->d_val32, 4 * s->nnzb_pad));
+		k_to_f32<<<grid(4 * s->nnzb_pad), kBlock, 0, s->stream>>>(4 * s->nnzb_pad, s->d_val, s->d_val32);
+		BFMX_HIP(hipGetLastError());
+	}
+	return BFMX_OK;
+}
+
+void launch_residual(bfmx_system* s, int pre, bool mixed, int mode, double rtol2) {
+	int const nb = static_cast<int>(s->n_nodes);
+#define BFMX_RES(P, M)                                                                                             \
+	k_residual<P, M><<<grid(nb), kBlock, 0, s->stream>>>(nb, mode, rtol2, s->d_slice_ptr, s->d_col, s->d_val,      \
+	                                                     s->d_bsolve, s->d_dinv, s->d_x, s->d_r, s->d_z, s->d_zs, \
+	                                                     s->d_p[0], s->d_p[1], s->d_xs[0], s->d_xs[1],            \
+	                                                     s->d_partial, s->d_scal)
+	if (pre == BFMX_PRECOND_JACOBI) {
+		if (mixed) {
+			BFMX_RES(BFMX_PRECOND_JACOBI, true);
+		} else {
+			BFMX_RES(BFMX_PRECOND_JACOBI, false);
+		}
+	} else {
+		if (mixed) {
+			BFMX_RES(BFMX_PRECOND_BLOCK_JACOBI, true);
+		} else {
+			BFMX_RES(BFMX_PRECOND_BLOCK_JACOBI, false);
+		}
+	}
+#undef BFMX_RES
+}
+
+// One CG iteration; it = position in the chunk selects the direction buffers.
+void launch_iteration(bfmx_system* s, int pre, bool mixed, int it) {
+	int const nb = static_cast<int>(s->n_nodes);
+	int const g = grid(nb);
+	int const a = it & 1, b = a ^ 1;
+	if (mixed) {
+		k_cg_spmv32<<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val32, s->d_zs, s->d_xs[a],
+		                                         s->d_xs[b], s->d_q, s->d_partial, s->d_scal);
+	} else {
+		k_cg_spmv64<<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, s->d_z, s->d_p[a],
+		                                         s->d_p[b], s->d_q, s->d_partial, s->d_scal);
+	}
+#define BFMX_UPD(P, M)                                                                                     \
+	k_cg_update<P, M><<<g, kBlock, 0, s->stream>>>(nb, s->d_p[b], s->d_xs[b], s->d_q, s->d_dinv, s->d_x,   \
+	                                               s->d_r, s->d_z, s->d_zs, s->d_partial, s->d_scal)
+	if (pre == BFMX_PRECOND_JACOBI) {
+		if (mixed) {
+			BFMX_UPD(BFMX_PRECOND_JACOBI, true);
+		} else {
+			BFMX_UPD(BFMX_PRECOND_JACOBI, false);
+		}
+	} else {
+		if (mixed) {
+			BFMX_UPD(BFMX_PRECOND_BLOCK_JACOBI, true);
+		} else {
+			BFMX_UPD(BFMX_PRECOND_BLOCK_JACOBI, false);
+		}
+	}
+#undef BFMX_UPD
+}
+
+int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
+	if (s->graph.exec && s->graph.precond == pre && s->graph.precision == (mixed ? 1 : 0) && s->graph.rtol2 == rtol2) {
+		return BFMX_OK;
+	}
+	if (s->graph.exec) {
+		BFMX_HIP(hipGraphExecDestroy(s->graph.exec));
+		s->graph.exec = nullptr;
+	}
+	hipGraph_t g = nullptr;
+	BFMX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+	for (int it = 0; it < kChunk; it++) {
+		launch_iteration(s, pre, mixed, it);
+	}
+	if (mixed) {
+		launch_residual(s, pre, true, 1, rtol2);
+	}
+	BFMX_HIP(hipStreamEndCapture(s->stream, &g));
+	hipError_t const e = hipGraphInstantiate(&s->graph.exec, g, nullptr, nullptr, 0);
+	(void) hipGraphDestroy(g);
+	if (e != hipSuccess) {
+		set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
+		return BFMX_E_HIP;
+	}
+	s->graph.precond = pre;
+	s->graph.precision = mixed ? 1 : 0;
+	s->graph.rtol2 = rtol2;
+	return BFMX_OK;
+}
+
+}  // namespace
+
+void release_solver(bfmx_system* s) {
+	if (s->graph.exec) {
+		(void) hipGraphExecDestroy(s->graph.exec);
+		s->graph.exec = nullptr;
+	}
+	dfree(s->d_x);
+	dfree(s->d_r);
+	dfree(s->d_z);
+	dfree(s->d_q);
+	dfree(s->d_p[0]);
+	dfree(s->d_p[1]);
+	dfree(s->d_xs[0]);
+	dfree(s->d_xs[1]);
+	dfree(s->d_zs);
+	dfree(s->d_dinv);
+	dfree(s->d_partial);
+	dfree(s->d_scal);
+	dfree(s->d_val32);
+	if (s->h_scal) {
+		(void) hipHostFree(s->h_scal);
+		s->h_scal = nullptr;
+	}
+	for (auto& e : s->ev) {
+		if (e) {
+			(void) hipEventDestroy(e);
+			e = nullptr;
+		}
+	}
+}
+
+int solve(bfmx_system* s, const double* b_in, int precond, int precision, double rtol, size_t max_iter,
+          bfmx_stats_t* stats) {
+	if (precond != BFMX_PRECOND_JACOBI && precond != BFMX_PRECOND_BLOCK_JACOBI) {
+		set_error("unknown preconditioner %d", precond);
+		return BFMX_E_UNSUPPORTED;
+	}
+	if (precision != BFMX_PRECISION_FP64 && precision != BFMX_PRECISION_MIXED) {
+		set_error("unknown precision %d", precision);
+		return BFMX_E_UNSUPPORTED;
+	}
+	if (!(rtol > 0)) {
+		set_error("rtol must be positive");
+		return BFMX_E_ARG;
+	}
+	bool const mixed = precision == BFMX_PRECISION_MIXED;
+	int64_t const n = s->n_nodes;
+	int64_t const cap = max_iter ? static_cast<int64_t>(max_iter) : 20 * 2 * n;
+	hipStream_t const st = s->stream;
+	BFMX_CHECK(ensure_workspace(s));
+	if (mixed) {
+		BFMX_CHECK(ensure_mixed(s));
+	}
+	if (b_in) {
+		std::vector<double> bp(2 * n);
+		for (int64_t i = 0; i < n; i++) {
+			bp[2 * s->perm[i]] = b_in[2 * i];
+			bp[2 * s->perm[i] + 1] = b_in[2 * i + 1];
+		}
+		BFMX_HIP(hipMemcpyAsync(s->d_bsolve, bp.data(), 2 * n * sizeof(double), hipMemcpyHostToDevice, st));
+		BFMX_HIP(hipStreamSynchronize(st));
+	} else {
+		BFMX_HIP(hipMemcpyAsync(s->d_bsolve, s->d_b, 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+	}
+	double const rtol2 = rtol * rtol;
+	BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2));
+
+	auto const t0 = std::chrono::steady_clock::now();
+	BFMX_HIP(hipEventRecord(s->ev[4], st));
+	k_precond<<<grid(n), kBlock, 0, st>>>(static_cast<int>(n), precond, s->d_diag_blk, s->d_val, s->d_dinv);
+	BFMX_HIP(hipMemsetAsync(s->d_x, 0, 2 * n * sizeof(double), st));
+	BFMX_HIP(hipMemsetAsync(s->d_scal, 0, sizeof(Scalars), st));
+	launch_residual(s, precond, mixed, 0, rtol2);
+	BFMX_HIP(hipGetLastError());
+
+	// Replay chunks, keeping one in flight while the previous one's scalars
+	// are inspected. Chunks always run whole (the FP64 path's recursive
+	// convergence only raises `conv`), so the direction ping-pong buffers stay
+	// in step with the chunk-relative iteration index.
+	int32_t status = BFMX_OK;
+	int restarts = 0;
+	int64_t launched = 0;
+	int64_t chunks = 0;
+	bool finished = false;
+	Scalars last{};
+	BFMX_HIP(hipMemcpyAsync(&s->h_scal[0], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	last = s->h_scal[0];
+	finished = last.done == 1;
+	while (!finished) {
+		bool const more = launched < cap;
+		if (more) {
+			BFMX_HIP(hipGraphLaunch(s->graph.exec, st));
+			launched += kChunk;
+			int const slot = static_cast<int>(chunks & 3);
+			BFMX_HIP(hipMemcpyAsync(&s->h_scal[slot], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+			BFMX_HIP(hipEventRecord(s->ev[slot], st));
+			chunks++;
+		}
+		int64_t const look = (more && chunks >= 2) ? chunks - 2 : chunks - 1;
+		BFMX_HIP(hipEventSynchronize(s->ev[look & 3]));
+		last = s->h_scal[look & 3];
+		if (last.done == 2) {
+			status = BFMX_E_BREAKDOWN;
+			break;
+		}
+		if (last.done == 1) {
+			finished = true;
+			break;
+		}
+		if (!mixed && last.conv) {
+			// confirm with the true residual; when it falls short, continue from it
+			BFMX_HIP(hipStreamSynchronize(st));
+			launch_residual(s, precond, false, 1, rtol2);
+			BFMX_HIP(hipGetLastError());
+			BFMX_HIP(hipMemcpyAsync(&s->h_scal[0], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+			BFMX_HIP(hipStreamSynchronize(st));
+			last = s->h_scal[0];
+			if (last.done == 1) {
+				finished = true;
+				break;
+			}
+			restarts++;
+			chunks = 0;
+			continue;
+		}
+		if (!more && look == chunks - 1) {
+			status = BFMX_E_MAXITER;
+			break;
+		}
+	}
+	BFMX_HIP(hipStreamSynchronize(st));
+	BFMX_HIP(hipEventRecord(s->ev[5], st));
+	BFMX_HIP(hipEventSynchronize(s->ev[5]));
+	auto const t1 = std::chrono::steady_clock::now();
+	float ms = 0;
+	BFMX_HIP(hipEventElapsedTime(&ms, s->ev[4], s->ev[5]));
+	BFMX_HIP(hipMemcpy(&last, s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost));
+	if (status == BFMX_OK && !finished) {
+		status = BFMX_E_MAXITER;
+	}
+	s->last_iters = last.iter;
+	s->has_u = true;
+	if (stats) {
+		stats->status = status;
+		stats->precond = precond;
+		stats->precision = precision;
+		stats->restarts = restarts;
+		stats->iterations = last.iter;
+		stats->n_dof = 2 * n;
+		stats->rtol = rtol;
+		stats->rel_residual = last.bb > 0 ? std::sqrt(last.rr_true / last.bb) : 0.0;
+		stats->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+		stats->b_norm = std::sqrt(last.bb);
+		(void) ms;
+	}
+	if (status == BFMX_E_BREAKDOWN) {
+		set_error("CG breakdown after %lld iterations (p.Ap = %g): matrix not SPD or singular",
+		          (long long) last.iter, last.pq);
+	} else if (status == BFMX_E_MAXITER) {
+		set_error("CG did not reach rtol %g in %lld iterations (relative residual %g)", rtol, (long long) last.iter,
+		          last.bb > 0 ? std::sqrt(last.rr / last.bb) : 0.0);
+	}
+	return status;
+}
+
+int time_spmv(bfmx_system* s, int precision, int reps, double* avg_ms, double* bytes) {
+	bool const mixed = precision == BFMX_PRECISION_MIXED;
+	int64_t const n = s->n_nodes;
+	hipStream_t const st = s->stream;
+	BFMX_CHECK(finalize_conditions(s));
+	BFMX_CHECK(ensure_workspace(s));
+	if (mixed) {
+		BFMX_CHECK(ensure_mixed(s));
+	}
+	// z = 1, p = 0: p.Ap = 1^T A 1 > 0, identical for every repetition
+	Scalars init{};
+	init.rz = 1.0;
+	BFMX_HIP(hipMemcpyAsync(s->d_scal, &init, sizeof(Scalars), hipMemcpyHostToDevice, st));
+	if (mixed) {
+		k_fill32<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 1.0f, s->d_zs);
+		k_fill32<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 0.0f, s->d_xs[0]);
+	} else {
+		k_fill<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 1.0, s->d_z);
+		k_fill<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 0.0, s->d_p[0]);
+	}
+	int const nb = static_cast<int>(n);
+	int const g = grid(nb);
+	auto launch = [&]() {
+		if (mixed) {
+			k_cg_spmv32<<<g, kBlock, 0, st>>>(nb, s->d_slice_ptr, s->d_col, s->d_val32, s->d_zs, s->d_xs[0],
+			                                  s->d_xs[1], s->d_q, s->d_partial, s->d_scal);
+		} else {
+			k_cg_spmv64<<<g, kBlock, 0, st>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, s->d_z, s->d_p[0], s->d_p[1],
+			                                  s->d_q, s->d_partial, s->d_scal);
+		}
+	};
+	for (int i = 0; i < 3; i++) {
+		launch();
+	}
+	BFMX_HIP(hipGetLastError());
+	BFMX_HIP(hipEventRecord(s->ev[4], st));
+	for (int i = 0; i < reps; i++) {
+		launch();
+	}
+	BFMX_HIP(hipEventRecord(s->ev[5], st));
+	BFMX_HIP(hipEventSynchronize(s->ev[5]));
+	float ms = 0;
+	BFMX_HIP(hipEventElapsedTime(&ms, s->ev[4], s->ev[5]));
+	Scalars chk{};
+	BFMX_HIP(hipMemcpy(&chk, s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost));
+	if (chk.done != 0) {
+		set_error("SpMV timing run flagged p.Ap <= 0 (p.Ap = %g)", chk.pq);
+		return BFMX_E_BREAKDOWN;
+	}
+	*avg_ms = static_cast<double>(ms) / reps;
+	if (bytes) {
+		double const nd = 2.0 * static_cast<double>(n);
+		double const vb = mixed ? 4.0 : 8.0;
+		// matrix blocks (values + column index) + slice pointers
+		// + gathered z and p_old (once each) + written p_new + written q (FP64)
+		*bytes = static_cast<double>(s->nnzb) * (4 * vb + 4) + 4.0 * (s->n_slices + 1) + 2 * nd * vb + nd * vb +
+		         nd * 8.0;
+	}
+	return BFMX_OK;
+}
+
+}  // namespace bfmx

@@ -1,0 +1,243 @@
+"""ctypes wrapper of libbfmhip.so, the GPU planar-elasticity path (include/bfmx.h).
+
+There is no CPU fallback: when the library is missing or no GPU is visible,
+constructing a System raises.
+"""
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libbfmhip.so")
+
+PRECOND_JACOBI, PRECOND_BLOCK_JACOBI = 0, 1
+FP64, MIXED = 0, 1
+DIRICHLET_X, DIRICHLET_Y, NEUMANN_X, NEUMANN_Y = 0, 1, 2, 3
+NEUMANN_NORMAL, NEUMANN_TANGENT, DIRICHLET_NORMAL, DIRICHLET_TANGENT = 4, 5, 6, 7
+
+ERRORS = {-1: "BFMX_E_ARG", -2: "BFMX_E_HIP", -3: "BFMX_E_NOMEM", -4: "BFMX_E_BREAKDOWN",
+          -5: "BFMX_E_MAXITER", -6: "BFMX_E_NODEV", -7: "BFMX_E_UNSUPPORTED"}
+
+# every entry point of include/bfmx.h
+EXPORTS = [
+    "bfmx_last_error", "bfmx_device_count", "bfmx_system_create", "bfmx_system_destroy",
+    "bfmx_system_set_edges", "bfmx_system_assemble", "bfmx_system_apply_condition",
+    "bfmx_system_solve", "bfmx_system_info", "bfmx_system_node_perm", "bfmx_system_get_bsr",
+    "bfmx_system_get_rhs", "bfmx_system_get_u", "bfmx_system_get_entry", "bfmx_system_time_spmv",
+]
+
+
+class Stats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("precond", C.c_int32), ("precision", C.c_int32),
+                ("restarts", C.c_int32), ("iterations", C.c_int64), ("n_dof", C.c_int64),
+                ("rtol", C.c_double), ("rel_residual", C.c_double), ("solve_ms", C.c_double),
+                ("b_norm", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BfmxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_u8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i32 = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64 = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+class _Opt:
+    def __init__(self, base):
+        self.base = base
+
+    def from_param(self, obj):
+        return None if obj is None else self.base.from_param(obj)
+
+
+def lib():
+    """Loads libbfmhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        sz, vp = C.c_size_t, C.c_void_p
+        L.bfmx_last_error.restype = C.c_char_p
+        L.bfmx_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.bfmx_system_create.argtypes = [C.POINTER(vp), C.c_int, sz, _dp, sz, _up, C.c_int]
+        L.bfmx_system_destroy.argtypes = [vp]
+        L.bfmx_system_set_edges.argtypes = [vp, sz, _up, _Opt(_u8)]
+        L.bfmx_system_assemble.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double, _Opt(_dp), _Opt(_dp),
+                                           C.c_double, C.c_double]
+        L.bfmx_system_apply_condition.argtypes = [vp, C.c_int, C.c_double, _u8]
+        L.bfmx_system_solve.argtypes = [vp, _Opt(_dp), _Opt(_dp), C.c_int, C.c_int, C.c_double, sz,
+                                        C.POINTER(Stats)]
+        L.bfmx_system_info.argtypes = [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]
+        L.bfmx_system_node_perm.argtypes = [vp, _i64]
+        L.bfmx_system_get_bsr.argtypes = [vp, _Opt(_i64), _Opt(_i32), _Opt(_dp)]
+        L.bfmx_system_get_rhs.argtypes = [vp, _dp]
+        L.bfmx_system_get_u.argtypes = [vp, _dp]
+        L.bfmx_system_get_entry.argtypes = [vp, sz, sz, C.POINTER(C.c_double)]
+        L.bfmx_system_time_spmv.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        for name in EXPORTS[1:]:
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise BfmxError(rc, lib().bfmx_last_error().decode())
+
+
+def device_count():
+    n = C.c_int()
+    _check(lib().bfmx_device_count(C.byref(n)))
+    return n.value
+
+
+class System:
+    """One planar system on the GPU: mesh upload + renumbering at creation."""
+
+    def __init__(self, kind, coords, elems, renumber=True, edges=None, edge_boundary=None):
+        L = lib()
+        coords = np.ascontiguousarray(np.asarray(coords, np.float64).reshape(-1))
+        elems = np.ascontiguousarray(np.asarray(elems, np.uint64).reshape(-1))
+        self.n_nodes = coords.size // 2
+        self.h = C.c_void_p()
+        _check(L.bfmx_system_create(C.byref(self.h), kind, self.n_nodes, coords, elems.size // kind, elems,
+                                    int(bool(renumber))))
+        if edges is not None:
+            self.set_edges(edges, edge_boundary)
+
+    @classmethod
+    def from_mesh(cls, mesh, renumber=True):
+        return cls(mesh.kind, mesh.coords, mesh.elems, renumber, edges=mesh.edges)
+
+    def close(self):
+        if self.h:
+            lib().bfmx_system_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_edges(self, edges, boundary=None):
+        e = np.ascontiguousarray(np.asarray(edges, np.uint64).reshape(-1))
+        b = None if boundary is None else np.ascontiguousarray(np.asarray(boundary, np.uint8))
+        _check(lib().bfmx_system_set_edges(self.h, e.size // 2, e, b))
+
+    def assemble(self, stress, E, nu, rho, elem_E=None, node_force=None, g=(0.0, 0.0)):
+        eE = None if elem_E is None else np.ascontiguousarray(elem_E, np.float64)
+        nf = None if node_force is None else np.ascontiguousarray(np.asarray(node_force, np.float64).reshape(-1))
+        _check(lib().bfmx_system_assemble(self.h, int(stress), E, nu, rho, eE, nf, g[0], g[1]))
+
+    def apply_condition(self, kind, value, flags):
+        _check(lib().bfmx_system_apply_condition(self.h, int(kind), float(value),
+                                                 np.ascontiguousarray(np.asarray(flags).astype(np.uint8))))
+
+    def apply_conditions(self, kinds, values, flags):
+        for k, v, f in zip(kinds, values, flags):
+            self.apply_condition(k, v, f)
+
+    def solve(self, b=None, precond=PRECOND_JACOBI, precision=FP64, rtol=1e-8, max_iter=0, want_u=True):
+        u = np.empty(2 * self.n_nodes) if want_u else None
+        st = Stats()
+        bb = None if b is None else np.ascontiguousarray(b, np.float64)
+        rc = lib().bfmx_system_solve(self.h, bb, u, precond, precision, rtol, max_iter, C.byref(st))
+        _check(rc)
+        return u, st.as_dict()
+
+    def info(self):
+        v = [C.c_size_t() for _ in range(4)]
+        _check(lib().bfmx_system_info(self.h, *[C.byref(x) for x in v]))
+        return dict(zip(["n_nodes", "nnzb", "nnzb_padded", "n_colors"], [x.value for x in v]))
+
+    def node_perm(self):
+        p = np.empty(self.n_nodes, np.int64)
+        _check(lib().bfmx_system_node_perm(self.h, p))
+        return p
+
+    def get_bsr(self):
+        """(rowptr, col, val (nnzb, 2, 2)) in the internal node order."""
+        info = self.info()
+        rowptr = np.empty(self.n_nodes + 1, np.int64)
+        col = np.empty(info["nnzb"], np.int32)
+        val = np.empty(4 * info["nnzb"])
+        _check(lib().bfmx_system_get_bsr(self.h, rowptr, col, val))
+        return rowptr, col, val.reshape(-1, 2, 2)
+
+    def get_dense(self):
+        """The condition-applied matrix as dense (small meshes), original DOF order."""
+        rowptr, col, val = self.get_bsr()
+        perm = self.node_perm()
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(perm.size)
+        m = 2 * self.n_nodes
+        K = np.zeros((m, m))
+        for i in range(self.n_nodes):
+            oi = inv[i]
+            for t in range(rowptr[i], rowptr[i + 1]):
+                oj = inv[col[t]]
+                K[2 * oi:2 * oi + 2, 2 * oj:2 * oj + 2] = val[t]
+        return K
+
+    def get_csr_scipy(self):
+        """The condition-applied matrix as scipy CSR in original DOF order."""
+        import scipy.sparse as sp
+        rowptr, col, val = self.get_bsr()
+        perm = self.node_perm()
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(perm.size)
+        counts = np.diff(rowptr)
+        rows_new = np.repeat(np.arange(self.n_nodes), counts)
+        bi = inv[rows_new]
+        bj = inv[col]
+        r = np.concatenate([2 * bi, 2 * bi, 2 * bi + 1, 2 * bi + 1])
+        c = np.concatenate([2 * bj, 2 * bj + 1, 2 * bj, 2 * bj + 1])
+        v = np.concatenate([val[:, 0, 0], val[:, 0, 1], val[:, 1, 0], val[:, 1, 1]])
+        m = 2 * self.n_nodes
+        return sp.csr_matrix((v, (r, c)), shape=(m, m))
+
+    def get_rhs(self):
+        b = np.empty(2 * self.n_nodes)
+        _check(lib().bfmx_system_get_rhs(self.h, b))
+        return b
+
+    def get_u(self):
+        u = np.empty(2 * self.n_nodes)
+        _check(lib().bfmx_system_get_u(self.h, u))
+        return u
+
+    def get_entry(self, i, j):
+        v = C.c_double()
+        _check(lib().bfmx_system_get_entry(self.h, i, j, C.byref(v)))
+        return v.value
+
+    def time_spmv(self, precision=FP64, reps=50):
+        ms, by = C.c_double(), C.c_double()
+        _check(lib().bfmx_system_time_spmv(self.h, precision, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+
+def beam_system(mesh, stress=True, material=None, traction=None, gravity=None, renumber=True):
+    """Builds, assembles and applies the beam problem's conditions."""
+    from . import meshgen
+    mat = dict(meshgen.BEAM_MATERIAL if material is None else material)
+    s = System.from_mesh(mesh, renumber=renumber)
+    s.assemble(stress, mat["E"], mat["nu"], mat["rho"], elem_E=mesh.elem_E,
+               g=gravity if gravity is not None else (0.0, 0.0))
+    kinds, values, flags = meshgen.beam_conditions(mesh, meshgen.BEAM_TRACTION if traction is None else traction)
+    s.apply_conditions(kinds, values, flags)
+    return s

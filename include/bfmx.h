@@ -1,0 +1,168 @@
+#pragma once
+
+/* C ABI of libbfmhip.so, the GPU (gfx950) implementation of libbfm's
+ * planar-elasticity path: mesh upload and node renumbering, element
+ * assembly, Dirichlet/Neumann application and the global linear solve.
+ *
+ * The libbfm host layer (bfm_system_create_planar_*, bfm_matrix_solve,
+ * bfm_sim_run) calls these entry points; tests and benchmarks may call them
+ * directly. Only plain pointers, sizes and scalars cross this boundary. All
+ * host arrays use libbfm's original numbering: node i, DOF 2*i + component.
+ * The node renumbering done at creation is internal to the system object.
+ *
+ * Every entry point returns BFMX_OK (0) on success or a negative BFMX_E_*
+ * code; bfmx_last_error() returns a message for the calling thread's last
+ * failure. Calls on one system object are not thread-safe; distinct system
+ * objects may be used from distinct threads.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference tree):
+ *   bfmx_system_create       bfm_system_create + get_elem + bfm_system_renumber
+ *                            (libbfm/src/system.c:5-34, 44-81, 93-107;
+ *                            libbfm/src/perm.c:117-331)
+ *   bfmx_system_set_edges    the mesh->edges loops of create_planar
+ *                            (libbfm/src/system.c:477-522, 397-420)
+ *   bfmx_system_assemble     the element loop of create_planar and
+ *                            fill_elasticity_elem (libbfm/src/system.c:109-227,
+ *                            454-466)
+ *   bfmx_system_apply_condition  one iteration of the condition loop of
+ *                            create_planar (libbfm/src/system.c:470-526)
+ *   bfmx_system_solve        bfm_matrix_solve + bfm_perm_perm_vec(inv) in
+ *                            run_elasticity (libbfm/src/sim.c:122-131;
+ *                            libbfm/src/matrix.c:531-541)
+ *   bfmx_system_get_entry    bfm_matrix_get (libbfm/src/matrix.c:455-465)
+ */
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+	BFMX_OK = 0,
+	BFMX_E_ARG = -1,         /* invalid argument or call order */
+	BFMX_E_HIP = -2,         /* HIP runtime error */
+	BFMX_E_NOMEM = -3,       /* host or device allocation failed */
+	BFMX_E_BREAKDOWN = -4,   /* CG breakdown: p.Ap <= 0 or not finite */
+	BFMX_E_MAXITER = -5,     /* tolerance not reached within max_iter */
+	BFMX_E_NODEV = -6,       /* no usable GPU */
+	BFMX_E_UNSUPPORTED = -7, /* element kind or option not supported */
+};
+
+/* Preconditioners. */
+enum {
+	BFMX_PRECOND_JACOBI = 0,       /* 1 / diag(A), per DOF */
+	BFMX_PRECOND_BLOCK_JACOBI = 1, /* inverse of each node's 2x2 diagonal block */
+};
+
+/* Arithmetic of the solve. */
+enum {
+	BFMX_PRECISION_FP64 = 0,  /* FP64 matrix, vectors and reductions */
+	BFMX_PRECISION_MIXED = 1, /* FP32 matrix and search directions inside FP64
+	                             refinement; FP64 solution, residual and dots */
+};
+
+/* Condition kinds, numbered as bfm_condition_kind_t (libbfm/src/bfm/condition.h:5-14). */
+enum {
+	BFMX_DIRICHLET_X = 0,
+	BFMX_DIRICHLET_Y = 1,
+	BFMX_NEUMANN_X = 2,
+	BFMX_NEUMANN_Y = 3,
+	BFMX_NEUMANN_NORMAL = 4,
+	BFMX_NEUMANN_TANGENT = 5,
+	BFMX_DIRICHLET_NORMAL = 6,
+	BFMX_DIRICHLET_TANGENT = 7,
+};
+
+typedef struct bfmx_system bfmx_system_t;
+
+typedef struct {
+	int32_t status;         /* BFMX_OK, BFMX_E_BREAKDOWN or BFMX_E_MAXITER */
+	int32_t precond;
+	int32_t precision;
+	int32_t restarts;       /* true-residual checks that restarted the iteration */
+	int64_t iterations;     /* CG iterations (sum over refinement sweeps) */
+	int64_t n_dof;
+	double rtol;
+	double rel_residual;    /* ||b - A u||_2 / ||b||_2, recomputed in FP64 at the end */
+	double solve_ms;        /* device time from the first solver kernel to convergence */
+	double b_norm;
+} bfmx_stats_t;
+
+const char* bfmx_last_error(void);
+
+/* Number of visible GPUs; 0 with BFMX_OK when none. */
+int bfmx_device_count(int* count);
+
+/* Uploads a planar mesh to the current device. kind is 3 (P1 triangles) or
+ * 4 (Q1 quads, nodes counter-clockwise from (+1,+1) as in
+ * libbfm/src/shape.c:19-26). coords holds 2 doubles per node, elems kind
+ * node indices per element. With renumber != 0 the nodes are reordered by
+ * reverse Cuthill-McKee on the node graph before anything is stored. */
+int bfmx_system_create(bfmx_system_t** out, int kind, size_t n_nodes, const double* coords,
+                       size_t n_elems, const size_t* elems, int renumber);
+int bfmx_system_destroy(bfmx_system_t* s);
+
+/* Edge list used by Neumann and normal/tangent Dirichlet conditions, in
+ * mesh->edges order: edge_nodes holds the two node indices of each edge and
+ * edge_boundary is 1 where the edge has a single adjacent element
+ * (bfm_edge_t.elems[1] == -1). */
+int bfmx_system_set_edges(bfmx_system_t* s, size_t n_edges, const size_t* edge_nodes,
+                          const uint8_t* edge_boundary);
+
+/* Assembles K and the body load on the device: plane stress when stress != 0,
+ * plane strain otherwise; Young's modulus E (or elem_E[e] per element when
+ * elem_E is not NULL), Poisson ratio nu, density rho. The body force per unit
+ * mass of node j is node_force[2j..2j+1] when node_force is not NULL, else
+ * (gx, gy). Clears all boundary conditions. */
+int bfmx_system_assemble(bfmx_system_t* s, int stress, double E, double nu, double rho,
+                         const double* elem_E, const double* node_force, double gx, double gy);
+
+/* Applies one boundary condition; node_flags has one byte per node (nonzero
+ * = in the set). Conditions act in call order with libbfm's semantics: a
+ * Dirichlet value overwrites the load of its DOF, a later Neumann load adds
+ * to it, and the symmetric elimination lifts the prescribed values into the
+ * other rows. */
+int bfmx_system_apply_condition(bfmx_system_t* s, int kind, double value, const uint8_t* node_flags);
+
+/* Solves A u = b. b is the assembled, condition-applied load when b_in is
+ * NULL, else b_in (2 * n_nodes doubles). u_out (2 * n_nodes doubles, may be
+ * NULL) receives the displacements. max_iter = 0 selects 20 * n_dof. stats
+ * may be NULL. The solution also stays on the device for bfmx_system_get_u. */
+int bfmx_system_solve(bfmx_system_t* s, const double* b_in, double* u_out, int precond,
+                      int precision, double rtol, size_t max_iter, bfmx_stats_t* stats);
+
+/* Sizes: nodes, stored 2x2 blocks, blocks including the sliced-ELL padding,
+ * element colours. Any pointer may be NULL. */
+int bfmx_system_info(bfmx_system_t* s, size_t* n_nodes, size_t* nnzb, size_t* nnzb_padded,
+                     size_t* n_colors);
+
+/* perm[old] = new node index (n_nodes values). */
+int bfmx_system_node_perm(bfmx_system_t* s, int64_t* perm);
+
+/* The matrix after boundary conditions as block-CSR in the internal node
+ * order: rowptr (n_nodes + 1), col (nnzb), val (4 * nnzb, row-major 2x2
+ * blocks). Any pointer may be NULL. */
+int bfmx_system_get_bsr(bfmx_system_t* s, int64_t* rowptr, int32_t* col, double* val);
+
+/* The condition-applied load vector, original DOF order. */
+int bfmx_system_get_rhs(bfmx_system_t* s, double* b);
+
+/* Last solution, original DOF order. */
+int bfmx_system_get_u(bfmx_system_t* s, double* u);
+
+/* Entry (i, j) of the condition-applied matrix in original DOF numbering;
+ * 0 for entries outside the sparsity pattern. */
+int bfmx_system_get_entry(bfmx_system_t* s, size_t i, size_t j, double* out);
+
+/* Times reps back-to-back launches of the solver's SpMV kernel (the one the
+ * FP64 or mixed CG iteration uses) with HIP events on the system's stream.
+ * avg_ms receives the mean duration of one launch; bytes (may be NULL) the
+ * kernel's algorithmic bytes per launch. */
+int bfmx_system_time_spmv(bfmx_system_t* s, int precision, int reps, double* avg_ms, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,241 @@
+"""GPU parity: the HIP path (through the bfmx C ABI) against the reference
+fixtures (tests/golden, written by libbfm compiled from its own sources) and
+the CPU oracle.
+
+Tolerances (floating point; the GPU sums Gauss points in registers, fuses
+multiply-adds and mirrors the element matrix, so it rounds differently):
+  K, b entries     |gpu - ref| <= 1e-13 * max|ref|
+  displacements    ||u_gpu - u_ref|| / ||u_ref|| <= 1e-8
+  residuals        ||b - K u|| / ||b|| <= the solver tolerance, K and b from
+                   the oracle's own assembly
+"""
+
+import numpy as np
+import pytest
+
+from conftest import REF_DATA, dense_from_golden, rel_err
+from make_golden import TINY_CONDITIONS, node_sets
+
+from bfm_amd import device as D
+from bfm_amd import meshgen
+from bfm_amd.pybfm import read_effects_file
+
+pytestmark = pytest.mark.gpu
+
+K_TOL = 1e-13
+U_TOL = 1e-8
+
+
+def close_entries(a, ref, tol=K_TOL):
+    scale = np.abs(ref).max()
+    err = np.abs(a - ref).max()
+    assert err <= tol * scale, f"max entry error {err:.3e} > {tol:g} * {scale:.3e}"
+
+
+def tiny_mesh(kind):
+    return meshgen.beam(3, 2, kind=kind, length=3.0, height=2.0, jitter=0.2, seed=11)
+
+
+def tiny_conditions(mesh):
+    sets = node_sets(mesh)
+    kinds = [c[0] for c in TINY_CONDITIONS]
+    values = [c[1] for c in TINY_CONDITIONS]
+    flags = np.stack([sets[c[2]] for c in TINY_CONDITIONS]).astype(np.uint8)
+    return kinds, values, flags
+
+
+def test_device_visible():
+    assert D.device_count() >= 1
+
+
+@pytest.mark.parametrize("kind,tag", [(4, "q1"), (3, "p1")])
+@pytest.mark.parametrize("stress", [0, 1])
+@pytest.mark.parametrize("renumber", [True, False])
+def test_tiny_assembly_and_conditions(golden, kind, tag, stress, renumber):
+    name = f"tiny_{tag}_{'stress' if stress else 'strain'}"
+    mesh = tiny_mesh(kind)
+    s = D.System.from_mesh(mesh, renumber=renumber)
+    s.assemble(stress, 7.0e10, 0.33, 2700.0, g=(0.0, -9.81))
+    close_entries(s.get_dense(), golden[f"{name}/K_pre"])
+    close_entries(s.get_rhs(), golden[f"{name}/b_pre"])
+    # every condition kind, in the golden order
+    s.assemble(stress, 7.0e10, 0.33, 2700.0, g=(0.0, -9.81))
+    s.apply_conditions(*tiny_conditions(mesh))
+    close_entries(s.get_dense(), golden[f"{name}/K"])
+    close_entries(s.get_rhs(), golden[f"{name}/b"])
+    u, st = s.solve(rtol=1e-14)
+    assert st["status"] == 0
+    assert rel_err(u, golden[f"{name}/u"]) <= U_TOL
+
+
+@pytest.mark.parametrize("tag", ["q1", "p1"])
+@pytest.mark.parametrize("t", [0, 1, 2])
+def test_single_element_matrix(golden, tag, t):
+    pre = f"elem_{tag}_{t}"
+    coords = golden[f"{pre}/coords"]
+    elems = golden[f"{pre}/elems"]
+    kind = elems.shape[1]
+    s = D.System(kind, coords, elems)
+    s.assemble(int(golden[f"{pre}/stress"]), 2.0e11, float(golden[f"{pre}/nu"]), 7800.0, g=(0.0, -9.81))
+    close_entries(s.get_dense(), golden[f"{pre}/K"])
+    close_entries(s.get_rhs(), golden[f"{pre}/b"])
+
+
+C1_CONDS = [(3, meshgen.BEAM_TRACTION, "top"), (0, 0.0, "left"), (1, 0.0, "left")]
+
+
+@pytest.mark.parametrize("name,conds,grav", [
+    ("c1", C1_CONDS, False),
+    ("c1_gravity", C1_CONDS, True),
+    ("c1_neumann_last", C1_CONDS[1:] + C1_CONDS[:1], False),
+])
+@pytest.mark.parametrize("precond", [D.PRECOND_JACOBI, D.PRECOND_BLOCK_JACOBI])
+def test_c1_beam(golden, name, conds, grav, precond):
+    mesh = meshgen.beam(40, 10, kind=3)
+    mat = meshgen.BEAM_MATERIAL
+    s = D.System.from_mesh(mesh)
+    s.assemble(1, mat["E"], mat["nu"], mat["rho"], g=(0.0, -9.81) if grav else (0.0, 0.0))
+    for kind, value, where in conds:
+        s.apply_condition(kind, value, mesh.node_sets[where])
+    K = dense_from_golden(golden, name, mesh.n_dof)
+    close_entries(s.get_dense(), K)
+    close_entries(s.get_rhs(), golden[f"{name}/b"])
+    u, st = s.solve(precond=precond, rtol=1e-13)
+    assert st["status"] == 0 and st["rel_residual"] <= 1e-12
+    assert rel_err(u, golden[f"{name}/u"]) <= U_TOL
+
+
+def test_lepl1110_mesh_problem(golden):
+    """8.lepl1110 + problem.txt as parsed by the reference, solved on the GPU,
+    against the reference's solve and the 8-digit data/U.txt, data/V.txt."""
+    coords = golden["lepl8/coords"]
+    elems = golden["lepl8/elems"]
+    E, nu, rho = golden["lepl8/E_nu_rho"]
+    s = D.System(elems.shape[1], coords, elems, edges=golden["lepl8/edge_nodes"],
+                 edge_boundary=(golden["lepl8/edge_elems"][:, 1] == -1).astype(np.uint8))
+    stress = int(golden["lepl8/sim_kind"]) == 2
+    s.assemble(stress, E, nu, rho, g=(0.0, -9.81))
+    flags = golden["lepl8/cond_flags"]
+    for i in range(int(golden["lepl8/n_conditions"])):
+        kind, value = golden[f"lepl8/cond{i}_kind_value"]
+        s.apply_condition(int(kind), value, flags[i])
+    u, st = s.solve(rtol=1e-13)
+    assert st["status"] == 0
+    assert rel_err(u, golden["lepl8/u"]) <= U_TOL
+    U = read_effects_file(f"{REF_DATA}/U.txt")
+    V = read_effects_file(f"{REF_DATA}/V.txt")
+    # the files hold 8 significant digits
+    assert np.abs(u[0::2] - U).max() <= 5e-8 * np.abs(U).max() + 1e-20
+    assert np.abs(u[1::2] - V).max() <= 5e-8 * np.abs(V).max() + 1e-20
+
+
+def oracle_csr(mesh, stress, forces=None):
+    import oracle as O
+    mat = meshgen.BEAM_MATERIAL
+    kinds, values, flags = meshgen.beam_conditions(mesh)
+    return O.assemble_csr(mesh, stress, mat["E"], mat["nu"], mat["rho"], forces=forces, kinds=kinds,
+                          values=values, flags=flags)
+
+
+@pytest.mark.parametrize("kind", [3, 4])
+def test_mid_mesh_matrix_and_solution(kind):
+    """A jittered 120 x 30 beam: every matrix entry against the oracle's CSR
+    assembly, and u against a direct sparse solve of the oracle's system."""
+    import scipy.sparse.linalg as spla
+    mesh = meshgen.beam(120, 30, kind=kind, jitter=0.2, seed=3)
+    s = D.beam_system(mesh, gravity=(0.0, -9.81))
+    A, b = oracle_csr(mesh, True, forces=[(0.0, -9.81)])
+    Kg = s.get_csr_scipy()
+    Ko = A.to_scipy()
+    diff = abs(Kg - Ko)
+    assert diff.max() <= K_TOL * abs(Ko).max()
+    close_entries(s.get_rhs(), b)
+    u_ref = spla.spsolve(Ko.tocsc(), b)
+    u, st = s.solve(rtol=1e-12)
+    assert st["status"] == 0
+    assert rel_err(u, u_ref) <= U_TOL
+    # block Jacobi reaches the same solution
+    u2, st2 = s.solve(precond=D.PRECOND_BLOCK_JACOBI, rtol=1e-12)
+    assert rel_err(u2, u_ref) <= U_TOL
+
+
+def test_determinism():
+    mesh = meshgen.beam(64, 16, kind=4, jitter=0.2, seed=9)
+    s1 = D.beam_system(mesh, gravity=(0.0, -9.81))
+    s2 = D.beam_system(mesh, gravity=(0.0, -9.81))
+    K1, K2 = s1.get_bsr()[2], s2.get_bsr()[2]
+    assert np.array_equal(K1, K2)
+    u1, _ = s1.solve(rtol=1e-10)
+    u2, _ = s2.solve(rtol=1e-10)
+    u3, _ = s1.solve(rtol=1e-10)
+    assert np.array_equal(u1, u2) and np.array_equal(u1, u3)
+
+
+def test_renumbering_invariance():
+    mesh = meshgen.beam(50, 12, kind=4, jitter=0.2, seed=4)
+    a = D.beam_system(mesh, renumber=True)
+    b = D.beam_system(mesh, renumber=False)
+    p = a.node_perm()
+    assert np.array_equal(np.sort(p), np.arange(mesh.n_nodes))
+    ua, _ = a.solve(rtol=1e-12)
+    ub, _ = b.solve(rtol=1e-12)
+    assert rel_err(ua, ub) <= 1e-9
+
+
+def test_mixed_precision_reaches_fp64_residual():
+    mesh = meshgen.beam(200, 50, kind=4, jitter=0.2, seed=2)
+    mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=2)
+    s = D.beam_system(mesh)
+    u64, st64 = s.solve(rtol=1e-10)
+    u32, st32 = s.solve(precision=D.MIXED, rtol=1e-10)
+    assert st32["status"] == 0 and st32["rel_residual"] <= 1e-10
+    # error of the mixed solution against the FP64 one, stated
+    err = rel_err(u32, u64)
+    print(f"mixed vs fp64: rel error {err:.3e}, iterations {st32['iterations']} vs {st64['iterations']}")
+    assert err <= 1e-7
+
+
+def test_zero_load_gives_zero_solution():
+    mesh = meshgen.beam(8, 4, kind=4)
+    s = D.System.from_mesh(mesh)
+    s.assemble(1, 1e9, 0.3, 0.0)
+    s.apply_condition(0, 0.0, mesh.node_sets["left"])
+    s.apply_condition(1, 0.0, mesh.node_sets["left"])
+    u, st = s.solve()
+    assert st["status"] == 0 and st["iterations"] == 0
+    assert np.all(u == 0)
+
+
+def test_singular_system_reports_failure():
+    mesh = meshgen.beam(8, 4, kind=4)
+    s = D.System.from_mesh(mesh)
+    s.assemble(1, 1e9, 0.3, 1000.0, g=(0.3, -9.81))
+    with pytest.raises(D.BfmxError) as e:
+        s.solve(rtol=1e-12, max_iter=4000)
+    assert e.value.code in (-4, -5)
+
+
+def test_invalid_inputs_are_rejected():
+    with pytest.raises(D.BfmxError) as e:
+        D.System(4, np.zeros((4, 2)), np.array([[0, 1, 2, 7]]))
+    assert e.value.code == -1
+    with pytest.raises(D.BfmxError):
+        D.System(5, np.zeros((4, 2)), np.array([[0, 1, 2, 3, 0]]))
+    mesh = meshgen.beam(4, 2, kind=4)
+    s = D.System.from_mesh(mesh)
+    with pytest.raises(D.BfmxError):
+        s.solve()          # not assembled
+
+
+def test_full_size_beam_residual_against_oracle():
+    """The 2000 x 500 Q1 beam (2,005,002 DOFs): the GPU solution satisfies the
+    oracle's own assembled system to the solver tolerance."""
+    import oracle as O
+    mesh, stress = meshgen.workload("q1_2m")
+    s = D.beam_system(mesh, stress=stress)
+    u, st = s.solve(rtol=1e-8)
+    assert st["status"] == 0 and st["rel_residual"] <= 1e-8
+    A, b = oracle_csr(mesh, stress)
+    r = b - A.matvec(u)
+    assert np.linalg.norm(r) / np.linalg.norm(b) <= 1.0001e-8
+    assert rel_err(s.get_rhs(), b) <= 1e-14
