@@ -610,6 +610,8 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	// in step with the chunk-relative iteration index.
 	int32_t status = BFMX_OK;
 	int restarts = 0;
+	int stalled = 0;
+	double prev_true = 0;
 	int64_t launched = 0;
 	int64_t chunks = 0;
 	bool finished = false;
@@ -651,6 +653,14 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 				finished = true;
 				break;
 			}
+			// The true residual has a floor (rounding of b - A x); three checks
+			// in a row that fail to halve it mean rtol is below that floor.
+			stalled = (prev_true > 0 && last.rr_true > 0.25 * prev_true) ? stalled + 1 : 0;
+			prev_true = last.rr_true;
+			if (stalled >= 3) {
+				status = BFMX_E_STAGNATION;
+				break;
+			}
 			restarts++;
 			chunks = 0;
 			continue;
@@ -688,6 +698,9 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	if (status == BFMX_E_BREAKDOWN) {
 		set_error("CG breakdown after %lld iterations (p.Ap = %g): matrix not SPD or singular",
 		          (long long) last.iter, last.pq);
+	} else if (status == BFMX_E_STAGNATION) {
+		set_error("CG stagnated: true relative residual %g stays above rtol %g (rounding floor)",
+		          last.bb > 0 ? std::sqrt(last.rr_true / last.bb) : 0.0, rtol);
 	} else if (status == BFMX_E_MAXITER) {
 		set_error("CG did not reach rtol %g in %lld iterations (relative residual %g)", rtol, (long long) last.iter,
 		          last.bb > 0 ? std::sqrt(last.rr / last.bb) : 0.0);

@@ -18,7 +18,7 @@ DIRICHLET_X, DIRICHLET_Y, NEUMANN_X, NEUMANN_Y = 0, 1, 2, 3
 NEUMANN_NORMAL, NEUMANN_TANGENT, DIRICHLET_NORMAL, DIRICHLET_TANGENT = 4, 5, 6, 7
 
 ERRORS = {-1: "BFMX_E_ARG", -2: "BFMX_E_HIP", -3: "BFMX_E_NOMEM", -4: "BFMX_E_BREAKDOWN",
-          -5: "BFMX_E_MAXITER", -6: "BFMX_E_NODEV", -7: "BFMX_E_UNSUPPORTED"}
+          -5: "BFMX_E_MAXITER", -6: "BFMX_E_NODEV", -7: "BFMX_E_UNSUPPORTED", -8: "BFMX_E_STAGNATION"}
 
 # every entry point of include/bfmx.h
 EXPORTS = [

@@ -49,6 +49,8 @@ enum {
 	BFMX_E_MAXITER = -5,     /* tolerance not reached within max_iter */
 	BFMX_E_NODEV = -6,       /* no usable GPU */
 	BFMX_E_UNSUPPORTED = -7, /* element kind or option not supported */
+	BFMX_E_STAGNATION = -8,  /* the true residual stopped decreasing above rtol
+	                            (rtol below the attainable accuracy) */
 };
 
 /* Preconditioners. */
