@@ -60,11 +60,14 @@ struct alignas(128) Scalars {
 	double pq;       // p . A p
 	double tol2;     // (rtol ||b||)^2
 	double bb;       // b . b
-	double rr_true;  // ||b - A x||^2 from the residual kernel
+	double rr_true;  // ||b - A x||^2 at the last recomputation
 	double rz_prev;  // r . z of the previous iteration
+	double rr_ref;   // r . r at the last recomputation (reliable updates)
+	double rr_fail;  // true r . r of the last failed tolerance check
 	int64_t iter;
-	int32_t done;    // 0 running, 1 converged (true residual), 2 breakdown
-	int32_t conv;    // recursive residual met the tolerance (FP64 path)
+	int32_t done;    // 0 running, 1 converged, 2 breakdown, 3 stagnated
+	int32_t stall;   // consecutive failed tolerance checks without progress
+	int32_t updates; // true-residual recomputations
 	uint32_t cnt[4]; // last-workgroup counters, reset by the last workgroup
 };
 
@@ -135,8 +138,6 @@ struct bfmx_system {
 	double *d_x = nullptr, *d_r = nullptr, *d_z = nullptr, *d_q = nullptr;
 	double* d_p[2] = {nullptr, nullptr};
 	double* d_dinv = nullptr;        // 4 per node (block inverse) or 2 per node (point)
-	float* d_xs[2] = {nullptr, nullptr};   // FP32 search directions (mixed)
-	float* d_zs = nullptr;                 // FP32 copy of z (mixed)
 	double* d_partial = nullptr;     // per-workgroup partial sums (3 per workgroup)
 	bfmx::Scalars* d_scal = nullptr;
 	bfmx::Scalars* h_scal = nullptr; // pinned host mirror ring (4 entries)

@@ -7,19 +7,21 @@
 //                the last workgroup to finish reduces them and sets
 //                alpha = (r . z) / (p . q), or flags a breakdown.
 //   k_cg_update  x += alpha p, r -= alpha q, z = M^-1 r and the partial sums of
-//                r . z and r . r; the last workgroup sets beta, counts the
-//                iteration and raises the convergence flag.
-// Scalars live in device memory and every kernel returns at once when the
-// flag is up, so a captured graph of kChunk iterations can be replayed
-// without the host looking at each iteration. All reductions are in FP64
-// and in a fixed order (deterministic).
+//                r . z and r . r; the last workgroup sets beta and counts the
+//                iteration.
+// A captured graph holds kChunk iterations followed by k_residual, which
+// recomputes r = b - A x with the FP64 matrix when the recursive residual has
+// met the tolerance (then the true residual decides convergence) or, on the
+// mixed path, has dropped by 1/delta since the last recomputation (a reliable
+// update; the directions are kept). Scalars live in device memory, every
+// kernel returns at once after convergence or breakdown, and the host only
+// reads the scalars between graph replays. All reductions are FP64 and in a
+// fixed order (deterministic).
 //
-// k_residual recomputes r = b - A x with the FP64 matrix. It starts a solve
-// (mode 0: zero directions, beta = 0) and replaces the recursive residual by
-// the true one (mode 1, keeping the directions): after the recursive residual
-// meets the tolerance (FP64 path), and at the end of every graph chunk
-// (mixed path, whose iteration uses the FP32 matrix and FP32 directions while
-// x, r, z, the dots and this check stay FP64).
+// Precision: FP64 uses the FP64 matrix. MIXED streams the FP32 copy of the
+// matrix values (half the bytes; the 2M-DOF copy fits the MI355X's 256 MB
+// Infinity Cache) with FP64 vectors, FP64 accumulation, FP64 dots and the
+// FP64 true-residual recomputation above.
 //
 // Preconditioners: point Jacobi (1 / diag) or 2x2 block Jacobi.
 
@@ -35,17 +37,23 @@ namespace {
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 typedef float flt2 __attribute__((ext_vector_type(2)));
 
+constexpr double kDelta2 = 1e-2;   // reliable update when ||r|| dropped 10x since the last one
+
 __device__ __forceinline__ dbl2 ld2(const double* p) {
 	return *reinterpret_cast<const dbl2*>(p);
 }
 __device__ __forceinline__ void st2(double* p, dbl2 v) {
 	*reinterpret_cast<dbl2*>(p) = v;
 }
-__device__ __forceinline__ dbl2 ld2_stream(const double* p) {
+
+// Two consecutive matrix values, streamed (non-temporal: the matrix is not
+// re-read within a kernel, keeping L2 for the gathered vectors).
+__device__ __forceinline__ dbl2 ldv(const double* p) {
 	return __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(p));
 }
-__device__ __forceinline__ flt2 ld2f_stream(const float* p) {
-	return __builtin_nontemporal_load(reinterpret_cast<const flt2*>(p));
+__device__ __forceinline__ dbl2 ldv(const float* p) {
+	flt2 const f = __builtin_nontemporal_load(reinterpret_cast<const flt2*>(p));
+	return dbl2{f.x, f.y};
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -145,17 +153,18 @@ __device__ __forceinline__ dbl2 apply_precond(const double* __restrict__ dinv, i
 	}
 }
 
-// y = A x for block row `row` with the FP64 matrix; ldx(c) returns x at node c.
-template <typename LoadX>
-__device__ __forceinline__ dbl2 row_spmv64(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ col,
-                                           const double* __restrict__ val, int row, LoadX ldx) {
+// y = A x for block row `row`; MT is the stored value type (double or float),
+// the accumulation is FP64. ldx(c) returns x at node c.
+template <typename MT, typename LoadX>
+__device__ __forceinline__ dbl2 row_spmv(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ col,
+                                         const MT* __restrict__ val, int row, LoadX ldx) {
 	int const s = row >> 6, r = row & 63;
 	int32_t const beg = slice_ptr[s], end = slice_ptr[s + 1];
-	const double* v = val + 4 * static_cast<int64_t>(beg) + 2 * r;
+	const MT* v = val + 4 * static_cast<int64_t>(beg) + 2 * r;
 	double y0 = 0, y1 = 0;
 	for (int32_t blk = beg + r; blk < end; blk += kSlice, v += 4 * kSlice) {
 		int32_t const c = __builtin_nontemporal_load(col + blk);
-		dbl2 const t = ld2_stream(v), u = ld2_stream(v + 2 * kSlice);
+		dbl2 const t = ldv(v), u = ldv(v + 2 * kSlice);
 		dbl2 const xv = ldx(c);
 		y0 += t.x * xv.x + t.y * xv.y;
 		y1 += u.x * xv.x + u.y * xv.y;
@@ -163,30 +172,12 @@ __device__ __forceinline__ dbl2 row_spmv64(const int32_t* __restrict__ slice_ptr
 	return dbl2{y0, y1};
 }
 
-// Same with the FP32 copy of the matrix (value layout identical, 4-byte entries).
-template <typename LoadX>
-__device__ __forceinline__ dbl2 row_spmv32(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ col,
-                                           const float* __restrict__ val, int row, LoadX ldx) {
-	int const s = row >> 6, r = row & 63;
-	int32_t const beg = slice_ptr[s], end = slice_ptr[s + 1];
-	const float* v = val + 4 * static_cast<int64_t>(beg) + 2 * r;
-	float y0 = 0, y1 = 0;
-	for (int32_t blk = beg + r; blk < end; blk += kSlice, v += 4 * kSlice) {
-		int32_t const c = __builtin_nontemporal_load(col + blk);
-		flt2 const t = ld2f_stream(v), u = ld2f_stream(v + 2 * kSlice);
-		flt2 const xv = ldx(c);
-		y0 += t.x * xv.x + t.y * xv.y;
-		y1 += u.x * xv.x + u.y * xv.y;
-	}
-	return dbl2{y0, y1};
-}
-
-__global__ __launch_bounds__(kBlock) void k_cg_spmv64(int nb, const int32_t* __restrict__ slice_ptr,
-                                                      const int32_t* __restrict__ col,
-                                                      const double* __restrict__ val, const double* __restrict__ z,
-                                                      const double* __restrict__ pold, double* __restrict__ pnew,
-                                                      double* __restrict__ q, double* __restrict__ partial,
-                                                      Scalars* __restrict__ S) {
+template <typename MT>
+__global__ __launch_bounds__(kBlock) void k_cg_spmv(int nb, const int32_t* __restrict__ slice_ptr,
+                                                    const int32_t* __restrict__ col, const MT* __restrict__ val,
+                                                    const double* __restrict__ z, const double* __restrict__ pold,
+                                                    double* __restrict__ pnew, double* __restrict__ q,
+                                                    double* __restrict__ partial, Scalars* __restrict__ S) {
 	if (S->done) {
 		return;
 	}
@@ -194,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv64(int nb, const int32_t* __r
 	int const row = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
 	double v[1] = {0};
 	if (row < nb) {
-		dbl2 const y = row_spmv64(slice_ptr, col, val, row, [&](int c) {
+		dbl2 const y = row_spmv(slice_ptr, col, val, row, [&](int c) {
 			return ld2(z + 2 * c) + beta * ld2(pold + 2 * c);
 		});
 		dbl2 const pn = ld2(z + 2 * row) + beta * ld2(pold + 2 * row);
@@ -214,48 +205,11 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv64(int nb, const int32_t* __r
 	}
 }
 
-__global__ __launch_bounds__(kBlock) void k_cg_spmv32(int nb, const int32_t* __restrict__ slice_ptr,
-                                                      const int32_t* __restrict__ col,
-                                                      const float* __restrict__ val, const float* __restrict__ z,
-                                                      const float* __restrict__ pold, float* __restrict__ pnew,
-                                                      double* __restrict__ q, double* __restrict__ partial,
-                                                      Scalars* __restrict__ S) {
-	if (S->done) {
-		return;
-	}
-	float const beta = static_cast<float>(S->beta);
-	int const row = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-	double v[1] = {0};
-	if (row < nb) {
-		auto ldx = [&](int c) {
-			return *reinterpret_cast<const flt2*>(z + 2 * c) + beta * *reinterpret_cast<const flt2*>(pold + 2 * c);
-		};
-		dbl2 const y = row_spmv32(slice_ptr, col, val, row, ldx);
-		flt2 const pn = ldx(row);
-		*reinterpret_cast<flt2*>(pnew + 2 * row) = pn;
-		st2(q + 2 * row, y);
-		v[0] = static_cast<double>(pn.x) * y.x + static_cast<double>(pn.y) * y.y;
-	}
-	double tot[1];
-	if (last_block_reduce<1>(v, partial, &S->cnt[0], tot) && threadIdx.x == 0) {
-		double const pq = tot[0];
-		S->pq = pq;
-		if (!(pq > 0) || !isfinite(pq)) {
-			S->done = 2;
-		} else {
-			S->alpha = S->rz / pq;
-		}
-	}
-}
-
-// MIXED: p is FP32 (p32), z is also written as FP32 for the next gather, and
-// the recursive residual never ends the run (the FP64 check does).
-template <int PRE, bool MIXED>
+template <int PRE>
 __global__ __launch_bounds__(kBlock) void k_cg_update(int nb, const double* __restrict__ p,
-                                                      const float* __restrict__ p32, const double* __restrict__ q,
-                                                      const double* __restrict__ dinv, double* __restrict__ x,
-                                                      double* __restrict__ r, double* __restrict__ z,
-                                                      float* __restrict__ z32, double* __restrict__ partial,
+                                                      const double* __restrict__ q, const double* __restrict__ dinv,
+                                                      double* __restrict__ x, double* __restrict__ r,
+                                                      double* __restrict__ z, double* __restrict__ partial,
                                                       Scalars* __restrict__ S) {
 	if (S->done) {
 		return;
@@ -264,23 +218,12 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int nb, const double* __re
 	int const i = blockIdx.x * kBlock + threadIdx.x;
 	double v[2] = {0, 0};
 	if (i < nb) {
-		dbl2 pv;
-		if (MIXED) {
-			flt2 const pf = *reinterpret_cast<const flt2*>(p32 + 2 * i);
-			pv = dbl2{pf.x, pf.y};
-		} else {
-			pv = ld2(p + 2 * i);
-		}
-		dbl2 const xv = ld2(x + 2 * i) + alpha * pv;
+		dbl2 const xv = ld2(x + 2 * i) + alpha * ld2(p + 2 * i);
 		dbl2 const rv = ld2(r + 2 * i) - alpha * ld2(q + 2 * i);
 		dbl2 const zv = apply_precond<PRE>(dinv, i, rv);
 		st2(x + 2 * i, xv);
 		st2(r + 2 * i, rv);
-		if (MIXED) {
-			*reinterpret_cast<flt2*>(z32 + 2 * i) = flt2{static_cast<float>(zv.x), static_cast<float>(zv.y)};
-		} else {
-			st2(z + 2 * i, zv);
-		}
+		st2(z + 2 * i, zv);
 		v[0] = rv.x * zv.x + rv.y * zv.y;
 		v[1] = rv.x * rv.x + rv.y * rv.y;
 	}
@@ -292,50 +235,44 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int nb, const double* __re
 		S->rz = rz;
 		S->rr = tot[1];
 		S->iter += 1;
-		if (!MIXED && tot[1] <= S->tol2) {
-			S->conv = 1;
-		}
 	}
 }
 
-// r = b - A x (FP64), z = M^-1 r. mode 0 starts a run (directions zeroed,
-// beta = 0); mode 1 replaces the recursive residual and keeps the directions
-// (beta = r'.z' / previous r.z). Sets done = 1 when ||r|| <= rtol ||b||, else 0.
-template <int PRE, bool MIXED>
-__global__ __launch_bounds__(kBlock) void k_residual(int nb, int mode, double rtol2,
+// r = b - A x with the FP64 matrix, z = M^-1 r.
+// mode 0 starts a run: directions zeroed, beta = 0.
+// mode 1 runs only when the recursive residual met the tolerance, or dropped
+// below delta2 times its value at the last recomputation; it replaces r and z
+// and keeps the directions (beta = r'.z' / previous r.z). When the true
+// residual meets the tolerance the run ends (done = 1); when a tolerance check
+// fails three times without halving the true residual, the run stagnated at
+// the rounding floor (done = 3).
+template <int PRE>
+__global__ __launch_bounds__(kBlock) void k_residual(int nb, int mode, double rtol2, double delta2,
                                                      const int32_t* __restrict__ slice_ptr,
                                                      const int32_t* __restrict__ col,
                                                      const double* __restrict__ val, const double* __restrict__ b,
                                                      const double* __restrict__ dinv, const double* __restrict__ x,
                                                      double* __restrict__ r, double* __restrict__ z,
-                                                     float* __restrict__ z32, double* __restrict__ p0,
-                                                     double* __restrict__ p1, float* __restrict__ ps0,
-                                                     float* __restrict__ ps1, double* __restrict__ partial,
-                                                     Scalars* __restrict__ S) {
-	if (S->done == 2) {
-		return;
+                                                     double* __restrict__ p0, double* __restrict__ p1,
+                                                     double* __restrict__ partial, Scalars* __restrict__ S) {
+	bool const at_tol = mode == 1 && S->rr <= S->tol2;
+	if (mode == 1) {
+		if (S->done || !(at_tol || S->rr <= delta2 * S->rr_ref)) {
+			return;
+		}
 	}
 	int const row = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
 	double v[3] = {0, 0, 0};
 	if (row < nb) {
-		dbl2 const ax = row_spmv64(slice_ptr, col, val, row, [&](int c) { return ld2(x + 2 * c); });
+		dbl2 const ax = row_spmv(slice_ptr, col, val, row, [&](int c) { return ld2(x + 2 * c); });
 		dbl2 const bv = ld2(b + 2 * row);
 		dbl2 const rv = bv - ax;
 		dbl2 const zv = apply_precond<PRE>(dinv, row, rv);
 		st2(r + 2 * row, rv);
-		if (MIXED) {
-			*reinterpret_cast<flt2*>(z32 + 2 * row) = flt2{static_cast<float>(zv.x), static_cast<float>(zv.y)};
-		} else {
-			st2(z + 2 * row, zv);
-		}
+		st2(z + 2 * row, zv);
 		if (mode == 0) {
-			if (MIXED) {
-				*reinterpret_cast<flt2*>(ps0 + 2 * row) = flt2{0.f, 0.f};
-				*reinterpret_cast<flt2*>(ps1 + 2 * row) = flt2{0.f, 0.f};
-			} else {
-				st2(p0 + 2 * row, dbl2{0, 0});
-				st2(p1 + 2 * row, dbl2{0, 0});
-			}
+			st2(p0 + 2 * row, dbl2{0, 0});
+			st2(p1 + 2 * row, dbl2{0, 0});
 		}
 		v[0] = rv.x * zv.x + rv.y * zv.y;
 		v[1] = rv.x * rv.x + rv.y * rv.y;
@@ -343,19 +280,31 @@ __global__ __launch_bounds__(kBlock) void k_residual(int nb, int mode, double rt
 	}
 	double tot[3];
 	if (last_block_reduce<3>(v, partial, &S->cnt[2], tot) && threadIdx.x == 0) {
+		double const rr = tot[1];
 		S->bb = tot[2];
 		S->tol2 = rtol2 * tot[2];
-		S->rr = tot[1];
-		S->rr_true = tot[1];
+		S->rr = rr;
+		S->rr_true = rr;
+		S->rr_ref = rr;
+		S->updates += 1;
 		if (mode == 0) {
 			S->beta = 0;
 			S->iter = 0;
+			S->stall = 0;
+			S->rr_fail = 0;
 		} else {
 			S->beta = S->rz_prev != 0 ? tot[0] / S->rz_prev : 0;
 		}
 		S->rz = tot[0];
-		S->conv = 0;
-		S->done = tot[1] <= S->tol2 ? 1 : 0;
+		if (rr <= S->tol2) {
+			S->done = 1;
+		} else if (at_tol) {
+			S->stall = (S->rr_fail > 0 && rr > 0.25 * S->rr_fail) ? S->stall + 1 : 0;
+			S->rr_fail = rr;
+			if (S->stall >= 3) {
+				S->done = 3;
+			}
+		}
 	}
 }
 
@@ -395,13 +344,6 @@ __global__ __launch_bounds__(kBlock) void k_fill(int64_t n, double v, double* __
 	}
 }
 
-__global__ __launch_bounds__(kBlock) void k_fill32(int64_t n, float v, float* __restrict__ a) {
-	int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-	if (i < n) {
-		a[i] = v;
-	}
-}
-
 int grid(int64_t n) {
 	return static_cast<int>((n + kBlock - 1) / kBlock);
 }
@@ -430,13 +372,7 @@ int ensure_workspace(bfmx_system* s) {
 	return BFMX_OK;
 }
 
-int ensure_mixed(bfmx_system* s) {
-	int64_t const n = s->n_nodes;
-	if (!s->d_xs[0]) {
-		BFMX_CHECK(dalloc(&s->d_xs[0], 2 * n));
-		BFMX_CHECK(dalloc(&s->d_xs[1], 2 * n));
-		BFMX_CHECK(dalloc(&s->d_zs, 2 * n));
-	}
+int ensure_f32(bfmx_system* s) {
 	if (!s->d_val32) {
 		BFMX_CHECK(dalloc(&s->d_val32, 4 * s->nnzb_pad));
 		k_to_f32<<<grid(4 * s->nnzb_pad), kBlock, 0, s->stream>>>(4 * s->nnzb_pad, s->d_val, s->d_val32);
@@ -445,58 +381,42 @@ int ensure_mixed(bfmx_system* s) {
 	return BFMX_OK;
 }
 
-void launch_residual(bfmx_system* s, int pre, bool mixed, int mode, double rtol2) {
+void launch_residual(bfmx_system* s, int pre, int mode, double rtol2, double delta2) {
 	int const nb = static_cast<int>(s->n_nodes);
-#define BFMX_RES(P, M)                                                                                             \
-	k_residual<P, M><<<grid(nb), kBlock, 0, s->stream>>>(nb, mode, rtol2, s->d_slice_ptr, s->d_col, s->d_val,      \
-	                                                     s->d_bsolve, s->d_dinv, s->d_x, s->d_r, s->d_z, s->d_zs, \
-	                                                     s->d_p[0], s->d_p[1], s->d_xs[0], s->d_xs[1],            \
-	                                                     s->d_partial, s->d_scal)
 	if (pre == BFMX_PRECOND_JACOBI) {
-		if (mixed) {
-			BFMX_RES(BFMX_PRECOND_JACOBI, true);
-		} else {
-			BFMX_RES(BFMX_PRECOND_JACOBI, false);
-		}
+		k_residual<BFMX_PRECOND_JACOBI><<<grid(nb), kBlock, 0, s->stream>>>(
+		    nb, mode, rtol2, delta2, s->d_slice_ptr, s->d_col, s->d_val, s->d_bsolve, s->d_dinv, s->d_x, s->d_r,
+		    s->d_z, s->d_p[0], s->d_p[1], s->d_partial, s->d_scal);
 	} else {
-		if (mixed) {
-			BFMX_RES(BFMX_PRECOND_BLOCK_JACOBI, true);
-		} else {
-			BFMX_RES(BFMX_PRECOND_BLOCK_JACOBI, false);
-		}
+		k_residual<BFMX_PRECOND_BLOCK_JACOBI><<<grid(nb), kBlock, 0, s->stream>>>(
+		    nb, mode, rtol2, delta2, s->d_slice_ptr, s->d_col, s->d_val, s->d_bsolve, s->d_dinv, s->d_x, s->d_r,
+		    s->d_z, s->d_p[0], s->d_p[1], s->d_partial, s->d_scal);
 	}
-#undef BFMX_RES
 }
 
-// One CG iteration; it = position in the chunk selects the direction buffers.
+void launch_spmv(bfmx_system* s, bool mixed, const double* pold, double* pnew) {
+	int const nb = static_cast<int>(s->n_nodes);
+	if (mixed) {
+		k_cg_spmv<float><<<grid(nb), kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val32, s->d_z, pold,
+		                                                     pnew, s->d_q, s->d_partial, s->d_scal);
+	} else {
+		k_cg_spmv<double><<<grid(nb), kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, s->d_z, pold,
+		                                                      pnew, s->d_q, s->d_partial, s->d_scal);
+	}
+}
+
+// One CG iteration; `it` (position in the chunk) selects the direction buffers.
 void launch_iteration(bfmx_system* s, int pre, bool mixed, int it) {
 	int const nb = static_cast<int>(s->n_nodes);
-	int const g = grid(nb);
 	int const a = it & 1, b = a ^ 1;
-	if (mixed) {
-		k_cg_spmv32<<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val32, s->d_zs, s->d_xs[a],
-		                                         s->d_xs[b], s->d_q, s->d_partial, s->d_scal);
-	} else {
-		k_cg_spmv64<<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, s->d_z, s->d_p[a],
-		                                         s->d_p[b], s->d_q, s->d_partial, s->d_scal);
-	}
-#define BFMX_UPD(P, M)                                                                                     \
-	k_cg_update<P, M><<<g, kBlock, 0, s->stream>>>(nb, s->d_p[b], s->d_xs[b], s->d_q, s->d_dinv, s->d_x,   \
-	                                               s->d_r, s->d_z, s->d_zs, s->d_partial, s->d_scal)
+	launch_spmv(s, mixed, s->d_p[a], s->d_p[b]);
 	if (pre == BFMX_PRECOND_JACOBI) {
-		if (mixed) {
-			BFMX_UPD(BFMX_PRECOND_JACOBI, true);
-		} else {
-			BFMX_UPD(BFMX_PRECOND_JACOBI, false);
-		}
+		k_cg_update<BFMX_PRECOND_JACOBI><<<grid(nb), kBlock, 0, s->stream>>>(
+		    nb, s->d_p[b], s->d_q, s->d_dinv, s->d_x, s->d_r, s->d_z, s->d_partial, s->d_scal);
 	} else {
-		if (mixed) {
-			BFMX_UPD(BFMX_PRECOND_BLOCK_JACOBI, true);
-		} else {
-			BFMX_UPD(BFMX_PRECOND_BLOCK_JACOBI, false);
-		}
+		k_cg_update<BFMX_PRECOND_BLOCK_JACOBI><<<grid(nb), kBlock, 0, s->stream>>>(
+		    nb, s->d_p[b], s->d_q, s->d_dinv, s->d_x, s->d_r, s->d_z, s->d_partial, s->d_scal);
 	}
-#undef BFMX_UPD
 }
 
 int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
@@ -512,9 +432,7 @@ int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
 	for (int it = 0; it < kChunk; it++) {
 		launch_iteration(s, pre, mixed, it);
 	}
-	if (mixed) {
-		launch_residual(s, pre, true, 1, rtol2);
-	}
+	launch_residual(s, pre, 1, rtol2, mixed ? kDelta2 : 0.0);
 	BFMX_HIP(hipStreamEndCapture(s->stream, &g));
 	hipError_t const e = hipGraphInstantiate(&s->graph.exec, g, nullptr, nullptr, 0);
 	(void) hipGraphDestroy(g);
@@ -541,9 +459,6 @@ void release_solver(bfmx_system* s) {
 	dfree(s->d_q);
 	dfree(s->d_p[0]);
 	dfree(s->d_p[1]);
-	dfree(s->d_xs[0]);
-	dfree(s->d_xs[1]);
-	dfree(s->d_zs);
 	dfree(s->d_dinv);
 	dfree(s->d_partial);
 	dfree(s->d_scal);
@@ -580,7 +495,7 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	hipStream_t const st = s->stream;
 	BFMX_CHECK(ensure_workspace(s));
 	if (mixed) {
-		BFMX_CHECK(ensure_mixed(s));
+		BFMX_CHECK(ensure_f32(s));
 	}
 	if (b_in) {
 		std::vector<double> bp(2 * n);
@@ -597,30 +512,18 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2));
 
 	auto const t0 = std::chrono::steady_clock::now();
-	BFMX_HIP(hipEventRecord(s->ev[4], st));
 	k_precond<<<grid(n), kBlock, 0, st>>>(static_cast<int>(n), precond, s->d_diag_blk, s->d_val, s->d_dinv);
 	BFMX_HIP(hipMemsetAsync(s->d_x, 0, 2 * n * sizeof(double), st));
 	BFMX_HIP(hipMemsetAsync(s->d_scal, 0, sizeof(Scalars), st));
-	launch_residual(s, precond, mixed, 0, rtol2);
+	launch_residual(s, precond, 0, rtol2, 0.0);
 	BFMX_HIP(hipGetLastError());
-
-	// Replay chunks, keeping one in flight while the previous one's scalars
-	// are inspected. Chunks always run whole (the FP64 path's recursive
-	// convergence only raises `conv`), so the direction ping-pong buffers stay
-	// in step with the chunk-relative iteration index.
-	int32_t status = BFMX_OK;
-	int restarts = 0;
-	int stalled = 0;
-	double prev_true = 0;
-	int64_t launched = 0;
-	int64_t chunks = 0;
-	bool finished = false;
-	Scalars last{};
 	BFMX_HIP(hipMemcpyAsync(&s->h_scal[0], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
 	BFMX_HIP(hipStreamSynchronize(st));
-	last = s->h_scal[0];
-	finished = last.done == 1;
-	while (!finished) {
+	Scalars last = s->h_scal[0];
+
+	// Replay chunks, keeping one in flight while the one before it is inspected.
+	int64_t launched = 0, chunks = 0;
+	while (last.done == 0) {
 		bool const more = launched < cap;
 		if (more) {
 			BFMX_HIP(hipGraphLaunch(s->graph.exec, st));
@@ -633,74 +536,40 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		int64_t const look = (more && chunks >= 2) ? chunks - 2 : chunks - 1;
 		BFMX_HIP(hipEventSynchronize(s->ev[look & 3]));
 		last = s->h_scal[look & 3];
-		if (last.done == 2) {
-			status = BFMX_E_BREAKDOWN;
-			break;
-		}
-		if (last.done == 1) {
-			finished = true;
-			break;
-		}
-		if (!mixed && last.conv) {
-			// confirm with the true residual; when it falls short, continue from it
-			BFMX_HIP(hipStreamSynchronize(st));
-			launch_residual(s, precond, false, 1, rtol2);
-			BFMX_HIP(hipGetLastError());
-			BFMX_HIP(hipMemcpyAsync(&s->h_scal[0], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
-			BFMX_HIP(hipStreamSynchronize(st));
-			last = s->h_scal[0];
-			if (last.done == 1) {
-				finished = true;
-				break;
-			}
-			// The true residual has a floor (rounding of b - A x); three checks
-			// in a row that fail to halve it mean rtol is below that floor.
-			stalled = (prev_true > 0 && last.rr_true > 0.25 * prev_true) ? stalled + 1 : 0;
-			prev_true = last.rr_true;
-			if (stalled >= 3) {
-				status = BFMX_E_STAGNATION;
-				break;
-			}
-			restarts++;
-			chunks = 0;
-			continue;
-		}
 		if (!more && look == chunks - 1) {
-			status = BFMX_E_MAXITER;
 			break;
 		}
 	}
 	BFMX_HIP(hipStreamSynchronize(st));
-	BFMX_HIP(hipEventRecord(s->ev[5], st));
-	BFMX_HIP(hipEventSynchronize(s->ev[5]));
 	auto const t1 = std::chrono::steady_clock::now();
-	float ms = 0;
-	BFMX_HIP(hipEventElapsedTime(&ms, s->ev[4], s->ev[5]));
 	BFMX_HIP(hipMemcpy(&last, s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost));
-	if (status == BFMX_OK && !finished) {
-		status = BFMX_E_MAXITER;
+	int32_t status = BFMX_OK;
+	switch (last.done) {
+		case 1: status = BFMX_OK; break;
+		case 2: status = BFMX_E_BREAKDOWN; break;
+		case 3: status = BFMX_E_STAGNATION; break;
+		default: status = BFMX_E_MAXITER; break;
 	}
 	s->last_iters = last.iter;
 	s->has_u = true;
+	double const rel = last.bb > 0 ? std::sqrt(last.rr_true / last.bb) : 0.0;
 	if (stats) {
 		stats->status = status;
 		stats->precond = precond;
 		stats->precision = precision;
-		stats->restarts = restarts;
+		stats->restarts = static_cast<int32_t>(last.updates);
 		stats->iterations = last.iter;
 		stats->n_dof = 2 * n;
 		stats->rtol = rtol;
-		stats->rel_residual = last.bb > 0 ? std::sqrt(last.rr_true / last.bb) : 0.0;
+		stats->rel_residual = rel;
 		stats->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
 		stats->b_norm = std::sqrt(last.bb);
-		(void) ms;
 	}
 	if (status == BFMX_E_BREAKDOWN) {
 		set_error("CG breakdown after %lld iterations (p.Ap = %g): matrix not SPD or singular",
 		          (long long) last.iter, last.pq);
 	} else if (status == BFMX_E_STAGNATION) {
-		set_error("CG stagnated: true relative residual %g stays above rtol %g (rounding floor)",
-		          last.bb > 0 ? std::sqrt(last.rr_true / last.bb) : 0.0, rtol);
+		set_error("CG stagnated: true relative residual %g stays above rtol %g (rounding floor)", rel, rtol);
 	} else if (status == BFMX_E_MAXITER) {
 		set_error("CG did not reach rtol %g in %lld iterations (relative residual %g)", rtol, (long long) last.iter,
 		          last.bb > 0 ? std::sqrt(last.rr / last.bb) : 0.0);
@@ -715,37 +584,22 @@ int time_spmv(bfmx_system* s, int precision, int reps, double* avg_ms, double* b
 	BFMX_CHECK(finalize_conditions(s));
 	BFMX_CHECK(ensure_workspace(s));
 	if (mixed) {
-		BFMX_CHECK(ensure_mixed(s));
+		BFMX_CHECK(ensure_f32(s));
 	}
-	// z = 1, p = 0: p.Ap = 1^T A 1 > 0, identical for every repetition
+	// z = 1, p = 0: p.Ap = 1^T A 1 > 0, identical for every repetition.
+	// Uses the solver's r/z/p/q buffers; x (the last solution) is untouched.
 	Scalars init{};
 	init.rz = 1.0;
 	BFMX_HIP(hipMemcpyAsync(s->d_scal, &init, sizeof(Scalars), hipMemcpyHostToDevice, st));
-	if (mixed) {
-		k_fill32<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 1.0f, s->d_zs);
-		k_fill32<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 0.0f, s->d_xs[0]);
-	} else {
-		k_fill<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 1.0, s->d_z);
-		k_fill<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 0.0, s->d_p[0]);
-	}
-	int const nb = static_cast<int>(n);
-	int const g = grid(nb);
-	auto launch = [&]() {
-		if (mixed) {
-			k_cg_spmv32<<<g, kBlock, 0, st>>>(nb, s->d_slice_ptr, s->d_col, s->d_val32, s->d_zs, s->d_xs[0],
-			                                  s->d_xs[1], s->d_q, s->d_partial, s->d_scal);
-		} else {
-			k_cg_spmv64<<<g, kBlock, 0, st>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, s->d_z, s->d_p[0], s->d_p[1],
-			                                  s->d_q, s->d_partial, s->d_scal);
-		}
-	};
+	k_fill<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 1.0, s->d_z);
+	k_fill<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 0.0, s->d_p[0]);
 	for (int i = 0; i < 3; i++) {
-		launch();
+		launch_spmv(s, mixed, s->d_p[0], s->d_p[1]);
 	}
 	BFMX_HIP(hipGetLastError());
 	BFMX_HIP(hipEventRecord(s->ev[4], st));
 	for (int i = 0; i < reps; i++) {
-		launch();
+		launch_spmv(s, mixed, s->d_p[0], s->d_p[1]);
 	}
 	BFMX_HIP(hipEventRecord(s->ev[5], st));
 	BFMX_HIP(hipEventSynchronize(s->ev[5]));
@@ -761,10 +615,9 @@ int time_spmv(bfmx_system* s, int precision, int reps, double* avg_ms, double* b
 	if (bytes) {
 		double const nd = 2.0 * static_cast<double>(n);
 		double const vb = mixed ? 4.0 : 8.0;
-		// matrix blocks (values + column index) + slice pointers
-		// + gathered z and p_old (once each) + written p_new + written q (FP64)
-		*bytes = static_cast<double>(s->nnzb) * (4 * vb + 4) + 4.0 * (s->n_slices + 1) + 2 * nd * vb + nd * vb +
-		         nd * 8.0;
+		// matrix blocks (4 values + column index) + slice pointers, the gathered
+		// z and p_old (read once each), the written p_new and q
+		*bytes = static_cast<double>(s->nnzb) * (4 * vb + 4) + 4.0 * (s->n_slices + 1) + 4 * nd * 8.0;
 	}
 	return BFMX_OK;
 }
