@@ -50,8 +50,8 @@ void set_error(const char* fmt, ...);
 		}                         \
 	} while (0)
 
-// Device-resident scalars of one CG run. Written by the last workgroup of a
-// kernel, read by the next kernel (kernel boundaries order the accesses).
+// Device-resident scalars of one CG run. Written by the one-workgroup reduce
+// kernels, read by later kernels (kernel boundaries order the accesses).
 struct alignas(128) Scalars {
 	double rz;       // r . z
 	double alpha;
@@ -68,7 +68,6 @@ struct alignas(128) Scalars {
 	int32_t done;    // 0 running, 1 converged, 2 breakdown, 3 stagnated
 	int32_t stall;   // consecutive failed tolerance checks without progress
 	int32_t updates; // true-residual recomputations
-	uint32_t cnt[4]; // last-workgroup counters, reset by the last workgroup
 };
 
 struct Graph {
