@@ -30,6 +30,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 
 #include "internal.h"
 
@@ -469,6 +470,22 @@ void launch_iteration(bfmx_system* s, int pre, bool mixed, int it) {
 	k_red_beta<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
 }
 
+// kChunk iterations and the conditional residual recomputation.
+void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
+	for (int it = 0; it < kChunk; it++) {
+		launch_iteration(s, pre, mixed, it);
+	}
+	launch_residual(s, pre, 1, rtol2, mixed ? kDelta2 : 0.0);
+}
+
+// Chunks are replayed from a captured HIP graph unless the environment sets
+// BFMX_GRAPHS=0, which launches the same kernels straight on the stream (for
+// tools that do not follow graph launches).
+bool graphs_enabled() {
+	const char* e = std::getenv("BFMX_GRAPHS");
+	return !(e && e[0] == '0');
+}
+
 int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
 	if (s->graph.exec && s->graph.precond == pre && s->graph.precision == (mixed ? 1 : 0) && s->graph.rtol2 == rtol2) {
 		return BFMX_OK;
@@ -479,10 +496,7 @@ int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
 	}
 	hipGraph_t g = nullptr;
 	BFMX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-	for (int it = 0; it < kChunk; it++) {
-		launch_iteration(s, pre, mixed, it);
-	}
-	launch_residual(s, pre, 1, rtol2, mixed ? kDelta2 : 0.0);
+	launch_chunk(s, pre, mixed, rtol2);
 	BFMX_HIP(hipStreamEndCapture(s->stream, &g));
 	hipError_t const e = hipGraphInstantiate(&s->graph.exec, g, nullptr, nullptr, 0);
 	(void) hipGraphDestroy(g);
@@ -559,7 +573,10 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		BFMX_HIP(hipMemcpyAsync(s->d_bsolve, s->d_b, 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
 	}
 	double const rtol2 = rtol * rtol;
-	BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2));
+	bool const use_graph = graphs_enabled();
+	if (use_graph) {
+		BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2));
+	}
 
 	auto const t0 = std::chrono::steady_clock::now();
 	k_precond<<<grid(n), kBlock, 0, st>>>(static_cast<int>(n), precond, s->d_diag_blk, s->d_val, s->d_dinv);
@@ -576,7 +593,12 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	while (last.done == 0) {
 		bool const more = launched < cap;
 		if (more) {
-			BFMX_HIP(hipGraphLaunch(s->graph.exec, st));
+			if (use_graph) {
+				BFMX_HIP(hipGraphLaunch(s->graph.exec, st));
+			} else {
+				launch_chunk(s, precond, mixed, rtol2);
+				BFMX_HIP(hipGetLastError());
+			}
 			launched += kChunk;
 			int const slot = static_cast<int>(chunks & 3);
 			BFMX_HIP(hipMemcpyAsync(&s->h_scal[slot], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
