@@ -47,6 +47,17 @@ def _sources(d, ext):
     return sorted(os.path.join(d, f) for f in os.listdir(d) if f.endswith(ext)) if os.path.isdir(d) else []
 
 
+def public_api():
+    """Function names declared by include/bfm/*.h."""
+    import re
+    names = set()
+    hdir = os.path.join(INC, "bfm")
+    for h in os.listdir(hdir):
+        text = re.sub(r"/\*.*?\*/", "", open(os.path.join(hdir, h)).read(), flags=re.S)
+        names |= set(re.findall(r"\b(bfm_\w+)\s*\(", text))
+    return names
+
+
 def build(jobs=8, verbose=False):
     os.makedirs(OBJ_DIR, exist_ok=True)
     hip_src = _sources(HIP_DIR, ".hip")
@@ -77,8 +88,12 @@ def build(jobs=8, verbose=False):
             hobjs.append(obj)
         bfm_lib = os.path.join(LIB_DIR, "libbfm.so")
         tmp = bfm_lib + ".tmp"
+        # export exactly the functions the public headers declare
+        vscript = os.path.join(OBJ_DIR, "bfm.map")
+        with open(vscript, "w") as f:
+            f.write("{ global: " + " ".join(f"{n};" for n in sorted(public_api())) + " local: *; };\n")
         _run([CC, "-shared", "-o", tmp, *hobjs, "-L", LIB_DIR, "-lbfmhip", "-Wl,-rpath,$ORIGIN",
-              "-Wl,-soname,libbfm.so.1", "-lm", "-lpthread"])
+              "-Wl,-soname,libbfm.so.1", f"-Wl,--version-script={vscript}", "-lm", "-lpthread"])
         os.replace(tmp, bfm_lib)
     shutil.rmtree(OBJ_DIR, ignore_errors=True)
     if verbose:

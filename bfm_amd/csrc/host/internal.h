@@ -1,6 +1,7 @@
 /* Internal helpers of the libbfm host layer. */
 #pragma once
 
+#include <bfm/ez.h>
 #include <bfm/gpu.h>
 #include <bfm/sim.h>
 #include <bfm/system.h>

@@ -56,9 +56,14 @@ def test_bfmx_without_gpu_refuses():
     assert e.value.code == -6
 
 
-def public_functions():
+EXTENSION_HEADERS = {"gpu.h"}   # this implementation's additions to the API
+
+
+def public_functions(include_extensions=False):
     names = set()
     for h in sorted(os.listdir(os.path.join(INC, "bfm"))):
+        if h in EXTENSION_HEADERS and not include_extensions:
+            continue
         names |= declared(open(os.path.join(INC, "bfm", h)).read(), "bfm_")
     return names
 
@@ -69,12 +74,22 @@ def test_header_function_set_matches_bindings():
     assert names == set(F.SIGNATURES)
 
 
+def test_reference_headers_declare_the_same_functions():
+    if not os.path.isdir(REF_INC):
+        pytest.skip("reference tree not present")
+    ref = set()
+    for h in os.listdir(os.path.join(REF_INC, "bfm")):
+        if h.endswith(".h"):
+            ref |= declared(open(os.path.join(REF_INC, "bfm", h)).read(), "bfm_")
+    assert ref == public_functions()
+
+
 def test_libbfm_exports():
     path = os.path.join(LIB_DIR, "libbfm.so")
     if not os.path.exists(path):
         pytest.fail("libbfm.so is not built")
     ex = {s for s in exported(path) if s.startswith("bfm_")}
-    assert ex == public_functions()
+    assert ex == public_functions(include_extensions=True)
 
 
 def layout(inc):
