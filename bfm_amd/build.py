@@ -1,8 +1,9 @@
 """Builds the native libraries of bfm_amd in-tree.
 
     lib/libbfmhip.so   HIP kernels for gfx950 + the bfmx C ABI (include/bfmx.h)
-    lib/libbfm.so      the libbfm C API (include/bfm/*.h), C99 host code that
+    lib/libbfm.so.1    the libbfm C API (include/bfm/*.h), C99 host code that
                        calls libbfmhip.so for the planar-elasticity path
+                       (lib/libbfm.so links to it)
 
 Usage: python -m bfm_amd.build [--jobs N]
 """
@@ -94,7 +95,12 @@ def build(jobs=8, verbose=False):
             f.write("{ global: " + " ".join(f"{n};" for n in sorted(public_api())) + " local: *; };\n")
         _run([CC, "-shared", "-o", tmp, *hobjs, "-L", LIB_DIR, "-lbfmhip", "-Wl,-rpath,$ORIGIN",
               "-Wl,-soname,libbfm.so.1", f"-Wl,--version-script={vscript}", "-lm", "-lpthread"])
-        os.replace(tmp, bfm_lib)
+        # libbfm.so.1 (the soname clients record, as the reference's SOVERSION 1)
+        # and libbfm.so, the link-time name, pointing at it
+        os.replace(tmp, bfm_lib + ".1")
+        if os.path.lexists(bfm_lib):
+            os.remove(bfm_lib)
+        os.symlink("libbfm.so.1", bfm_lib)
     shutil.rmtree(OBJ_DIR, ignore_errors=True)
     if verbose:
         print("built", os.listdir(LIB_DIR))
