@@ -169,11 +169,12 @@ def test_several_instances_in_one_sim(L, golden):
 
 def obj_conditions(mesh, lib):
     xy = mesh.coords.reshape(-1, 2)
+    x0, x1 = xy[:, 0].min(), xy[:, 0].max()
     y0, y1 = xy[:, 1].min(), xy[:, 1].max()
-    bottom = xy[:, 1] <= y0 + 1e-9
+    ends = (xy[:, 0] <= x0 + 0.02 * (x1 - x0)) | (xy[:, 0] >= x1 - 0.02 * (x1 - x0))   # both supports
     top = xy[:, 1] >= y1 - 0.05 * (y1 - y0)
     out = []
-    for kind, value, flags in [(0, 0.0, bottom), (1, 0.0, bottom), (3, -1e5, top), (5, 2e4, top)]:
+    for kind, value, flags in [(0, 0.0, ends), (1, 0.0, ends), (3, -1e5, top), (5, 2e4, top)]:
         c = pb.Condition(mesh, kind, value, lib=lib)
         c.set_nodes(flags)
         out.append(c)
