@@ -63,10 +63,11 @@ struct alignas(128) Scalars {
 	double rr_true;  // ||b - A x||^2 at the last recomputation
 	double rz_prev;  // r . z of the previous iteration
 	double rr_ref;   // r . r at the last recomputation (reliable updates)
-	double rr_fail;  // true r . r of the last failed tolerance check
+	double rr_best;  // lowest true r . r seen by the tolerance checks
 	int64_t iter;
 	int32_t done;    // 0 running, 1 converged, 2 breakdown, 3 stagnated
-	int32_t stall;   // consecutive failed tolerance checks without progress
+	int32_t stall;   // consecutive tolerance checks without halving the true residual
+	int32_t checking;// a tolerance check has failed: check at every chunk end
 	int32_t updates; // true-residual recomputations
 };
 

@@ -270,9 +270,10 @@ __global__ __launch_bounds__(kRed) void k_red_beta(int nwg, const double* __rest
 
 // Whether the residual recomputation runs (mode 1): the recursive residual
 // met the tolerance, or dropped below delta2 times its value at the last
-// recomputation.
+// recomputation, or a true-residual check has already failed (from then on
+// the true residual is checked at the end of every chunk).
 __device__ __forceinline__ bool residual_due(const Scalars* S, double delta2) {
-	return !S->done && (S->rr <= S->tol2 || S->rr <= delta2 * S->rr_ref);
+	return !S->done && (S->rr <= S->tol2 || S->rr <= delta2 * S->rr_ref || S->checking);
 }
 
 // r = b - A x with the FP64 matrix, z = M^-1 r; partials r.z, r.r, b.b.
@@ -312,15 +313,19 @@ __global__ __launch_bounds__(kBlock) void k_residual(int nb, int mode, double de
 // Completes k_residual. mode 0: starts the run. mode 1 (when it ran): the
 // recomputed residual replaces the recursive one and the directions are kept
 // (beta = r'.z' / previous r.z). When the true residual meets the tolerance
-// the run ends (done = 1); when a tolerance check fails three times without
-// halving the true residual, the run stagnated at the rounding floor (done = 3).
+// the run ends (done = 1). Once a tolerance check has failed, every chunk ends
+// with a check; when the true residual has not halved (in norm) below its best
+// for kStallChecks checks, it sits at its rounding floor and the run ends
+// (done = 3).
+constexpr int kStallChecks = 8;
+
 __global__ __launch_bounds__(kRed) void k_red_residual(int nwg, int mode, double rtol2, double delta2,
                                                        const double* __restrict__ partial,
                                                        Scalars* __restrict__ S) {
 	if (mode == 1 && !residual_due(S, delta2)) {
 		return;
 	}
-	bool const at_tol = mode == 1 && S->rr <= S->tol2;
+	bool const check = mode == 1 && (S->rr <= S->tol2 || S->checking);
 	double tot[3];
 	reduce_partials<3>(partial, nwg, tot);
 	if (threadIdx.x == 0) {
@@ -335,17 +340,20 @@ __global__ __launch_bounds__(kRed) void k_red_residual(int nwg, int mode, double
 			S->beta = 0;
 			S->iter = 0;
 			S->stall = 0;
-			S->rr_fail = 0;
+			S->checking = 0;
+			S->rr_best = 0;
 		} else {
 			S->beta = S->rz_prev != 0 ? tot[0] / S->rz_prev : 0;
 		}
 		S->rz = tot[0];
 		if (rr <= S->tol2) {
 			S->done = 1;
-		} else if (at_tol) {
-			S->stall = (S->rr_fail > 0 && rr > 0.25 * S->rr_fail) ? S->stall + 1 : 0;
-			S->rr_fail = rr;
-			if (S->stall >= 3) {
+		} else if (check) {
+			S->checking = 1;
+			if (S->rr_best == 0 || rr < 0.25 * S->rr_best) {
+				S->rr_best = rr;
+				S->stall = 0;
+			} else if (++S->stall >= kStallChecks) {
 				S->done = 3;
 			}
 		}

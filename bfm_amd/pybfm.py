@@ -36,6 +36,12 @@ class Library:
     def __getattr__(self, name):
         return getattr(self.lib, name)
 
+    def check(self, rc):
+        """pybfm's `assert not lib.bfm_...(...)`, with the library's message."""
+        if rc:
+            err = self.state.err
+            raise AssertionError(err.msg.decode() if err.has and err.msg else f"libbfm call failed ({rc})")
+
 
 _default = None
 
@@ -316,7 +322,7 @@ class Sim:
         self.forces.append(force)
 
     def run(self):
-        assert not self.L.bfm_sim_run(C.byref(self.c_sim))
+        self.L.check(self.L.bfm_sim_run(C.byref(self.c_sim)))
 
 
 class _SimView:
@@ -326,7 +332,7 @@ class _SimView:
         self.L, self.c_sim = L, c_sim
 
     def run(self):
-        assert not self.L.bfm_sim_run(C.byref(self.c_sim))
+        self.L.check(self.L.bfm_sim_run(C.byref(self.c_sim)))
 
 
 class Ez_lepl1110:
