@@ -1,0 +1,1108 @@
+// Unsmoothed aggregation multigrid: the FP32 preconditioner of the
+// flexible-CG path.
+//
+// Hierarchy topology (built once per system on the host from the node graph):
+//   * aggregates: a node whose neighbours are all free starts an aggregate
+//     with them (about 3 x 3 nodes on a quad mesh); a leftover node joins the
+//     aggregate of its first aggregated neighbour;
+//   * the coarse block pattern (two aggregates are coupled when two of their
+//     members are) in sliced ELL, and for every coarse block slot the list of
+//     fine blocks it gathers, so the Galerkin product P^T A P is formed with
+//     one thread per coarse block, without atomics and in a fixed order.
+// Numeric setup (every solve, on the GPU, because the Dirichlet set enters):
+//   * level-0 modes B: the three rigid-body motions (two translations and the
+//     rotation, in coordinates centred on the mesh and scaled to its size),
+//     with the rows of constrained DOFs zeroed;
+//   * per aggregate, a Cholesky QR of its rows of B: P = B R^-1 (orthonormal
+//     columns, stored FP32) and the coarse modes B_c = R. A column whose
+//     remainder is below kDropTol of its norm (a mode that the constrained
+//     DOFs make dependent) is dropped: its coarse DOF is decoupled;
+//   * Galerkin operators in FP64 with FP32 copies, inverse diagonal blocks,
+//     Gershgorin bounds of lambda_max(D^-1 A), Chebyshev coefficients, and a
+//     dense Gauss-Jordan inverse of the coarsest operator.
+// V-cycle (FP32 vectors and matrices, FP32 arithmetic): Chebyshev smoothing
+// with block Jacobi (kSmoothDeg steps before and after the coarse
+// correction), residual, restriction P^T r, recursion, prolongation; the
+// coarsest level applies its dense inverse.
+
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+namespace bfmx {
+
+namespace {
+
+constexpr int kSmoothDeg = 2;   // Chebyshev steps before and after the coarse correction
+constexpr int kCoarseDeg = 16;  // Chebyshev steps on a coarsest level too large for the dense inverse
+constexpr int kMaxDeg = 16;
+constexpr int kMaxLevels = 12;
+constexpr int kCoarsest = 64;   // stop coarsening at this many block rows
+constexpr int kDenseMax = 768;  // largest coarsest operator (DOFs) inverted densely
+constexpr double kSmoothRatio = 20.0;   // Chebyshev interval [lmax / ratio, lmax]
+constexpr double kCoarseRatio = 200.0;
+constexpr double kDropTol = 1e-10;      // relative remainder below which a mode column is dropped
+constexpr int kGj = 1024;               // threads of the Gauss-Jordan workgroup
+
+int grid(int64_t n) {
+	return static_cast<int>((n + kBlock - 1) / kBlock);
+}
+
+// Index of entry (a, b) of block slot blk (lane r = row & 63 of its row).
+template <int BS>
+struct Blk;
+template <>
+struct Blk<2> {   // the system's split layout: top rows, then bottom rows of 64 blocks
+	__device__ __forceinline__ static int64_t at(int64_t blk, int r, int a, int b) {
+		return 4 * blk - 2 * r + a * 2 * kSlice + b;
+	}
+};
+template <>
+struct Blk<3> {   // nine entries, each contiguous over the 64 blocks of a chunk
+	__device__ __forceinline__ static int64_t at(int64_t blk, int r, int a, int b) {
+		return (blk - r) * 9 + (a * 3 + b) * kSlice + r;
+	}
+};
+
+// Loads block slot blk into A (two 8-byte loads per row for 2x2 blocks).
+template <int BS, typename T>
+__device__ __forceinline__ void ld_block(const T* __restrict__ val, int64_t blk, int r, T (&A)[BS][BS]) {
+	if constexpr (BS == 2) {
+		typedef T t2 __attribute__((ext_vector_type(2)));
+		t2 const u = *reinterpret_cast<const t2*>(val + Blk<2>::at(blk, r, 0, 0));
+		t2 const v = *reinterpret_cast<const t2*>(val + Blk<2>::at(blk, r, 1, 0));
+		A[0][0] = u.x;
+		A[0][1] = u.y;
+		A[1][0] = v.x;
+		A[1][1] = v.y;
+	} else {
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+#pragma unroll
+			for (int b = 0; b < BS; b++) {
+				A[a][b] = val[Blk<BS>::at(blk, r, a, b)];
+			}
+		}
+	}
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		v = fmax(v, __shfl_xor(v, off, 64));
+	}
+	return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		v += __shfl_xor(v, off, 64);
+	}
+	return v;
+}
+
+// Per-workgroup sum (SUM) or maximum of v, written to partial[blockIdx.x].
+template <bool SUM>
+__device__ __forceinline__ void wg_partial(double v, double* __restrict__ partial) {
+	__shared__ double lds[kBlock / 64];
+	v = SUM ? wave_sum(v) : wave_max(v);
+	if ((threadIdx.x & 63) == 0) {
+		lds[threadIdx.x >> 6] = v;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double m = lds[0];
+		for (int w = 1; w < kBlock / 64; w++) {
+			m = SUM ? m + lds[w] : fmax(m, lds[w]);
+		}
+		partial[blockIdx.x] = m;
+	}
+}
+
+// ------------------------------------------------------------ numeric setup
+
+// Level-0 modes with the rows of constrained DOFs zeroed.
+__global__ __launch_bounds__(kBlock) void k_fine_modes(int nb, const double* __restrict__ B0,
+                                                       const uint8_t* __restrict__ fixed, double* __restrict__ B) {
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= nb) {
+		return;
+	}
+#pragma unroll
+	for (int a = 0; a < 2; a++) {
+		bool const f = fixed[2 * i + a] != 0;
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			B[6 * i + 3 * a + c] = f ? 0.0 : B0[6 * i + 3 * a + c];
+		}
+	}
+}
+
+// One aggregate K per thread: G = sum_i B_i^T B_i, Cholesky G = R^T R with
+// dropped columns, P_i = B_i R^-1 on the kept columns (0 on dropped ones),
+// B_c[K] = R.
+template <int BSF>
+__global__ __launch_bounds__(kBlock) void k_tentative(int nc, const int32_t* __restrict__ agg_ptr,
+                                                      const int32_t* __restrict__ agg_nodes,
+                                                      const double* __restrict__ B, float* __restrict__ P,
+                                                      double* __restrict__ Bc) {
+	int const K = blockIdx.x * kBlock + threadIdx.x;
+	if (K >= nc) {
+		return;
+	}
+	int32_t const beg = agg_ptr[K], end = agg_ptr[K + 1];
+	double G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+	for (int32_t t = beg; t < end; t++) {
+		const double* b = B + BSF * 3 * static_cast<int64_t>(agg_nodes[t]);
+#pragma unroll
+		for (int a = 0; a < BSF; a++) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+#pragma unroll
+				for (int d = 0; d < 3; d++) {
+					G[c][d] += b[3 * a + c] * b[3 * a + d];
+				}
+			}
+		}
+	}
+	double R[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+	bool keep[3];
+#pragma unroll
+	for (int k = 0; k < 3; k++) {
+		double s = G[k][k];
+#pragma unroll
+		for (int m = 0; m < k; m++) {
+			s -= R[m][k] * R[m][k];
+		}
+		keep[k] = G[k][k] > 0 && s > kDropTol * G[k][k];
+		if (keep[k]) {
+			double const rkk = sqrt(s);
+			R[k][k] = rkk;
+#pragma unroll
+			for (int j = k + 1; j < 3; j++) {
+				double t = G[k][j];
+#pragma unroll
+				for (int m = 0; m < k; m++) {
+					t -= R[m][k] * R[m][j];
+				}
+				R[k][j] = t / rkk;
+			}
+		}
+	}
+	for (int32_t t = beg; t < end; t++) {
+		int64_t const i = agg_nodes[t];
+		const double* b = B + BSF * 3 * i;
+#pragma unroll
+		for (int a = 0; a < BSF; a++) {
+			double q[3];
+#pragma unroll
+			for (int k = 0; k < 3; k++) {
+				double v = b[3 * a + k];
+#pragma unroll
+				for (int m = 0; m < k; m++) {
+					v -= q[m] * R[m][k];
+				}
+				q[k] = keep[k] ? v / R[k][k] : 0.0;
+				P[BSF * 3 * i + 3 * a + k] = static_cast<float>(q[k]);
+			}
+		}
+	}
+#pragma unroll
+	for (int k = 0; k < 3; k++) {
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			Bc[9 * static_cast<int64_t>(K) + 3 * k + c] = R[k][c];
+		}
+	}
+}
+
+// Coarse block slot b = sum over its fine blocks (i, j) of P_i^T A_ij P_j.
+template <int BSF>
+__global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32_t* __restrict__ gal_ptr,
+                                                     const int32_t* __restrict__ gal_blk,
+                                                     const int32_t* __restrict__ gal_i,
+                                                     const int32_t* __restrict__ gal_j,
+                                                     const double* __restrict__ fval, const float* __restrict__ P,
+                                                     double* __restrict__ cval64, float* __restrict__ cval32) {
+	int64_t const b = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+	if (b >= nslots) {
+		return;
+	}
+	double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+	for (int32_t e = gal_ptr[b]; e < gal_ptr[b + 1]; e++) {
+		int32_t const i = gal_i[e], j = gal_j[e];
+		double A[BSF][BSF];
+		ld_block<BSF>(fval, gal_blk[e], i & 63, A);
+		double Pj[BSF][3], Pi[BSF][3];
+#pragma unroll
+		for (int a = 0; a < BSF; a++) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				Pj[a][c] = P[BSF * 3 * static_cast<int64_t>(j) + 3 * a + c];
+				Pi[a][c] = P[BSF * 3 * static_cast<int64_t>(i) + 3 * a + c];
+			}
+		}
+		double T[BSF][3];   // A_ij P_j
+#pragma unroll
+		for (int a = 0; a < BSF; a++) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				double t = 0;
+#pragma unroll
+				for (int k = 0; k < BSF; k++) {
+					t += A[a][k] * Pj[k][c];
+				}
+				T[a][c] = t;
+			}
+		}
+#pragma unroll
+		for (int a = 0; a < 3; a++) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				double t = 0;
+#pragma unroll
+				for (int k = 0; k < BSF; k++) {
+					t += Pi[k][a] * T[k][c];
+				}
+				C[a][c] += t;
+			}
+		}
+	}
+	int const r = static_cast<int>(b & 63);
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			int64_t const k = Blk<3>::at(b, r, a, c);
+			cval64[k] = C[a][c];
+			cval32[k] = static_cast<float>(C[a][c]);
+		}
+	}
+}
+
+// Inverse diagonal blocks (an exactly zero diagonal entry belongs to a
+// decoupled DOF, whose row and column are zero: it is treated as 1) and the
+// per-workgroup maxima of the Gershgorin bound max_k sum_j |(D^-1 A)_kj|.
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_diag_gersh(int nb, const int32_t* __restrict__ slice_ptr,
+                                                       const int32_t* __restrict__ diag_blk,
+                                                       const double* __restrict__ val, float* __restrict__ dinv,
+                                                       double* __restrict__ partial) {
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	double g = 0;
+	if (i < nb) {
+		int const r = i & 63;
+		int32_t const dblk = diag_blk[i];
+		double D[BS][BS], Di[BS][BS];
+		ld_block<BS>(val, dblk, r, D);
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+			if (D[a][a] == 0) {
+				D[a][a] = 1;
+			}
+		}
+		if constexpr (BS == 2) {
+			double const inv = 1.0 / (D[0][0] * D[1][1] - D[0][1] * D[1][0]);
+			Di[0][0] = D[1][1] * inv;
+			Di[0][1] = -D[0][1] * inv;
+			Di[1][0] = -D[1][0] * inv;
+			Di[1][1] = D[0][0] * inv;
+		} else {
+			double const c00 = D[1][1] * D[2][2] - D[1][2] * D[2][1];
+			double const c01 = D[1][2] * D[2][0] - D[1][0] * D[2][2];
+			double const c02 = D[1][0] * D[2][1] - D[1][1] * D[2][0];
+			double const inv = 1.0 / (D[0][0] * c00 + D[0][1] * c01 + D[0][2] * c02);
+			Di[0][0] = c00 * inv;
+			Di[1][0] = c01 * inv;
+			Di[2][0] = c02 * inv;
+			Di[0][1] = (D[0][2] * D[2][1] - D[0][1] * D[2][2]) * inv;
+			Di[1][1] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) * inv;
+			Di[2][1] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) * inv;
+			Di[0][2] = (D[0][1] * D[1][2] - D[0][2] * D[1][1]) * inv;
+			Di[1][2] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) * inv;
+			Di[2][2] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * inv;
+		}
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+#pragma unroll
+			for (int c = 0; c < BS; c++) {
+				dinv[BS * BS * static_cast<int64_t>(i) + BS * a + c] = static_cast<float>(Di[a][c]);
+			}
+		}
+		double rs[BS];
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+			rs[a] = 0;
+		}
+		for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
+			double A[BS][BS];
+			ld_block<BS>(val, blk, r, A);
+			if (blk == dblk) {
+#pragma unroll
+				for (int a = 0; a < BS; a++) {
+					A[a][a] = D[a][a];
+				}
+			}
+#pragma unroll
+			for (int a = 0; a < BS; a++) {
+#pragma unroll
+				for (int c = 0; c < BS; c++) {
+					double t = 0;
+#pragma unroll
+					for (int k = 0; k < BS; k++) {
+						t += Di[a][k] * A[k][c];
+					}
+					rs[a] += fabs(t);
+				}
+			}
+		}
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+			g = fmax(g, rs[a]);
+		}
+	}
+	wg_partial<false>(g, partial);
+}
+
+// lambda_max = max of the nwg partials; Chebyshev coefficients for the
+// interval [lambda_max / ratio, lambda_max]: scal[0] = lambda_max and step k
+// updates d = scal[1 + 2k] d + scal[2 + 2k] D^-1 r, x += d (step 0 has no d
+// term). This is the three-term Chebyshev recurrence for a polynomial
+// smoother (Saad, Iterative Methods for Sparse Linear Systems, 2nd ed.,
+// algorithm 12.1): sigma = theta / delta, rho_0 = 1 / sigma,
+// rho_k = 1 / (2 sigma - rho_{k-1}), d_k = rho_k rho_{k-1} d_{k-1}
+// + (2 rho_k / delta) D^-1 r_k, d_0 = D^-1 r_0 / theta.
+__global__ __launch_bounds__(kBlock) void k_cheb_coef(int nwg, const double* __restrict__ partial, double ratio,
+                                                      int degree, double* __restrict__ scal) {
+	__shared__ double lds[kBlock / 64];
+	double m = 0;
+	for (int i = threadIdx.x; i < nwg; i += kBlock) {
+		m = fmax(m, partial[i]);
+	}
+	m = wave_max(m);
+	if ((threadIdx.x & 63) == 0) {
+		lds[threadIdx.x >> 6] = m;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double lmax = 0;
+		for (int w = 0; w < kBlock / 64; w++) {
+			lmax = fmax(lmax, lds[w]);
+		}
+		double const lmin = lmax / ratio;
+		double const theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+		double const sigma = theta / delta;
+		double rho = 1.0 / sigma;
+		scal[0] = lmax;
+		scal[1] = 0;
+		scal[2] = 1.0 / theta;
+		for (int k = 1; k < degree; k++) {
+			double const rn = 1.0 / (2.0 * sigma - rho);
+			scal[1 + 2 * k] = rn * rho;
+			scal[2 + 2 * k] = 2.0 * rn / delta;
+			rho = rn;
+		}
+	}
+}
+
+// Dense copy of a level's operator (n = BS * nb), one block row per thread;
+// padding blocks carry zeros and add nothing.
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_dense(int nb, const int32_t* __restrict__ slice_ptr,
+                                                  const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                  double* __restrict__ A) {
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= nb) {
+		return;
+	}
+	int const n = BS * nb, r = i & 63;
+	for (int a = 0; a < BS; a++) {
+		for (int j = 0; j < n; j++) {
+			A[static_cast<int64_t>(BS * i + a) * n + j] = 0;
+		}
+	}
+	for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
+		int32_t const c = col[blk];
+		double B[BS][BS];
+		ld_block<BS>(val, blk, r, B);
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+#pragma unroll
+			for (int b = 0; b < BS; b++) {
+				A[static_cast<int64_t>(BS * i + a) * n + BS * c + b] += B[a][b];
+			}
+		}
+	}
+}
+
+// In-place Gauss-Jordan inverse of the SPD n x n matrix A by one workgroup,
+// without pivoting (the pivots of an SPD matrix are positive). A zero diagonal
+// entry marks a decoupled DOF (zero row and column) and is replaced by 1.
+// Step k: a_kk <- 1/p, a_kj <- a_kj/p, a_ik <- -a_ik/p,
+// a_ij <- a_ij - a_ik a_kj / p (p = a_kk), with row and column k staged in LDS
+// so every update reads the values of the step's start.
+__global__ __launch_bounds__(kGj) void k_gauss_jordan(int n, double* __restrict__ A) {
+	__shared__ double rowk[kDenseMax], colk[kDenseMax];
+	for (int i = threadIdx.x; i < n; i += kGj) {
+		if (A[static_cast<int64_t>(i) * n + i] == 0) {
+			A[static_cast<int64_t>(i) * n + i] = 1;
+		}
+	}
+	__syncthreads();
+	int64_t const nn = static_cast<int64_t>(n) * n;
+	for (int k = 0; k < n; k++) {
+		for (int j = threadIdx.x; j < n; j += kGj) {
+			rowk[j] = A[static_cast<int64_t>(k) * n + j];
+			colk[j] = A[static_cast<int64_t>(j) * n + k];
+		}
+		__syncthreads();
+		double const p = 1.0 / rowk[k];
+		for (int64_t e = threadIdx.x; e < nn; e += kGj) {
+			int const i = static_cast<int>(e / n), j = static_cast<int>(e - static_cast<int64_t>(i) * n);
+			double v;
+			if (i == k) {
+				v = j == k ? p : rowk[j] * p;
+			} else if (j == k) {
+				v = -colk[i] * p;
+			} else {
+				v = A[e] - colk[i] * rowk[j] * p;
+			}
+			A[e] = v;
+		}
+		__syncthreads();
+	}
+}
+
+// ------------------------------------------------------------------ V-cycle
+
+// One Chebyshev step on a level: r = b - A x_in (skipped when FIRST: x_in = 0),
+// d = c1 d + c2 D^-1 r (c1 unused at step 0), x_out = x_in + d. With DOT, also
+// the per-workgroup sums of x_out . q (q an FP64 vector over the same rows).
+template <int BS, typename BT, bool FIRST, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_cheb(int nb, const int32_t* __restrict__ slice_ptr,
+                                                 const int32_t* __restrict__ col, const float* __restrict__ val,
+                                                 const BT* __restrict__ b, const float* __restrict__ dinv,
+                                                 const float* __restrict__ xin, float* __restrict__ xout,
+                                                 float* __restrict__ d, const double* __restrict__ scal, int step,
+                                                 const double* __restrict__ q, double* __restrict__ partial,
+                                                 const Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	float const c1 = static_cast<float>(scal[1 + 2 * step]);
+	float const c2 = static_cast<float>(scal[2 + 2 * step]);
+	double dot = 0;
+	if (i < nb) {
+		float res[BS];
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+			res[a] = static_cast<float>(b[BS * static_cast<int64_t>(i) + a]);
+		}
+		if (!FIRST) {
+			int const r = i & 63;
+			for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
+				int32_t const c = col[blk];
+				float A[BS][BS];
+				ld_block<BS>(val, blk, r, A);
+				float xv[BS];
+#pragma unroll
+				for (int k = 0; k < BS; k++) {
+					xv[k] = xin[BS * static_cast<int64_t>(c) + k];
+				}
+#pragma unroll
+				for (int a = 0; a < BS; a++) {
+#pragma unroll
+					for (int k = 0; k < BS; k++) {
+						res[a] -= A[a][k] * xv[k];
+					}
+				}
+			}
+		}
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+			float z = 0;
+#pragma unroll
+			for (int k = 0; k < BS; k++) {
+				z += dinv[BS * BS * static_cast<int64_t>(i) + BS * a + k] * res[k];
+			}
+			int64_t const o = BS * static_cast<int64_t>(i) + a;
+			float const dn = (FIRST || step == 0) ? c2 * z : c1 * d[o] + c2 * z;
+			float const xn = FIRST ? dn : xin[o] + dn;
+			d[o] = dn;
+			xout[o] = xn;
+			if (DOT) {
+				dot += static_cast<double>(xn) * q[o];
+			}
+		}
+	}
+	if (DOT) {
+		wg_partial<true>(dot, partial);
+	}
+}
+
+// r = b - A x on a level.
+template <int BS, typename BT>
+__global__ __launch_bounds__(kBlock) void k_level_resid(int nb, const int32_t* __restrict__ slice_ptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const float* __restrict__ val, const BT* __restrict__ b,
+                                                        const float* __restrict__ x, float* __restrict__ rv,
+                                                        const Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	if (i >= nb) {
+		return;
+	}
+	int const r = i & 63;
+	float res[BS];
+#pragma unroll
+	for (int a = 0; a < BS; a++) {
+		res[a] = static_cast<float>(b[BS * static_cast<int64_t>(i) + a]);
+	}
+	for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
+		int32_t const c = col[blk];
+		float A[BS][BS];
+		ld_block<BS>(val, blk, r, A);
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+#pragma unroll
+			for (int k = 0; k < BS; k++) {
+				res[a] -= A[a][k] * x[BS * static_cast<int64_t>(c) + k];
+			}
+		}
+	}
+#pragma unroll
+	for (int a = 0; a < BS; a++) {
+		rv[BS * static_cast<int64_t>(i) + a] = res[a];
+	}
+}
+
+// b_c[I] = sum over the members i of aggregate I of P_i^T r_i.
+template <int BSF>
+__global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int32_t* __restrict__ agg_ptr,
+                                                     const int32_t* __restrict__ agg_nodes,
+                                                     const float* __restrict__ P, const float* __restrict__ r,
+                                                     float* __restrict__ bc, const Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	int const I = blockIdx.x * kBlock + threadIdx.x;
+	if (I >= nc) {
+		return;
+	}
+	float acc[3] = {0, 0, 0};
+	for (int32_t t = agg_ptr[I]; t < agg_ptr[I + 1]; t++) {
+		int64_t const i = agg_nodes[t];
+#pragma unroll
+		for (int k = 0; k < BSF; k++) {
+			float const rk = r[BSF * i + k];
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				acc[c] += P[BSF * 3 * i + 3 * k + c] * rk;
+			}
+		}
+	}
+#pragma unroll
+	for (int c = 0; c < 3; c++) {
+		bc[3 * static_cast<int64_t>(I) + c] = acc[c];
+	}
+}
+
+// x_i += P_i x_c[agg(i)].
+template <int BSF>
+__global__ __launch_bounds__(kBlock) void k_prolong(int nb, const int32_t* __restrict__ agg,
+                                                    const float* __restrict__ P, const float* __restrict__ xc,
+                                                    float* __restrict__ x, const Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= nb) {
+		return;
+	}
+	int64_t const I = agg[i];
+	float const e0 = xc[3 * I], e1 = xc[3 * I + 1], e2 = xc[3 * I + 2];
+	const float* p = P + BSF * 3 * static_cast<int64_t>(i);
+#pragma unroll
+	for (int k = 0; k < BSF; k++) {
+		x[BSF * static_cast<int64_t>(i) + k] += p[3 * k] * e0 + p[3 * k + 1] * e1 + p[3 * k + 2] * e2;
+	}
+}
+
+// x = A^-1 b with the dense inverse (symmetric: read by columns, coalesced);
+// with DOT, the per-workgroup sums of x . q.
+template <typename BT, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_dense_solve(int n, const double* __restrict__ Ainv,
+                                                        const BT* __restrict__ b, float* __restrict__ x,
+                                                        const double* __restrict__ q, double* __restrict__ partial,
+                                                        const Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	double dot = 0;
+	if (i < n) {
+		double acc = 0;
+		for (int j = 0; j < n; j++) {
+			acc += Ainv[static_cast<int64_t>(j) * n + i] * static_cast<double>(b[j]);
+		}
+		float const xi = static_cast<float>(acc);
+		x[i] = xi;
+		if (DOT) {
+			dot = static_cast<double>(xi) * q[i];
+		}
+	}
+	if (DOT) {
+		wg_partial<true>(dot, partial);
+	}
+}
+
+// ------------------------------------------------------------------ host topology
+
+// Aggregates of the graph (ptr, adj) (rows include themselves); returns their count.
+int aggregate(int n, const int64_t* ptr, const int32_t* adj, std::vector<int32_t>& agg) {
+	agg.assign(n, -1);
+	int na = 0;
+	for (int i = 0; i < n; i++) {
+		if (agg[i] >= 0) {
+			continue;
+		}
+		bool free_nbhd = true;
+		for (int64_t t = ptr[i]; t < ptr[i + 1]; t++) {
+			if (agg[adj[t]] >= 0) {
+				free_nbhd = false;
+				break;
+			}
+		}
+		if (free_nbhd) {
+			for (int64_t t = ptr[i]; t < ptr[i + 1]; t++) {
+				agg[adj[t]] = na;
+			}
+			agg[i] = na++;
+		}
+	}
+	std::vector<int32_t> out = agg;
+	for (int i = 0; i < n; i++) {
+		if (agg[i] >= 0) {
+			continue;
+		}
+		for (int64_t t = ptr[i]; t < ptr[i + 1]; t++) {
+			if (agg[adj[t]] >= 0) {
+				out[i] = agg[adj[t]];
+				break;
+			}
+		}
+		if (out[i] < 0) {
+			out[i] = na++;
+		}
+	}
+	agg.swap(out);
+	return na;
+}
+
+struct Sell {
+	std::vector<int32_t> slice_ptr, col, diag_blk;
+	int64_t n_slices = 0, nnzb = 0, nnzb_pad = 0;
+	std::vector<int32_t> slot_of;   // block slot of each adjacency entry
+};
+
+// Sliced ELL of 3x3 blocks for the graph (ptr, adj); padding repeats the
+// row's own column (with zero values).
+int build_sell(int n, const std::vector<int64_t>& ptr, const std::vector<int32_t>& adj, Sell& S) {
+	S.n_slices = (n + kSlice - 1) / kSlice;
+	S.slice_ptr.assign(S.n_slices + 1, 0);
+	for (int64_t sl = 0; sl < S.n_slices; sl++) {
+		int64_t kmax = 0;
+		for (int64_t r = sl * kSlice; r < std::min<int64_t>(n, (sl + 1) * kSlice); r++) {
+			kmax = std::max(kmax, ptr[r + 1] - ptr[r]);
+		}
+		int64_t const next = S.slice_ptr[sl] + kmax * kSlice;
+		if (next * 9 >= (int64_t(1) << 31)) {
+			set_error("multigrid level too large for 32-bit block indices");
+			return BFMX_E_ARG;
+		}
+		S.slice_ptr[sl + 1] = static_cast<int32_t>(next);
+	}
+	S.nnzb = ptr[n];
+	S.nnzb_pad = S.slice_ptr[S.n_slices];
+	S.col.assign(S.nnzb_pad, 0);
+	S.diag_blk.assign(n, 0);
+	S.slot_of.assign(S.nnzb, 0);
+	for (int64_t sl = 0; sl < S.n_slices; sl++) {
+		int64_t const kmax = (S.slice_ptr[sl + 1] - S.slice_ptr[sl]) / kSlice;
+		for (int r = 0; r < kSlice; r++) {
+			int64_t const row = sl * kSlice + r;
+			int64_t const len = row < n ? ptr[row + 1] - ptr[row] : 0;
+			for (int64_t k = 0; k < kmax; k++) {
+				int64_t const slot = S.slice_ptr[sl] + k * kSlice + r;
+				if (k < len) {
+					int32_t const c = adj[ptr[row] + k];
+					S.col[slot] = c;
+					S.slot_of[ptr[row] + k] = static_cast<int32_t>(slot);
+					if (c == row) {
+						S.diag_blk[row] = static_cast<int32_t>(slot);
+					}
+				} else {
+					S.col[slot] = row < n ? static_cast<int32_t>(row) : 0;
+				}
+			}
+		}
+	}
+	return BFMX_OK;
+}
+
+template <typename T>
+int upload(T** d, const std::vector<T>& h, hipStream_t st) {
+	BFMX_CHECK(dalloc(d, h.size()));
+	if (!h.empty()) {
+		BFMX_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+	}
+	return BFMX_OK;
+}
+
+bool dense_coarsest(const AmgLevel& L) {
+	return L.bs * static_cast<int64_t>(L.nb) <= kDenseMax;
+}
+
+}  // namespace
+
+void amg_release(bfmx_system* s) {
+	for (size_t l = 0; l < s->amg.size(); l++) {
+		AmgLevel& L = s->amg[l];
+		if (l > 0) {
+			dfree(L.d_slice_ptr);
+			dfree(L.d_col);
+			dfree(L.d_diag_blk);
+			dfree(L.d_val64);
+			dfree(L.d_val32);
+		}
+		dfree(L.d_dinv);
+		dfree(L.d_scal);
+		dfree(L.d_B);
+		dfree(L.d_B0);
+		dfree(L.d_agg);
+		dfree(L.d_P);
+		dfree(L.d_agg_ptr);
+		dfree(L.d_agg_nodes);
+		dfree(L.d_gal_ptr);
+		dfree(L.d_gal_blk);
+		dfree(L.d_gal_i);
+		dfree(L.d_gal_j);
+		dfree(L.d_dense);
+		dfree(L.d_b);
+		dfree(L.d_x[0]);
+		dfree(L.d_x[1]);
+		dfree(L.d_d);
+		dfree(L.d_r);
+		dfree(L.d_partial);
+	}
+	s->amg.clear();
+}
+
+// Needs the system's FP32 matrix buffer (s->d_val32) allocated: level 0
+// reads it.
+int amg_build(bfmx_system* s) {
+	if (!s->amg.empty()) {
+		return BFMX_OK;
+	}
+	if (!s->d_val32) {
+		set_error("multigrid set-up before the FP32 matrix buffer");
+		return BFMX_E_ARG;
+	}
+	int const rc = [&]() -> int {
+		hipStream_t const st = s->stream;
+		std::vector<AmgLevel>& lv = s->amg;
+		int64_t const n0 = s->n_nodes;
+		lv.emplace_back();
+		{
+			AmgLevel& L0 = lv[0];
+			L0.nb = static_cast<int>(n0);
+			L0.bs = 2;
+			L0.n_slices = s->n_slices;
+			L0.nnzb = s->nnzb;
+			L0.nnzb_pad = s->nnzb_pad;
+			L0.d_slice_ptr = s->d_slice_ptr;
+			L0.d_col = s->d_col;
+			L0.d_diag_blk = s->d_diag_blk;
+			L0.d_val64 = s->d_val;
+			L0.d_val32 = s->d_val32;
+			// rigid-body modes in coordinates centred on the mesh, scaled to its half-size
+			double lo[2] = {s->coords[0], s->coords[1]}, hi[2] = {s->coords[0], s->coords[1]};
+			for (int64_t i = 0; i < n0; i++) {
+				for (int a = 0; a < 2; a++) {
+					lo[a] = std::min(lo[a], s->coords[2 * i + a]);
+					hi[a] = std::max(hi[a], s->coords[2 * i + a]);
+				}
+			}
+			double const cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]);
+			double L = 0.5 * std::max(hi[0] - lo[0], hi[1] - lo[1]);
+			if (!(L > 0)) {
+				L = 1;
+			}
+			std::vector<double> B0(6 * n0);
+			for (int64_t i = 0; i < n0; i++) {
+				double const x = (s->coords[2 * i] - cx) / L, y = (s->coords[2 * i + 1] - cy) / L;
+				double const m[6] = {1, 0, -y, 0, 1, x};
+				std::copy(m, m + 6, &B0[6 * i]);
+			}
+			BFMX_CHECK(upload(&L0.d_B0, B0, st));
+			BFMX_HIP(hipStreamSynchronize(st));
+		}
+		// graph of the current finest-so-far level and the block slot of each entry
+		std::vector<int64_t> fptr;
+		std::vector<int32_t> fadj;
+		std::vector<int32_t> fslot(s->nnzb);
+		for (int64_t i = 0; i < n0; i++) {
+			for (int64_t k = 0; k < s->adj_ptr[i + 1] - s->adj_ptr[i]; k++) {
+				fslot[s->adj_ptr[i] + k] = static_cast<int32_t>(s->slice_ptr[i >> 6] + k * kSlice + (i & 63));
+			}
+		}
+		while (static_cast<int>(lv.size()) < kMaxLevels && lv.back().nb > kCoarsest) {
+			int const lf = static_cast<int>(lv.size()) - 1;
+			const int64_t* ptr = lf == 0 ? s->adj_ptr.data() : fptr.data();
+			const int32_t* adj = lf == 0 ? s->adj.data() : fadj.data();
+			int const nf = lv[lf].nb;
+			std::vector<int32_t> agg;
+			int const na = aggregate(nf, ptr, adj, agg);
+			if (na * 10 > nf * 8) {
+				break;   // coarsening stalled
+			}
+			// members of each aggregate
+			std::vector<int32_t> aptr(na + 1, 0), anodes(nf);
+			for (int i = 0; i < nf; i++) {
+				aptr[agg[i] + 1]++;
+			}
+			for (int I = 0; I < na; I++) {
+				aptr[I + 1] += aptr[I];
+			}
+			{
+				std::vector<int32_t> fill(aptr.begin(), aptr.end() - 1);
+				for (int i = 0; i < nf; i++) {
+					anodes[fill[agg[i]]++] = i;
+				}
+			}
+			// coarse graph (sorted rows, including the diagonal)
+			std::vector<int64_t> cptr(na + 1, 0);
+			std::vector<int32_t> cadj;
+			{
+				std::vector<int32_t> row;
+				for (int I = 0; I < na; I++) {
+					row.clear();
+					for (int32_t t = aptr[I]; t < aptr[I + 1]; t++) {
+						int const i = anodes[t];
+						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+							row.push_back(agg[adj[e]]);
+						}
+					}
+					std::sort(row.begin(), row.end());
+					row.erase(std::unique(row.begin(), row.end()), row.end());
+					cadj.insert(cadj.end(), row.begin(), row.end());
+					cptr[I + 1] = static_cast<int64_t>(cadj.size());
+				}
+			}
+			Sell CS;
+			BFMX_CHECK(build_sell(na, cptr, cadj, CS));
+			// Galerkin gather lists, grouped by coarse slot, in fine row order
+			int64_t const nfe = ptr[nf];
+			std::vector<int32_t> cslot(nfe);
+			std::vector<int32_t> gptr(CS.nnzb_pad + 1, 0);
+			for (int i = 0; i < nf; i++) {
+				int const I = agg[i];
+				for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+					int32_t const J = agg[adj[e]];
+					auto const beg = cadj.begin() + cptr[I], end = cadj.begin() + cptr[I + 1];
+					int64_t const k = std::lower_bound(beg, end, J) - cadj.begin();
+					cslot[e] = CS.slot_of[k];
+					gptr[cslot[e] + 1]++;
+				}
+			}
+			for (int64_t b = 0; b < CS.nnzb_pad; b++) {
+				gptr[b + 1] += gptr[b];
+			}
+			std::vector<int32_t> gblk(nfe), gi(nfe), gj(nfe);
+			{
+				std::vector<int32_t> fill(gptr.begin(), gptr.end() - 1);
+				for (int i = 0; i < nf; i++) {
+					for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+						int32_t const at = fill[cslot[e]]++;
+						gblk[at] = fslot[e];
+						gi[at] = i;
+						gj[at] = adj[e];
+					}
+				}
+			}
+			AmgLevel& F = lv[lf];
+			BFMX_CHECK(upload(&F.d_agg, agg, st));
+			BFMX_CHECK(dalloc(&F.d_P, static_cast<size_t>(F.bs) * 3 * nf));
+			BFMX_CHECK(upload(&F.d_agg_ptr, aptr, st));
+			BFMX_CHECK(upload(&F.d_agg_nodes, anodes, st));
+			BFMX_CHECK(upload(&F.d_gal_ptr, gptr, st));
+			BFMX_CHECK(upload(&F.d_gal_blk, gblk, st));
+			BFMX_CHECK(upload(&F.d_gal_i, gi, st));
+			BFMX_CHECK(upload(&F.d_gal_j, gj, st));
+			lv.emplace_back();   // its buffers are released with the hierarchy, also on failure
+			AmgLevel& C = lv.back();
+			C.nb = na;
+			C.bs = 3;
+			C.n_slices = CS.n_slices;
+			C.nnzb = CS.nnzb;
+			C.nnzb_pad = CS.nnzb_pad;
+			BFMX_CHECK(upload(&C.d_slice_ptr, CS.slice_ptr, st));
+			BFMX_CHECK(upload(&C.d_col, CS.col, st));
+			BFMX_CHECK(upload(&C.d_diag_blk, CS.diag_blk, st));
+			BFMX_CHECK(dalloc(&C.d_val64, 9 * static_cast<size_t>(CS.nnzb_pad)));
+			BFMX_CHECK(dalloc(&C.d_val32, 9 * static_cast<size_t>(CS.nnzb_pad)));
+			BFMX_HIP(hipStreamSynchronize(st));   // the host sources above are temporaries
+			fptr.swap(cptr);
+			fadj.swap(cadj);
+			fslot.swap(CS.slot_of);
+		}
+		for (auto& L : lv) {
+			size_t const m = static_cast<size_t>(L.bs) * L.nb;
+			BFMX_CHECK(dalloc(&L.d_b, m));
+			BFMX_CHECK(dalloc(&L.d_x[0], m));
+			BFMX_CHECK(dalloc(&L.d_x[1], m));
+			BFMX_CHECK(dalloc(&L.d_d, m));
+			BFMX_CHECK(dalloc(&L.d_r, m));
+			BFMX_CHECK(dalloc(&L.d_dinv, static_cast<size_t>(L.bs) * L.bs * L.nb));
+			BFMX_CHECK(dalloc(&L.d_scal, 2 + 2 * kMaxDeg));
+			BFMX_CHECK(dalloc(&L.d_partial, grid(L.nb)));
+			BFMX_CHECK(dalloc(&L.d_B, 3 * m));
+		}
+		if (dense_coarsest(lv.back())) {
+			size_t const n = static_cast<size_t>(lv.back().bs) * lv.back().nb;
+			BFMX_CHECK(dalloc(&lv.back().d_dense, n * n));
+		}
+		return BFMX_OK;
+	}();
+	if (rc != BFMX_OK) {
+		amg_release(s);
+	}
+	return rc;
+}
+
+// Modes, prolongators, Galerkin operators, smoother data and the coarsest
+// inverse, from the current FP64 matrix and Dirichlet set.
+int amg_setup_numeric(bfmx_system* s) {
+	hipStream_t const st = s->stream;
+	std::vector<AmgLevel>& lv = s->amg;
+	k_fine_modes<<<grid(lv[0].nb), kBlock, 0, st>>>(lv[0].nb, lv[0].d_B0, s->d_fixed, lv[0].d_B);
+	for (size_t l = 0; l + 1 < lv.size(); l++) {
+		AmgLevel& F = lv[l];
+		AmgLevel& C = lv[l + 1];
+		int64_t const ns = C.nnzb_pad;
+		if (F.bs == 2) {
+			k_tentative<2><<<grid(C.nb), kBlock, 0, st>>>(C.nb, F.d_agg_ptr, F.d_agg_nodes, F.d_B, F.d_P, C.d_B);
+			k_galerkin<2><<<grid(ns), kBlock, 0, st>>>(ns, F.d_gal_ptr, F.d_gal_blk, F.d_gal_i, F.d_gal_j, F.d_val64,
+			                                          F.d_P, C.d_val64, C.d_val32);
+		} else {
+			k_tentative<3><<<grid(C.nb), kBlock, 0, st>>>(C.nb, F.d_agg_ptr, F.d_agg_nodes, F.d_B, F.d_P, C.d_B);
+			k_galerkin<3><<<grid(ns), kBlock, 0, st>>>(ns, F.d_gal_ptr, F.d_gal_blk, F.d_gal_i, F.d_gal_j, F.d_val64,
+			                                          F.d_P, C.d_val64, C.d_val32);
+		}
+	}
+	for (size_t l = 0; l < lv.size(); l++) {
+		AmgLevel& L = lv[l];
+		bool const coarsest = l + 1 == lv.size();
+		if (coarsest && L.d_dense) {
+			int const n = L.bs * L.nb;
+			if (L.bs == 2) {
+				k_dense<2><<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_dense);
+			} else {
+				k_dense<3><<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_dense);
+			}
+			k_gauss_jordan<<<1, kGj, 0, st>>>(n, L.d_dense);
+			continue;
+		}
+		int const g = grid(L.nb);
+		if (L.bs == 2) {
+			k_diag_gersh<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_diag_blk, L.d_val64, L.d_dinv, L.d_partial);
+		} else {
+			k_diag_gersh<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_diag_blk, L.d_val64, L.d_dinv, L.d_partial);
+		}
+		k_cheb_coef<<<1, kBlock, 0, st>>>(g, L.d_partial, coarsest ? kCoarseRatio : kSmoothRatio,
+		                                  coarsest ? kCoarseDeg : kSmoothDeg, L.d_scal);
+	}
+	BFMX_HIP(hipGetLastError());
+	return BFMX_OK;
+}
+
+namespace {
+
+template <int BS, typename BT>
+void cheb(hipStream_t st, AmgLevel& L, const BT* b, bool first, int step, const float* xin, float* xout,
+          const double* q, double* partial, const Scalars* S) {
+	int const g = grid(L.nb);
+	if (first) {
+		k_cheb<BS, BT, true, false><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_dinv, xin,
+		                                                  xout, L.d_d, L.d_scal, step, nullptr, nullptr, S);
+	} else if (q) {
+		k_cheb<BS, BT, false, true><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_dinv, xin,
+		                                                  xout, L.d_d, L.d_scal, step, q, partial, S);
+	} else {
+		k_cheb<BS, BT, false, false><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_dinv,
+		                                                   xin, xout, L.d_d, L.d_scal, step, nullptr, nullptr, S);
+	}
+}
+
+// V-cycle on level l for the right-hand side b (FP64 on level 0, FP32
+// below); returns the buffer that holds the level's correction. With q, the
+// last kernel also writes the per-workgroup partial sums of x . q.
+template <int BS, typename BT>
+float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* partial, const Scalars* S) {
+	hipStream_t const st = s->stream;
+	AmgLevel& L = s->amg[l];
+	bool const coarsest = l + 1 == s->amg.size();
+	if (coarsest && L.d_dense) {
+		int const n = BS * L.nb;
+		if (q) {
+			k_dense_solve<BT, true><<<grid(n), kBlock, 0, st>>>(n, L.d_dense, b, L.d_x[0], q, partial, S);
+		} else {
+			k_dense_solve<BT, false><<<grid(n), kBlock, 0, st>>>(n, L.d_dense, b, L.d_x[0], nullptr, nullptr, S);
+		}
+		return L.d_x[0];
+	}
+	int const deg = coarsest ? kCoarseDeg : kSmoothDeg;
+	int cur = 0;
+	cheb<BS, BT>(st, L, b, true, 0, nullptr, L.d_x[0], nullptr, nullptr, S);
+	if (coarsest) {
+		for (int k = 1; k < deg; k++) {
+			bool const last = k + 1 == deg;
+			cheb<BS, BT>(st, L, b, false, k, L.d_x[cur], L.d_x[cur ^ 1], last ? q : nullptr, last ? partial : nullptr,
+			             S);
+			cur ^= 1;
+		}
+		return L.d_x[cur];
+	}
+	for (int k = 1; k < deg; k++) {
+		cheb<BS, BT>(st, L, b, false, k, L.d_x[cur], L.d_x[cur ^ 1], nullptr, nullptr, S);
+		cur ^= 1;
+	}
+	AmgLevel& C = s->amg[l + 1];
+	int const g = grid(L.nb);
+	k_level_resid<BS, BT><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_x[cur], L.d_r, S);
+	k_restrict<BS><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_P, L.d_r, C.d_b, S);
+	float* xc = vcycle<3, float>(s, l + 1, C.d_b, nullptr, nullptr, S);
+	k_prolong<BS><<<g, kBlock, 0, st>>>(L.nb, L.d_agg, L.d_P, xc, L.d_x[cur], S);
+	for (int k = 0; k < deg; k++) {
+		bool const last = k + 1 == deg;
+		cheb<BS, BT>(st, L, b, false, k, L.d_x[cur], L.d_x[cur ^ 1], last ? q : nullptr, last ? partial : nullptr, S);
+		cur ^= 1;
+	}
+	return L.d_x[cur];
+}
+
+}  // namespace
+
+float* amg_vcycle(bfmx_system* s, const double* r, const double* q, double* partial, const Scalars* S,
+                  int* n_partials) {
+	AmgLevel const& L0 = s->amg[0];
+	*n_partials = s->amg.size() == 1 && L0.d_dense ? grid(2 * static_cast<int64_t>(L0.nb)) : grid(L0.nb);
+	return vcycle<2, double>(s, 0, r, q, partial, S);
+}
+
+}  // namespace bfmx

@@ -427,9 +427,6 @@ int assemble(bfmx_system* s, int stress, double E, double nu, double rho, const 
 	s->finalized = false;
 	s->has_u = false;
 	s->dirichlet_nodes.assign(n, 0);
-	if (s->d_val32) {
-		dfree(s->d_val32);   // stale FP32 copy
-	}
 	return BFMX_OK;
 }
 

@@ -71,6 +71,44 @@ struct alignas(128) Scalars {
 	int32_t updates; // true-residual recomputations
 };
 
+// One level of the aggregation multigrid used as the FP32 preconditioner of
+// the flexible-CG path. Level 0 is the system's own matrix (2x2 blocks);
+// coarser levels have three DOFs per node (the coefficients of the three
+// rigid-body modes of the aggregate). Block storage is sliced ELL as on
+// level 0; 3x3 blocks keep each of their nine entries contiguous across a
+// slice's 64 rows.
+struct AmgLevel {
+	int nb = 0;                 // block rows
+	int bs = 2;                 // DOFs per block row
+	int64_t n_slices = 0, nnzb = 0, nnzb_pad = 0;
+	int32_t* d_slice_ptr = nullptr;   // level 0: the system's arrays (not owned)
+	int32_t* d_col = nullptr;
+	int32_t* d_diag_blk = nullptr;
+	double* d_val64 = nullptr;        // level 0: the system's FP64 values (not owned)
+	float* d_val32 = nullptr;         // level 0: the system's FP32 copy (not owned)
+	float* d_dinv = nullptr;          // bs*bs per row: inverse diagonal blocks
+	double* d_scal = nullptr;         // lambda_max, then the Chebyshev coefficients
+	double* d_B = nullptr;            // bs x 3 per row: rigid-body modes on this level
+	double* d_B0 = nullptr;           // level 0: the modes before fixed DOFs are zeroed
+	// transfer to the next coarser level (absent on the coarsest)
+	int32_t* d_agg = nullptr;         // next-level row of each row
+	float* d_P = nullptr;             // bs x 3 prolongator block per row
+	int32_t* d_agg_ptr = nullptr;     // members of each next-level row
+	int32_t* d_agg_nodes = nullptr;
+	int32_t* d_gal_ptr = nullptr;     // per next-level block slot: range of contributions
+	int32_t* d_gal_blk = nullptr;     // this level's block slot
+	int32_t* d_gal_i = nullptr;       // this level's row
+	int32_t* d_gal_j = nullptr;       // this level's column
+	// coarsest level: dense inverse (n x n, n = bs * nb) when small enough
+	double* d_dense = nullptr;
+	// FP32 work vectors (bs*nb each)
+	float* d_b = nullptr;
+	float* d_x[2] = {nullptr, nullptr};
+	float* d_d = nullptr;
+	float* d_r = nullptr;
+	double* d_partial = nullptr;      // per-workgroup maxima / sums
+};
+
 struct Graph {
 	hipGraphExec_t exec = nullptr;
 	int precond = -1;
@@ -143,6 +181,7 @@ struct bfmx_system {
 	bfmx::Scalars* h_scal = nullptr; // pinned host mirror ring (4 entries)
 	hipEvent_t ev[6] = {};
 	bfmx::Graph graph;
+	std::vector<bfmx::AmgLevel> amg;   // multigrid hierarchy (built on first use)
 
 	bool assembled = false;
 	bool finalized = false;
@@ -163,6 +202,16 @@ int assemble(bfmx_system* s, int stress, double E, double nu, double rho, const 
              const double* node_force, double gx, double gy);
 int apply_condition(bfmx_system* s, int kind, double value, const uint8_t* flags);
 int finalize_conditions(bfmx_system* s);
+// amg.hip
+int amg_build(bfmx_system* s);
+void amg_release(bfmx_system* s);
+int amg_setup_numeric(bfmx_system* s);
+// V-cycle on level 0 for the FP64 residual r; returns the FP32 correction z.
+// The last kernel also writes per-workgroup partial sums of z . q into
+// partial (*n_partials of them), the flexible-CG orthogonalisation. Every
+// kernel returns at once when S->done.
+float* amg_vcycle(bfmx_system* s, const double* r, const double* q, double* partial, const Scalars* S,
+                  int* n_partials);
 // pcg.hip
 int solve(bfmx_system* s, const double* b_in, int precond, int precision, double rtol, size_t max_iter,
           bfmx_stats_t* stats);

@@ -27,6 +27,17 @@
 // FP64 true-residual recomputation above.
 //
 // Preconditioners: point Jacobi (1 / diag) or 2x2 block Jacobi.
+//
+// Multigrid (BFMX_PRECOND_AMG): flexible CG in FP64 (matrix, vectors, dots)
+// preconditioned by one FP32 V-cycle of aggregation multigrid (amg.hip). The
+// V-cycle is not an exactly symmetric operator in FP32, so the direction is
+// orthogonalised against the previous one explicitly (Notay's flexible CG
+// with one stored direction): beta = -(z . q_prev) / (p_prev . q_prev),
+// alpha = (p . r) / (p . q). One iteration is the V-cycle (its last kernel
+// also forms the partials of z . q_prev) and five kernels:
+//   k_red_fcg_beta, k_fcg_spmv (p = z + beta p_old, q = A p, partials of p . q
+//   and p . r), k_red_fcg_alpha, k_fcg_update (x, r and partials of r . r),
+//   k_red_fcg_rr.
 
 #include <chrono>
 #include <cmath>
@@ -268,6 +279,105 @@ __global__ __launch_bounds__(kRed) void k_red_beta(int nwg, const double* __rest
 	}
 }
 
+// ------------------------------------------------------------ flexible CG
+
+// beta = -(z . q_prev) / (p_prev . q_prev), 0 before the first direction.
+__global__ __launch_bounds__(kRed) void k_red_fcg_beta(int nwg, const double* __restrict__ partial,
+                                                       Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	double tot[1];
+	reduce_partials<1>(partial, nwg, tot);
+	if (threadIdx.x == 0) {
+		S->beta = S->pq > 0 ? -tot[0] / S->pq : 0.0;
+	}
+}
+
+// p_new = z + beta p_old (z the FP32 V-cycle output), q = A p_new with the
+// FP64 matrix, partials p_new . q and p_new . r.
+__global__ __launch_bounds__(kBlock) void k_fcg_spmv(int nb, const int32_t* __restrict__ slice_ptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const double* __restrict__ val, const float* __restrict__ z,
+                                                     const double* __restrict__ pold, double* __restrict__ pnew,
+                                                     double* __restrict__ q, const double* __restrict__ r,
+                                                     double* __restrict__ partial, const Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	double const beta = S->beta;
+	int const row = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	auto ldz = [&](int c) {
+		flt2 const f = *reinterpret_cast<const flt2*>(z + 2 * c);
+		return dbl2{f.x, f.y};
+	};
+	double v[2] = {0, 0};
+	if (row < nb) {
+		dbl2 const y = row_spmv(slice_ptr, col, val, row, [&](int c) { return ldz(c) + beta * ld2(pold + 2 * c); });
+		dbl2 const pn = ldz(row) + beta * ld2(pold + 2 * row);
+		dbl2 const rv = ld2(r + 2 * row);
+		st2(pnew + 2 * row, pn);
+		st2(q + 2 * row, y);
+		v[0] = pn.x * y.x + pn.y * y.y;
+		v[1] = pn.x * rv.x + pn.y * rv.y;
+	}
+	write_partials<2>(v, partial);
+}
+
+// alpha = (p . r) / (p . q), or a breakdown.
+__global__ __launch_bounds__(kRed) void k_red_fcg_alpha(int nwg, const double* __restrict__ partial,
+                                                        Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	double tot[2];
+	reduce_partials<2>(partial, nwg, tot);
+	if (threadIdx.x == 0) {
+		double const pq = tot[0];
+		S->pq = pq;
+		if (!(pq > 0) || !isfinite(pq)) {
+			S->done = 2;
+		} else {
+			S->alpha = tot[1] / pq;
+		}
+	}
+}
+
+// x += alpha p, r -= alpha q, partials r . r.
+__global__ __launch_bounds__(kBlock) void k_fcg_update(int nb, const double* __restrict__ p,
+                                                       const double* __restrict__ q, double* __restrict__ x,
+                                                       double* __restrict__ r, double* __restrict__ partial,
+                                                       const Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	double const alpha = S->alpha;
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	double v[1] = {0};
+	if (i < nb) {
+		dbl2 const xv = ld2(x + 2 * i) + alpha * ld2(p + 2 * i);
+		dbl2 const rv = ld2(r + 2 * i) - alpha * ld2(q + 2 * i);
+		st2(x + 2 * i, xv);
+		st2(r + 2 * i, rv);
+		v[0] = rv.x * rv.x + rv.y * rv.y;
+	}
+	write_partials<1>(v, partial);
+}
+
+// r . r and the iteration count.
+__global__ __launch_bounds__(kRed) void k_red_fcg_rr(int nwg, const double* __restrict__ partial,
+                                                     Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	double tot[1];
+	reduce_partials<1>(partial, nwg, tot);
+	if (threadIdx.x == 0) {
+		S->rr = tot[0];
+		S->iter += 1;
+	}
+}
+
 // Whether the residual recomputation runs (mode 1): the recursive residual
 // met the tolerance, or dropped below delta2 times its value at the last
 // recomputation, or a true-residual check has already failed (from then on
@@ -424,12 +534,16 @@ int ensure_workspace(bfmx_system* s) {
 	return BFMX_OK;
 }
 
+// FP32 copy of the matrix values, refreshed at every call (the values change
+// with assembly and conditions; the buffer, and so every captured graph that
+// reads it, stays).
 int ensure_f32(bfmx_system* s) {
 	if (!s->d_val32) {
-		BFMX_CHECK(dalloc(&s->d_val32, 4 * s->nnzb_pad));
-		k_to_f32<<<grid(4 * s->nnzb_pad), kBlock, 0, s->stream>>>(4 * s->nnzb_pad, s->d_val, s->d_val32);
-		BFMX_HIP(hipGetLastError());
+		set_error("FP32 matrix buffer not allocated");
+		return BFMX_E_ARG;
 	}
+	k_to_f32<<<grid(4 * s->nnzb_pad), kBlock, 0, s->stream>>>(4 * s->nnzb_pad, s->d_val, s->d_val32);
+	BFMX_HIP(hipGetLastError());
 	return BFMX_OK;
 }
 
@@ -478,8 +592,38 @@ void launch_iteration(bfmx_system* s, int pre, bool mixed, int it) {
 	k_red_beta<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
 }
 
-// kChunk iterations and the conditional residual recomputation.
+// One flexible-CG iteration with the multigrid V-cycle.
+void launch_fcg_iteration(bfmx_system* s, int it) {
+	int const nb = static_cast<int>(s->n_nodes);
+	int const g = grid(nb);
+	int const a = it & 1, b = a ^ 1;
+	int nwz = 0;
+	float* z = amg_vcycle(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nwz);
+	k_red_fcg_beta<<<1, kRed, 0, s->stream>>>(nwz, s->d_partial, s->d_scal);
+	k_fcg_spmv<<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[a], s->d_p[b], s->d_q,
+	                                        s->d_r, s->d_partial, s->d_scal);
+	k_red_fcg_alpha<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
+	k_fcg_update<<<g, kBlock, 0, s->stream>>>(nb, s->d_p[b], s->d_q, s->d_x, s->d_r, s->d_partial, s->d_scal);
+	k_red_fcg_rr<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
+}
+
+constexpr int kFcgChunk = 4;   // flexible-CG iterations per captured graph
+
+int chunk_iters(int pre) {
+	return pre == BFMX_PRECOND_AMG ? kFcgChunk : kChunk;
+}
+
+// One chunk of iterations and the conditional residual recomputation. The
+// multigrid path recomputes the residual only for the tolerance checks (its
+// recursive residual is FP64 throughout).
 void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
+	if (pre == BFMX_PRECOND_AMG) {
+		for (int it = 0; it < kFcgChunk; it++) {
+			launch_fcg_iteration(s, it);
+		}
+		launch_residual(s, BFMX_PRECOND_JACOBI, 1, rtol2, 0.0);
+		return;
+	}
 	for (int it = 0; it < kChunk; it++) {
 		launch_iteration(s, pre, mixed, it);
 	}
@@ -521,6 +665,7 @@ int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
 }  // namespace
 
 void release_solver(bfmx_system* s) {
+	amg_release(s);
 	if (s->graph.exec) {
 		(void) hipGraphExecDestroy(s->graph.exec);
 		s->graph.exec = nullptr;
@@ -549,7 +694,7 @@ void release_solver(bfmx_system* s) {
 
 int solve(bfmx_system* s, const double* b_in, int precond, int precision, double rtol, size_t max_iter,
           bfmx_stats_t* stats) {
-	if (precond != BFMX_PRECOND_JACOBI && precond != BFMX_PRECOND_BLOCK_JACOBI) {
+	if (precond != BFMX_PRECOND_JACOBI && precond != BFMX_PRECOND_BLOCK_JACOBI && precond != BFMX_PRECOND_AMG) {
 		set_error("unknown preconditioner %d", precond);
 		return BFMX_E_UNSUPPORTED;
 	}
@@ -561,13 +706,17 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		set_error("rtol must be positive");
 		return BFMX_E_ARG;
 	}
-	bool const mixed = precision == BFMX_PRECISION_MIXED;
+	bool const amg = precond == BFMX_PRECOND_AMG;
+	bool const mixed = !amg && precision == BFMX_PRECISION_MIXED;
 	int64_t const n = s->n_nodes;
 	int64_t const cap = max_iter ? static_cast<int64_t>(max_iter) : 20 * 2 * n;
 	hipStream_t const st = s->stream;
 	BFMX_CHECK(ensure_workspace(s));
-	if (mixed) {
-		BFMX_CHECK(ensure_f32(s));
+	if ((mixed || amg) && !s->d_val32) {
+		BFMX_CHECK(dalloc(&s->d_val32, 4 * s->nnzb_pad));   // before any capture reads its address
+	}
+	if (amg) {
+		BFMX_CHECK(amg_build(s));   // topology, once per system
 	}
 	if (b_in) {
 		std::vector<double> bp(2 * n);
@@ -587,10 +736,18 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	}
 
 	auto const t0 = std::chrono::steady_clock::now();
-	k_precond<<<grid(n), kBlock, 0, st>>>(static_cast<int>(n), precond, s->d_diag_blk, s->d_val, s->d_dinv);
+	if (mixed || amg) {
+		BFMX_CHECK(ensure_f32(s));
+	}
+	if (amg) {
+		BFMX_CHECK(amg_setup_numeric(s));
+	}
+	int const pre_res = amg ? BFMX_PRECOND_JACOBI : precond;   // z of the residual kernel (unused by FCG)
+	k_precond<<<grid(n), kBlock, 0, st>>>(static_cast<int>(n), pre_res, s->d_diag_blk, s->d_val, s->d_dinv);
 	BFMX_HIP(hipMemsetAsync(s->d_x, 0, 2 * n * sizeof(double), st));
+	BFMX_HIP(hipMemsetAsync(s->d_q, 0, 2 * n * sizeof(double), st));
 	BFMX_HIP(hipMemsetAsync(s->d_scal, 0, sizeof(Scalars), st));
-	launch_residual(s, precond, 0, rtol2, 0.0);
+	launch_residual(s, pre_res, 0, rtol2, 0.0);
 	BFMX_HIP(hipGetLastError());
 	BFMX_HIP(hipMemcpyAsync(&s->h_scal[0], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
 	BFMX_HIP(hipStreamSynchronize(st));
@@ -607,7 +764,7 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 				launch_chunk(s, precond, mixed, rtol2);
 				BFMX_HIP(hipGetLastError());
 			}
-			launched += kChunk;
+			launched += chunk_iters(precond);
 			int const slot = static_cast<int>(chunks & 3);
 			BFMX_HIP(hipMemcpyAsync(&s->h_scal[slot], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
 			BFMX_HIP(hipEventRecord(s->ev[slot], st));

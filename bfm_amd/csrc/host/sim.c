@@ -92,7 +92,7 @@ int bfm_sim_store_stats(bfm_sim_t const* sim, size_t n, bfm_solve_stats_t const*
 }
 
 int bfm_sim_set_solver(bfm_sim_t* sim, bfm_solver_t const* solver) {
-	if (!(solver->rtol > 0) || solver->precond < 0 || solver->precond > 1 || solver->precision < 0 ||
+	if (!(solver->rtol > 0) || solver->precond < 0 || solver->precond > 2 || solver->precision < 0 ||
 	    solver->precision > 1) {
 		BFM_ERROR(sim->state, "invalid solver settings");
 		return -1;

@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libbfmhip.so")
 
-PRECOND_JACOBI, PRECOND_BLOCK_JACOBI = 0, 1
+PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_AMG = 0, 1, 2
 FP64, MIXED = 0, 1
 DIRICHLET_X, DIRICHLET_Y, NEUMANN_X, NEUMANN_Y = 0, 1, 2, 3
 NEUMANN_NORMAL, NEUMANN_TANGENT, DIRICHLET_NORMAL, DIRICHLET_TANGENT = 4, 5, 6, 7

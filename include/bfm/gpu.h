@@ -12,7 +12,8 @@
 #include <bfm/sim.h>
 
 typedef struct {
-	int precond;     /* 0 point Jacobi, 1 2x2 block Jacobi */
+	int precond;     /* 0 point Jacobi, 1 2x2 block Jacobi, 2 FP32 multigrid V-cycle
+	                    inside FP64 flexible CG */
 	int precision;   /* 0 FP64, 1 FP32 matrix inside FP64 refinement */
 	double rtol;     /* ||b - K u|| / ||b|| target, FP64 */
 	size_t max_iter; /* 0 = 20 x DOFs */

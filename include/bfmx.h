@@ -57,6 +57,9 @@ enum {
 enum {
 	BFMX_PRECOND_JACOBI = 0,       /* 1 / diag(A), per DOF */
 	BFMX_PRECOND_BLOCK_JACOBI = 1, /* inverse of each node's 2x2 diagonal block */
+	BFMX_PRECOND_AMG = 2,          /* flexible CG (FP64) preconditioned by one FP32
+	                                  V-cycle of aggregation multigrid built on the
+	                                  rigid-body modes; precision is ignored */
 };
 
 /* Arithmetic of the solve. */
