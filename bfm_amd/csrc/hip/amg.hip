@@ -1098,6 +1098,37 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 
 }  // namespace
 
+// reps launches of the finest level's Chebyshev step with its SpMV (the
+// V-cycle's step k = 1), bracketed by events e0 and e1 on the system's
+// stream; b is the solver's residual buffer. Algorithmic bytes per launch:
+// FP32 blocks and column indices, slice pointers, and per node the FP64 b,
+// the FP32 inverse diagonal block, x_in (gathered once), d read and written
+// and x_out written.
+int amg_time_smoother(bfmx_system* s, int reps, hipEvent_t e0, hipEvent_t e1, double* bytes) {
+	hipStream_t const st = s->stream;
+	AmgLevel& L = s->amg[0];
+	if (s->amg.size() == 1 && L.d_dense) {
+		set_error("the finest level is solved densely (no smoother to time)");
+		return BFMX_E_UNSUPPORTED;
+	}
+	const Scalars* S = s->d_scal;
+	cheb<2, double>(st, L, s->d_r, true, 0, nullptr, L.d_x[0], nullptr, nullptr, S);
+	for (int i = 0; i < 3; i++) {
+		cheb<2, double>(st, L, s->d_r, false, 1, L.d_x[0], L.d_x[1], nullptr, nullptr, S);
+	}
+	BFMX_HIP(hipGetLastError());
+	BFMX_HIP(hipEventRecord(e0, st));
+	for (int i = 0; i < reps; i++) {
+		cheb<2, double>(st, L, s->d_r, false, 1, L.d_x[0], L.d_x[1], nullptr, nullptr, S);
+	}
+	BFMX_HIP(hipEventRecord(e1, st));
+	BFMX_HIP(hipGetLastError());
+	if (bytes) {
+		*bytes = static_cast<double>(L.nnzb) * 20 + 4.0 * (L.n_slices + 1) + 64.0 * L.nb;
+	}
+	return BFMX_OK;
+}
+
 float* amg_vcycle(bfmx_system* s, const double* r, const double* q, double* partial, const Scalars* S,
                   int* n_partials) {
 	AmgLevel const& L0 = s->amg[0];

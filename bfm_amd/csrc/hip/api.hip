@@ -361,5 +361,18 @@ int bfmx_system_time_spmv(bfmx_system_t* s, int precision, int reps, double* avg
 	return guarded([&] { return time_spmv(s, precision, reps, avg_ms, bytes); });
 }
 
+int bfmx_system_time_kernel(bfmx_system_t* s, int kernel, int reps, double* avg_ms, double* bytes) {
+	BFMX_CHECK(check(s));
+	if (!avg_ms || reps <= 0 || !s->assembled) {
+		set_error("bad arguments to time_kernel");
+		return BFMX_E_ARG;
+	}
+	if (kernel < BFMX_KERNEL_CG_SPMV64 || kernel > BFMX_KERNEL_AMG_SMOOTH) {
+		set_error("unknown kernel %d", kernel);
+		return BFMX_E_UNSUPPORTED;
+	}
+	return guarded([&] { return time_kernel(s, kernel, reps, avg_ms, bytes); });
+}
+
 }  // extern "C"
 #pragma GCC visibility pop

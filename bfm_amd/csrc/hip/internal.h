@@ -216,6 +216,9 @@ float* amg_vcycle(bfmx_system* s, const double* r, const double* q, double* part
 int solve(bfmx_system* s, const double* b_in, int precond, int precision, double rtol, size_t max_iter,
           bfmx_stats_t* stats);
 int time_spmv(bfmx_system* s, int precision, int reps, double* avg_ms, double* bytes);
+int time_kernel(bfmx_system* s, int kernel, int reps, double* avg_ms, double* bytes);
+// amg.hip: reps launches of one finest-level smoothing step (set-up done)
+int amg_time_smoother(bfmx_system* s, int reps, hipEvent_t e0, hipEvent_t e1, double* bytes);
 void release_solver(bfmx_system* s);
 
 template <typename T>

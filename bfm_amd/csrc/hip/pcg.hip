@@ -539,8 +539,7 @@ int ensure_workspace(bfmx_system* s) {
 // reads it, stays).
 int ensure_f32(bfmx_system* s) {
 	if (!s->d_val32) {
-		set_error("FP32 matrix buffer not allocated");
-		return BFMX_E_ARG;
+		BFMX_CHECK(dalloc(&s->d_val32, 4 * s->nnzb_pad));
 	}
 	k_to_f32<<<grid(4 * s->nnzb_pad), kBlock, 0, s->stream>>>(4 * s->nnzb_pad, s->d_val, s->d_val32);
 	BFMX_HIP(hipGetLastError());
@@ -812,6 +811,61 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		          last.bb > 0 ? std::sqrt(last.rr / last.bb) : 0.0);
 	}
 	return status;
+}
+
+// Prepares the solver buffers and the multigrid hierarchy as a solve does.
+int prepare_amg(bfmx_system* s) {
+	BFMX_CHECK(ensure_f32(s));
+	BFMX_CHECK(amg_build(s));
+	return amg_setup_numeric(s);
+}
+
+int time_kernel(bfmx_system* s, int kernel, int reps, double* avg_ms, double* bytes) {
+	if (kernel == BFMX_KERNEL_CG_SPMV64 || kernel == BFMX_KERNEL_CG_SPMV32) {
+		return time_spmv(s, kernel == BFMX_KERNEL_CG_SPMV64 ? BFMX_PRECISION_FP64 : BFMX_PRECISION_MIXED, reps,
+		                 avg_ms, bytes);
+	}
+	int64_t const n = s->n_nodes;
+	hipStream_t const st = s->stream;
+	BFMX_CHECK(finalize_conditions(s));
+	BFMX_CHECK(ensure_workspace(s));
+	BFMX_CHECK(prepare_amg(s));
+	Scalars init{};
+	BFMX_HIP(hipMemcpyAsync(s->d_scal, &init, sizeof(Scalars), hipMemcpyHostToDevice, st));
+	k_fill<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 1.0, s->d_r);
+	k_fill<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 0.0, s->d_p[0]);
+	float ms = 0;
+	if (kernel == BFMX_KERNEL_FCG_SPMV) {
+		int const nb = static_cast<int>(n);
+		int const g = grid(nb);
+		int nwz = 0;
+		float* z = amg_vcycle(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nwz);   // an FP32 z
+		auto launch = [&] {
+			k_fcg_spmv<<<g, kBlock, 0, st>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[0], s->d_p[1], s->d_q,
+			                                 s->d_r, s->d_partial, s->d_scal);
+		};
+		for (int i = 0; i < 3; i++) {
+			launch();
+		}
+		BFMX_HIP(hipGetLastError());
+		BFMX_HIP(hipEventRecord(s->ev[4], st));
+		for (int i = 0; i < reps; i++) {
+			launch();
+		}
+		BFMX_HIP(hipEventRecord(s->ev[5], st));
+		if (bytes) {
+			// matrix blocks + slice pointers, gathered z (FP32) and p_old, read r,
+			// written p_new and q
+			double const nd = 2.0 * static_cast<double>(n);
+			*bytes = static_cast<double>(s->nnzb) * 36 + 4.0 * (s->n_slices + 1) + nd * (4 + 8 + 8 + 8 + 8);
+		}
+	} else {
+		BFMX_CHECK(amg_time_smoother(s, reps, s->ev[4], s->ev[5], bytes));
+	}
+	BFMX_HIP(hipEventSynchronize(s->ev[5]));
+	BFMX_HIP(hipEventElapsedTime(&ms, s->ev[4], s->ev[5]));
+	*avg_ms = static_cast<double>(ms) / reps;
+	return BFMX_OK;
 }
 
 int time_spmv(bfmx_system* s, int precision, int reps, double* avg_ms, double* bytes) {

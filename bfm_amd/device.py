@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libbfmhip.so")
 
 PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_AMG = 0, 1, 2
 FP64, MIXED = 0, 1
+KERNEL_CG_SPMV64, KERNEL_CG_SPMV32, KERNEL_FCG_SPMV, KERNEL_AMG_SMOOTH = 0, 1, 2, 3
 DIRICHLET_X, DIRICHLET_Y, NEUMANN_X, NEUMANN_Y = 0, 1, 2, 3
 NEUMANN_NORMAL, NEUMANN_TANGENT, DIRICHLET_NORMAL, DIRICHLET_TANGENT = 4, 5, 6, 7
 
@@ -26,6 +27,7 @@ EXPORTS = [
     "bfmx_system_set_edges", "bfmx_system_assemble", "bfmx_system_apply_condition",
     "bfmx_system_solve", "bfmx_system_info", "bfmx_system_node_perm", "bfmx_system_get_bsr",
     "bfmx_system_get_rhs", "bfmx_system_get_u", "bfmx_system_get_entry", "bfmx_system_time_spmv",
+    "bfmx_system_time_kernel",
 ]
 
 
@@ -87,6 +89,7 @@ def lib():
         L.bfmx_system_get_u.argtypes = [vp, _dp]
         L.bfmx_system_get_entry.argtypes = [vp, sz, sz, C.POINTER(C.c_double)]
         L.bfmx_system_time_spmv.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.bfmx_system_time_kernel.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         for name in EXPORTS[1:]:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -228,6 +231,12 @@ class System:
     def time_spmv(self, precision=FP64, reps=50):
         ms, by = C.c_double(), C.c_double()
         _check(lib().bfmx_system_time_spmv(self.h, precision, reps, C.byref(ms), C.byref(by)))
+        return ms.value, by.value
+
+    def time_kernel(self, kernel, reps=50):
+        """(mean ms per launch, algorithmic bytes per launch) of one solver kernel."""
+        ms, by = C.c_double(), C.c_double()
+        _check(lib().bfmx_system_time_kernel(self.h, kernel, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
 

@@ -168,6 +168,22 @@ int bfmx_system_get_entry(bfmx_system_t* s, size_t i, size_t j, double* out);
  * kernel's algorithmic bytes per launch. */
 int bfmx_system_time_spmv(bfmx_system_t* s, int precision, int reps, double* avg_ms, double* bytes);
 
+/* Kernels that bfmx_system_time_kernel can time. */
+enum {
+	BFMX_KERNEL_CG_SPMV64 = 0,   /* FP64 CG SpMV (as bfmx_system_time_spmv, FP64) */
+	BFMX_KERNEL_CG_SPMV32 = 1,   /* FP32-matrix CG SpMV (as bfmx_system_time_spmv, mixed) */
+	BFMX_KERNEL_FCG_SPMV = 2,    /* flexible-CG SpMV: FP32 z, FP64 matrix and directions */
+	BFMX_KERNEL_AMG_SMOOTH = 3,  /* one finest-level Chebyshev smoothing step of the
+	                                multigrid V-cycle: FP32 SpMV + block Jacobi */
+};
+
+/* Times reps back-to-back launches of one solver kernel on the system's
+ * stream with HIP events, on the current assembled system (the multigrid
+ * kernel sets the hierarchy up first). avg_ms receives the mean duration of
+ * one launch; bytes (may be NULL) the kernel's algorithmic bytes per launch.
+ * No reference counterpart: the benchmark's roofline figure. */
+int bfmx_system_time_kernel(bfmx_system_t* s, int kernel, int reps, double* avg_ms, double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -89,7 +89,7 @@ C1_CONDS = [(3, meshgen.BEAM_TRACTION, "top"), (0, 0.0, "left"), (1, 0.0, "left"
     ("c1_gravity", C1_CONDS, True),
     ("c1_neumann_last", C1_CONDS[1:] + C1_CONDS[:1], False),
 ])
-@pytest.mark.parametrize("precond", [D.PRECOND_JACOBI, D.PRECOND_BLOCK_JACOBI])
+@pytest.mark.parametrize("precond", [D.PRECOND_JACOBI, D.PRECOND_BLOCK_JACOBI, D.PRECOND_AMG])
 def test_c1_beam(golden, name, conds, grav, precond):
     mesh = meshgen.beam(40, 10, kind=3)
     mat = meshgen.BEAM_MATERIAL
@@ -107,7 +107,8 @@ def test_c1_beam(golden, name, conds, grav, precond):
     assert err <= U_TOL
 
 
-def test_lepl1110_mesh_problem(golden):
+@pytest.mark.parametrize("precond", [D.PRECOND_JACOBI, D.PRECOND_AMG])
+def test_lepl1110_mesh_problem(golden, precond):
     """8.lepl1110 + problem.txt as parsed by the reference, solved on the GPU,
     against the reference's solve and the 8-digit data/U.txt, data/V.txt."""
     coords = golden["lepl8/coords"]
@@ -121,7 +122,7 @@ def test_lepl1110_mesh_problem(golden):
     for i in range(int(golden["lepl8/n_conditions"])):
         kind, value = golden[f"lepl8/cond{i}_kind_value"]
         s.apply_condition(int(kind), value, flags[i])
-    u, st = s.solve(rtol=1e-10)
+    u, st = s.solve(precond=precond, rtol=1e-10)
     assert st["status"] == 0
     err = rel_err(u, golden["lepl8/u"])
     print(f"lepl8: {st['iterations']} iterations, residual {st['rel_residual']:.2e}, error vs libbfm {err:.2e}")
@@ -258,3 +259,91 @@ def test_full_size_beam_residual_against_oracle():
     print(f"GPU u against the oracle's system: relative residual {res:.3e} ({st['iterations']} iterations)")
     assert res <= 2e-8
     assert rel_err(s.get_rhs(), b) <= 1e-14
+    u2, st2 = s.solve(precond=D.PRECOND_AMG, rtol=1e-8)
+    assert st2["status"] == 0 and st2["rel_residual"] <= 1e-8
+    res2 = np.linalg.norm(b - A.matvec(u2)) / np.linalg.norm(b)
+    print(f"multigrid FCG: relative residual {res2:.3e} against the oracle's system ({st2['iterations']} "
+          f"iterations, {st2['solve_ms']:.1f} ms vs {st['solve_ms']:.1f} ms), "
+          f"difference to PCG {rel_err(u2, u):.2e}")
+    assert res2 <= 2e-8
+    assert rel_err(u2, u) <= 1e-7
+
+
+# ---------------------------------------------------------------- multigrid FCG
+
+
+def test_amg_heterogeneous_beam_matches_fp64_pcg():
+    """Per-element E over two decades (log-uniform [1e9, 1e11]): the FP32
+    multigrid inside FP64 flexible CG reaches the same FP64 residual as plain
+    PCG, in far fewer iterations; the error between the two is stated."""
+    mesh = meshgen.beam(200, 50, kind=4, jitter=0.2, seed=3)
+    mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=3)
+    s = D.beam_system(mesh)
+    u_pcg, st_pcg = s.solve(rtol=1e-9)    # the residual floor of this system is ~6e-10
+    u_amg, st_amg = s.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+    assert st_amg["status"] == 0 and st_amg["rel_residual"] <= 1e-9
+    err = rel_err(u_amg, u_pcg)
+    print(f"amg vs pcg: {st_amg['iterations']} vs {st_pcg['iterations']} iterations, rel difference {err:.2e}")
+    assert err <= 1e-6
+    assert st_amg["iterations"] * 5 < st_pcg["iterations"]
+
+
+def test_amg_p1_mesh_against_oracle():
+    import oracle as O
+    mesh = meshgen.beam(60, 15, kind=3, jitter=0.15, seed=4)
+    mat = meshgen.BEAM_MATERIAL
+    kinds, values, flags = meshgen.beam_conditions(mesh)
+    s = D.beam_system(mesh, stress=False)
+    u, st = s.solve(precond=D.PRECOND_AMG, rtol=1e-10)
+    assert st["status"] == 0
+    u_ref, *_ = O.solve_dense(mesh, False, mat["E"], mat["nu"], mat["rho"], kinds, values, flags)
+    err = rel_err(u, u_ref)
+    print(f"P1 multigrid: {st['iterations']} iterations, error vs oracle {err:.2e}")
+    assert err <= 1e-8
+
+
+def test_amg_is_deterministic_and_reusable():
+    """Two solves give bitwise identical solutions; re-assembly with other
+    conditions re-runs the numeric set-up (the hierarchy topology and the
+    captured graphs stay)."""
+    mesh = meshgen.beam(120, 30, kind=4, jitter=0.2, seed=5)
+    s = D.beam_system(mesh)
+    u1, st1 = s.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+    u2, st2 = s.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+    assert st1["iterations"] == st2["iterations"]
+    assert np.array_equal(u1, u2)
+    # clamp the right edge too: new Dirichlet set, new matrix
+    mat = meshgen.BEAM_MATERIAL
+    s.assemble(1, mat["E"], mat["nu"], mat["rho"])
+    kinds, values, flags = meshgen.beam_conditions(mesh)
+    s.apply_conditions(kinds, values, flags)
+    s.apply_condition(D.DIRICHLET_X, 0.0, mesh.node_sets["right"])
+    s.apply_condition(D.DIRICHLET_Y, 0.0, mesh.node_sets["right"])
+    u3, st3 = s.solve(precond=D.PRECOND_AMG, rtol=1e-10)
+    u4, st4 = s.solve(precond=D.PRECOND_JACOBI, rtol=1e-10)
+    assert st3["status"] == 0
+    err = rel_err(u3, u4)
+    print(f"re-assembled: {st3['iterations']} vs {st4['iterations']} iterations, rel difference {err:.2e}")
+    assert err <= 1e-8
+    assert np.abs(u3).max() < np.abs(u1).max()   # both ends clamped: stiffer
+
+
+def test_amg_single_level_is_a_direct_solve():
+    """A mesh small enough for one level: the preconditioner is the dense
+    FP32 inverse, so FCG stops after very few iterations."""
+    mesh = meshgen.beam(6, 3, kind=4)
+    s = D.beam_system(mesh)
+    u, st = s.solve(precond=D.PRECOND_AMG, rtol=1e-10)
+    u_ref, _ = s.solve(precond=D.PRECOND_JACOBI, rtol=1e-12)
+    assert st["status"] == 0 and st["iterations"] <= 4
+    assert rel_err(u, u_ref) <= 1e-9
+
+
+def test_kernel_timing_entry_point():
+    mesh = meshgen.beam(100, 25, kind=4)
+    s = D.beam_system(mesh)
+    for k in (D.KERNEL_CG_SPMV64, D.KERNEL_CG_SPMV32, D.KERNEL_FCG_SPMV, D.KERNEL_AMG_SMOOTH):
+        ms, by = s.time_kernel(k, reps=5)
+        assert ms > 0 and by > 0
+    u, st = s.solve(precond=D.PRECOND_AMG, rtol=1e-9)   # timing leaves the system solvable
+    assert st["status"] == 0
