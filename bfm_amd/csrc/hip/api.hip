@@ -374,5 +374,182 @@ int bfmx_system_time_kernel(bfmx_system_t* s, int kernel, int reps, double* avg_
 	return guarded([&] { return time_kernel(s, kernel, reps, avg_ms, bytes); });
 }
 
+// ---- learned preconditioner
+
+int bfmx_model_n_params(size_t* n) {
+	if (!n) {
+		set_error("null argument");
+		return BFMX_E_ARG;
+	}
+	*n = static_cast<size_t>(mpnn_n_params());
+	return BFMX_OK;
+}
+
+int bfmx_model_create(bfmx_model_t** out, uint64_t seed) {
+	if (!out) {
+		set_error("null argument");
+		return BFMX_E_ARG;
+	}
+	int count = 0;
+	if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+		set_error("no GPU visible");
+		return BFMX_E_NODEV;
+	}
+	return guarded([&] { return model_create(out, seed); });
+}
+
+int bfmx_model_destroy(bfmx_model_t* m) {
+	model_destroy(m);
+	return BFMX_OK;
+}
+
+int bfmx_model_get_params(bfmx_model_t* m, float* params) {
+	if (!m || !params) {
+		set_error("null argument");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return model_get_params(m, params); });
+}
+
+int bfmx_model_set_params(bfmx_model_t* m, const float* params) {
+	if (!m || !params) {
+		set_error("null argument");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return model_set_params(m, params); });
+}
+
+int bfmx_trainset_create(bfmx_trainset_t** out) {
+	if (!out) {
+		set_error("null argument");
+		return BFMX_E_ARG;
+	}
+	*out = new (std::nothrow) bfmx_trainset();
+	if (!*out) {
+		set_error("out of host memory");
+		return BFMX_E_NOMEM;
+	}
+	return BFMX_OK;
+}
+
+int bfmx_trainset_destroy(bfmx_trainset_t* ts) {
+	trainset_destroy(ts);
+	return BFMX_OK;
+}
+
+int bfmx_trainset_add(bfmx_trainset_t* ts, bfmx_system_t* s, int max_pairs, int every, double rtol, int* n_added) {
+	BFMX_CHECK(check(s));
+	if (!ts || max_pairs < 1 || every < 1 || !(rtol > 0)) {
+		set_error("bad arguments to trainset_add");
+		return BFMX_E_ARG;
+	}
+	if (!s->assembled) {
+		set_error("trainset_add before assemble");
+		return BFMX_E_ARG;
+	}
+	return guarded([&]() -> int {
+		size_t const before = ts->entries.size();
+		BFMX_CHECK(trainset_add(ts, s, max_pairs, every, rtol));
+		if (n_added) {
+			*n_added = ts->entries.size() > before ? ts->entries.back().P : 0;
+		}
+		return BFMX_OK;
+	});
+}
+
+int bfmx_trainset_info(bfmx_trainset_t* ts, int entry, size_t* n_nodes, int* n_pairs) {
+	if (!ts || entry < 0 || entry >= static_cast<int>(ts->entries.size())) {
+		set_error("no training entry %d", entry);
+		return BFMX_E_ARG;
+	}
+	if (n_nodes) {
+		*n_nodes = static_cast<size_t>(ts->entries[entry].n);
+	}
+	if (n_pairs) {
+		*n_pairs = ts->entries[entry].P;
+	}
+	return BFMX_OK;
+}
+
+int bfmx_trainset_get(bfmx_trainset_t* ts, int entry, float* feat, float* r, float* e, float* s_den) {
+	if (!ts) {
+		set_error("null training set");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return trainset_get(ts, entry, feat, r, e, s_den); });
+}
+
+int bfmx_model_loss_grad(bfmx_model_t* m, bfmx_trainset_t* ts, int entry, int p0, int batch, double* loss,
+                         float* grad) {
+	if (!m || !ts || entry < 0 || entry >= static_cast<int>(ts->entries.size())) {
+		set_error("bad model, training set or entry");
+		return BFMX_E_ARG;
+	}
+	if (batch < 1 || p0 < 0 || p0 + batch > ts->entries[entry].P) {
+		set_error("pairs %d .. %d outside the entry's %d", p0, p0 + batch - 1, ts->entries[entry].P);
+		return BFMX_E_ARG;
+	}
+	return guarded([&]() -> int {
+		BFMX_CHECK(model_loss_grad(m, ts, entry, p0, batch));
+		BFMX_HIP(hipStreamSynchronize(m->stream));
+		if (loss) {
+			BFMX_HIP(hipMemcpy(loss, m->loss, sizeof(double), hipMemcpyDeviceToHost));
+		}
+		if (grad) {
+			BFMX_HIP(hipMemcpy(grad, m->d_grad, mpnn_n_params() * sizeof(float), hipMemcpyDeviceToHost));
+		}
+		return BFMX_OK;
+	});
+}
+
+int bfmx_model_train(bfmx_model_t* m, bfmx_trainset_t* ts, int steps, int batch, float lr, uint64_t seed,
+                     double* losses) {
+	if (!m || !ts || steps < 0 || batch < 1 || !(lr > 0)) {
+		set_error("bad arguments to model_train");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return model_train(m, ts, steps, batch, lr, seed, losses); });
+}
+
+int bfmx_system_set_model(bfmx_system_t* s, bfmx_model_t* m) {
+	BFMX_CHECK(check(s));
+	s->model = m;
+	return BFMX_OK;
+}
+
+int bfmx_system_apply_model(bfmx_system_t* s, const double* r, double* z) {
+	BFMX_CHECK(check(s));
+	if (!r || !z || !s->assembled) {
+		set_error("bad arguments to apply_model");
+		return BFMX_E_ARG;
+	}
+	return guarded([&]() -> int {
+		int64_t const n = s->n_nodes;
+		hipStream_t const st = s->stream;
+		BFMX_CHECK(finalize_conditions(s));
+		BFMX_CHECK(ensure_workspace_public(s));
+		BFMX_CHECK(mpnn_prepare(s));
+		BFMX_CHECK(ensure_f32_public(s));
+		BFMX_CHECK(mpnn_setup_numeric(s));
+		// S: running, r . r of the given residual
+		double rr = 0;
+		for (int64_t k = 0; k < 2 * n; k++) {
+			rr += r[k] * r[k];
+		}
+		Scalars init{};
+		init.rr = rr;
+		BFMX_HIP(hipMemcpyAsync(s->d_scal, &init, sizeof(Scalars), hipMemcpyHostToDevice, st));
+		BFMX_HIP(hipMemcpyAsync(s->d_r, r, 2 * n * sizeof(double), hipMemcpyHostToDevice, st));
+		BFMX_HIP(hipMemsetAsync(s->d_q, 0, 2 * n * sizeof(double), st));
+		int nw = 0;
+		precond_block_jacobi(s);
+		double* zd = mpnn_apply(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nw);
+		BFMX_HIP(hipGetLastError());
+		BFMX_HIP(hipMemcpyAsync(z, zd, 2 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+		BFMX_HIP(hipStreamSynchronize(st));
+		return BFMX_OK;
+	});
+}
+
 }  // extern "C"
 #pragma GCC visibility pop

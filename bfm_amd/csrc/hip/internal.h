@@ -20,6 +20,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -114,9 +115,13 @@ struct Graph {
 	int precond = -1;
 	int precision = -1;
 	double rtol2 = -1;
+	const void* model = nullptr;
 };
 
 }  // namespace bfmx
+
+struct bfmx_model;
+struct bfmx_trainset;
 
 struct bfmx_system {
 	int device = 0;
@@ -176,12 +181,17 @@ struct bfmx_system {
 	double *d_x = nullptr, *d_r = nullptr, *d_z = nullptr, *d_q = nullptr;
 	double* d_p[2] = {nullptr, nullptr};
 	double* d_dinv = nullptr;        // 4 per node (block inverse) or 2 per node (point)
-	double* d_partial = nullptr;     // per-workgroup partial sums (3 per workgroup)
+	double* d_partial = nullptr;     // per-workgroup partial sums
+	int64_t partial_cap = 0;         // their number
 	bfmx::Scalars* d_scal = nullptr;
 	bfmx::Scalars* h_scal = nullptr; // pinned host mirror ring (4 entries)
 	hipEvent_t ev[6] = {};
-	bfmx::Graph graph;
+	std::vector<bfmx::Graph> graphs;   // captured chunks, one per solver configuration
 	std::vector<bfmx::AmgLevel> amg;   // multigrid hierarchy (built on first use)
+	bfmx_model* model = nullptr;       // learned preconditioner (borrowed)
+	float* d_mp_feat = nullptr;        // its node features (8 per node)
+	float* d_mp_h[4] = {nullptr, nullptr, nullptr, nullptr};   // hidden features (FP32, 64 per node)
+	double* d_mp_z = nullptr;          // its output (FP64, 2 per node)
 
 	bool assembled = false;
 	bool finalized = false;
@@ -189,6 +199,54 @@ struct bfmx_system {
 	std::vector<uint8_t> dirichlet_nodes;  // union of Dirichlet-flagged nodes (renumbered)
 	int64_t last_iters = 0;
 	bool has_u = false;
+};
+
+// Learned preconditioner: parameters, optimiser state and training workspace.
+struct bfmx_model {
+	int device = 0;
+	hipStream_t stream = nullptr;
+	float* d_params = nullptr;
+	float* d_grad = nullptr;
+	float* d_m = nullptr;
+	float* d_v = nullptr;
+	uint16_t* d_W = nullptr;    // 3 x 128 x 64 BF16 (layer weights)
+	uint16_t* d_Wt = nullptr;   // 3 x 64 x 128 BF16 (transposed)
+	int64_t adam_t = 0;
+	// training workspace (rows capacity): forward passes of x and of g
+	// (activations h, messages m), the adjoint's buffers, gradients
+	int64_t cap = 0;
+	uint16_t* h[4] = {nullptr, nullptr, nullptr, nullptr};
+	uint16_t* m[3] = {nullptr, nullptr, nullptr};
+	uint16_t* hg[4] = {nullptr, nullptr, nullptr, nullptr};
+	uint16_t* mg[3] = {nullptr, nullptr, nullptr};
+	uint16_t* adj[4] = {nullptr, nullptr, nullptr, nullptr};
+	float* x = nullptr;
+	float* xg = nullptr;
+	float* z = nullptr;
+	float* dA = nullptr;
+	float* dB = nullptr;
+	float* dM = nullptr;
+	float* up = nullptr;
+	float* partial = nullptr;   // gradient partials
+	double* lpart = nullptr;    // loss partials
+	double* loss = nullptr;
+};
+
+// Recorded (residual, error) pairs of one or more systems.
+struct bfmx_trainset {
+	struct Entry {
+		bfmx_system* sys = nullptr;
+		int64_t n = 0;          // nodes
+		int P = 0;              // pairs
+		float* d_val32 = nullptr;   // FP32 matrix snapshot
+		float* d_feat = nullptr;    // node features
+		float* d_r = nullptr;       // P x 2n
+		float* d_e = nullptr;
+		float* d_s = nullptr;       // P
+		float* d_den = nullptr;     // P
+	};
+	std::vector<Entry> entries;
+	int64_t pairs = 0;
 };
 
 namespace bfmx {
@@ -212,7 +270,38 @@ int amg_setup_numeric(bfmx_system* s);
 // kernel returns at once when S->done.
 float* amg_vcycle(bfmx_system* s, const double* r, const double* q, double* partial, const Scalars* S,
                   int* n_partials);
+// mpnn.hip
+int mpnn_n_params();
+int model_create(bfmx_model** out, uint64_t seed);
+void model_destroy(bfmx_model* m);
+int model_get_params(bfmx_model* m, float* p);
+int model_set_params(bfmx_model* m, const float* p);
+int model_loss_grad(bfmx_model* m, bfmx_trainset* ts, int entry, int p0, int B);
+int model_train(bfmx_model* m, bfmx_trainset* ts, int steps, int batch, float lr, uint64_t seed, double* losses);
+void trainset_destroy(bfmx_trainset* ts);
+int trainset_add(bfmx_trainset* ts, bfmx_system* s, int max_pairs, int every, double rtol);
+int trainset_get(bfmx_trainset* ts, int entry, float* feat, float* r, float* e, float* s_den);
+int mpnn_prepare(bfmx_system* s);
+int mpnn_setup_numeric(bfmx_system* s);
+// One application of the learned preconditioner to the FP64 residual r;
+// returns z (FP64, 2 per node: the FP64 block-Jacobi step of the solver's
+// inverses in s->d_dinv plus the network's FP32 correction); the last kernel
+// writes *n_partials partial sums of z . q. Every kernel returns at once when
+// S->done.
+double* mpnn_apply(bfmx_system* s, const double* r, const double* q, double* partial, const Scalars* S,
+                   int* n_partials);
+void mpnn_release(bfmx_system* s);
 // pcg.hip
+// Plain Jacobi PCG from x = 0 on the current right-hand side, calling rec(p)
+// after iteration p * every (p = 0 .. P-1) with the iterate in s->d_x and
+// its residual in s->d_r.
+int record_pcg(bfmx_system* s, int P, int every, const std::function<int(int)>& rec);
+// FP32 copy of the current matrix values into dst (4 * nnzb_pad floats).
+int snapshot_f32(bfmx_system* s, float* dst);
+int ensure_workspace_public(bfmx_system* s);
+// 2x2 block-Jacobi inverses of the current matrix into s->d_dinv.
+void precond_block_jacobi(bfmx_system* s);
+int ensure_f32_public(bfmx_system* s);
 int solve(bfmx_system* s, const double* b_in, int precond, int precision, double rtol, size_t max_iter,
           bfmx_stats_t* stats);
 int time_spmv(bfmx_system* s, int precision, int reps, double* avg_ms, double* bytes);

@@ -39,6 +39,7 @@
 //   and p . r), k_red_fcg_alpha, k_fcg_update (x, r and partials of r . r),
 //   k_red_fcg_rr.
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -294,11 +295,13 @@ __global__ __launch_bounds__(kRed) void k_red_fcg_beta(int nwg, const double* __
 	}
 }
 
-// p_new = z + beta p_old (z the FP32 V-cycle output), q = A p_new with the
-// FP64 matrix, partials p_new . q and p_new . r.
+// p_new = z + beta p_old (z the preconditioner's output: FP32 from the
+// V-cycle, FP64 from the learned preconditioner), q = A p_new with the FP64
+// matrix, partials p_new . q and p_new . r.
+template <typename ZT>
 __global__ __launch_bounds__(kBlock) void k_fcg_spmv(int nb, const int32_t* __restrict__ slice_ptr,
                                                      const int32_t* __restrict__ col,
-                                                     const double* __restrict__ val, const float* __restrict__ z,
+                                                     const double* __restrict__ val, const ZT* __restrict__ z,
                                                      const double* __restrict__ pold, double* __restrict__ pnew,
                                                      double* __restrict__ q, const double* __restrict__ r,
                                                      double* __restrict__ partial, const Scalars* __restrict__ S) {
@@ -308,8 +311,12 @@ __global__ __launch_bounds__(kBlock) void k_fcg_spmv(int nb, const int32_t* __re
 	double const beta = S->beta;
 	int const row = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
 	auto ldz = [&](int c) {
-		flt2 const f = *reinterpret_cast<const flt2*>(z + 2 * c);
-		return dbl2{f.x, f.y};
+		if constexpr (sizeof(ZT) == 4) {
+			flt2 const f = *reinterpret_cast<const flt2*>(z + 2 * c);
+			return dbl2{f.x, f.y};
+		} else {
+			return ld2(reinterpret_cast<const double*>(z) + 2 * c);
+		}
 	};
 	double v[2] = {0, 0};
 	if (row < nb) {
@@ -522,7 +529,10 @@ int ensure_workspace(bfmx_system* s) {
 	BFMX_CHECK(dalloc(&s->d_p[0], 2 * n));
 	BFMX_CHECK(dalloc(&s->d_p[1], 2 * n));
 	BFMX_CHECK(dalloc(&s->d_dinv, 4 * n));
-	BFMX_CHECK(dalloc(&s->d_partial, 3 * static_cast<size_t>(grid(n))));
+	// per-workgroup partial sums: three per 256-row workgroup (the residual
+	// kernel), one per 64-row workgroup (the learned preconditioner's decoder)
+	s->partial_cap = std::max<int64_t>(3 * static_cast<int64_t>(grid(n)), (n + 63) / 64);
+	BFMX_CHECK(dalloc(&s->d_partial, static_cast<size_t>(s->partial_cap)));
 	BFMX_CHECK(dalloc(&s->d_scal, 1));
 	BFMX_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_scal), 4 * sizeof(Scalars), hipHostMallocDefault));
 	for (auto& e : s->ev) {
@@ -591,16 +601,24 @@ void launch_iteration(bfmx_system* s, int pre, bool mixed, int it) {
 	k_red_beta<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
 }
 
-// One flexible-CG iteration with the multigrid V-cycle.
-void launch_fcg_iteration(bfmx_system* s, int it) {
+// One flexible-CG iteration with the multigrid V-cycle or the learned
+// preconditioner.
+void launch_fcg_iteration(bfmx_system* s, int pre, int it) {
 	int const nb = static_cast<int>(s->n_nodes);
 	int const g = grid(nb);
 	int const a = it & 1, b = a ^ 1;
 	int nwz = 0;
-	float* z = amg_vcycle(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nwz);
-	k_red_fcg_beta<<<1, kRed, 0, s->stream>>>(nwz, s->d_partial, s->d_scal);
-	k_fcg_spmv<<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[a], s->d_p[b], s->d_q,
-	                                        s->d_r, s->d_partial, s->d_scal);
+	if (pre == BFMX_PRECOND_AMG) {
+		float* z = amg_vcycle(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nwz);
+		k_red_fcg_beta<<<1, kRed, 0, s->stream>>>(nwz, s->d_partial, s->d_scal);
+		k_fcg_spmv<float><<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[a],
+		                                               s->d_p[b], s->d_q, s->d_r, s->d_partial, s->d_scal);
+	} else {
+		double* z = mpnn_apply(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nwz);
+		k_red_fcg_beta<<<1, kRed, 0, s->stream>>>(nwz, s->d_partial, s->d_scal);
+		k_fcg_spmv<double><<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[a],
+		                                                s->d_p[b], s->d_q, s->d_r, s->d_partial, s->d_scal);
+	}
 	k_red_fcg_alpha<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
 	k_fcg_update<<<g, kBlock, 0, s->stream>>>(nb, s->d_p[b], s->d_q, s->d_x, s->d_r, s->d_partial, s->d_scal);
 	k_red_fcg_rr<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
@@ -608,19 +626,24 @@ void launch_fcg_iteration(bfmx_system* s, int it) {
 
 constexpr int kFcgChunk = 4;   // flexible-CG iterations per captured graph
 
+bool is_fcg(int pre) {
+	return pre == BFMX_PRECOND_AMG || pre == BFMX_PRECOND_LEARNED;
+}
+
 int chunk_iters(int pre) {
-	return pre == BFMX_PRECOND_AMG ? kFcgChunk : kChunk;
+	return is_fcg(pre) ? kFcgChunk : kChunk;
 }
 
 // One chunk of iterations and the conditional residual recomputation. The
 // multigrid path recomputes the residual only for the tolerance checks (its
 // recursive residual is FP64 throughout).
 void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
-	if (pre == BFMX_PRECOND_AMG) {
+	if (is_fcg(pre)) {
 		for (int it = 0; it < kFcgChunk; it++) {
-			launch_fcg_iteration(s, it);
+			launch_fcg_iteration(s, pre, it);
 		}
-		launch_residual(s, BFMX_PRECOND_JACOBI, 1, rtol2, 0.0);
+		launch_residual(s, pre == BFMX_PRECOND_LEARNED ? BFMX_PRECOND_BLOCK_JACOBI : BFMX_PRECOND_JACOBI, 1, rtol2,
+		                0.0);
 		return;
 	}
 	for (int it = 0; it < kChunk; it++) {
@@ -637,27 +660,107 @@ bool graphs_enabled() {
 	return !(e && e[0] == '0');
 }
 
-int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
-	if (s->graph.exec && s->graph.precond == pre && s->graph.precision == (mixed ? 1 : 0) && s->graph.rtol2 == rtol2) {
-		return BFMX_OK;
-	}
-	if (s->graph.exec) {
-		BFMX_HIP(hipGraphExecDestroy(s->graph.exec));
-		s->graph.exec = nullptr;
+// The captured chunk for this configuration (captured on first use).
+int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, hipGraphExec_t* out) {
+	int const prec = mixed ? 1 : 0;
+	const void* model = pre == BFMX_PRECOND_LEARNED ? static_cast<const void*>(s->model) : nullptr;
+	for (auto const& gr : s->graphs) {
+		if (gr.precond == pre && gr.precision == prec && gr.rtol2 == rtol2 && gr.model == model) {
+			*out = gr.exec;
+			return BFMX_OK;
+		}
 	}
 	hipGraph_t g = nullptr;
 	BFMX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
 	launch_chunk(s, pre, mixed, rtol2);
 	BFMX_HIP(hipStreamEndCapture(s->stream, &g));
-	hipError_t const e = hipGraphInstantiate(&s->graph.exec, g, nullptr, nullptr, 0);
+	Graph gr;
+	hipError_t const e = hipGraphInstantiate(&gr.exec, g, nullptr, nullptr, 0);
 	(void) hipGraphDestroy(g);
 	if (e != hipSuccess) {
 		set_error("hipGraphInstantiate: %s", hipGetErrorString(e));
 		return BFMX_E_HIP;
 	}
-	s->graph.precond = pre;
-	s->graph.precision = mixed ? 1 : 0;
-	s->graph.rtol2 = rtol2;
+	gr.precond = pre;
+	gr.precision = prec;
+	gr.rtol2 = rtol2;
+	gr.model = model;
+	s->graphs.push_back(gr);
+	*out = gr.exec;
+	return BFMX_OK;
+}
+
+// A learned run stalls when its best r.r has not halved for twice as many
+// iterations as the run has taken (and at least kLearnedStall): a window
+// that grows with the run, as CG's residual plateaus do. The network's
+// operator is SPD, so this is a safeguard, not the expected end of a run.
+constexpr int64_t kLearnedStall = 4000;
+
+// Runs chunks of one solver from the current state (warm: x kept, else x = 0)
+// until the device flags an end, cap iterations are launched or, with
+// detect_stall, the recursive residual stops falling. *out: final scalars.
+int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t cap, bool warm, bool detect_stall,
+               Scalars* out) {
+	int64_t const n = s->n_nodes;
+	hipStream_t const st = s->stream;
+	bool const use_graph = graphs_enabled();
+	hipGraphExec_t exec = nullptr;
+	if (use_graph) {
+		BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2, &exec));
+	}
+	// the residual kernel's z (unused by flexible CG); the learned decoder reads
+	// the 2x2 block inverses
+	int const pre_res = precond == BFMX_PRECOND_AMG      ? BFMX_PRECOND_JACOBI
+	                    : precond == BFMX_PRECOND_LEARNED ? BFMX_PRECOND_BLOCK_JACOBI
+	                                                      : precond;
+	k_precond<<<grid(n), kBlock, 0, st>>>(static_cast<int>(n), pre_res, s->d_diag_blk, s->d_val, s->d_dinv);
+	if (!warm) {
+		BFMX_HIP(hipMemsetAsync(s->d_x, 0, 2 * n * sizeof(double), st));
+	}
+	BFMX_HIP(hipMemsetAsync(s->d_q, 0, 2 * n * sizeof(double), st));
+	BFMX_HIP(hipMemsetAsync(s->d_scal, 0, sizeof(Scalars), st));
+	launch_residual(s, pre_res, 0, rtol2, 0.0);
+	BFMX_HIP(hipGetLastError());
+	BFMX_HIP(hipMemcpyAsync(&s->h_scal[0], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	Scalars last = s->h_scal[0];
+	double mark = last.rr;
+	int64_t mark_iter = 0;
+
+	// Replay chunks, keeping one in flight while the one before it is inspected.
+	int64_t launched = 0, chunks = 0;
+	while (last.done == 0) {
+		bool const more = launched < cap;
+		if (more) {
+			if (use_graph) {
+				BFMX_HIP(hipGraphLaunch(exec, st));
+			} else {
+				launch_chunk(s, precond, mixed, rtol2);
+				BFMX_HIP(hipGetLastError());
+			}
+			launched += chunk_iters(precond);
+			int const slot = static_cast<int>(chunks & 3);
+			BFMX_HIP(hipMemcpyAsync(&s->h_scal[slot], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+			BFMX_HIP(hipEventRecord(s->ev[slot], st));
+			chunks++;
+		}
+		int64_t const look = (more && chunks >= 2) ? chunks - 2 : chunks - 1;
+		BFMX_HIP(hipEventSynchronize(s->ev[look & 3]));
+		last = s->h_scal[look & 3];
+		if (!more && look == chunks - 1) {
+			break;
+		}
+		if (detect_stall && last.done == 0) {
+			if (last.rr < 0.5 * mark) {
+				mark = last.rr;
+				mark_iter = last.iter;
+			} else if (last.iter - mark_iter >= std::max(kLearnedStall, 2 * mark_iter)) {
+				break;
+			}
+		}
+	}
+	BFMX_HIP(hipStreamSynchronize(st));
+	BFMX_HIP(hipMemcpy(out, s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost));
 	return BFMX_OK;
 }
 
@@ -665,10 +768,11 @@ int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
 
 void release_solver(bfmx_system* s) {
 	amg_release(s);
-	if (s->graph.exec) {
-		(void) hipGraphExecDestroy(s->graph.exec);
-		s->graph.exec = nullptr;
+	mpnn_release(s);
+	for (auto& gr : s->graphs) {
+		(void) hipGraphExecDestroy(gr.exec);
 	}
+	s->graphs.clear();
 	dfree(s->d_x);
 	dfree(s->d_r);
 	dfree(s->d_z);
@@ -693,7 +797,7 @@ void release_solver(bfmx_system* s) {
 
 int solve(bfmx_system* s, const double* b_in, int precond, int precision, double rtol, size_t max_iter,
           bfmx_stats_t* stats) {
-	if (precond != BFMX_PRECOND_JACOBI && precond != BFMX_PRECOND_BLOCK_JACOBI && precond != BFMX_PRECOND_AMG) {
+	if (precond < BFMX_PRECOND_JACOBI || precond > BFMX_PRECOND_LEARNED) {
 		set_error("unknown preconditioner %d", precond);
 		return BFMX_E_UNSUPPORTED;
 	}
@@ -706,16 +810,20 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		return BFMX_E_ARG;
 	}
 	bool const amg = precond == BFMX_PRECOND_AMG;
-	bool const mixed = !amg && precision == BFMX_PRECISION_MIXED;
+	bool const learned = precond == BFMX_PRECOND_LEARNED;
+	bool const mixed = !is_fcg(precond) && precision == BFMX_PRECISION_MIXED;
 	int64_t const n = s->n_nodes;
 	int64_t const cap = max_iter ? static_cast<int64_t>(max_iter) : 20 * 2 * n;
 	hipStream_t const st = s->stream;
 	BFMX_CHECK(ensure_workspace(s));
-	if ((mixed || amg) && !s->d_val32) {
+	if ((mixed || amg || learned) && !s->d_val32) {
 		BFMX_CHECK(dalloc(&s->d_val32, 4 * s->nnzb_pad));   // before any capture reads its address
 	}
 	if (amg) {
 		BFMX_CHECK(amg_build(s));   // topology, once per system
+	}
+	if (learned) {
+		BFMX_CHECK(mpnn_prepare(s));
 	}
 	if (b_in) {
 		std::vector<double> bp(2 * n);
@@ -729,56 +837,31 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		BFMX_HIP(hipMemcpyAsync(s->d_bsolve, s->d_b, 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
 	}
 	double const rtol2 = rtol * rtol;
-	bool const use_graph = graphs_enabled();
-	if (use_graph) {
-		BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2));
-	}
 
 	auto const t0 = std::chrono::steady_clock::now();
-	if (mixed || amg) {
+	if (mixed || amg || learned) {
 		BFMX_CHECK(ensure_f32(s));
 	}
 	if (amg) {
 		BFMX_CHECK(amg_setup_numeric(s));
 	}
-	int const pre_res = amg ? BFMX_PRECOND_JACOBI : precond;   // z of the residual kernel (unused by FCG)
-	k_precond<<<grid(n), kBlock, 0, st>>>(static_cast<int>(n), pre_res, s->d_diag_blk, s->d_val, s->d_dinv);
-	BFMX_HIP(hipMemsetAsync(s->d_x, 0, 2 * n * sizeof(double), st));
-	BFMX_HIP(hipMemsetAsync(s->d_q, 0, 2 * n * sizeof(double), st));
-	BFMX_HIP(hipMemsetAsync(s->d_scal, 0, sizeof(Scalars), st));
-	launch_residual(s, pre_res, 0, rtol2, 0.0);
-	BFMX_HIP(hipGetLastError());
-	BFMX_HIP(hipMemcpyAsync(&s->h_scal[0], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
-	BFMX_HIP(hipStreamSynchronize(st));
-	Scalars last = s->h_scal[0];
-
-	// Replay chunks, keeping one in flight while the one before it is inspected.
-	int64_t launched = 0, chunks = 0;
-	while (last.done == 0) {
-		bool const more = launched < cap;
-		if (more) {
-			if (use_graph) {
-				BFMX_HIP(hipGraphLaunch(s->graph.exec, st));
-			} else {
-				launch_chunk(s, precond, mixed, rtol2);
-				BFMX_HIP(hipGetLastError());
-			}
-			launched += chunk_iters(precond);
-			int const slot = static_cast<int>(chunks & 3);
-			BFMX_HIP(hipMemcpyAsync(&s->h_scal[slot], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
-			BFMX_HIP(hipEventRecord(s->ev[slot], st));
-			chunks++;
-		}
-		int64_t const look = (more && chunks >= 2) ? chunks - 2 : chunks - 1;
-		BFMX_HIP(hipEventSynchronize(s->ev[look & 3]));
-		last = s->h_scal[look & 3];
-		if (!more && look == chunks - 1) {
-			break;
-		}
+	if (learned) {
+		BFMX_CHECK(mpnn_setup_numeric(s));
 	}
-	BFMX_HIP(hipStreamSynchronize(st));
+	Scalars last{};
+	BFMX_CHECK(run_chunks(s, precond, mixed, rtol2, cap, false, learned, &last));
+	int64_t iters = last.iter;
+	int32_t fallback = 0;
+	int64_t fb_iters = 0;
+	if (learned && (last.done == 0 || last.done == 2)) {
+		// breakdown, iteration limit or stall of the learned run: block-Jacobi
+		// PCG from the current iterate
+		BFMX_CHECK(run_chunks(s, BFMX_PRECOND_BLOCK_JACOBI, false, rtol2, cap, true, false, &last));
+		fallback = 1;
+		fb_iters = last.iter;
+		iters += last.iter;
+	}
 	auto const t1 = std::chrono::steady_clock::now();
-	BFMX_HIP(hipMemcpy(&last, s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost));
 	int32_t status = BFMX_OK;
 	switch (last.done) {
 		case 1: status = BFMX_OK; break;
@@ -786,7 +869,7 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		case 3: status = BFMX_E_STAGNATION; break;
 		default: status = BFMX_E_MAXITER; break;
 	}
-	s->last_iters = last.iter;
+	s->last_iters = iters;
 	s->has_u = true;
 	double const rel = last.bb > 0 ? std::sqrt(last.rr_true / last.bb) : 0.0;
 	if (stats) {
@@ -794,12 +877,14 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		stats->precond = precond;
 		stats->precision = precision;
 		stats->restarts = static_cast<int32_t>(last.updates);
-		stats->iterations = last.iter;
+		stats->iterations = iters;
 		stats->n_dof = 2 * n;
 		stats->rtol = rtol;
 		stats->rel_residual = rel;
 		stats->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
 		stats->b_norm = std::sqrt(last.bb);
+		stats->fallback = fallback;
+		stats->fallback_iterations = static_cast<int32_t>(fb_iters);
 	}
 	if (status == BFMX_E_BREAKDOWN) {
 		set_error("CG breakdown after %lld iterations (p.Ap = %g): matrix not SPD or singular",
@@ -811,6 +896,50 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		          last.bb > 0 ? std::sqrt(last.rr / last.bb) : 0.0);
 	}
 	return status;
+}
+
+int record_pcg(bfmx_system* s, int P, int every, const std::function<int(int)>& rec) {
+	int64_t const n = s->n_nodes;
+	hipStream_t const st = s->stream;
+	BFMX_CHECK(ensure_workspace(s));
+	k_precond<<<grid(n), kBlock, 0, st>>>(static_cast<int>(n), BFMX_PRECOND_JACOBI, s->d_diag_blk, s->d_val,
+	                                      s->d_dinv);
+	BFMX_HIP(hipMemsetAsync(s->d_x, 0, 2 * n * sizeof(double), st));
+	BFMX_HIP(hipMemsetAsync(s->d_q, 0, 2 * n * sizeof(double), st));
+	BFMX_HIP(hipMemsetAsync(s->d_scal, 0, sizeof(Scalars), st));
+	BFMX_HIP(hipMemcpyAsync(s->d_bsolve, s->d_b, 2 * n * sizeof(double), hipMemcpyDeviceToDevice, st));
+	launch_residual(s, BFMX_PRECOND_JACOBI, 0, 0.0, 0.0);
+	int it = 0;
+	for (int p = 0; p < P; p++) {
+		if (p > 0) {
+			for (int k = 0; k < every; k++) {
+				launch_iteration(s, BFMX_PRECOND_JACOBI, false, it++);
+			}
+		}
+		BFMX_CHECK(rec(p));
+	}
+	BFMX_HIP(hipGetLastError());
+	BFMX_HIP(hipStreamSynchronize(st));
+	return BFMX_OK;
+}
+
+void precond_block_jacobi(bfmx_system* s) {
+	k_precond<<<grid(s->n_nodes), kBlock, 0, s->stream>>>(static_cast<int>(s->n_nodes), BFMX_PRECOND_BLOCK_JACOBI,
+	                                                      s->d_diag_blk, s->d_val, s->d_dinv);
+}
+
+int ensure_workspace_public(bfmx_system* s) {
+	return ensure_workspace(s);
+}
+
+int ensure_f32_public(bfmx_system* s) {
+	return ensure_f32(s);
+}
+
+int snapshot_f32(bfmx_system* s, float* dst) {
+	k_to_f32<<<grid(4 * s->nnzb_pad), kBlock, 0, s->stream>>>(4 * s->nnzb_pad, s->d_val, dst);
+	BFMX_HIP(hipGetLastError());
+	return BFMX_OK;
 }
 
 // Prepares the solver buffers and the multigrid hierarchy as a solve does.
@@ -841,8 +970,8 @@ int time_kernel(bfmx_system* s, int kernel, int reps, double* avg_ms, double* by
 		int nwz = 0;
 		float* z = amg_vcycle(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nwz);   // an FP32 z
 		auto launch = [&] {
-			k_fcg_spmv<<<g, kBlock, 0, st>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[0], s->d_p[1], s->d_q,
-			                                 s->d_r, s->d_partial, s->d_scal);
+			k_fcg_spmv<float><<<g, kBlock, 0, st>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[0], s->d_p[1],
+			                                        s->d_q, s->d_r, s->d_partial, s->d_scal);
 		};
 		for (int i = 0; i < 3; i++) {
 			launch();

@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libbfmhip.so")
 
-PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_AMG = 0, 1, 2
+PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_AMG, PRECOND_LEARNED = 0, 1, 2, 3
 FP64, MIXED = 0, 1
 KERNEL_CG_SPMV64, KERNEL_CG_SPMV32, KERNEL_FCG_SPMV, KERNEL_AMG_SMOOTH = 0, 1, 2, 3
 DIRICHLET_X, DIRICHLET_Y, NEUMANN_X, NEUMANN_Y = 0, 1, 2, 3
@@ -27,7 +27,10 @@ EXPORTS = [
     "bfmx_system_set_edges", "bfmx_system_assemble", "bfmx_system_apply_condition",
     "bfmx_system_solve", "bfmx_system_info", "bfmx_system_node_perm", "bfmx_system_get_bsr",
     "bfmx_system_get_rhs", "bfmx_system_get_u", "bfmx_system_get_entry", "bfmx_system_time_spmv",
-    "bfmx_system_time_kernel",
+    "bfmx_system_time_kernel", "bfmx_model_n_params", "bfmx_model_create", "bfmx_model_destroy",
+    "bfmx_model_get_params", "bfmx_model_set_params", "bfmx_trainset_create", "bfmx_trainset_destroy",
+    "bfmx_trainset_add", "bfmx_trainset_info", "bfmx_trainset_get", "bfmx_model_loss_grad", "bfmx_model_train",
+    "bfmx_system_set_model", "bfmx_system_apply_model",
 ]
 
 
@@ -35,7 +38,7 @@ class Stats(C.Structure):
     _fields_ = [("status", C.c_int32), ("precond", C.c_int32), ("precision", C.c_int32),
                 ("restarts", C.c_int32), ("iterations", C.c_int64), ("n_dof", C.c_int64),
                 ("rtol", C.c_double), ("rel_residual", C.c_double), ("solve_ms", C.c_double),
-                ("b_norm", C.c_double)]
+                ("b_norm", C.c_double), ("fallback", C.c_int32), ("fallback_iterations", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -90,6 +93,21 @@ def lib():
         L.bfmx_system_get_entry.argtypes = [vp, sz, sz, C.POINTER(C.c_double)]
         L.bfmx_system_time_spmv.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.bfmx_system_time_kernel.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.bfmx_model_n_params.argtypes = [C.POINTER(sz)]
+        L.bfmx_model_create.argtypes = [C.POINTER(vp), C.c_uint64]
+        L.bfmx_model_destroy.argtypes = [vp]
+        L.bfmx_model_get_params.argtypes = [vp, f32p]
+        L.bfmx_model_set_params.argtypes = [vp, f32p]
+        L.bfmx_trainset_create.argtypes = [C.POINTER(vp)]
+        L.bfmx_trainset_destroy.argtypes = [vp]
+        L.bfmx_trainset_add.argtypes = [vp, vp, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)]
+        L.bfmx_trainset_info.argtypes = [vp, C.c_int, C.POINTER(sz), C.POINTER(C.c_int)]
+        L.bfmx_trainset_get.argtypes = [vp, C.c_int, _Opt(f32p), _Opt(f32p), _Opt(f32p), _Opt(f32p)]
+        L.bfmx_model_loss_grad.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), _Opt(f32p)]
+        L.bfmx_model_train.argtypes = [vp, vp, C.c_int, C.c_int, C.c_float, C.c_uint64, _Opt(_dp)]
+        L.bfmx_system_set_model.argtypes = [vp, vp]
+        L.bfmx_system_apply_model.argtypes = [vp, _dp, _dp]
         for name in EXPORTS[1:]:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -233,11 +251,121 @@ class System:
         _check(lib().bfmx_system_time_spmv(self.h, precision, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
 
+    def set_model(self, model):
+        _check(lib().bfmx_system_set_model(self.h, None if model is None else model.h))
+        self._model = model   # keep it alive while attached
+
+    def apply_model(self, r):
+        """The learned preconditioner applied once, internal node order (2 per node)."""
+        r = np.ascontiguousarray(r, np.float64).reshape(-1)
+        z = np.empty_like(r)
+        _check(lib().bfmx_system_apply_model(self.h, r, z))
+        return z
+
     def time_kernel(self, kernel, reps=50):
         """(mean ms per launch, algorithmic bytes per launch) of one solver kernel."""
         ms, by = C.c_double(), C.c_double()
         _check(lib().bfmx_system_time_kernel(self.h, kernel, reps, C.byref(ms), C.byref(by)))
         return ms.value, by.value
+
+
+class Model:
+    """The learned preconditioner's parameters on the GPU (include/bfmx.h)."""
+
+    def __init__(self, seed=0, params=None):
+        self.h = C.c_void_p()
+        _check(lib().bfmx_model_create(C.byref(self.h), int(seed)))
+        if params is not None:
+            self.set_params(params)
+
+    @staticmethod
+    def n_params():
+        n = C.c_size_t()
+        _check(lib().bfmx_model_n_params(C.byref(n)))
+        return n.value
+
+    def get_params(self):
+        p = np.empty(self.n_params(), np.float32)
+        _check(lib().bfmx_model_get_params(self.h, p))
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params()
+        _check(lib().bfmx_model_set_params(self.h, p))
+
+    def save(self, path):
+        self.get_params().tofile(path)
+
+    @classmethod
+    def load(cls, path):
+        return cls(params=np.fromfile(path, dtype=np.float32))
+
+    def loss_grad(self, ts, entry, p0, batch):
+        loss = C.c_double()
+        g = np.empty(self.n_params(), np.float32)
+        _check(lib().bfmx_model_loss_grad(self.h, ts.h, entry, p0, batch, C.byref(loss), g))
+        return loss.value, g
+
+    def train(self, ts, steps, batch=8, lr=1e-3, seed=0):
+        losses = np.zeros(steps)
+        _check(lib().bfmx_model_train(self.h, ts.h, steps, batch, lr, seed, losses))
+        return losses
+
+    def close(self):
+        if self.h:
+            lib().bfmx_model_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class TrainSet:
+    """(residual, error) pairs recorded from plain PCG runs (include/bfmx.h)."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        _check(lib().bfmx_trainset_create(C.byref(self.h)))
+        self.systems = []   # the systems must outlive the set
+
+    def add(self, system, max_pairs=64, every=8, rtol=1e-10):
+        n = C.c_int()
+        _check(lib().bfmx_trainset_add(self.h, system.h, max_pairs, every, rtol, C.byref(n)))
+        self.systems.append(system)
+        return n.value
+
+    def __len__(self):
+        return len(self.systems)
+
+    def info(self, entry):
+        n, p = C.c_size_t(), C.c_int()
+        _check(lib().bfmx_trainset_info(self.h, entry, C.byref(n), C.byref(p)))
+        return n.value, p.value
+
+    def get(self, entry):
+        """(feat (n, 8), r (P, n, 2), e (P, n, 2), s (P,), den (P,)) in the internal node order."""
+        n, P = self.info(entry)
+        feat = np.empty(8 * n, np.float32)
+        r = np.empty(2 * n * P, np.float32)
+        e = np.empty(2 * n * P, np.float32)
+        sd = np.empty(2 * P, np.float32)
+        _check(lib().bfmx_trainset_get(self.h, entry, feat, r, e, sd))
+        return feat.reshape(n, 8), r.reshape(P, n, 2), e.reshape(P, n, 2), sd[:P], sd[P:]
+
+    def close(self):
+        if self.h:
+            lib().bfmx_trainset_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def beam_system(mesh, stress=True, material=None, traction=None, gravity=None, renumber=True):

@@ -1,0 +1,131 @@
+"""Training and evaluation of the learned preconditioner (GPU).
+
+    python -m bfm_amd.learned train [--pairs 20000] [--steps 4000] [--out PATH]
+    python -m bfm_amd.learned eval  [--weights PATH] [--workload q1_2m_het] [--seed 7]
+
+train: builds seeded planar problems from the beam distributions (20%
+jittered Q1 beams, homogeneous and with per-element E log-uniform in
+[1e9, 1e11], plane stress, sizes from 60x15 to 300x75 cells), records
+(residual, error) pairs from plain Jacobi-PCG runs on each
+(bfmx_trainset_add), and trains the network with Adam on random batches of
+one problem's pairs (bfmx_model_train; forward and backward run on the GPU).
+Writes the FP32 parameters (raw little-endian) and a JSON log beside them.
+
+eval: solves a held-out problem (default: the 2M-DOF heterogeneous beam with
+a seed not used in training) with the learned flexible CG, block-Jacobi PCG
+(the method without the network) and Jacobi PCG; prints iterations and times.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+WEIGHTS = os.path.join(HERE, "weights", "mpnn_elasticity.f32")
+
+# training problems: (cells x, cells y, heterogeneous E), seeds 100 + k
+TRAIN_SHAPES = [(60, 15), (100, 25), (160, 40), (200, 50), (300, 75)]
+
+
+def training_problems(n_problems, seed0=100):
+    from . import meshgen
+    for k in range(n_problems):
+        nx, ny = TRAIN_SHAPES[k % len(TRAIN_SHAPES)]
+        hetero = (k // len(TRAIN_SHAPES)) % 2 == 1
+        seed = seed0 + k
+        mesh = meshgen.beam(nx, ny, kind=4, jitter=0.2, seed=seed)
+        if hetero:
+            mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=seed)
+        yield f"beam{nx}x{ny}{'_het' if hetero else ''}_s{seed}", mesh
+
+
+def train(args):
+    from . import device as D
+    t0 = time.perf_counter()
+    ts = D.TrainSet()
+    log = {"problems": [], "pairs": 0}
+    per = max(1, args.pairs // args.problems)
+    systems = []
+    for name, mesh in training_problems(args.problems):
+        s = D.beam_system(mesh)
+        # Jacobi-PCG takes about 6 iterations per cell along a 4:1 beam
+        # (about 12 sqrt(nodes)): spread the recorded iterates over the run
+        every = max(1, int(12 * np.sqrt(mesh.n_nodes) / per))
+        n = ts.add(s, max_pairs=per, every=every, rtol=1e-9)
+        systems.append(s)
+        log["problems"].append({"name": name, "dofs": mesh.n_dof, "pairs": n, "every": every})
+        log["pairs"] += n
+        print(f"{name}: {mesh.n_dof} DOFs, {n} pairs (every {every})", flush=True)
+    t1 = time.perf_counter()
+    m = D.Model(seed=args.seed)
+    losses = []
+    chunk = 250
+    for k in range(0, args.steps, chunk):
+        lr = args.lr * (0.1 if k >= 0.75 * args.steps else 1.0)
+        ls = m.train(ts, min(chunk, args.steps - k), batch=args.batch, lr=lr, seed=args.seed + k)
+        losses.extend(ls.tolist())
+        print(f"step {k + len(ls)}: loss {np.mean(ls[-50:]):.4f} (lr {lr:g})", flush=True)
+    t2 = time.perf_counter()
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    m.save(args.out)
+    log.update({"steps": args.steps, "batch": args.batch, "lr": args.lr, "seed": args.seed,
+                "record_s": t1 - t0, "train_s": t2 - t1, "loss_first": float(np.mean(losses[:50])),
+                "loss_last": float(np.mean(losses[-50:]))})
+    with open(os.path.splitext(args.out)[0] + ".json", "w") as f:
+        json.dump(log, f, indent=1)
+    print(json.dumps({k: v for k, v in log.items() if k != "problems"}), flush=True)
+
+
+def evaluate(args):
+    from . import device as D
+    from . import meshgen
+    mesh, stress = meshgen.workload(args.workload, seed=args.seed)
+    s = D.beam_system(mesh, stress=stress)
+    m = D.Model.load(args.weights)
+    s.set_model(m)
+    out = {"workload": args.workload, "seed": args.seed, "dofs": mesh.n_dof}
+    for name, pre in (("learned_fcg", D.PRECOND_LEARNED), ("block_jacobi_pcg", D.PRECOND_BLOCK_JACOBI),
+                      ("jacobi_pcg", D.PRECOND_JACOBI), ("amg_fcg", D.PRECOND_AMG)):
+        try:
+            s.solve(precond=pre, rtol=args.rtol, want_u=False)   # warm-up (graph capture, set-up)
+            t = time.perf_counter()
+            _, st = s.solve(precond=pre, rtol=args.rtol, want_u=False)
+            dt = time.perf_counter() - t
+            out[name] = {"ms": 1e3 * dt, "iterations": int(st["iterations"]), "rel_residual": st["rel_residual"],
+                         "fallback": int(st["fallback"])}
+        except D.BfmxError as e:
+            out[name] = {"error": str(e)}
+        print(name, out[name], flush=True)
+    print(json.dumps(out))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--pairs", type=int, default=20000)
+    t.add_argument("--problems", type=int, default=40)
+    t.add_argument("--steps", type=int, default=4000)
+    t.add_argument("--batch", type=int, default=8)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--out", default=WEIGHTS)
+    e = sub.add_parser("eval")
+    e.add_argument("--weights", default=WEIGHTS)
+    e.add_argument("--workload", default="q1_2m_het")
+    e.add_argument("--seed", type=int, default=7)
+    e.add_argument("--rtol", type=float, default=1e-8)
+    e.add_argument("--het", type=int, default=1, help="per-element E log-uniform (1) or uniform (0)")
+    args = ap.parse_args()
+    if args.cmd == "train":
+        train(args)
+    else:
+        evaluate(args)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -60,6 +60,11 @@ enum {
 	BFMX_PRECOND_AMG = 2,          /* flexible CG (FP64) preconditioned by one FP32
 	                                  V-cycle of aggregation multigrid built on the
 	                                  rigid-body modes; precision is ignored */
+	BFMX_PRECOND_LEARNED = 3,      /* flexible CG (FP64) preconditioned by the
+	                                  message-passing network attached with
+	                                  bfmx_system_set_model (BF16/FP32); falls back
+	                                  to block-Jacobi PCG from the current iterate
+	                                  on breakdown, the iteration limit or a stall */
 };
 
 /* Arithmetic of the solve. */
@@ -94,6 +99,8 @@ typedef struct {
 	double rel_residual;    /* ||b - A u||_2 / ||b||_2, recomputed in FP64 at the end */
 	double solve_ms;        /* device time from the first solver kernel to convergence */
 	double b_norm;
+	int32_t fallback;       /* BFMX_PRECOND_LEARNED: 1 when block-Jacobi PCG finished the solve */
+	int32_t fallback_iterations;
 } bfmx_stats_t;
 
 const char* bfmx_last_error(void);
@@ -183,6 +190,52 @@ enum {
  * one launch; bytes (may be NULL) the kernel's algorithmic bytes per launch.
  * No reference counterpart: the benchmark's roofline figure. */
 int bfmx_system_time_kernel(bfmx_system_t* s, int kernel, int reps, double* avg_ms, double* bytes);
+
+/* ---- learned preconditioner (no reference counterpart) ----------------
+ * A 3-layer message-passing network over the node graph (hidden width 64,
+ * BF16 layer weights and activations, FP32 accumulation) used as the
+ * preconditioner of flexible CG, trained on the GPU on (residual, error)
+ * pairs recorded from plain PCG runs. */
+typedef struct bfmx_model bfmx_model_t;
+typedef struct bfmx_trainset bfmx_trainset_t;
+
+/* Number of FP32 parameters of a model. */
+int bfmx_model_n_params(size_t* n);
+/* New model on the current device (seeded): Glorot-uniform layers and a
+ * small encoder, so the untrained preconditioner is block Jacobi plus a small
+ * symmetric positive semi-definite term (exactly block Jacobi with a zero
+ * encoder). */
+int bfmx_model_create(bfmx_model_t** out, uint64_t seed);
+int bfmx_model_destroy(bfmx_model_t* m);
+int bfmx_model_get_params(bfmx_model_t* m, float* params);
+int bfmx_model_set_params(bfmx_model_t* m, const float* params);
+
+int bfmx_trainset_create(bfmx_trainset_t** out);
+int bfmx_trainset_destroy(bfmx_trainset_t* ts);
+/* Solves the system's current problem with Jacobi PCG to rtol (x*), then
+ * reruns it recording up to max_pairs pairs (r_k, x* - x_k), one every
+ * `every` iterations from k = 0. The matrix values and node features are
+ * snapshotted; the system must outlive the set (its graph is shared). */
+int bfmx_trainset_add(bfmx_trainset_t* ts, bfmx_system_t* s, int max_pairs, int every, double rtol,
+                      int* n_added);
+int bfmx_trainset_info(bfmx_trainset_t* ts, int entry, size_t* n_nodes, int* n_pairs);
+/* An entry's data in the internal node order: feat (8 per node), r and e
+ * (n_pairs x 2 n_nodes), s_den (n_pairs scales s, then n_pairs e^T A e). */
+int bfmx_trainset_get(bfmx_trainset_t* ts, int entry, float* feat, float* r, float* e, float* s_den);
+
+/* Loss (mean over pairs p0 .. p0 + batch - 1 of ||z - e||_A^2 / ||e||_A^2)
+ * and its gradient with respect to the parameters (FP32), without an update. */
+int bfmx_model_loss_grad(bfmx_model_t* m, bfmx_trainset_t* ts, int entry, int p0, int batch, double* loss,
+                         float* grad);
+/* steps Adam steps (lr; beta 0.9, 0.999) on random batches of one entry's
+ * pairs; losses (may be NULL) receives each step's loss. */
+int bfmx_model_train(bfmx_model_t* m, bfmx_trainset_t* ts, int steps, int batch, float lr, uint64_t seed,
+                     double* losses);
+
+/* Attaches the model used by BFMX_PRECOND_LEARNED (borrowed; NULL detaches). */
+int bfmx_system_set_model(bfmx_system_t* s, bfmx_model_t* m);
+/* z = preconditioner(r) once, both in the internal node order (2 per node). */
+int bfmx_system_apply_model(bfmx_system_t* s, const double* r, double* z);
 
 #ifdef __cplusplus
 }
