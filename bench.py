@@ -99,6 +99,7 @@ def main():
     ap.add_argument("--rtol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-pcg", action="store_true", help="skip the plain Jacobi-PCG leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--mixed-cg", action="store_true",
                     help="also run CG on the FP32 matrix with FP64 reliable updates")
     args = ap.parse_args()
@@ -161,6 +162,24 @@ def main():
         }
         extra["speedup_vs_pcg"] = p_ms / ms
         log(f"speedup of multigrid FCG over Jacobi-PCG: {p_ms / ms:.2f}x")
+    weights = os.path.join(ROOT, "bfm_amd", "weights", "mpnn_elasticity.f32")
+    if not args.no_learned and os.path.exists(weights):
+        # the learned preconditioner beside the method without it (block-Jacobi PCG)
+        s.set_model(D.Model.load(weights))
+        w_save, args.warmup = args.warmup, 0
+        b_ms, b_st = leg("pcg_block_jacobi", D.PRECOND_BLOCK_JACOBI, steps=1)
+        l_ms, l_st = leg("learned_fcg", D.PRECOND_LEARNED, steps=1)
+        args.warmup = w_save
+        extra["learned_fcg"] = {
+            "ms_per_step": l_ms, "value": mesh.n_dof / (l_ms / 1e3), "iterations": int(l_st["iterations"]),
+            "rel_residual": l_st["rel_residual"], "fallback": int(l_st["fallback"]),
+            "block_jacobi_pcg": {"ms_per_step": b_ms, "iterations": int(b_st["iterations"])},
+            "iteration_ratio_vs_block_jacobi": b_st["iterations"] / max(1, l_st["iterations"]),
+        }
+        if "pcg_jacobi_fp64" in extra:
+            extra["learned_fcg"]["speedup_vs_pcg"] = extra["pcg_jacobi_fp64"]["ms_per_step"] / l_ms
+        log(f"learned FCG: {l_ms:.1f} ms, {l_st['iterations']} iterations (block-Jacobi PCG "
+            f"{b_st['iterations']}, {b_ms:.1f} ms)")
     if args.mixed_cg:
         try:
             m_ms, m_st = leg("cg_fp32_matrix", D.PRECOND_JACOBI, D.MIXED, steps=1)
