@@ -19,6 +19,10 @@ void set_error(const char* fmt, ...) {
 	va_end(ap);
 }
 
+const char* last_error_text() {
+	return g_err;
+}
+
 namespace {
 
 int check(bfmx_system* s) {
@@ -30,6 +34,17 @@ int check(bfmx_system* s) {
 	if (e != hipSuccess) {
 		set_error("hipSetDevice(%d): %s", s->device, hipGetErrorString(e));
 		return BFMX_E_HIP;
+	}
+	return BFMX_OK;
+}
+
+// Calls that rebuild a system's topology, matrix or conditions (or free it)
+// are refused on the systems a batch built from meshes owns.
+int check_owned(bfmx_system* s) {
+	BFMX_CHECK(check(s));
+	if (s->borrowed) {
+		set_error("the system belongs to a batch built from meshes: it cannot be changed or destroyed");
+		return BFMX_E_ARG;
 	}
 	return BFMX_OK;
 }
@@ -161,6 +176,10 @@ int bfmx_system_destroy(bfmx_system_t* s) {
 	if (!s) {
 		return BFMX_OK;
 	}
+	if (s->borrowed) {
+		set_error("the system belongs to a batch built from meshes: destroy the batch");
+		return BFMX_E_ARG;
+	}
 	(void) hipSetDevice(s->device);
 	if (s->stream) {
 		(void) hipStreamSynchronize(s->stream);
@@ -171,7 +190,7 @@ int bfmx_system_destroy(bfmx_system_t* s) {
 }
 
 int bfmx_system_set_edges(bfmx_system_t* s, size_t n_edges, const size_t* edge_nodes, const uint8_t* edge_boundary) {
-	BFMX_CHECK(check(s));
+	BFMX_CHECK(check_owned(s));
 	if (n_edges && !edge_nodes) {
 		set_error("null edge list");
 		return BFMX_E_ARG;
@@ -181,7 +200,7 @@ int bfmx_system_set_edges(bfmx_system_t* s, size_t n_edges, const size_t* edge_n
 
 int bfmx_system_assemble(bfmx_system_t* s, int stress, double E, double nu, double rho, const double* elem_E,
                          const double* node_force, double gx, double gy) {
-	BFMX_CHECK(check(s));
+	BFMX_CHECK(check_owned(s));
 	return guarded([&]() -> int {
 		BFMX_CHECK(assemble(s, stress, E, nu, rho, elem_E, node_force, gx, gy));
 		BFMX_HIP(hipStreamSynchronize(s->stream));
@@ -190,7 +209,7 @@ int bfmx_system_assemble(bfmx_system_t* s, int stress, double E, double nu, doub
 }
 
 int bfmx_system_apply_condition(bfmx_system_t* s, int kind, double value, const uint8_t* node_flags) {
-	BFMX_CHECK(check(s));
+	BFMX_CHECK(check_owned(s));
 	if (!node_flags) {
 		set_error("null node flags");
 		return BFMX_E_ARG;
@@ -549,6 +568,70 @@ int bfmx_system_apply_model(bfmx_system_t* s, const double* r, double* z) {
 		BFMX_HIP(hipStreamSynchronize(st));
 		return BFMX_OK;
 	});
+}
+
+// ---- batched solves
+
+int bfmx_batch_create(bfmx_batch_t** out, size_t n, bfmx_system_t* const* systems) {
+	if (!out || !systems || n == 0) {
+		set_error("bad arguments to batch_create");
+		return BFMX_E_ARG;
+	}
+	BFMX_CHECK(check(systems[0]));
+	return guarded([&] { return batch_create(out, n, systems); });
+}
+
+int bfmx_batch_create_planar(bfmx_batch_t** out, size_t n, const bfmx_instance_t* inst, int renumber) {
+	if (!out || !inst || n == 0 || n > (size_t(1) << 30)) {
+		set_error("bad arguments to batch_create_planar");
+		return BFMX_E_ARG;
+	}
+	*out = nullptr;
+	int ndev = 0;
+	if (bfmx_device_count(&ndev) != BFMX_OK || ndev == 0) {
+		set_error("no GPU visible to HIP");
+		return BFMX_E_NODEV;
+	}
+	return guarded([&] { return batch_create_planar(out, n, inst, renumber); });
+}
+
+int bfmx_batch_system(bfmx_batch_t* b, size_t index, bfmx_system_t** s) {
+	if (!b || !s) {
+		set_error("null argument");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return batch_system(b, index, s); });
+}
+
+int bfmx_batch_destroy(bfmx_batch_t* b) {
+	batch_destroy(b);
+	return BFMX_OK;
+}
+
+int bfmx_batch_autotune(bfmx_batch_t* b, int precond, int sample, int iters, int32_t* picks, double* times_ms) {
+	if (!b || !picks || sample < 1 || iters < 1) {
+		set_error("bad arguments to batch_autotune");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return batch_autotune(b, precond, sample, iters, picks, times_ms); });
+}
+
+int bfmx_batch_solve(bfmx_batch_t* b, int precond, double rtol, size_t max_iter, const int32_t* picks,
+                     bfmx_stats_t* stats, double* total_ms) {
+	if (!b || !(rtol > 0)) {
+		set_error("bad arguments to batch_solve");
+		return BFMX_E_ARG;
+	}
+	return guarded(
+	    [&] { return batch_solve(b, precond, rtol, static_cast<int64_t>(max_iter), picks, stats, total_ms); });
+}
+
+int bfmx_batch_get_u(bfmx_batch_t* b, size_t index, double* u) {
+	if (!b || !u) {
+		set_error("null argument");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return batch_get_u(b, index, u); });
 }
 
 }  // extern "C"

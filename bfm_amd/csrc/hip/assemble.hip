@@ -18,74 +18,17 @@
 //   Neumann: each node adds its incident flagged edges' loads in edge order,
 //     the order in which the reference adds them, so b matches it bit for bit
 //     given the same body load.
+//
+// The per-element, per-node and per-row bodies are the device functions of
+// assemble_dev.h, which the batched persistent kernel shares.
 
 #include <cmath>
 
-#include "internal.h"
+#include "assemble_dev.h"
 
 namespace bfmx {
 
 namespace {
-
-template <int KIND>
-struct Rule;
-
-// 3-point triangle rule (libbfm/src/rule.c:94-102).
-template <>
-struct Rule<3> {
-	static constexpr int NG = 3;
-	__device__ static constexpr double xi(int g) { return g == 1 ? 1.0 - 1.0 / 3 : 1.0 / 6; }
-	__device__ static constexpr double eta(int g) { return g == 2 ? 1.0 - 1.0 / 3 : 1.0 / 6; }
-	__device__ static constexpr double w(int) { return 1.0 / 6; }
-	__device__ static void shape(double xs, double et, double* phi, double* dxs, double* det) {
-		phi[0] = 1 - xs - et;
-		phi[1] = xs;
-		phi[2] = et;
-		dxs[0] = -1, dxs[1] = 1, dxs[2] = 0;
-		det[0] = -1, det[1] = 0, det[2] = 1;
-	}
-};
-
-// 2x2 Gauss-Legendre rule with libbfm's abscissa literal (rule.c:92, 104-111).
-template <>
-struct Rule<4> {
-	static constexpr int NG = 4;
-	static constexpr double kA = 0.577350269189626;
-	__device__ static constexpr double xi(int g) { return g < 2 ? -kA : kA; }
-	__device__ static constexpr double eta(int g) { return (g == 0 || g == 3) ? kA : -kA; }
-	__device__ static constexpr double w(int) { return 1.0; }
-	__device__ static void shape(double xs, double et, double* phi, double* dxs, double* det) {
-		phi[0] = (1 + xs) * (1 + et) / 4;
-		phi[1] = (1 - xs) * (1 + et) / 4;
-		phi[2] = (1 - xs) * (1 - et) / 4;
-		phi[3] = (1 + xs) * (1 - et) / 4;
-		dxs[0] = (1 + et) / 4;
-		dxs[1] = (-1 - et) / 4;
-		dxs[2] = (-1 + et) / 4;
-		dxs[3] = (1 - et) / 4;
-		det[0] = (1 + xs) / 4;
-		det[1] = (1 - xs) / 4;
-		det[2] = (-1 + xs) / 4;
-		det[3] = (-1 - xs) / 4;
-	}
-};
-
-__device__ __forceinline__ void add_block(double* __restrict__ val, int blk, int r, double a00, double a01,
-                                          double a10, double a11) {
-	if (blk < 0) {
-		return;   // pair missing from the pattern (cannot happen for a pattern built from these elements)
-	}
-	double2* top = reinterpret_cast<double2*>(val + 4 * static_cast<int64_t>(blk) - 2 * r);
-	double2* bot = top + kSlice;
-	double2 t = *top;
-	double2 u = *bot;
-	t.x += a00;
-	t.y += a01;
-	u.x += a10;
-	u.y += a11;
-	*top = t;
-	*bot = u;
-}
 
 // slots[pos * K*K + a*K + b] = block index of (node a, node b) of element pos.
 template <int KIND>
@@ -100,136 +43,33 @@ __global__ __launch_bounds__(kBlock) void k_slots(int64_t ne, const int32_t* __r
 	int64_t const e = t / KIND;
 	int const a = static_cast<int>(t % KIND);
 	int32_t const row = el[e * KIND + a];
-	int32_t const base = slice_ptr[row >> 6] + (row & 63);
-	int32_t const len = row_len[row];
 	for (int b = 0; b < KIND; b++) {
-		int32_t const target = el[e * KIND + b];
-		int32_t found = -1;
-		for (int k = 0; k < len; k++) {
-			if (col[base + k * kSlice] == target) {
-				found = base + k * kSlice;
-				break;
-			}
-		}
-		slots[e * KIND * KIND + a * KIND + b] = found;
+		slots[e * KIND * KIND + a * KIND + b] = dev::find_slot(slice_ptr, row_len, col, row, el[e * KIND + b]);
 	}
 }
 
 template <int KIND>
 __global__ __launch_bounds__(kBlock) void k_assemble(int64_t e0, int64_t e1, const int32_t* __restrict__ el,
                                                      const int32_t* __restrict__ slots,
-                                                     const double* __restrict__ xy,
-                                                     const double* __restrict__ elemE, double E0, double nu,
-                                                     int stress, double rho, const double* __restrict__ nforce,
-                                                     double gfx, double gfy, double* __restrict__ val,
-                                                     double* __restrict__ b) {
-	using R = Rule<KIND>;
+                                                     const double* __restrict__ xy, dev::Material M,
+                                                     double* __restrict__ val, double* __restrict__ b) {
 	int64_t const e = e0 + static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
 	if (e >= e1) {
 		return;
 	}
-	int n[KIND];
-	double x[KIND], y[KIND];
-#pragma unroll
-	for (int j = 0; j < KIND; j++) {
-		n[j] = el[e * KIND + j];
-		double2 const p = reinterpret_cast<const double2*>(xy)[n[j]];
-		x[j] = p.x;
-		y[j] = p.y;
-	}
-	double const E = elemE ? elemE[e] : E0;
-	// D-matrix constants (system.c:454-458)
-	double const a = !stress ? E * (1 - nu) / (1 + nu) / (1 - 2 * nu) : E / (1 - nu * nu);
-	double const bb = !stress ? E * nu / (1 + nu) / (1 - 2 * nu) : E * nu / (1 - nu * nu);
-	double const c = E / (2 * (1 + nu));
-
-	double gxs[R::NG][KIND], gys[R::NG][KIND], phis[R::NG][KIND], djw[R::NG];
-#pragma unroll
-	for (int g = 0; g < R::NG; g++) {
-		double phi[KIND], dxs[KIND], det[KIND];
-		R::shape(R::xi(g), R::eta(g), phi, dxs, det);
-		double dx_dxsi = 0, dx_deta = 0, dy_dxsi = 0, dy_deta = 0;
-#pragma unroll
-		for (int j = 0; j < KIND; j++) {
-			dx_dxsi += x[j] * dxs[j];
-			dx_deta += x[j] * det[j];
-			dy_dxsi += y[j] * dxs[j];
-			dy_deta += y[j] * det[j];
-		}
-		double const det_J = fabs(dx_dxsi * dy_deta - dx_deta * dy_dxsi);
-#pragma unroll
-		for (int j = 0; j < KIND; j++) {
-			gxs[g][j] = (dxs[j] * dy_deta - det[j] * dy_dxsi) / det_J;
-			gys[g][j] = (det[j] * dx_dxsi - dxs[j] * dx_deta) / det_J;
-			phis[g][j] = phi[j];
-		}
-		djw[g] = det_J * R::w(g);
-	}
-
-	// body load: rho f(x_j) phi_j, f evaluated at the node (system.c:190-203)
-#pragma unroll
-	for (int j = 0; j < KIND; j++) {
-		double fx = gfx, fy = gfy;
-		if (nforce) {
-			fx = nforce[2 * n[j]];
-			fy = nforce[2 * n[j] + 1];
-		}
-		double bx = 0, by = 0;
-#pragma unroll
-		for (int g = 0; g < R::NG; g++) {
-			bx += djw[g] * fx * rho * phis[g][j];
-			by += djw[g] * fy * rho * phis[g][j];
-		}
-		double2* bp = reinterpret_cast<double2*>(b) + n[j];
-		double2 v = *bp;
-		v.x += bx;
-		v.y += by;
-		*bp = v;
-	}
-
-	// stiffness blocks (system.c:207-223); upper triangle, mirrored
 	const int32_t* sl = slots + e * KIND * KIND;
-#pragma unroll
-	for (int j = 0; j < KIND; j++) {
-#pragma unroll
-		for (int k = j; k < KIND; k++) {
-			double s11 = 0, s12 = 0, s21 = 0, s22 = 0;
-#pragma unroll
-			for (int g = 0; g < R::NG; g++) {
-				double const xj = gxs[g][j], yj = gys[g][j], xk = gxs[g][k], yk = gys[g][k];
-				s11 += djw[g] * (a * xj * xk + c * yj * yk);
-				s12 += djw[g] * (bb * xj * yk + c * yj * xk);
-				s21 += djw[g] * (bb * yj * xk + c * xj * yk);
-				s22 += djw[g] * (a * yj * yk + c * xj * xk);
-			}
-			if (j == k) {
-				s21 = s12;
-			}
-			add_block(val, sl[j * KIND + k], n[j] & 63, s11, s12, s21, s22);
-			if (k != j) {
-				add_block(val, sl[k * KIND + j], n[k] & 63, s11, s21, s12, s22);
-			}
-		}
-	}
+	dev::assemble_element<KIND>(e, el, xy, M, [&](int j, int k, const int*) { return sl[j * KIND + k]; }, val, b);
 }
 
 __global__ __launch_bounds__(kBlock) void k_dirichlet_xy(int cnt, const int32_t* __restrict__ nodes, int shift,
                                                          double v, uint8_t* __restrict__ fixed,
                                                          double* __restrict__ vfirst, double* __restrict__ b) {
 	int const t = blockIdx.x * kBlock + threadIdx.x;
-	if (t >= cnt) {
-		return;
+	if (t < cnt) {
+		dev::dirichlet_xy(nodes[t], shift, v, fixed, vfirst, b);
 	}
-	int64_t const d = 2 * static_cast<int64_t>(nodes[t]) + shift;
-	if (!fixed[d]) {
-		fixed[d] = 1;
-		vfirst[d] = v;
-	}
-	b[d] = v;
 }
 
-// Dirichlet normal/tangent (system.c:388-425): tangent = half the sum of unit
-// vectors from each adjacent boundary-edge neighbour to the node.
 __global__ __launch_bounds__(kBlock) void k_dirichlet_nt(int cnt, const int32_t* __restrict__ nodes, int tangent,
                                                          double v, const int64_t* __restrict__ inc_ptr,
                                                          const int32_t* __restrict__ inc,
@@ -238,38 +78,11 @@ __global__ __launch_bounds__(kBlock) void k_dirichlet_nt(int cnt, const int32_t*
                                                          const double* __restrict__ xy, uint8_t* __restrict__ fixed,
                                                          double* __restrict__ vfirst, double* __restrict__ b) {
 	int const t = blockIdx.x * kBlock + threadIdx.x;
-	if (t >= cnt) {
-		return;
-	}
-	int32_t const i = nodes[t];
-	double const xi = xy[2 * i], yi = xy[2 * i + 1];
-	double tx = 0, ty = 0;
-	for (int64_t q = inc_ptr[i]; q < inc_ptr[i + 1]; q++) {
-		int32_t const e = inc[q];
-		if (q > inc_ptr[i] && inc[q - 1] == e) {
-			continue;   // degenerate edge listed twice; the reference visits it once
-		}
-		if (!edge_bnd[e]) {
-			continue;
-		}
-		int32_t const n2 = edge_nodes[2 * e] == i ? edge_nodes[2 * e + 1] : edge_nodes[2 * e];
-		double const dx = xi - xy[2 * n2], dy = yi - xy[2 * n2 + 1];
-		double const length = sqrt(dx * dx + dy * dy);
-		tx += dx / length / 2;
-		ty += dy / length / 2;
-	}
-	double const val[2] = {v * (tangent ? tx : -ty), v * (tangent ? ty : tx)};
-	for (int k = 0; k < 2; k++) {
-		int64_t const d = 2 * static_cast<int64_t>(i) + k;
-		if (!fixed[d]) {
-			fixed[d] = 1;
-			vfirst[d] = val[k];
-		}
-		b[d] = val[k];
+	if (t < cnt) {
+		dev::dirichlet_nt(nodes[t], tangent, v, inc_ptr, inc, edge_nodes, edge_bnd, xy, fixed, vfirst, b);
 	}
 }
 
-// Neumann loads of one condition (system.c:477-522), per flagged node.
 __global__ __launch_bounds__(kBlock) void k_neumann(int cnt, const int32_t* __restrict__ nodes, int kind, double v,
                                                     const uint8_t* __restrict__ flags,
                                                     const int64_t* __restrict__ inc_ptr,
@@ -277,38 +90,11 @@ __global__ __launch_bounds__(kBlock) void k_neumann(int cnt, const int32_t* __re
                                                     const int32_t* __restrict__ edge_nodes,
                                                     const double* __restrict__ xy, double* __restrict__ b) {
 	int const t = blockIdx.x * kBlock + threadIdx.x;
-	if (t >= cnt) {
-		return;
+	if (t < cnt) {
+		dev::neumann(nodes[t], kind, v, flags, inc_ptr, inc, edge_nodes, xy, b);
 	}
-	int32_t const i = nodes[t];
-	double bx = b[2 * i], by = b[2 * i + 1];
-	for (int64_t q = inc_ptr[i]; q < inc_ptr[i + 1]; q++) {
-		int32_t const e = inc[q];
-		int32_t const n1 = edge_nodes[2 * e], n2 = edge_nodes[2 * e + 1];
-		if (!flags[n1] || !flags[n2]) {
-			continue;
-		}
-		double const dx = xy[2 * n1] - xy[2 * n2];
-		double const dy = xy[2 * n1 + 1] - xy[2 * n2 + 1];
-		if (kind == BFMX_NEUMANN_X || kind == BFMX_NEUMANN_Y) {
-			double const jac = sqrt(dx * dx + dy * dy) / 2;
-			if (kind == BFMX_NEUMANN_X) {
-				bx += jac * v;
-			} else {
-				by += jac * v;
-			}
-		} else {
-			bool const tan = kind == BFMX_NEUMANN_TANGENT;
-			bx += 0.5 * v * (tan ? dx : -dy);
-			by += 0.5 * v * (tan ? dy : dx);
-		}
-	}
-	b[2 * i] = bx;
-	b[2 * i + 1] = by;
 }
 
-// Symmetric elimination of all constrained DOFs on the rows that hold them or
-// reference them.
 __global__ __launch_bounds__(kBlock) void k_eliminate(int cnt, const int32_t* __restrict__ rows,
                                                       const int32_t* __restrict__ slice_ptr,
                                                       const int32_t* __restrict__ row_len,
@@ -317,37 +103,8 @@ __global__ __launch_bounds__(kBlock) void k_eliminate(int cnt, const int32_t* __
                                                       const double* __restrict__ vfirst, double* __restrict__ val,
                                                       double* __restrict__ b) {
 	int const t = blockIdx.x * kBlock + threadIdx.x;
-	if (t >= cnt) {
-		return;
-	}
-	int32_t const i = rows[t];
-	int const r = i & 63;
-	int32_t const base = slice_ptr[i >> 6] + r;
-	int32_t const len = row_len[i];
-	bool const fr[2] = {fixed[2 * i] != 0, fixed[2 * i + 1] != 0};
-	double bl[2] = {b[2 * i], b[2 * i + 1]};
-	for (int k = 0; k < len; k++) {
-		int32_t const blk = base + k * kSlice;
-		int32_t const j = col[blk];
-		double* top = val + 4 * static_cast<int64_t>(blk) - 2 * r;
-		double* rowp[2] = {top, top + 2 * kSlice};
-		bool const fc[2] = {fixed[2 * j] != 0, fixed[2 * j + 1] != 0};
-		for (int rr = 0; rr < 2; rr++) {
-			for (int cc = 0; cc < 2; cc++) {
-				double& a = rowp[rr][cc];
-				if (fr[rr]) {
-					a = (j == i && rr == cc) ? 1.0 : 0.0;
-				} else if (fc[cc]) {
-					bl[rr] -= vfirst[2 * j + cc] * a;
-					a = 0.0;
-				}
-			}
-		}
-	}
-	for (int rr = 0; rr < 2; rr++) {
-		if (!fr[rr]) {
-			b[2 * i + rr] = bl[rr];
-		}
+	if (t < cnt) {
+		dev::eliminate_row(rows[t], slice_ptr, row_len, col, fixed, vfirst, val, b);
 	}
 }
 
@@ -403,18 +160,18 @@ int assemble(bfmx_system* s, int stress, double E, double nu, double rho, const 
 	BFMX_HIP(hipMemsetAsync(s->d_b, 0, 2 * n * sizeof(double), st));
 	BFMX_HIP(hipMemsetAsync(s->d_fixed, 0, 2 * n, st));
 	BFMX_HIP(hipMemsetAsync(s->d_vfirst, 0, 2 * n * sizeof(double), st));
-	const double* eE = elemE ? s->d_elemE : nullptr;
+	dev::Material const M{elemE ? s->d_elemE : nullptr, E, nu, rho, stress, d_force, gx, gy};
 	for (int c = 0; c < s->n_colors; c++) {
 		int64_t const e0 = s->color_ptr[c], e1 = s->color_ptr[c + 1];
 		if (e1 == e0) {
 			continue;
 		}
 		if (s->kind == 3) {
-			k_assemble<3><<<grid(e1 - e0), kBlock, 0, st>>>(e0, e1, s->d_elem, s->d_slots, s->d_coords, eE, E, nu,
-			                                               stress, rho, d_force, gx, gy, s->d_val, s->d_b);
+			k_assemble<3><<<grid(e1 - e0), kBlock, 0, st>>>(e0, e1, s->d_elem, s->d_slots, s->d_coords, M, s->d_val,
+			                                               s->d_b);
 		} else {
-			k_assemble<4><<<grid(e1 - e0), kBlock, 0, st>>>(e0, e1, s->d_elem, s->d_slots, s->d_coords, eE, E, nu,
-			                                               stress, rho, d_force, gx, gy, s->d_val, s->d_b);
+			k_assemble<4><<<grid(e1 - e0), kBlock, 0, st>>>(e0, e1, s->d_elem, s->d_slots, s->d_coords, M, s->d_val,
+			                                               s->d_b);
 		}
 		BFMX_HIP(hipGetLastError());
 	}

@@ -122,6 +122,7 @@ struct Graph {
 
 struct bfmx_model;
 struct bfmx_trainset;
+struct bfmx_batch;
 
 struct bfmx_system {
 	int device = 0;
@@ -195,6 +196,7 @@ struct bfmx_system {
 
 	bool assembled = false;
 	bool finalized = false;
+	bool borrowed = false;   // belongs to a batch built from meshes: device arrays inside its arenas
 	int stress = 0;
 	std::vector<uint8_t> dirichlet_nodes;  // union of Dirichlet-flagged nodes (renumbered)
 	int64_t last_iters = 0;
@@ -252,7 +254,14 @@ struct bfmx_trainset {
 namespace bfmx {
 
 // prep.hip (host)
+// Host half of build_topology: renumbering, adjacency, the sliced-ELL
+// pattern (col: nnzb_pad block columns) and the colour-ordered elements
+// (el_col: kind per element), without device work.
+int prepare_topology(bfmx_system* s, const double* coords, const size_t* elems, int renumber,
+                     std::vector<int32_t>& col, std::vector<int32_t>& el_col);
 int build_topology(bfmx_system* s, const double* coords, const size_t* elems, int renumber);
+// Host half of set_edges: renumbered edges and the node -> edge incidence.
+int prepare_edges(bfmx_system* s, size_t n_edges, const size_t* edge_nodes, const uint8_t* edge_boundary);
 int set_edges(bfmx_system* s, size_t n_edges, const size_t* edge_nodes, const uint8_t* edge_boundary);
 // assemble.hip
 int compute_slots(bfmx_system* s);
@@ -291,6 +300,19 @@ int mpnn_setup_numeric(bfmx_system* s);
 double* mpnn_apply(bfmx_system* s, const double* r, const double* q, double* partial, const Scalars* S,
                    int* n_partials);
 void mpnn_release(bfmx_system* s);
+// batch.hip
+int batch_n_configs();
+int batch_n_classes();
+int batch_create(bfmx_batch** out, size_t n, bfmx_system* const* systems);
+void batch_destroy(bfmx_batch* b);
+int batch_solve(bfmx_batch* b, int pre, double rtol, int64_t max_iter, const int32_t* picks, bfmx_stats_t* stats,
+                double* total_ms);
+int batch_autotune(bfmx_batch* b, int pre, int sample, int iters, int32_t* picks, double* times_ms);
+int batch_get_u(bfmx_batch* b, size_t k, double* u);
+int batch_create_planar(bfmx_batch** out, size_t n, const bfmx_instance_t* inst, int renumber);
+int batch_system(bfmx_batch* b, size_t k, bfmx_system** s);
+// api.hip: the calling thread's last error message
+const char* last_error_text();
 // pcg.hip
 // Plain Jacobi PCG from x = 0 on the current right-hand side, calling rec(p)
 // after iteration p * every (p = 0 .. P-1) with the iterate in s->d_x and

@@ -113,7 +113,8 @@ void rcm(int64_t n, const std::vector<int64_t>& ptr, const std::vector<int32_t>&
 
 }  // namespace
 
-int build_topology(bfmx_system* s, const double* coords, const size_t* elems, int renumber) {
+int prepare_topology(bfmx_system* s, const double* coords, const size_t* elems, int renumber,
+                     std::vector<int32_t>& col, std::vector<int32_t>& el_col) {
 	int64_t const n = s->n_nodes;
 	int64_t const ne = s->n_elems;
 	int const kind = s->kind;
@@ -179,7 +180,7 @@ int build_topology(bfmx_system* s, const double* coords, const size_t* elems, in
 	}
 	s->nnzb = s->adj_ptr[n];
 	s->nnzb_pad = s->slice_ptr[s->n_slices];
-	std::vector<int32_t> col(s->nnzb_pad);
+	col.assign(s->nnzb_pad, 0);
 	for (int64_t sl = 0; sl < s->n_slices; sl++) {
 		int64_t const kmax = (s->slice_ptr[sl + 1] - s->slice_ptr[sl]) / kSlice;
 		for (int r = 0; r < kSlice; r++) {
@@ -252,14 +253,22 @@ int build_topology(bfmx_system* s, const double* coords, const size_t* elems, in
 	}
 	std::vector<int64_t> cfill(s->color_ptr.begin(), s->color_ptr.end() - 1);
 	s->elem_order.assign(ne, 0);
-	std::vector<int32_t> el_col(ne * kind);
+	el_col.assign(ne * kind, 0);
 	for (int64_t t = 0; t < ne; t++) {
 		int32_t const e = by_min[t];
 		int64_t const pos = cfill[color[e]]++;
 		s->elem_order[pos] = e;
 		std::memcpy(&el_col[pos * kind], &el[e * kind], kind * sizeof(int32_t));
 	}
+	return BFMX_OK;
+}
 
+int build_topology(bfmx_system* s, const double* coords, const size_t* elems, int renumber) {
+	std::vector<int32_t> col, el_col;
+	BFMX_CHECK(prepare_topology(s, coords, elems, renumber, col, el_col));
+	int64_t const n = s->n_nodes;
+	int64_t const ne = s->n_elems;
+	int const kind = s->kind;
 	// uploads
 	BFMX_CHECK(dalloc(&s->d_coords, 2 * n));
 	BFMX_CHECK(dalloc(&s->d_elem, ne * kind));
@@ -289,7 +298,7 @@ int build_topology(bfmx_system* s, const double* coords, const size_t* elems, in
 	return BFMX_OK;
 }
 
-int set_edges(bfmx_system* s, size_t n_edges, const size_t* edge_nodes, const uint8_t* edge_boundary) {
+int prepare_edges(bfmx_system* s, size_t n_edges, const size_t* edge_nodes, const uint8_t* edge_boundary) {
 	int64_t const n = s->n_nodes;
 	s->n_edges = static_cast<int64_t>(n_edges);
 	s->edge_nodes.resize(2 * n_edges);
@@ -321,6 +330,12 @@ int set_edges(bfmx_system* s, size_t n_edges, const size_t* edge_nodes, const ui
 		s->inc[fill[s->edge_nodes[2 * e]]++] = static_cast<int32_t>(e);
 		s->inc[fill[s->edge_nodes[2 * e + 1]]++] = static_cast<int32_t>(e);
 	}
+	return BFMX_OK;
+}
+
+int set_edges(bfmx_system* s, size_t n_edges, const size_t* edge_nodes, const uint8_t* edge_boundary) {
+	BFMX_CHECK(prepare_edges(s, n_edges, edge_nodes, edge_boundary));
+	int64_t const n = s->n_nodes;
 	dfree(s->d_edge_nodes);
 	dfree(s->d_edge_bnd);
 	dfree(s->d_inc_ptr);

@@ -30,7 +30,9 @@ EXPORTS = [
     "bfmx_system_time_kernel", "bfmx_model_n_params", "bfmx_model_create", "bfmx_model_destroy",
     "bfmx_model_get_params", "bfmx_model_set_params", "bfmx_trainset_create", "bfmx_trainset_destroy",
     "bfmx_trainset_add", "bfmx_trainset_info", "bfmx_trainset_get", "bfmx_model_loss_grad", "bfmx_model_train",
-    "bfmx_system_set_model", "bfmx_system_apply_model",
+    "bfmx_system_set_model", "bfmx_system_apply_model", "bfmx_batch_create", "bfmx_batch_destroy",
+    "bfmx_batch_autotune", "bfmx_batch_solve", "bfmx_batch_get_u", "bfmx_batch_create_planar",
+    "bfmx_batch_system",
 ]
 
 
@@ -42,6 +44,16 @@ class Stats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Instance(C.Structure):
+    """bfmx_instance_t: one planar problem of a batch built from meshes."""
+    _fields_ = [("kind", C.c_int32), ("stress", C.c_int32), ("n_nodes", C.c_size_t), ("coords", C.c_void_p),
+                ("n_elems", C.c_size_t), ("elems", C.c_void_p), ("n_edges", C.c_size_t),
+                ("edge_nodes", C.c_void_p), ("edge_boundary", C.c_void_p), ("E", C.c_double), ("nu", C.c_double),
+                ("rho", C.c_double), ("elem_E", C.c_void_p), ("gx", C.c_double), ("gy", C.c_double),
+                ("n_conditions", C.c_size_t), ("cond_kind", C.c_void_p), ("cond_value", C.c_void_p),
+                ("cond_flags", C.c_void_p)]
 
 
 class BfmxError(RuntimeError):
@@ -108,6 +120,14 @@ def lib():
         L.bfmx_model_train.argtypes = [vp, vp, C.c_int, C.c_int, C.c_float, C.c_uint64, _Opt(_dp)]
         L.bfmx_system_set_model.argtypes = [vp, vp]
         L.bfmx_system_apply_model.argtypes = [vp, _dp, _dp]
+        L.bfmx_batch_create.argtypes = [C.POINTER(vp), sz, C.POINTER(vp)]
+        L.bfmx_batch_destroy.argtypes = [vp]
+        L.bfmx_batch_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, _i32, _Opt(_dp)]
+        L.bfmx_batch_solve.argtypes = [vp, C.c_int, C.c_double, sz, _Opt(_i32), C.c_void_p,
+                                       C.POINTER(C.c_double)]
+        L.bfmx_batch_get_u.argtypes = [vp, sz, _dp]
+        L.bfmx_batch_create_planar.argtypes = [C.POINTER(vp), sz, C.POINTER(Instance), C.c_int]
+        L.bfmx_batch_system.argtypes = [vp, sz, C.POINTER(vp)]
         for name in EXPORTS[1:]:
             getattr(L, name).restype = C.c_int
         _lib = L
@@ -366,6 +386,134 @@ class TrainSet:
             self.close()
         except Exception:
             pass
+
+
+BATCH_CLASSES, BATCH_CONFIGS = 3, 5
+
+
+def planar_instance(mesh, stress, material, kinds=(), values=(), flags=(), g=(0.0, 0.0), elem_E=None):
+    """The arrays of one problem of a batch built from meshes (Batch.from_instances)."""
+    n = mesh.n_nodes
+    kinds = np.ascontiguousarray(kinds, np.int32)
+    return {
+        "kind": int(mesh.kind), "stress": int(bool(stress)), "n_nodes": n,
+        "coords": np.ascontiguousarray(np.asarray(mesh.coords, np.float64).reshape(-1)),
+        "elems": np.ascontiguousarray(np.asarray(mesh.elems, np.uint64).reshape(-1)),
+        "edges": np.ascontiguousarray(np.asarray(mesh.edges, np.uint64).reshape(-1)),
+        "E": float(material["E"]), "nu": float(material["nu"]), "rho": float(material["rho"]),
+        "elem_E": None if elem_E is None else np.ascontiguousarray(elem_E, np.float64),
+        "g": (float(g[0]), float(g[1])), "kinds": kinds,
+        "values": np.ascontiguousarray(values, np.float64),
+        "flags": np.ascontiguousarray(np.asarray(flags, np.uint8).reshape(len(kinds), n)),
+    }
+
+
+def panel_instance(mesh, traction, material=None):
+    """A holed panel of the batched workload as a batch instance (as panel_system)."""
+    from . import meshgen
+    kinds, values, flags = meshgen.panel_conditions(mesh, traction)
+    return planar_instance(mesh, False, meshgen.PANEL_MATERIAL if material is None else material, kinds, values,
+                           flags)
+
+
+class _BatchSystem(System):
+    """A system owned by a batch built from meshes (bfmx_batch_system): the
+    getters and solo solves of System; never destroyed on its own."""
+
+    def __init__(self, batch, h, n_nodes):
+        self.batch = batch   # keeps the batch alive
+        self.h = h
+        self.n_nodes = n_nodes
+
+    def close(self):
+        self.h = C.c_void_p()
+
+
+class Batch:
+    """Systems solved together by one persistent launch per size class (include/bfmx.h)."""
+
+    def __init__(self, systems):
+        self.systems = list(systems)   # kept alive with the batch
+        self.sizes = [s.n_nodes for s in self.systems]
+        arr = (C.c_void_p * len(self.systems))(*[s.h.value for s in self.systems])
+        self.h = C.c_void_p()
+        _check(lib().bfmx_batch_create(C.byref(self.h), len(self.systems), arr))
+
+    @classmethod
+    def from_instances(cls, instances, renumber=True):
+        """A batch built from meshes (dicts of planar_instance): every solve
+        launch assembles and solves them."""
+        self = cls.__new__(cls)
+        self.systems = []
+        self.sizes = [d["n_nodes"] for d in instances]
+        recs = (Instance * len(instances))()
+        ptr = lambda a: None if a is None or a.size == 0 else a.ctypes.data
+        for r, d in zip(recs, instances):
+            r.kind, r.stress, r.n_nodes = d["kind"], d["stress"], d["n_nodes"]
+            r.coords, r.n_elems, r.elems = ptr(d["coords"]), d["elems"].size // d["kind"], ptr(d["elems"])
+            r.n_edges, r.edge_nodes, r.edge_boundary = d["edges"].size // 2, ptr(d["edges"]), None
+            r.E, r.nu, r.rho, r.elem_E = d["E"], d["nu"], d["rho"], ptr(d["elem_E"])
+            r.gx, r.gy = d["g"]
+            r.n_conditions = d["kinds"].size
+            r.cond_kind, r.cond_value, r.cond_flags = ptr(d["kinds"]), ptr(d["values"]), ptr(d["flags"])
+        self.h = C.c_void_p()
+        _check(lib().bfmx_batch_create_planar(C.byref(self.h), len(instances), recs, int(bool(renumber))))
+        return self
+
+    def system(self, k):
+        """System k: the batch's own for a batch built from meshes (available
+        after a solve), else the System the batch was made of."""
+        if self.systems:
+            return self.systems[k]
+        h = C.c_void_p()
+        _check(lib().bfmx_batch_system(self.h, k, C.byref(h)))
+        return _BatchSystem(self, h, self.sizes[k])
+
+    def autotune(self, precond=PRECOND_JACOBI, sample=64, iters=64):
+        """(picks (3,), times_ms (3, 5)): the fastest configuration of each size class."""
+        picks = np.zeros(BATCH_CLASSES, np.int32)
+        times = np.zeros(BATCH_CLASSES * BATCH_CONFIGS)
+        _check(lib().bfmx_batch_autotune(self.h, precond, sample, iters, picks, times))
+        return picks, times.reshape(BATCH_CLASSES, BATCH_CONFIGS)
+
+    def solve(self, precond=PRECOND_JACOBI, rtol=1e-8, max_iter=0, picks=None, check=True):
+        """(list of stats dicts, device ms of the launch)."""
+        st = (Stats * len(self.sizes))()
+        ms = C.c_double()
+        p = None if picks is None else np.ascontiguousarray(picks, np.int32)
+        rc = lib().bfmx_batch_solve(self.h, precond, rtol, max_iter, p, C.cast(st, C.c_void_p), C.byref(ms))
+        stats = [x.as_dict() for x in st]
+        if check:
+            _check(rc)
+        return stats, ms.value
+
+    def get_u(self, k):
+        u = np.empty(self.sizes[k] * 2)
+        _check(lib().bfmx_batch_get_u(self.h, k, u))
+        return u
+
+    def close(self):
+        if self.h:
+            lib().bfmx_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def panel_system(mesh, traction, material=None):
+    """A holed panel of the batched workload: plane strain, traction on top,
+    bottom corners pinned."""
+    from . import meshgen
+    mat = dict(meshgen.PANEL_MATERIAL if material is None else material)
+    s = System.from_mesh(mesh)
+    s.assemble(0, mat["E"], mat["nu"], mat["rho"])
+    kinds, values, flags = meshgen.panel_conditions(mesh, traction)
+    s.apply_conditions(kinds, values, flags)
+    return s
 
 
 def beam_system(mesh, stress=True, material=None, traction=None, gravity=None, renumber=True):

@@ -127,6 +127,87 @@ def beam_conditions(mesh, traction=BEAM_TRACTION):
     return kinds, values, flags
 
 
+# ---------------------------------------------------------------- holed panels
+
+PANEL_MATERIAL = dict(E=211e9, nu=0.3, rho=7850.0)
+
+
+def panel(seed, dofs_lo=1000, dofs_hi=60000, height=1.0):
+    """A bridge-like truss panel of the batched workload (P1, plane strain).
+
+    The rectangle [0, W] x [0, H] (W / H uniform in [3, 6]) on a structured
+    grid of about U(dofs_lo, dofs_hi) DOFs, each cell split along its
+    (i, j)-(i+1, j+1) diagonal, with 0-12 seeded circular holes of radius
+    2-8% of H: triangles whose centroid lies in a hole are removed, nodes no
+    longer used are dropped. Holes keep two cells of material to every side
+    and to each other (a drawn hole that would not is drawn again, at most 50
+    times), so the panel stays one piece, every kept triangle shares an edge
+    with another, and the top edge and the bottom corners are kept.
+    Returns (mesh, traction): traction is U(1e5, 1e7) N/m, downward.
+    Node sets: "top" (the top edge), "pins" (the two bottom corners)."""
+    rng = np.random.default_rng(1_000_003 + seed)
+    H = height
+    W = H * rng.uniform(3.0, 6.0)
+    dofs = rng.uniform(dofs_lo, dofs_hi)
+    # (nx + 1)(ny + 1) nodes, nx / ny = W / H, before the holes remove a few
+    ny = max(2, int(round(np.sqrt(dofs / 2.0 / (W / H)))))
+    nx = max(2, int(round(ny * W / H)))
+    gap = 2.0 * max(W / nx, H / ny)
+    holes = []
+    for _ in range(int(rng.integers(0, 13))):
+        for _try in range(50):
+            r = H * rng.uniform(0.02, 0.08)
+            m = r + gap
+            if 2 * m >= H:
+                continue
+            hx, hy = rng.uniform(m, W - m), rng.uniform(m, H - m)
+            if all((hx - x) ** 2 + (hy - y) ** 2 > (r + q + gap) ** 2 for x, y, q in holes):
+                holes.append((hx, hy, r))
+                break
+    ii, jj = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1))
+    coords = np.stack([ii.ravel() * W / nx, jj.ravel() * H / ny], 1)
+
+    def node(i, j):
+        return j * (nx + 1) + i
+
+    ci, cj = np.meshgrid(np.arange(nx), np.arange(ny))
+    ci, cj = ci.ravel(), cj.ravel()
+    t1 = np.stack([node(ci, cj), node(ci + 1, cj), node(ci + 1, cj + 1)], 1)
+    t2 = np.stack([node(ci, cj), node(ci + 1, cj + 1), node(ci, cj + 1)], 1)
+    tris = np.concatenate([t1, t2])
+    cen = coords[tris].mean(axis=1)
+    keep = np.ones(len(tris), bool)
+    for (hx, hy, r) in holes:
+        keep &= (cen[:, 0] - hx) ** 2 + (cen[:, 1] - hy) ** 2 > r * r
+    tris = tris[keep]
+    used = np.zeros(len(coords), bool)
+    used[tris.ravel()] = True
+    new = -np.ones(len(coords), np.int64)
+    new[used] = np.arange(used.sum())
+    top_old = np.array([node(i, ny) for i in range(nx, -1, -1)])   # right to left: counter-clockwise
+    assert used[top_old].all() and used[node(0, 0)] and used[node(nx, 0)]
+    coords = coords[used]
+    tris = new[tris]
+    top = new[top_old]
+    edges = np.stack([top[:-1], top[1:]], 1).astype(np.uint64)
+    # the element of each top edge: the upper-left triangle t2 of cell (i, ny - 1)
+    sets = {"top": np.zeros(len(coords), bool), "pins": np.zeros(len(coords), bool)}
+    sets["top"][top] = True
+    sets["pins"][new[node(0, 0)]] = True
+    sets["pins"][new[node(nx, 0)]] = True
+    mesh = Mesh(3, coords, tris.astype(np.uint64), edges, -np.ones(len(edges), np.int64), sets)
+    traction = -rng.uniform(1e5, 1e7)
+    return mesh, traction
+
+
+def panel_conditions(mesh, traction):
+    """(kinds, values, flags): traction on the top edge, bottom corners pinned."""
+    kinds = np.array([3, 0, 1], dtype=np.int32)
+    values = np.array([traction, 0.0, 0.0])
+    flags = np.stack([mesh.node_sets["top"], mesh.node_sets["pins"], mesh.node_sets["pins"]]).astype(np.uint8)
+    return kinds, values, flags
+
+
 def workload(name, seed=0):
     """Named workloads: (mesh, stress flag).
 

@@ -237,6 +237,76 @@ int bfmx_system_set_model(bfmx_system_t* s, bfmx_model_t* m);
 /* z = preconditioner(r) once, both in the internal node order (2 per node). */
 int bfmx_system_apply_model(bfmx_system_t* s, const double* r, double* z);
 
+/* ---- batched solves (no reference counterpart) -------------------------
+ * Many systems (uneven sizes) solved by Jacobi or block-Jacobi PCG, one
+ * workgroup per system, in one persistent launch per size class (systems of
+ * < 2048, < 8192, >= 8192 nodes), the launches captured in one HIP graph.
+ * A batch holds either existing assembled systems (bfmx_batch_create) or
+ * meshes that every launch assembles before solving
+ * (bfmx_batch_create_planar). A configuration is a workgroup size x rows-per-thread pair:
+ * 0 = 256 x 1, 1 = 256 x 2, 2 = 512 x 1, 3 = 512 x 2, 4 = 1024 x 1. A
+ * system's solution does not depend on the rest of the batch (bitwise, for
+ * the same configuration). The systems must outlive the batch and stay
+ * unchanged while it is used. */
+typedef struct bfmx_batch bfmx_batch_t;
+enum { BFMX_BATCH_CLASSES = 3, BFMX_BATCH_CONFIGS = 5 };
+
+int bfmx_batch_create(bfmx_batch_t** out, size_t n, bfmx_system_t* const* systems);
+int bfmx_batch_destroy(bfmx_batch_t* b);
+/* Times every configuration on up to `sample` systems of each class for
+ * `iters` iterations; picks[c] receives the fastest configuration of class c
+ * and times_ms (may be NULL) the BFMX_BATCH_CLASSES x BFMX_BATCH_CONFIGS
+ * timings. The picks are replayed by passing them to bfmx_batch_solve. */
+int bfmx_batch_autotune(bfmx_batch_t* b, int precond, int sample, int iters, int32_t* picks, double* times_ms);
+/* Solves every system from u = 0 to rtol (true FP64 residual); picks NULL =
+ * the default configurations, max_iter 0 = 20 x the largest DOF count.
+ * stats (may be NULL) receives one entry per system, total_ms the launch's
+ * device time. Returns BFMX_OK when every system converged. */
+int bfmx_batch_solve(bfmx_batch_t* b, int precond, double rtol, size_t max_iter, const int32_t* picks,
+                     bfmx_stats_t* stats, double* total_ms);
+/* Solution of system `index`, in its original DOF order. */
+int bfmx_batch_get_u(bfmx_batch_t* b, size_t index, double* u);
+
+/* One planar problem of a batch built from meshes: the arguments of
+ * bfmx_system_create, bfmx_system_set_edges, bfmx_system_assemble (with a
+ * constant body force) and the bfmx_system_apply_condition calls of one
+ * system, gathered in one record. */
+typedef struct {
+	int32_t kind;                  /* 3 (P1 triangles) or 4 (Q1 quads) */
+	int32_t stress;                /* 1 plane stress, 0 plane strain */
+	size_t n_nodes;
+	const double* coords;          /* 2 per node */
+	size_t n_elems;
+	const size_t* elems;           /* kind per element */
+	size_t n_edges;                /* 0: no edges (Neumann loads add nothing) */
+	const size_t* edge_nodes;      /* 2 per edge */
+	const uint8_t* edge_boundary;  /* 1 per edge, or NULL: every edge on the boundary */
+	double E, nu, rho;
+	const double* elem_E;          /* per element Young's modulus, or NULL */
+	double gx, gy;                 /* body force per unit mass */
+	size_t n_conditions;           /* applied in this order */
+	const int32_t* cond_kind;
+	const double* cond_value;
+	const uint8_t* cond_flags;     /* n_conditions x n_nodes bytes */
+} bfmx_instance_t;
+
+/* Builds a batch from n meshes. The host prepares every instance as
+ * bfmx_system_create would (node renumbering when renumber != 0, block
+ * pattern, element colouring; instances are spread over up to 16 threads)
+ * and packs all device arrays into a few allocations. Each bfmx_batch_solve
+ * launch then assembles K and the load of every instance, applies its
+ * conditions and solves it, one workgroup per instance. The input arrays
+ * are copied; they need not outlive the call. */
+int bfmx_batch_create_planar(bfmx_batch_t** out, size_t n, const bfmx_instance_t* inst, int renumber);
+
+/* Instance `index` of a batch built from meshes, as a system object owned by
+ * the batch (valid until bfmx_batch_destroy; not to be destroyed, assembled
+ * or given conditions), available once a solve launch has assembled it:
+ * the bfmx_system_* getters read the batch-assembled matrix and load, and
+ * bfmx_system_solve solves it on its own. For a batch of existing systems,
+ * returns that system. */
+int bfmx_batch_system(bfmx_batch_t* b, size_t index, bfmx_system_t** s);
+
 #ifdef __cplusplus
 }
 #endif
