@@ -11,7 +11,10 @@ value = DOFs / mean step time of the headline solver, flexible CG in FP64
 preconditioned by one FP32 aggregation-multigrid V-cycle. Plain FP64
 Jacobi-PCG is timed the same way beside it (the speedup is reported).
 
-Also reported: the roofline of the headline solver's dominant kernel (the
+Also reported: the batched workload (4096 seeded holed panels of 1k-60k DOFs,
+assembled and solved by block-Jacobi PCG in one graph launch, one workgroup per
+panel: DOF/s and the launch's algorithmic GB/s); the roofline of the headline
+solver's dominant kernel (the
 finest-level smoothing step: FP32 SpMV + block Jacobi) and of the FP64 CG
 SpMV (algorithmic bytes / mean launch time from HIP events on the solver's
 stream), the end-to-end time (upload + renumbering + assembly + conditions +
@@ -34,6 +37,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch on the q1_2m beam measured by two rocprofv3 counter runs
+# (FETCH_SIZE x 1.9998 + WRITE_SIZE, the factors read off a 256 MiB
+# device-to-device copy in the same runs): profiles/hbm_traffic_q1_2m.md
+TRAFFIC_Q1_2M = {"k_cheb": 238.2e6, "k_cg_spmv": 389.3e6, "k_fcg_spmv": 392.0e6}
 CPU_SAMPLE = (180, 45)   # Q1 beam cells for the reference CPU path (16,652 DOFs)
 
 
@@ -90,6 +97,73 @@ def cpu_baseline_reference():
     }
 
 
+def copy_calibration(nbytes):
+    """One hipMemcpy of nbytes between two device buffers (the runtime's copy
+    kernel reads and writes nbytes): the reference that scales the
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0 and hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0          # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: {nbytes} bytes device to device")
+
+
+def batched_leg(D, meshgen, count, rtol):
+    """The batched workload: `count` seeded holed panels (1k-60k DOFs, plane
+    strain, traction U(1e5, 1e7) N/m on top, bottom corners pinned), built
+    into one batch from their meshes; one timed graph launch assembles every
+    panel (one workgroup each) and solves it by block-Jacobi PCG (one
+    workgroup each, the autotuner's configurations) to rtol."""
+    t = time.perf_counter()
+    inst, sizes, dofs = [], [], 0
+    for k in range(count):
+        m, tr = meshgen.panel(k)
+        inst.append(D.panel_instance(m, tr))
+        sizes.append(m.n_nodes)
+        dofs += m.n_dof
+    t_mesh = time.perf_counter() - t
+    t = time.perf_counter()
+    b = D.Batch.from_instances(inst)
+    del inst
+    t_create = time.perf_counter() - t
+    log(f"batched: {count} panels, {dofs} DOFs; meshes {t_mesh:.1f} s, batch set-up {t_create:.1f} s")
+    pre = D.PRECOND_BLOCK_JACOBI
+    t = time.perf_counter()
+    picks, times = b.autotune(precond=pre)
+    t_tune = time.perf_counter() - t
+    log(f"batched: autotune {t_tune:.1f} s, picks {picks.tolist()}")
+    b.solve(precond=pre, rtol=rtol, picks=picks)         # graph capture + first launch
+    t = time.perf_counter()
+    st, ms = b.solve(precond=pre, rtol=rtol, picks=picks)
+    wall = time.perf_counter() - t
+    assert all(x["status"] == 0 and x["rel_residual"] <= rtol for x in st)
+    its = np.array([x["iterations"] for x in st])
+    nnzb = np.array([b.system(k).info()["nnzb"] for k in range(count)])
+    byt = sum(D.batch_bytes(int(nnzb[k]), sizes[k], int(st[k]["iterations"]), int(st[k]["restarts"]), pre)
+              for k in range(count))
+    gbs = byt / (ms / 1e3) / 1e9
+    log(f"batched: {ms:.1f} ms device ({wall * 1e3:.1f} ms wall), {dofs / (ms / 1e3):.3e} DOF/s, "
+        f"iterations mean {its.mean():.0f} max {its.max()}, {gbs:.0f} GB/s algorithmic")
+    return {
+        "workload": f"{count} holed panels (seeds 0-{count - 1}), 1k-60k DOFs, plane strain, block-Jacobi PCG "
+                    f"to rtol {rtol:g}, assembled and solved in one graph launch",
+        "n_dof": int(dofs), "ms_per_step": ms, "wall_ms": wall * 1e3, "value": dofs / (ms / 1e3),
+        "unit": "DOF/s", "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
+        "picks": picks.tolist(), "autotune_ms": times.tolist(), "setup_s": {"meshes": t_mesh,
+                                                                              "batch_create": t_create,
+                                                                              "autotune": t_tune},
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_batch_pcg (block Jacobi, autotuned) + k_batch_assemble (whole graph launch)",
+                     "algorithmic_bytes": byt},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +174,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-pcg", action="store_true", help="skip the plain Jacobi-PCG leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched holed-panel leg")
+    ap.add_argument("--batch-count", type=int, default=4096, help="panels in the batched leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run one device-to-device copy of BYTES (the HBM counters' reference)")
     ap.add_argument("--mixed-cg", action="store_true",
                     help="also run CG on the FP32 matrix with FP64 reliable updates")
     args = ap.parse_args()
@@ -111,6 +189,8 @@ def main():
     from bfm_amd import device as D
     from bfm_amd import meshgen
 
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     mesh, stress = meshgen.workload(args.workload)
     log(f"workload {args.workload}: {mesh.n_nodes} nodes, {mesh.n_dof} DOFs, {mesh.n_elems} elements")
 
@@ -145,8 +225,9 @@ def main():
         k_ms, k_bytes = s.time_kernel(kernel, reps=100)
         gbs = k_bytes / (k_ms / 1e3) / 1e9
         log(f"{label}: {k_ms * 1e3:.1f} us, {k_bytes / 1e6:.1f} MB, {gbs:.0f} GB/s")
+        traffic = TRAFFIC_Q1_2M.get(label.split("<")[0].split(" ")[0]) if args.workload == "q1_2m" else None
         return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": gbs / HBM_PEAK_GBS, "traffic": None, "kernel": label,
+                "frac": gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": label,
                 "kernel_us": k_ms * 1e3, "algorithmic_bytes": k_bytes}
 
     roof_smooth = roof(D.KERNEL_AMG_SMOOTH, "k_cheb<2,double,false,false> (finest-level smoothing step)")
@@ -187,6 +268,9 @@ def main():
         except (D.BfmxError, AssertionError) as e:
             extra["cg_fp32_matrix"] = {"status": str(e)}
             log(f"cg_fp32_matrix: {e}")
+
+    if not args.no_batch:
+        extra["batched_panels"] = batched_leg(D, meshgen, args.batch_count, args.rtol)
 
     cpu = None
     if not args.no_cpu:

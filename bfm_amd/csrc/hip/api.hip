@@ -609,7 +609,7 @@ int bfmx_batch_destroy(bfmx_batch_t* b) {
 }
 
 int bfmx_batch_autotune(bfmx_batch_t* b, int precond, int sample, int iters, int32_t* picks, double* times_ms) {
-	if (!b || !picks || sample < 1 || iters < 1) {
+	if (!b || !picks || sample < 0 || iters < 1) {
 		set_error("bad arguments to batch_autotune");
 		return BFMX_E_ARG;
 	}

@@ -426,6 +426,7 @@ struct bfmx_batch {
 	bfmx::BatchOut* d_out = nullptr;
 	int32_t* d_order = nullptr;             // per class: systems largest first
 	std::vector<int32_t> class_off;         // kClasses + 1 offsets into d_order
+	int32_t* d_order_all = nullptr;         // all systems largest first (one launch for equal picks)
 	double* d_work = nullptr;
 	std::vector<bfmx::BatchOut> h_out;
 	struct Exec {
@@ -490,6 +491,7 @@ void batch_destroy(bfmx_batch* b) {
 	dfree(b->d_inst);
 	dfree(b->d_out);
 	dfree(b->d_order);
+	dfree(b->d_order_all);
 	dfree(b->d_work);
 	if (b->planar) {
 		for (auto* s : b->systems) {
@@ -535,11 +537,17 @@ int finish_create(bfmx_batch* b, const std::vector<BatchInst>& h) {
 	for (int c = 0; c < kClasses; c++) {
 		b->class_off[c + 1] += b->class_off[c];
 	}
+	std::vector<int32_t> order_all(n);
+	std::iota(order_all.begin(), order_all.end(), 0);
+	std::stable_sort(order_all.begin(), order_all.end(), [&](int32_t a, int32_t c) { return h[a].n > h[c].n; });
 	BFMX_CHECK(dalloc(&b->d_inst, n));
 	BFMX_CHECK(dalloc(&b->d_out, n));
 	BFMX_CHECK(dalloc(&b->d_order, n));
+	BFMX_CHECK(dalloc(&b->d_order_all, n));
 	BFMX_HIP(hipMemcpyAsync(b->d_inst, h.data(), n * sizeof(BatchInst), hipMemcpyHostToDevice, b->stream));
 	BFMX_HIP(hipMemcpyAsync(b->d_order, order.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, b->stream));
+	BFMX_HIP(hipMemcpyAsync(b->d_order_all, order_all.data(), n * sizeof(int32_t), hipMemcpyHostToDevice,
+	                        b->stream));
 	BFMX_HIP(hipStreamSynchronize(b->stream));
 	b->h_out.resize(n);
 	return BFMX_OK;
@@ -611,7 +619,8 @@ namespace {
 // Assembly of a batch built from meshes (largest systems first).
 void launch_assemble(bfmx_batch* b) {
 	if (b->planar) {
-		k_batch_assemble<<<static_cast<int>(b->systems.size()), kAsmBlock, 0, b->stream>>>(b->d_inst, b->d_order);
+		k_batch_assemble<<<static_cast<int>(b->systems.size()), kAsmBlock, 0, b->stream>>>(b->d_inst,
+		                                                                                   b->d_order_all);
 	}
 }
 
@@ -629,6 +638,13 @@ void mark_assembled(bfmx_batch* b) {
 
 void launch_all(bfmx_batch* b, int pre, double rtol2, int64_t max_iter, const int* picks) {
 	launch_assemble(b);
+	if (std::all_of(picks, picks + kClasses, [&](int p) { return p == picks[0]; })) {
+		// one configuration for all: one launch, largest systems first, so no
+		// class waits for the previous one to drain
+		launch(pre, picks[0], static_cast<int>(b->systems.size()), b->stream, b->d_inst, b->d_order_all, rtol2,
+		       max_iter, b->d_out);
+		return;
+	}
 	for (int c = kClasses - 1; c >= 0; c--) {   // largest systems first
 		int const cnt = b->class_off[c + 1] - b->class_off[c];
 		if (cnt > 0) {
@@ -660,12 +676,16 @@ int batch_solve(bfmx_batch* b, int pre, double rtol, int64_t max_iter, const int
 	}
 	double const rtol2 = rtol * rtol;
 	hipGraphExec_t exec = nullptr;
+	bool const use_graph = graphs_enabled();
 	for (auto const& g : b->graphs) {
+		if (!use_graph) {
+			break;
+		}
 		if (g.pre == pre && g.rtol2 == rtol2 && g.max_iter == max_iter && std::equal(picks, picks + kClasses, g.picks)) {
 			exec = g.exec;
 		}
 	}
-	if (!exec) {
+	if (!exec && use_graph) {
 		hipGraph_t g = nullptr;
 		BFMX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
 		launch_all(b, pre, rtol2, max_iter, picks);
@@ -681,7 +701,12 @@ int batch_solve(bfmx_batch* b, int pre, double rtol, int64_t max_iter, const int
 		b->graphs.push_back(ex);
 	}
 	BFMX_HIP(hipEventRecord(b->ev[0], b->stream));
-	BFMX_HIP(hipGraphLaunch(exec, b->stream));
+	if (use_graph) {
+		BFMX_HIP(hipGraphLaunch(exec, b->stream));
+	} else {
+		launch_all(b, pre, rtol2, max_iter, picks);
+		BFMX_HIP(hipGetLastError());
+	}
 	BFMX_HIP(hipEventRecord(b->ev[1], b->stream));
 	BFMX_HIP(hipMemcpyAsync(b->h_out.data(), b->d_out, b->h_out.size() * sizeof(BatchOut), hipMemcpyDeviceToHost,
 	                        b->stream));
@@ -731,7 +756,8 @@ int batch_autotune(bfmx_batch* b, int pre, int sample, int iters, int32_t* picks
 	launch_assemble(b);
 	BFMX_HIP(hipGetLastError());
 	for (int c = 0; c < kClasses; c++) {
-		int const cnt = std::min(sample, b->class_off[c + 1] - b->class_off[c]);
+		int const all = b->class_off[c + 1] - b->class_off[c];
+		int const cnt = sample > 0 ? std::min(sample, all) : all;
 		int best = kDefaultPick[c];
 		float best_ms = 0;
 		for (int cfg = 0; cfg < kConfigs; cfg++) {

@@ -331,6 +331,10 @@ int time_kernel(bfmx_system* s, int kernel, int reps, double* avg_ms, double* by
 // amg.hip: reps launches of one finest-level smoothing step (set-up done)
 int amg_time_smoother(bfmx_system* s, int reps, hipEvent_t e0, hipEvent_t e1, double* bytes);
 void release_solver(bfmx_system* s);
+// false when the environment sets BFMX_GRAPHS=0: launches go straight to the
+// stream instead of through captured graphs (for tools that do not follow
+// graph launches)
+bool graphs_enabled();
 
 template <typename T>
 int dalloc(T** p, size_t n) {

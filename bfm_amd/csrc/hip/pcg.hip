@@ -48,6 +48,14 @@
 
 namespace bfmx {
 
+// Chunks are replayed from a captured HIP graph unless the environment sets
+// BFMX_GRAPHS=0, which launches the same kernels straight on the stream (for
+// tools that do not follow graph launches).
+bool graphs_enabled() {
+	const char* e = std::getenv("BFMX_GRAPHS");
+	return !(e && e[0] == '0');
+}
+
 namespace {
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
@@ -652,13 +660,6 @@ void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
 	launch_residual(s, pre, 1, rtol2, mixed ? kDelta2 : 0.0);
 }
 
-// Chunks are replayed from a captured HIP graph unless the environment sets
-// BFMX_GRAPHS=0, which launches the same kernels straight on the stream (for
-// tools that do not follow graph launches).
-bool graphs_enabled() {
-	const char* e = std::getenv("BFMX_GRAPHS");
-	return !(e && e[0] == '0');
-}
 
 // The captured chunk for this configuration (captured on first use).
 int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, hipGraphExec_t* out) {

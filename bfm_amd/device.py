@@ -408,6 +408,18 @@ def planar_instance(mesh, stress, material, kinds=(), values=(), flags=(), g=(0.
     }
 
 
+def batch_bytes(nnzb, n, iterations, checks, precond=PRECOND_JACOBI):
+    """Algorithmic HBM bytes of the batched PCG of one system: per iteration
+    the SpMV pass (36 B per stored block, 4 B row length per row, the
+    gathered z and previous direction read once, new direction and A p
+    written) and the update pass (x, p, r, A p and the Jacobi inverse read;
+    x, r, z written); per true-residual check one residual pass."""
+    dinv = 16 if precond == PRECOND_JACOBI else 32
+    it = 36 * nnzb + 4 * n + 64 * n + (64 + dinv + 48) * n
+    check = 36 * nnzb + 4 * n + 16 * n + 16 * n + dinv * n + 32 * n
+    return iterations * it + (checks + 1) * check
+
+
 def panel_instance(mesh, traction, material=None):
     """A holed panel of the batched workload as a batch instance (as panel_system)."""
     from . import meshgen
@@ -469,8 +481,9 @@ class Batch:
         _check(lib().bfmx_batch_system(self.h, k, C.byref(h)))
         return _BatchSystem(self, h, self.sizes[k])
 
-    def autotune(self, precond=PRECOND_JACOBI, sample=64, iters=64):
-        """(picks (3,), times_ms (3, 5)): the fastest configuration of each size class."""
+    def autotune(self, precond=PRECOND_JACOBI, sample=0, iters=32):
+        """(picks (3,), times_ms (3, 5)): the fastest configuration of each size
+        class, timed on `sample` systems of the class (0: all)."""
         picks = np.zeros(BATCH_CLASSES, np.int32)
         times = np.zeros(BATCH_CLASSES * BATCH_CONFIGS)
         _check(lib().bfmx_batch_autotune(self.h, precond, sample, iters, picks, times))

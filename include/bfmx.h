@@ -253,10 +253,12 @@ enum { BFMX_BATCH_CLASSES = 3, BFMX_BATCH_CONFIGS = 5 };
 
 int bfmx_batch_create(bfmx_batch_t** out, size_t n, bfmx_system_t* const* systems);
 int bfmx_batch_destroy(bfmx_batch_t* b);
-/* Times every configuration on up to `sample` systems of each class for
- * `iters` iterations; picks[c] receives the fastest configuration of class c
- * and times_ms (may be NULL) the BFMX_BATCH_CLASSES x BFMX_BATCH_CONFIGS
- * timings. The picks are replayed by passing them to bfmx_batch_solve. */
+/* Times every configuration on up to `sample` systems of each class (0: all
+ * of them, which measures throughput with the GPU full; a small sample
+ * measures single-workgroup latency) for `iters` iterations; picks[c]
+ * receives the fastest configuration of class c and times_ms (may be NULL)
+ * the BFMX_BATCH_CLASSES x BFMX_BATCH_CONFIGS timings. The picks are
+ * replayed by passing them to bfmx_batch_solve. */
 int bfmx_batch_autotune(bfmx_batch_t* b, int precond, int sample, int iters, int32_t* picks, double* times_ms);
 /* Solves every system from u = 0 to rtol (true FP64 residual); picks NULL =
  * the default configurations, max_iter 0 = 20 x the largest DOF count.
