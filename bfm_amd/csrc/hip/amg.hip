@@ -1,27 +1,37 @@
-// Unsmoothed aggregation multigrid: the FP32 preconditioner of the
+// Smoothed aggregation multigrid: the FP32 preconditioner of the
 // flexible-CG path.
 //
 // Hierarchy topology (built once per system on the host from the node graph):
 //   * aggregates: a node whose neighbours are all free starts an aggregate
 //     with them (about 3 x 3 nodes on a quad mesh); a leftover node joins the
 //     aggregate of its first aggregated neighbour;
-//   * the coarse block pattern (two aggregates are coupled when two of their
-//     members are) in sliced ELL, and for every coarse block slot the list of
-//     fine blocks it gathers, so the Galerkin product P^T A P is formed with
-//     one thread per coarse block, without atomics and in a fixed order.
-// Numeric setup (every solve, on the GPU, because the Dirichlet set enters):
+//   * the prolongator pattern: row i couples to the aggregates of its
+//     neighbours (P = (I - w D^-1 A) P_t has the sparsity of A P_t), stored
+//     as CSR of bs x 3 blocks, with its transpose as per-aggregate lists for
+//     the restriction;
+//   * the coarse block pattern (that of P^T A P) in sliced ELL, and for every
+//     coarse block slot the list of its contributions (fine block A_ij, entry
+//     P_iI, entry P_jJ), so the Galerkin product is formed without atomics
+//     and in a fixed order; the coarse pattern is the next level's graph.
+// Numeric setup (every solve, on the GPU, because the Dirichlet set enters),
+// level by level:
 //   * level-0 modes B: the three rigid-body motions (two translations and the
 //     rotation, in coordinates centred on the mesh and scaled to its size),
 //     with the rows of constrained DOFs zeroed;
-//   * per aggregate, a Cholesky QR of its rows of B: P = B R^-1 (orthonormal
-//     columns, stored FP32) and the coarse modes B_c = R. A column whose
-//     remainder is below kDropTol of its norm (a mode that the constrained
-//     DOFs make dependent) is dropped: its coarse DOF is decoupled;
-//   * Galerkin operators in FP64 with FP32 copies, inverse diagonal blocks,
-//     Gershgorin bounds of lambda_max(D^-1 A), Chebyshev coefficients, and a
-//     dense Gauss-Jordan inverse of the coarsest operator.
+//   * inverse diagonal blocks, the Gershgorin bound lambda_max of D^-1 A and
+//     the Chebyshev coefficients of the level;
+//   * per aggregate, a Cholesky QR of its rows of B: P_t = B R^-1 (orthonormal
+//     columns) and the coarse modes B_c = R. A column whose remainder is below
+//     kDropTol of its norm (a mode that the constrained DOFs make dependent) is
+//     dropped: its coarse DOF is decoupled;
+//   * the smoothed prolongator P = (I - w D^-1 A) P_t, w = 4 / (3 lambda_max),
+//     stored FP32 (Vanek, Mandel and Brezina, Computing 56 (1996) 179-196);
+//   * the Galerkin operator P^T A P in FP64 with an FP32 copy, and a dense
+//     Gauss-Jordan inverse of the coarsest operator.
+// BFMX_AMG_SMOOTH=0 in the environment (read when a hierarchy is built)
+// keeps P = P_t: plain aggregation, for comparisons.
 // V-cycle (FP32 vectors and matrices, FP32 arithmetic): Chebyshev smoothing
-// with block Jacobi (kSmoothDeg steps before and after the coarse
+// with block Jacobi (AmgParams::deg steps before and after the coarse
 // correction), residual, restriction P^T r, recursion, prolongation; the
 // coarsest level applies its dense inverse.
 
@@ -34,14 +44,18 @@ namespace bfmx {
 
 namespace {
 
-constexpr int kSmoothDeg = 2;   // Chebyshev steps before and after the coarse correction
 constexpr int kCoarseDeg = 16;  // Chebyshev steps on a coarsest level too large for the dense inverse
 constexpr int kMaxDeg = 16;
 constexpr int kMaxLevels = 12;
-constexpr int kCoarsest = 64;   // stop coarsening at this many block rows
 constexpr int kDenseMax = 768;  // largest coarsest operator (DOFs) inverted densely
-constexpr double kSmoothRatio = 20.0;   // Chebyshev interval [lmax / ratio, lmax]
-constexpr double kCoarseRatio = 200.0;
+constexpr double kCoarseRatio = 200.0;  // Chebyshev interval of an iterated coarsest level
+// A level's scalars: [0] lambda_max of its Chebyshev interval (the
+// Gershgorin bound), [1 .. 2 kMaxDeg] the Chebyshev coefficients, then the
+// power iteration's current scale (kScalScale) and its lambda_max estimate
+// (kScalLambda), which sets the prolongator damping.
+constexpr int kScalScale = 2 + 2 * kMaxDeg;
+constexpr int kScalLambda = kScalScale + 1;
+constexpr int kScalSize = kScalLambda + 1;
 constexpr double kDropTol = 1e-10;      // relative remainder below which a mode column is dropped
 constexpr int kGj = 1024;               // threads of the Gauss-Jordan workgroup
 
@@ -218,33 +232,143 @@ __global__ __launch_bounds__(kBlock) void k_tentative(int nc, const int32_t* __r
 	}
 }
 
-// Coarse block slot b = sum over its fine blocks (i, j) of P_i^T A_ij P_j.
+// Inverse of the diagonal block of row i in FP64 (a zero diagonal entry,
+// a decoupled DOF, is treated as 1).
+template <int BS>
+__device__ __forceinline__ void diag_inverse(const double* __restrict__ val, int32_t dblk, int r,
+                                             double (&Di)[BS][BS]) {
+	double D[BS][BS];
+	ld_block<BS>(val, dblk, r, D);
+#pragma unroll
+	for (int a = 0; a < BS; a++) {
+		if (D[a][a] == 0) {
+			D[a][a] = 1;
+		}
+	}
+	if constexpr (BS == 2) {
+		double const inv = 1.0 / (D[0][0] * D[1][1] - D[0][1] * D[1][0]);
+		Di[0][0] = D[1][1] * inv;
+		Di[0][1] = -D[0][1] * inv;
+		Di[1][0] = -D[1][0] * inv;
+		Di[1][1] = D[0][0] * inv;
+	} else {
+		double const c00 = D[1][1] * D[2][2] - D[1][2] * D[2][1];
+		double const c01 = D[1][2] * D[2][0] - D[1][0] * D[2][2];
+		double const c02 = D[1][0] * D[2][1] - D[1][1] * D[2][0];
+		double const inv = 1.0 / (D[0][0] * c00 + D[0][1] * c01 + D[0][2] * c02);
+		Di[0][0] = c00 * inv;
+		Di[1][0] = c01 * inv;
+		Di[2][0] = c02 * inv;
+		Di[0][1] = (D[0][2] * D[2][1] - D[0][1] * D[2][2]) * inv;
+		Di[1][1] = (D[0][0] * D[2][2] - D[0][2] * D[2][0]) * inv;
+		Di[2][1] = (D[0][1] * D[2][0] - D[0][0] * D[2][1]) * inv;
+		Di[0][2] = (D[0][1] * D[1][2] - D[0][2] * D[1][1]) * inv;
+		Di[1][2] = (D[0][2] * D[1][0] - D[0][0] * D[1][2]) * inv;
+		Di[2][2] = (D[0][0] * D[1][1] - D[0][1] * D[1][0]) * inv;
+	}
+}
+
+// Smoothed prolongator, one fine row i per thread: entry k of row i (the
+// aggregate J = pcol[k]) is [J = agg(i)] Pt_i - w D_i^-1 sum over the row's
+// blocks A_ij with agg(j) = J of A_ij Pt_j, w = omega 4 / (3 lambda_max(D^-1 A))
+// with lambda_max the level's power-iteration estimate (or Gershgorin bound);
+// omega = 0 keeps the tentative prolongator. Padding blocks are zero and add
+// nothing.
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_smooth_p(int nb, const int32_t* __restrict__ slice_ptr,
+                                                     const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                     const int32_t* __restrict__ diag_blk,
+                                                     const float* __restrict__ Pt, const int32_t* __restrict__ pptr,
+                                                     const uint8_t* __restrict__ ppos, const double* __restrict__ scal,
+                                                     double omega, float* __restrict__ P) {
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= nb) {
+		return;
+	}
+	int const r = i & 63;
+	double const w = omega * 4.0 / (3.0 * scal[kScalLambda]);
+	int32_t const dblk = diag_blk[i];
+	double Di[BS][BS];
+	diag_inverse<BS>(val, dblk, r, Di);
+	int const self = ppos[dblk];
+	int32_t const p0 = pptr[i], np = pptr[i + 1] - p0;
+	int32_t const sbeg = slice_ptr[i >> 6] + r, send = slice_ptr[(i >> 6) + 1];
+	for (int k = 0; k < np; k++) {
+		double acc[BS][3];
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				acc[a][c] = 0;
+			}
+		}
+		if (w != 0) {
+			for (int32_t blk = sbeg; blk < send; blk += kSlice) {
+				if (ppos[blk] != k) {
+					continue;
+				}
+				int64_t const j = col[blk];
+				double A[BS][BS];
+				ld_block<BS>(val, blk, r, A);
+#pragma unroll
+				for (int a = 0; a < BS; a++) {
+#pragma unroll
+					for (int c = 0; c < 3; c++) {
+						double t = 0;
+#pragma unroll
+						for (int m = 0; m < BS; m++) {
+							t += A[a][m] * static_cast<double>(Pt[BS * 3 * j + 3 * m + c]);
+						}
+						acc[a][c] += t;
+					}
+				}
+			}
+		}
+		float* out = P + BS * 3 * static_cast<int64_t>(p0 + k);
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				double t = 0;
+#pragma unroll
+				for (int m = 0; m < BS; m++) {
+					t += Di[a][m] * acc[m][c];
+				}
+				double v = -w * t;
+				if (k == self) {
+					v += static_cast<double>(Pt[BS * 3 * static_cast<int64_t>(i) + 3 * a + c]);
+				}
+				out[3 * a + c] = static_cast<float>(v);
+			}
+		}
+	}
+}
+
+// Coarse block slot b = sum over its contributions (fine block A_ij, entries
+// pi = P_iI and pj = P_jJ) of P_iI^T A_ij P_jJ, in FP64 (P as stored, FP32).
+// kGalLanes lanes share a slot: lane l takes contributions l, l + kGalLanes,
+// ..., and the lanes' sums are combined by a fixed butterfly.
+constexpr int kGalLanes = 16;
+
 template <int BSF>
 __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32_t* __restrict__ gal_ptr,
                                                      const int32_t* __restrict__ gal_blk,
-                                                     const int32_t* __restrict__ gal_i,
-                                                     const int32_t* __restrict__ gal_j,
+                                                     const int32_t* __restrict__ gal_pi,
+                                                     const int32_t* __restrict__ gal_pj,
                                                      const double* __restrict__ fval, const float* __restrict__ P,
                                                      double* __restrict__ cval64, float* __restrict__ cval32) {
-	int64_t const b = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-	if (b >= nslots) {
-		return;
-	}
+	int64_t const b = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGalLanes;
+	int const lane = threadIdx.x % kGalLanes;
+	bool const live = b < nslots;   // every lane stays for the butterfly
 	double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-	for (int32_t e = gal_ptr[b]; e < gal_ptr[b + 1]; e++) {
-		int32_t const i = gal_i[e], j = gal_j[e];
+	int32_t const e0 = live ? gal_ptr[b] : 0, e1 = live ? gal_ptr[b + 1] : 0;
+	for (int32_t e = e0 + lane; e < e1; e += kGalLanes) {
+		int32_t const blk = gal_blk[e];
 		double A[BSF][BSF];
-		ld_block<BSF>(fval, gal_blk[e], i & 63, A);
-		double Pj[BSF][3], Pi[BSF][3];
-#pragma unroll
-		for (int a = 0; a < BSF; a++) {
-#pragma unroll
-			for (int c = 0; c < 3; c++) {
-				Pj[a][c] = P[BSF * 3 * static_cast<int64_t>(j) + 3 * a + c];
-				Pi[a][c] = P[BSF * 3 * static_cast<int64_t>(i) + 3 * a + c];
-			}
-		}
-		double T[BSF][3];   // A_ij P_j
+		ld_block<BSF>(fval, blk, blk & 63, A);
+		const float* pi = P + BSF * 3 * static_cast<int64_t>(gal_pi[e]);
+		const float* pj = P + BSF * 3 * static_cast<int64_t>(gal_pj[e]);
+		double T[BSF][3];   // A_ij P_jJ
 #pragma unroll
 		for (int a = 0; a < BSF; a++) {
 #pragma unroll
@@ -252,7 +376,7 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32
 				double t = 0;
 #pragma unroll
 				for (int k = 0; k < BSF; k++) {
-					t += A[a][k] * Pj[k][c];
+					t += A[a][k] * static_cast<double>(pj[3 * k + c]);
 				}
 				T[a][c] = t;
 			}
@@ -264,11 +388,26 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32
 				double t = 0;
 #pragma unroll
 				for (int k = 0; k < BSF; k++) {
-					t += Pi[k][a] * T[k][c];
+					t += static_cast<double>(pi[3 * k + a]) * T[k][c];
 				}
 				C[a][c] += t;
 			}
 		}
+	}
+#pragma unroll
+	for (int a = 0; a < 3; a++) {
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			double v = C[a][c];
+#pragma unroll
+			for (int off = kGalLanes / 2; off > 0; off >>= 1) {
+				v += __shfl_xor(v, off, 64);
+			}
+			C[a][c] = v;
+		}
+	}
+	if (!live || lane != 0) {
+		return;
 	}
 	int const r = static_cast<int>(b & 63);
 #pragma unroll
@@ -366,6 +505,92 @@ __global__ __launch_bounds__(kBlock) void k_diag_gersh(int nb, const int32_t* __
 	wg_partial<false>(g, partial);
 }
 
+// Power iteration for lambda_max(D^-1 A) on a level (FP32 matrix and
+// inverse diagonal blocks): x_0 has entries 1 + (hash of the index) / 2^32
+// (a fixed start with every component), then y = D^-1 A (s x) with s the
+// previous step's 1 / ||y|| (scal[kScalScale]); the estimate after a step
+// is ||D^-1 A x|| for the unit x (scal[kScalLambda]).
+__global__ __launch_bounds__(kBlock) void k_power_init(int64_t m, float* __restrict__ x, double* __restrict__ partial) {
+	int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+	double v = 0;
+	if (i < m) {
+		uint32_t h = static_cast<uint32_t>(i) * 2654435761u;
+		h ^= h >> 15;
+		float const xi = 1.0f + static_cast<float>(h) * 2.3283064e-10f;
+		x[i] = xi;
+		v = static_cast<double>(xi) * xi;
+	}
+	wg_partial<true>(v, partial);
+}
+
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_power_step(int nb, const int32_t* __restrict__ slice_ptr,
+                                                       const int32_t* __restrict__ col, const float* __restrict__ val,
+                                                       const float* __restrict__ dinv, const float* __restrict__ x,
+                                                       const double* __restrict__ scal, float* __restrict__ y,
+                                                       double* __restrict__ partial) {
+	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	double v = 0;
+	if (i < nb) {
+		float const sc = static_cast<float>(scal[kScalScale]);
+		int const r = i & 63;
+		float ax[BS];
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+			ax[a] = 0;
+		}
+		for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
+			int32_t const c = col[blk];
+			float A[BS][BS];
+			ld_block<BS>(val, blk, r, A);
+#pragma unroll
+			for (int a = 0; a < BS; a++) {
+#pragma unroll
+				for (int k = 0; k < BS; k++) {
+					ax[a] += A[a][k] * x[BS * static_cast<int64_t>(c) + k];
+				}
+			}
+		}
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+			float z = 0;
+#pragma unroll
+			for (int k = 0; k < BS; k++) {
+				z += dinv[BS * BS * static_cast<int64_t>(i) + BS * a + k] * ax[k];
+			}
+			z *= sc;
+			y[BS * static_cast<int64_t>(i) + a] = z;
+			v += static_cast<double>(z) * z;
+		}
+	}
+	wg_partial<true>(v, partial);
+}
+
+// ||y||^2 = sum of the nwg partials: scal[kScalLambda] = ||y|| (the
+// estimate, when y came from a unit x), scal[kScalScale] = 1 / ||y||.
+__global__ __launch_bounds__(kBlock) void k_power_norm(int nwg, const double* __restrict__ partial,
+                                                       double* __restrict__ scal) {
+	__shared__ double lds[kBlock / 64];
+	double m = 0;
+	for (int i = threadIdx.x; i < nwg; i += kBlock) {
+		m += partial[i];
+	}
+	m = wave_sum(m);
+	if ((threadIdx.x & 63) == 0) {
+		lds[threadIdx.x >> 6] = m;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double t = 0;
+		for (int w = 0; w < kBlock / 64; w++) {
+			t += lds[w];
+		}
+		double const nrm = sqrt(t);
+		scal[kScalLambda] = nrm;
+		scal[kScalScale] = nrm > 0 ? 1.0 / nrm : 0.0;
+	}
+}
+
 // lambda_max = max of the nwg partials; Chebyshev coefficients for the
 // interval [lambda_max / ratio, lambda_max]: scal[0] = lambda_max and step k
 // updates d = scal[1 + 2k] d + scal[2 + 2k] D^-1 r, x += d (step 0 has no d
@@ -375,7 +600,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_gersh(int nb, const int32_t* __
 // rho_k = 1 / (2 sigma - rho_{k-1}), d_k = rho_k rho_{k-1} d_{k-1}
 // + (2 rho_k / delta) D^-1 r_k, d_0 = D^-1 r_0 / theta.
 __global__ __launch_bounds__(kBlock) void k_cheb_coef(int nwg, const double* __restrict__ partial, double ratio,
-                                                      int degree, double* __restrict__ scal) {
+                                                      int degree, int use_power, double* __restrict__ scal) {
 	__shared__ double lds[kBlock / 64];
 	double m = 0;
 	for (int i = threadIdx.x; i < nwg; i += kBlock) {
@@ -390,6 +615,13 @@ __global__ __launch_bounds__(kBlock) void k_cheb_coef(int nwg, const double* __r
 		double lmax = 0;
 		for (int w = 0; w < kBlock / 64; w++) {
 			lmax = fmax(lmax, lds[w]);
+		}
+		// the smoother's interval ends at the Gershgorin bound, a guaranteed
+		// upper bound: below lambda_max the Chebyshev polynomial would amplify
+		// the modes above its interval. The power estimate (a lower bound)
+		// only sets the prolongator damping.
+		if (!use_power) {
+			scal[kScalLambda] = lmax;
 		}
 		double const lmin = lmax / ratio;
 		double const theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
@@ -581,84 +813,121 @@ __global__ __launch_bounds__(kBlock) void k_level_resid(int nb, const int32_t* _
 	}
 }
 
-// b_c[I] = sum over the members i of aggregate I of P_i^T r_i.
+// b_c[J] = sum over the entries of column J of P (row i, entry e) of
+// P_e^T r_i. kRestrictLanes lanes share a coarse row: lane l takes entries
+// l, l + kRestrictLanes, ..., combined by a fixed butterfly.
+constexpr int kRestrictLanes = 8;
+
 template <int BSF>
-__global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int32_t* __restrict__ agg_ptr,
-                                                     const int32_t* __restrict__ agg_nodes,
-                                                     const float* __restrict__ P, const float* __restrict__ r,
-                                                     float* __restrict__ bc, const Scalars* __restrict__ S) {
+__global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int32_t* __restrict__ rptr,
+                                                     const int32_t* __restrict__ rent,
+                                                     const int32_t* __restrict__ rrow, const float* __restrict__ P,
+                                                     const float* __restrict__ r, float* __restrict__ bc,
+                                                     const Scalars* __restrict__ S) {
 	if (S->done) {
 		return;
 	}
-	int const I = blockIdx.x * kBlock + threadIdx.x;
-	if (I >= nc) {
-		return;
-	}
+	int64_t const J = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kRestrictLanes;
+	int const lane = threadIdx.x % kRestrictLanes;
+	bool const live = J < nc;   // every lane stays for the butterfly
 	float acc[3] = {0, 0, 0};
-	for (int32_t t = agg_ptr[I]; t < agg_ptr[I + 1]; t++) {
-		int64_t const i = agg_nodes[t];
+	int32_t const t0 = live ? rptr[J] : 0, t1 = live ? rptr[J + 1] : 0;
+	for (int32_t t = t0 + lane; t < t1; t += kRestrictLanes) {
+		int64_t const i = rrow[t];
+		const float* p = P + BSF * 3 * static_cast<int64_t>(rent[t]);
 #pragma unroll
 		for (int k = 0; k < BSF; k++) {
 			float const rk = r[BSF * i + k];
 #pragma unroll
 			for (int c = 0; c < 3; c++) {
-				acc[c] += P[BSF * 3 * i + 3 * k + c] * rk;
+				acc[c] += p[3 * k + c] * rk;
 			}
 		}
 	}
 #pragma unroll
 	for (int c = 0; c < 3; c++) {
-		bc[3 * static_cast<int64_t>(I) + c] = acc[c];
+#pragma unroll
+		for (int off = kRestrictLanes / 2; off > 0; off >>= 1) {
+			acc[c] += __shfl_xor(acc[c], off, 64);
+		}
+	}
+	if (live && lane == 0) {
+#pragma unroll
+		for (int c = 0; c < 3; c++) {
+			bc[3 * J + c] = acc[c];
+		}
 	}
 }
 
-// x_i += P_i x_c[agg(i)].
+// x_i += sum over the entries e of row i of P_e x_c[pcol[e]].
 template <int BSF>
-__global__ __launch_bounds__(kBlock) void k_prolong(int nb, const int32_t* __restrict__ agg,
-                                                    const float* __restrict__ P, const float* __restrict__ xc,
-                                                    float* __restrict__ x, const Scalars* __restrict__ S) {
+__global__ __launch_bounds__(kBlock) void k_prolong(int nb, const int32_t* __restrict__ pptr,
+                                                    const int32_t* __restrict__ pcol, const float* __restrict__ P,
+                                                    const float* __restrict__ xc, float* __restrict__ x,
+                                                    const Scalars* __restrict__ S) {
 	if (S->done) {
 		return;
 	}
-	int const i = blockIdx.x * kBlock + threadIdx.x;
+	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
 	if (i >= nb) {
 		return;
 	}
-	int64_t const I = agg[i];
-	float const e0 = xc[3 * I], e1 = xc[3 * I + 1], e2 = xc[3 * I + 2];
-	const float* p = P + BSF * 3 * static_cast<int64_t>(i);
+	float acc[BSF];
 #pragma unroll
 	for (int k = 0; k < BSF; k++) {
-		x[BSF * static_cast<int64_t>(i) + k] += p[3 * k] * e0 + p[3 * k + 1] * e1 + p[3 * k + 2] * e2;
+		acc[k] = x[BSF * static_cast<int64_t>(i) + k];
+	}
+	for (int32_t e = pptr[i]; e < pptr[i + 1]; e++) {
+		int64_t const J = pcol[e];
+		float const e0 = xc[3 * J], e1 = xc[3 * J + 1], e2 = xc[3 * J + 2];
+		const float* p = P + BSF * 3 * static_cast<int64_t>(e);
+#pragma unroll
+		for (int k = 0; k < BSF; k++) {
+			acc[k] += p[3 * k] * e0 + p[3 * k + 1] * e1 + p[3 * k + 2] * e2;
+		}
+	}
+#pragma unroll
+	for (int k = 0; k < BSF; k++) {
+		x[BSF * static_cast<int64_t>(i) + k] = acc[k];
 	}
 }
 
-// x = A^-1 b with the dense inverse (symmetric: read by columns, coalesced);
-// with DOT, the per-workgroup sums of x . q.
-template <typename BT, bool DOT>
+// x = A^-1 b with the dense inverse, one wavefront per row (the inverse is
+// symmetric, so row i is read as column i: coalesced across the lanes).
+constexpr int kDenseRowsPerBlock = kBlock / 64;
+
+template <typename BT>
 __global__ __launch_bounds__(kBlock) void k_dense_solve(int n, const double* __restrict__ Ainv,
                                                         const BT* __restrict__ b, float* __restrict__ x,
-                                                        const double* __restrict__ q, double* __restrict__ partial,
                                                         const Scalars* __restrict__ S) {
 	if (S->done) {
 		return;
 	}
+	int const row = blockIdx.x * kDenseRowsPerBlock + (threadIdx.x >> 6);
+	int const lane = threadIdx.x & 63;
+	if (row >= n) {
+		return;   // whole wavefronts leave together
+	}
+	double acc = 0;
+	for (int j = lane; j < n; j += 64) {
+		acc += Ainv[static_cast<int64_t>(row) * n + j] * static_cast<double>(b[j]);
+	}
+	acc = wave_sum(acc);
+	if (lane == 0) {
+		x[row] = static_cast<float>(acc);
+	}
+}
+
+// Per-workgroup partial sums of x . q over n entries (x FP32, q FP64).
+__global__ __launch_bounds__(kBlock) void k_dot_partials(int n, const float* __restrict__ x,
+                                                         const double* __restrict__ q, double* __restrict__ partial,
+                                                         const Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
 	int const i = blockIdx.x * kBlock + threadIdx.x;
-	double dot = 0;
-	if (i < n) {
-		double acc = 0;
-		for (int j = 0; j < n; j++) {
-			acc += Ainv[static_cast<int64_t>(j) * n + i] * static_cast<double>(b[j]);
-		}
-		float const xi = static_cast<float>(acc);
-		x[i] = xi;
-		if (DOT) {
-			dot = static_cast<double>(xi) * q[i];
-		}
-	}
-	if (DOT) {
-		wg_partial<true>(dot, partial);
-	}
+	double const v = i < n ? static_cast<double>(x[i]) * q[i] : 0.0;
+	wg_partial<true>(v, partial);
 }
 
 // ------------------------------------------------------------------ host topology
@@ -764,6 +1033,13 @@ int upload(T** d, const std::vector<T>& h, hipStream_t st) {
 	return BFMX_OK;
 }
 
+// Smoothed aggregation unless the environment sets BFMX_AMG_SMOOTH=0
+// (unsmoothed aggregation, P = P_t: for comparisons).
+bool amg_smoothing() {
+	const char* e = std::getenv("BFMX_AMG_SMOOTH");
+	return !(e && e[0] == '0');
+}
+
 bool dense_coarsest(const AmgLevel& L) {
 	return L.bs * static_cast<int64_t>(L.nb) <= kDenseMax;
 }
@@ -785,13 +1061,20 @@ void amg_release(bfmx_system* s) {
 		dfree(L.d_B);
 		dfree(L.d_B0);
 		dfree(L.d_agg);
-		dfree(L.d_P);
+		dfree(L.d_Pt);
 		dfree(L.d_agg_ptr);
 		dfree(L.d_agg_nodes);
+		dfree(L.d_pptr);
+		dfree(L.d_pcol);
+		dfree(L.d_P);
+		dfree(L.d_ppos);
+		dfree(L.d_rptr);
+		dfree(L.d_rent);
+		dfree(L.d_rrow);
 		dfree(L.d_gal_ptr);
 		dfree(L.d_gal_blk);
-		dfree(L.d_gal_i);
-		dfree(L.d_gal_j);
+		dfree(L.d_gal_pi);
+		dfree(L.d_gal_pj);
 		dfree(L.d_dense);
 		dfree(L.d_b);
 		dfree(L.d_x[0]);
@@ -852,16 +1135,39 @@ int amg_build(bfmx_system* s) {
 			BFMX_CHECK(upload(&L0.d_B0, B0, st));
 			BFMX_HIP(hipStreamSynchronize(st));
 		}
-		// graph of the current finest-so-far level and the block slot of each entry
+		// graph of the current finest-so-far level, the block slot of each
+		// entry and the level's slice pointers
+		bool const smooth = amg_smoothing();
 		std::vector<int64_t> fptr;
 		std::vector<int32_t> fadj;
 		std::vector<int32_t> fslot(s->nnzb);
+		std::vector<int32_t> fslice(s->slice_ptr);
 		for (int64_t i = 0; i < n0; i++) {
 			for (int64_t k = 0; k < s->adj_ptr[i + 1] - s->adj_ptr[i]; k++) {
 				fslot[s->adj_ptr[i] + k] = static_cast<int32_t>(s->slice_ptr[i >> 6] + k * kSlice + (i & 63));
 			}
 		}
-		while (static_cast<int>(lv.size()) < kMaxLevels && lv.back().nb > kCoarsest) {
+		// coarsen to at most par.coarsest block rows (the one-workgroup
+		// dense inverse of the coarsest operator costs O(n^3) per set-up:
+		// ~0.1 ms at 192 DOFs, ~40 ms at 500)
+		AmgParams& par = s->amg_par;
+		par = AmgParams{};
+		if (const char* e = std::getenv("BFMX_AMG_DEG")) {
+			par.deg = std::min(kMaxDeg, std::max(1, std::atoi(e)));
+		}
+		if (const char* e = std::getenv("BFMX_AMG_RATIO")) {
+			par.ratio = std::max(1.5, std::atof(e));
+		}
+		if (const char* e = std::getenv("BFMX_AMG_COARSEST")) {
+			par.coarsest = std::max(1, std::atoi(e));
+		}
+		if (const char* e = std::getenv("BFMX_AMG_OMEGA")) {
+			par.omega = std::atof(e);
+		}
+		if (const char* e = std::getenv("BFMX_AMG_POWER")) {
+			par.power = std::max(0, std::atoi(e));
+		}
+		while (static_cast<int>(lv.size()) < kMaxLevels && lv.back().nb > par.coarsest) {
 			int const lf = static_cast<int>(lv.size()) - 1;
 			const int64_t* ptr = lf == 0 ? s->adj_ptr.data() : fptr.data();
 			const int32_t* adj = lf == 0 ? s->adj.data() : fadj.data();
@@ -885,17 +1191,88 @@ int amg_build(bfmx_system* s) {
 					anodes[fill[agg[i]]++] = i;
 				}
 			}
-			// coarse graph (sorted rows, including the diagonal)
+			// prolongator pattern: row i couples to the aggregates of its
+			// neighbours (smoothed) or to its own aggregate (unsmoothed)
+			std::vector<int32_t> pptr(nf + 1, 0), pcol;
+			pcol.reserve(static_cast<size_t>(smooth ? 3 : 1) * nf);
+			{
+				std::vector<int32_t> row;
+				for (int i = 0; i < nf; i++) {
+					row.clear();
+					if (smooth) {
+						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+							row.push_back(agg[adj[e]]);
+						}
+						std::sort(row.begin(), row.end());
+						row.erase(std::unique(row.begin(), row.end()), row.end());
+					} else {
+						row.push_back(agg[i]);
+					}
+					if (row.size() > 255) {
+						set_error("multigrid: a row couples to more than 255 aggregates");
+						return BFMX_E_UNSUPPORTED;
+					}
+					pcol.insert(pcol.end(), row.begin(), row.end());
+					pptr[i + 1] = static_cast<int32_t>(pcol.size());
+				}
+			}
+			if (pcol.size() >= (size_t(1) << 31) / 9) {
+				set_error("multigrid prolongator too large for 32-bit indices");
+				return BFMX_E_ARG;
+			}
+			auto entry_of = [&](int i, int32_t J) -> int32_t {   // entry of (i, J) in P; J must be in row i
+				auto const beg = pcol.begin() + pptr[i], end = pcol.begin() + pptr[i + 1];
+				return static_cast<int32_t>(std::lower_bound(beg, end, J) - pcol.begin());
+			};
+			// per fine block slot, the entry (within its row) of its column's
+			// aggregate; padding slots and the unsmoothed case point at the
+			// row's own aggregate
+			std::vector<uint8_t> ppos(fslice.back(), 0);
+			for (int64_t sl = 0; sl + 1 < static_cast<int64_t>(fslice.size()); sl++) {
+				for (int32_t slot = fslice[sl]; slot < fslice[sl + 1]; slot++) {
+					int64_t const row = sl * kSlice + (slot & 63);
+					if (row < nf) {
+						ppos[slot] = static_cast<uint8_t>(entry_of(static_cast<int>(row), agg[row]) - pptr[row]);
+					}
+				}
+			}
+			if (smooth) {
+				for (int i = 0; i < nf; i++) {
+					for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+						ppos[fslot[e]] = static_cast<uint8_t>(entry_of(i, agg[adj[e]]) - pptr[i]);
+					}
+				}
+			}
+			// columns of P (restriction), entries in row order
+			std::vector<int32_t> rptr(na + 1, 0), rent(pcol.size()), rrow(pcol.size());
+			for (int32_t J : pcol) {
+				rptr[J + 1]++;
+			}
+			for (int I = 0; I < na; I++) {
+				rptr[I + 1] += rptr[I];
+			}
+			{
+				std::vector<int32_t> fill(rptr.begin(), rptr.end() - 1);
+				for (int i = 0; i < nf; i++) {
+					for (int32_t e = pptr[i]; e < pptr[i + 1]; e++) {
+						int32_t const at = fill[pcol[e]]++;
+						rent[at] = e;
+						rrow[at] = i;
+					}
+				}
+			}
+			// coarse graph, the pattern of P^T A P (sorted rows, with the diagonal)
 			std::vector<int64_t> cptr(na + 1, 0);
 			std::vector<int32_t> cadj;
 			{
 				std::vector<int32_t> row;
 				for (int I = 0; I < na; I++) {
 					row.clear();
-					for (int32_t t = aptr[I]; t < aptr[I + 1]; t++) {
-						int const i = anodes[t];
+					for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
+						int const i = rrow[t];
 						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
-							row.push_back(agg[adj[e]]);
+							int const j = adj[e];
+							row.insert(row.end(), pcol.begin() + pptr[j], pcol.begin() + pptr[j + 1]);
 						}
 					}
 					std::sort(row.begin(), row.end());
@@ -906,44 +1283,70 @@ int amg_build(bfmx_system* s) {
 			}
 			Sell CS;
 			BFMX_CHECK(build_sell(na, cptr, cadj, CS));
-			// Galerkin gather lists, grouped by coarse slot, in fine row order
-			int64_t const nfe = ptr[nf];
-			std::vector<int32_t> cslot(nfe);
+			// Galerkin gather lists: every (fine block A_ij, entry P_iI, entry
+			// P_jJ) grouped by the coarse slot (I, J), in fine row order
+			auto coarse_slot = [&](int32_t I, int32_t J) -> int32_t {
+				auto const beg = cadj.begin() + cptr[I], end = cadj.begin() + cptr[I + 1];
+				return CS.slot_of[std::lower_bound(beg, end, J) - cadj.begin()];
+			};
 			std::vector<int32_t> gptr(CS.nnzb_pad + 1, 0);
-			for (int i = 0; i < nf; i++) {
-				int const I = agg[i];
-				for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
-					int32_t const J = agg[adj[e]];
-					auto const beg = cadj.begin() + cptr[I], end = cadj.begin() + cptr[I + 1];
-					int64_t const k = std::lower_bound(beg, end, J) - cadj.begin();
-					cslot[e] = CS.slot_of[k];
-					gptr[cslot[e] + 1]++;
+			int64_t nterms = 0;
+			for (int I = 0; I < na; I++) {
+				for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
+					int const i = rrow[t];
+					for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+						int const j = adj[e];
+						for (int32_t f = pptr[j]; f < pptr[j + 1]; f++) {
+							gptr[coarse_slot(I, pcol[f]) + 1]++;
+							nterms++;
+						}
+					}
 				}
+			}
+			if (nterms >= (int64_t(1) << 31)) {
+				set_error("multigrid Galerkin product too large for 32-bit indices");
+				return BFMX_E_ARG;
 			}
 			for (int64_t b = 0; b < CS.nnzb_pad; b++) {
 				gptr[b + 1] += gptr[b];
 			}
-			std::vector<int32_t> gblk(nfe), gi(nfe), gj(nfe);
+			std::vector<int32_t> gblk(nterms), gpi(nterms), gpj(nterms);
 			{
 				std::vector<int32_t> fill(gptr.begin(), gptr.end() - 1);
-				for (int i = 0; i < nf; i++) {
-					for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
-						int32_t const at = fill[cslot[e]]++;
-						gblk[at] = fslot[e];
-						gi[at] = i;
-						gj[at] = adj[e];
+				for (int I = 0; I < na; I++) {
+					for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
+						int const i = rrow[t];
+						int32_t const pi = rent[t];
+						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+							int const j = adj[e];
+							for (int32_t f = pptr[j]; f < pptr[j + 1]; f++) {
+								int32_t const at = fill[coarse_slot(I, pcol[f])]++;
+								gblk[at] = fslot[e];
+								gpi[at] = pi;
+								gpj[at] = f;
+							}
+						}
 					}
 				}
 			}
 			AmgLevel& F = lv[lf];
+			F.sa = smooth;
+			F.nnzp = static_cast<int64_t>(pcol.size());
 			BFMX_CHECK(upload(&F.d_agg, agg, st));
-			BFMX_CHECK(dalloc(&F.d_P, static_cast<size_t>(F.bs) * 3 * nf));
+			BFMX_CHECK(dalloc(&F.d_Pt, static_cast<size_t>(F.bs) * 3 * nf));
 			BFMX_CHECK(upload(&F.d_agg_ptr, aptr, st));
 			BFMX_CHECK(upload(&F.d_agg_nodes, anodes, st));
+			BFMX_CHECK(upload(&F.d_pptr, pptr, st));
+			BFMX_CHECK(upload(&F.d_pcol, pcol, st));
+			BFMX_CHECK(dalloc(&F.d_P, static_cast<size_t>(F.bs) * 3 * pcol.size()));
+			BFMX_CHECK(upload(&F.d_ppos, ppos, st));
+			BFMX_CHECK(upload(&F.d_rptr, rptr, st));
+			BFMX_CHECK(upload(&F.d_rent, rent, st));
+			BFMX_CHECK(upload(&F.d_rrow, rrow, st));
 			BFMX_CHECK(upload(&F.d_gal_ptr, gptr, st));
 			BFMX_CHECK(upload(&F.d_gal_blk, gblk, st));
-			BFMX_CHECK(upload(&F.d_gal_i, gi, st));
-			BFMX_CHECK(upload(&F.d_gal_j, gj, st));
+			BFMX_CHECK(upload(&F.d_gal_pi, gpi, st));
+			BFMX_CHECK(upload(&F.d_gal_pj, gpj, st));
 			lv.emplace_back();   // its buffers are released with the hierarchy, also on failure
 			AmgLevel& C = lv.back();
 			C.nb = na;
@@ -960,6 +1363,7 @@ int amg_build(bfmx_system* s) {
 			fptr.swap(cptr);
 			fadj.swap(cadj);
 			fslot.swap(CS.slot_of);
+			fslice.swap(CS.slice_ptr);
 		}
 		for (auto& L : lv) {
 			size_t const m = static_cast<size_t>(L.bs) * L.nb;
@@ -969,8 +1373,8 @@ int amg_build(bfmx_system* s) {
 			BFMX_CHECK(dalloc(&L.d_d, m));
 			BFMX_CHECK(dalloc(&L.d_r, m));
 			BFMX_CHECK(dalloc(&L.d_dinv, static_cast<size_t>(L.bs) * L.bs * L.nb));
-			BFMX_CHECK(dalloc(&L.d_scal, 2 + 2 * kMaxDeg));
-			BFMX_CHECK(dalloc(&L.d_partial, grid(L.nb)));
+			BFMX_CHECK(dalloc(&L.d_scal, kScalSize));
+			BFMX_CHECK(dalloc(&L.d_partial, grid(L.nb) + grid(static_cast<int64_t>(L.bs) * L.nb)));
 			BFMX_CHECK(dalloc(&L.d_B, 3 * m));
 		}
 		if (dense_coarsest(lv.back())) {
@@ -986,25 +1390,13 @@ int amg_build(bfmx_system* s) {
 }
 
 // Modes, prolongators, Galerkin operators, smoother data and the coarsest
-// inverse, from the current FP64 matrix and Dirichlet set.
+// inverse, from the current FP64 matrix and Dirichlet set, level by level:
+// a level's smoother data (inverse diagonal blocks, lambda_max) come first,
+// since its prolongator smoothing uses them.
 int amg_setup_numeric(bfmx_system* s) {
 	hipStream_t const st = s->stream;
 	std::vector<AmgLevel>& lv = s->amg;
 	k_fine_modes<<<grid(lv[0].nb), kBlock, 0, st>>>(lv[0].nb, lv[0].d_B0, s->d_fixed, lv[0].d_B);
-	for (size_t l = 0; l + 1 < lv.size(); l++) {
-		AmgLevel& F = lv[l];
-		AmgLevel& C = lv[l + 1];
-		int64_t const ns = C.nnzb_pad;
-		if (F.bs == 2) {
-			k_tentative<2><<<grid(C.nb), kBlock, 0, st>>>(C.nb, F.d_agg_ptr, F.d_agg_nodes, F.d_B, F.d_P, C.d_B);
-			k_galerkin<2><<<grid(ns), kBlock, 0, st>>>(ns, F.d_gal_ptr, F.d_gal_blk, F.d_gal_i, F.d_gal_j, F.d_val64,
-			                                          F.d_P, C.d_val64, C.d_val32);
-		} else {
-			k_tentative<3><<<grid(C.nb), kBlock, 0, st>>>(C.nb, F.d_agg_ptr, F.d_agg_nodes, F.d_B, F.d_P, C.d_B);
-			k_galerkin<3><<<grid(ns), kBlock, 0, st>>>(ns, F.d_gal_ptr, F.d_gal_blk, F.d_gal_i, F.d_gal_j, F.d_val64,
-			                                          F.d_P, C.d_val64, C.d_val32);
-		}
-	}
 	for (size_t l = 0; l < lv.size(); l++) {
 		AmgLevel& L = lv[l];
 		bool const coarsest = l + 1 == lv.size();
@@ -1024,8 +1416,47 @@ int amg_setup_numeric(bfmx_system* s) {
 		} else {
 			k_diag_gersh<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_diag_blk, L.d_val64, L.d_dinv, L.d_partial);
 		}
-		k_cheb_coef<<<1, kBlock, 0, st>>>(g, L.d_partial, coarsest ? kCoarseRatio : kSmoothRatio,
-		                                  coarsest ? kCoarseDeg : kSmoothDeg, L.d_scal);
+		int const power = coarsest ? 0 : s->amg_par.power;
+		if (power > 0) {
+			// the Gershgorin maxima stay in d_partial's first g entries; the
+			// power sums use the entries after them
+			double* const pp = L.d_partial + g;
+			int64_t const m = static_cast<int64_t>(L.bs) * L.nb;
+			k_power_init<<<grid(m), kBlock, 0, st>>>(m, L.d_x[0], pp);
+			k_power_norm<<<1, kBlock, 0, st>>>(grid(m), pp, L.d_scal);
+			for (int k = 0; k < power; k++) {
+				float* const xin = L.d_x[k & 1];
+				float* const yout = L.d_x[(k & 1) ^ 1];
+				if (L.bs == 2) {
+					k_power_step<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, L.d_dinv, xin,
+					                                      L.d_scal, yout, pp);
+				} else {
+					k_power_step<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, L.d_dinv, xin,
+					                                      L.d_scal, yout, pp);
+				}
+				k_power_norm<<<1, kBlock, 0, st>>>(g, pp, L.d_scal);
+			}
+		}
+		k_cheb_coef<<<1, kBlock, 0, st>>>(g, L.d_partial, coarsest ? kCoarseRatio : s->amg_par.ratio,
+		                                  coarsest ? kCoarseDeg : s->amg_par.deg, power > 0, L.d_scal);
+		if (coarsest) {
+			continue;
+		}
+		AmgLevel& C = lv[l + 1];
+		int64_t const ns = C.nnzb_pad;
+		if (L.bs == 2) {
+			k_tentative<2><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
+			k_smooth_p<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_diag_blk, L.d_Pt,
+			                                    L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0, L.d_P);
+			k_galerkin<2><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_ptr, L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
+			                                          L.d_val64, L.d_P, C.d_val64, C.d_val32);
+		} else {
+			k_tentative<3><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
+			k_smooth_p<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_diag_blk, L.d_Pt,
+			                                    L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0, L.d_P);
+			k_galerkin<3><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_ptr, L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
+			                                          L.d_val64, L.d_P, C.d_val64, C.d_val32);
+		}
 	}
 	BFMX_HIP(hipGetLastError());
 	return BFMX_OK;
@@ -1059,14 +1490,14 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 	bool const coarsest = l + 1 == s->amg.size();
 	if (coarsest && L.d_dense) {
 		int const n = BS * L.nb;
+		int const g = (n + kDenseRowsPerBlock - 1) / kDenseRowsPerBlock;
+		k_dense_solve<BT><<<g, kBlock, 0, st>>>(n, L.d_dense, b, L.d_x[0], S);
 		if (q) {
-			k_dense_solve<BT, true><<<grid(n), kBlock, 0, st>>>(n, L.d_dense, b, L.d_x[0], q, partial, S);
-		} else {
-			k_dense_solve<BT, false><<<grid(n), kBlock, 0, st>>>(n, L.d_dense, b, L.d_x[0], nullptr, nullptr, S);
+			k_dot_partials<<<grid(n), kBlock, 0, st>>>(n, L.d_x[0], q, partial, S);
 		}
 		return L.d_x[0];
 	}
-	int const deg = coarsest ? kCoarseDeg : kSmoothDeg;
+	int const deg = coarsest ? kCoarseDeg : s->amg_par.deg;
 	int cur = 0;
 	cheb<BS, BT>(st, L, b, true, 0, nullptr, L.d_x[0], nullptr, nullptr, S);
 	if (coarsest) {
@@ -1085,9 +1516,10 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 	AmgLevel& C = s->amg[l + 1];
 	int const g = grid(L.nb);
 	k_level_resid<BS, BT><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_x[cur], L.d_r, S);
-	k_restrict<BS><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_P, L.d_r, C.d_b, S);
+	k_restrict<BS><<<grid(static_cast<int64_t>(C.nb) * kRestrictLanes), kBlock, 0, st>>>(
+	    C.nb, L.d_rptr, L.d_rent, L.d_rrow, L.d_P, L.d_r, C.d_b, S);
 	float* xc = vcycle<3, float>(s, l + 1, C.d_b, nullptr, nullptr, S);
-	k_prolong<BS><<<g, kBlock, 0, st>>>(L.nb, L.d_agg, L.d_P, xc, L.d_x[cur], S);
+	k_prolong<BS><<<g, kBlock, 0, st>>>(L.nb, L.d_pptr, L.d_pcol, L.d_P, xc, L.d_x[cur], S);
 	for (int k = 0; k < deg; k++) {
 		bool const last = k + 1 == deg;
 		cheb<BS, BT>(st, L, b, false, k, L.d_x[cur], L.d_x[cur ^ 1], last ? q : nullptr, last ? partial : nullptr, S);

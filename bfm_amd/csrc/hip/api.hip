@@ -201,6 +201,10 @@ int bfmx_system_set_edges(bfmx_system_t* s, size_t n_edges, const size_t* edge_n
 int bfmx_system_assemble(bfmx_system_t* s, int stress, double E, double nu, double rho, const double* elem_E,
                          const double* node_force, double gx, double gy) {
 	BFMX_CHECK(check_owned(s));
+	if (stress < BFMX_PLANE_STRAIN || stress > BFMX_AXISYMMETRIC) {
+		set_error("formulation %d unknown (0 plane strain, 1 plane stress, 2 axisymmetric)", stress);
+		return BFMX_E_ARG;
+	}
 	return guarded([&]() -> int {
 		BFMX_CHECK(assemble(s, stress, E, nu, rho, elem_E, node_force, gx, gy));
 		BFMX_HIP(hipStreamSynchronize(s->stream));

@@ -58,7 +58,7 @@ __global__ __launch_bounds__(kBlock) void k_assemble(int64_t e0, int64_t e1, con
 		return;
 	}
 	const int32_t* sl = slots + e * KIND * KIND;
-	dev::assemble_element<KIND>(e, el, xy, M, [&](int j, int k, const int*) { return sl[j * KIND + k]; }, val, b);
+	dev::assemble_any<KIND>(e, el, xy, M, [&](int j, int k, const int*) { return sl[j * KIND + k]; }, val, b);
 }
 
 __global__ __launch_bounds__(kBlock) void k_dirichlet_xy(int cnt, const int32_t* __restrict__ nodes, int shift,
@@ -88,10 +88,10 @@ __global__ __launch_bounds__(kBlock) void k_neumann(int cnt, const int32_t* __re
                                                     const int64_t* __restrict__ inc_ptr,
                                                     const int32_t* __restrict__ inc,
                                                     const int32_t* __restrict__ edge_nodes,
-                                                    const double* __restrict__ xy, double* __restrict__ b) {
+                                                    const double* __restrict__ xy, int axi, double* __restrict__ b) {
 	int const t = blockIdx.x * kBlock + threadIdx.x;
 	if (t < cnt) {
-		dev::neumann(nodes[t], kind, v, flags, inc_ptr, inc, edge_nodes, xy, b);
+		dev::neumann(nodes[t], kind, v, flags, inc_ptr, inc, edge_nodes, xy, axi != 0, b);
 	}
 }
 
@@ -198,6 +198,9 @@ int apply_condition(bfmx_system* s, int kind, double value, const uint8_t* flags
 	}
 	bool const neumann = kind >= BFMX_NEUMANN_X && kind <= BFMX_NEUMANN_TANGENT;
 	bool const nt = kind == BFMX_DIRICHLET_NORMAL || kind == BFMX_DIRICHLET_TANGENT;
+	if (s->stress == 2 && (kind == BFMX_NEUMANN_NORMAL || kind == BFMX_NEUMANN_TANGENT)) {
+		return BFMX_OK;   // the axisymmetric condition loop has no such branch (system.c:575-619)
+	}
 	if ((neumann || nt) && s->inc_ptr.empty()) {
 		// no edge list: Neumann adds nothing, tangents are zero
 		if (neumann) {
@@ -236,7 +239,7 @@ int apply_condition(bfmx_system* s, int kind, double value, const uint8_t* flags
 	} else {
 		BFMX_HIP(hipMemcpyAsync(s->d_flags, fl.data(), n, hipMemcpyHostToDevice, st));
 		k_neumann<<<grid(cnt), kBlock, 0, st>>>(cnt, s->d_list, kind, value, s->d_flags, s->d_inc_ptr, s->d_inc,
-		                                        s->d_edge_nodes, s->d_coords, s->d_b);
+		                                        s->d_edge_nodes, s->d_coords, s->stress == 2, s->d_b);
 	}
 	BFMX_HIP(hipGetLastError());
 	if (!neumann) {

@@ -107,7 +107,7 @@ __device__ __forceinline__ int32_t find_slot(const int32_t* __restrict__ slice_p
 struct Material {
 	const double* elemE;   // per element (colour order) or null
 	double E, nu, rho;
-	int stress;            // 1 plane stress, 0 plane strain
+	int stress;            // 0 plane strain, 1 plane stress, 2 axisymmetric
 	const double* nforce;  // per node body acceleration (2 per node) or null
 	double gx, gy;         // constant body acceleration when nforce is null
 };
@@ -205,6 +205,114 @@ __device__ __forceinline__ void assemble_element(int64_t e, const int32_t* __res
 	}
 }
 
+// Axisymmetric element (fill_axisymmetric_elem, system.c:228-356), r = x:
+// the reference's constants (a multiplied by 1 - 2 nu where plane strain
+// divides, system.c:241-243) and its f_11 .. f_22 (f_22's second term without
+// r, system.c:339), Gauss-point sums in registers, upper triangle mirrored
+// (f_21(k, j) equals f_12(j, k)); body load rho f phi_j r.
+template <int KIND, typename Slot>
+__device__ __forceinline__ void assemble_element_axi(int64_t e, const int32_t* __restrict__ el,
+                                                     const double* __restrict__ xy, const Material& M, Slot slot,
+                                                     double* __restrict__ val, double* __restrict__ b) {
+#pragma clang fp contract(on)
+	using R = Rule<KIND>;
+	int n[KIND];
+	double x[KIND], y[KIND];
+#pragma unroll
+	for (int j = 0; j < KIND; j++) {
+		n[j] = el[e * KIND + j];
+		double2 const p = reinterpret_cast<const double2*>(xy)[n[j]];
+		x[j] = p.x;
+		y[j] = p.y;
+	}
+	double const E = M.elemE ? M.elemE[e] : M.E;
+	double const nu = M.nu;
+	double const a = E * (1 - nu) / (1 + nu) * (1 - 2 * nu);
+	double const bb = E * nu / (1 + nu) / (1 - 2 * nu);
+	double const c = E / (2 * (1 + nu));
+
+	double gxs[R::NG][KIND], gys[R::NG][KIND], phis[R::NG][KIND], djw[R::NG], rg[R::NG];
+#pragma unroll
+	for (int g = 0; g < R::NG; g++) {
+		double phi[KIND], dxs[KIND], det[KIND];
+		R::shape(R::xi(g), R::eta(g), phi, dxs, det);
+		double dx_dxsi = 0, dx_deta = 0, dy_dxsi = 0, dy_deta = 0, r = 0;
+#pragma unroll
+		for (int j = 0; j < KIND; j++) {
+			dx_dxsi += x[j] * dxs[j];
+			dx_deta += x[j] * det[j];
+			dy_dxsi += y[j] * dxs[j];
+			dy_deta += y[j] * det[j];
+			r += x[j] * phi[j];
+		}
+		double const det_J = fabs(dx_dxsi * dy_deta - dx_deta * dy_dxsi);
+#pragma unroll
+		for (int j = 0; j < KIND; j++) {
+			gxs[g][j] = (dxs[j] * dy_deta - det[j] * dy_dxsi) / det_J;
+			gys[g][j] = (det[j] * dx_dxsi - dxs[j] * dx_deta) / det_J;
+			phis[g][j] = phi[j];
+		}
+		djw[g] = det_J * R::w(g);
+		rg[g] = r;
+	}
+
+#pragma unroll
+	for (int j = 0; j < KIND; j++) {
+		double fx = M.gx, fy = M.gy;
+		if (M.nforce) {
+			fx = M.nforce[2 * n[j]];
+			fy = M.nforce[2 * n[j] + 1];
+		}
+		double bx = 0, by = 0;
+#pragma unroll
+		for (int g = 0; g < R::NG; g++) {
+			bx += djw[g] * fx * M.rho * phis[g][j] * rg[g];
+			by += djw[g] * fy * M.rho * phis[g][j] * rg[g];
+		}
+		double2* bp = reinterpret_cast<double2*>(b) + n[j];
+		double2 v = *bp;
+		v.x += bx;
+		v.y += by;
+		*bp = v;
+	}
+
+#pragma unroll
+	for (int j = 0; j < KIND; j++) {
+#pragma unroll
+		for (int k = j; k < KIND; k++) {
+			double s11 = 0, s12 = 0, s21 = 0, s22 = 0;
+#pragma unroll
+			for (int g = 0; g < R::NG; g++) {
+				double const xj = gxs[g][j], yj = gys[g][j], xk = gxs[g][k], yk = gys[g][k];
+				double const pj = phis[g][j], pk = phis[g][k], r = rg[g];
+				s11 += djw[g] * (a * xj * xk * r + c * yj * yk * r + pj * (bb * xk + a * pk / r) + xj * bb * pk);
+				s12 += djw[g] * (bb * xj * yk * r + c * yj * xk * r + pj * bb * yk);
+				s21 += djw[g] * (bb * yj * xk * r + c * xj * yk * r + yj * bb * pk);
+				s22 += djw[g] * (a * yj * yk * r + c * xj * xk);
+			}
+			if (j == k) {
+				s21 = s12;
+			}
+			add_block(val, slot(j, k, n), n[j] & 63, s11, s12, s21, s22);
+			if (k != j) {
+				add_block(val, slot(k, j, n), n[k] & 63, s11, s21, s12, s22);
+			}
+		}
+	}
+}
+
+// Element e with the material's formulation.
+template <int KIND, typename Slot>
+__device__ __forceinline__ void assemble_any(int64_t e, const int32_t* __restrict__ el, const double* __restrict__ xy,
+                                             const Material& M, Slot slot, double* __restrict__ val,
+                                             double* __restrict__ b) {
+	if (M.stress == 2) {
+		assemble_element_axi<KIND>(e, el, xy, M, slot, val, b);
+	} else {
+		assemble_element<KIND>(e, el, xy, M, slot, val, b);
+	}
+}
+
 // Dirichlet X or Y on node i: b[d] = v; the first value of each DOF is kept
 // for the symmetric elimination.
 __device__ __forceinline__ void dirichlet_xy(int32_t i, int shift, double v, uint8_t* __restrict__ fixed,
@@ -255,12 +363,15 @@ __device__ __forceinline__ void dirichlet_nt(int32_t i, int tangent, double v, c
 }
 
 // Neumann loads of one condition on flagged node i (system.c:477-522): the
-// incident flagged edges' loads in edge order.
+// incident flagged edges' loads in edge order. Axisymmetric (axi,
+// system.c:584-617): X/Y loads carry the factor r1 + r2, each r formed as
+// the reference forms it from one node's x and y; normal/tangent loads are
+// not applied (the caller skips them).
 template <typename IP>
 __device__ __forceinline__ void neumann(int32_t i, int kind, double v, const uint8_t* __restrict__ flags,
                                         const IP* __restrict__ inc_ptr, const int32_t* __restrict__ inc,
                                         const int32_t* __restrict__ edge_nodes, const double* __restrict__ xy,
-                                        double* __restrict__ b) {
+                                        bool axi, double* __restrict__ b) {
 #pragma clang fp contract(on)
 	double bx = b[2 * i], by = b[2 * i + 1];
 	for (int64_t q = inc_ptr[i]; q < inc_ptr[i + 1]; q++) {
@@ -273,10 +384,17 @@ __device__ __forceinline__ void neumann(int32_t i, int kind, double v, const uin
 		double const dy = xy[2 * n1 + 1] - xy[2 * n2 + 1];
 		if (kind == BFMX_NEUMANN_X || kind == BFMX_NEUMANN_Y) {
 			double const jac = sqrt(dx * dx + dy * dy) / 2;
+			double load = jac * v;
+			if (axi) {
+				double const s3 = 1 / sqrt(3.0);
+				double const r1 = xy[2 * n1] * (1 - s3) / 2 + xy[2 * n1 + 1] * (1 + s3) / 2;
+				double const r2 = xy[2 * n2] * (1 - s3) / 2 + xy[2 * n2 + 1] * (1 + s3) / 2;
+				load = (r1 + r2) * jac * v;
+			}
 			if (kind == BFMX_NEUMANN_X) {
-				bx += jac * v;
+				bx += load;
 			} else {
-				by += jac * v;
+				by += load;
 			}
 		} else {
 			bool const tan = kind == BFMX_NEUMANN_TANGENT;

@@ -198,7 +198,7 @@ __device__ void assemble_elements(const BatchInst& S) {
 	for (int c = 0; c < S.n_colors; c++) {
 		int32_t const e1 = S.color_ptr[c + 1];
 		for (int64_t e = S.color_ptr[c] + threadIdx.x; e < e1; e += BS) {
-			dev::assemble_element<KIND>(
+			dev::assemble_any<KIND>(
 			    e, S.elem, S.coords, S.M,
 			    [&](int j, int k, const int* nd) { return dev::find_slot(S.slice_ptr, S.row_len, S.col, nd[j], nd[k]); },
 			    S.val, S.b);
@@ -236,6 +236,7 @@ __device__ void assemble_system(const BatchInst& S) {
 				                  S.edge_bnd, S.coords, S.fixed, S.vfirst, S.b);
 			} else {
 				dev::neumann(i, C.kind, C.value, S.nflags + C.flags_off, S.inc_ptr, S.inc, S.edge_nodes, S.coords,
+				             S.M.stress == 2,
 				             S.b);
 			}
 		}
@@ -829,7 +830,11 @@ int prepare_instance(const bfmx_instance_t& I, int renumber, int device, Prepare
 			P.elemE[p] = I.elem_E[s->elem_order[p]];
 		}
 	}
-	s->stress = I.stress ? 1 : 0;
+	if (I.stress < 0 || I.stress > 2) {
+		set_error("formulation %d unknown (0 plane strain, 1 plane stress, 2 axisymmetric)", I.stress);
+		return BFMX_E_ARG;
+	}
+	s->stress = I.stress;
 	s->dirichlet_nodes.assign(n, 0);
 	std::vector<uint8_t> fl(n);
 	for (size_t q = 0; q < I.n_conditions; q++) {
@@ -839,6 +844,9 @@ int prepare_instance(const bfmx_instance_t& I, int renumber, int device, Prepare
 			return BFMX_E_ARG;
 		}
 		bool const neumann = kind >= BFMX_NEUMANN_X && kind <= BFMX_NEUMANN_TANGENT;
+		if (s->stress == 2 && (kind == BFMX_NEUMANN_NORMAL || kind == BFMX_NEUMANN_TANGENT)) {
+			continue;   // the axisymmetric condition loop has no such branch
+		}
 		const uint8_t* flags = I.cond_flags + q * I.n_nodes;
 		std::fill(fl.begin(), fl.end(), 0);
 		for (int64_t i = 0; i < n; i++) {

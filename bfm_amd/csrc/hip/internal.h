@@ -92,14 +92,25 @@ struct AmgLevel {
 	double* d_B = nullptr;            // bs x 3 per row: rigid-body modes on this level
 	double* d_B0 = nullptr;           // level 0: the modes before fixed DOFs are zeroed
 	// transfer to the next coarser level (absent on the coarsest)
-	int32_t* d_agg = nullptr;         // next-level row of each row
-	float* d_P = nullptr;             // bs x 3 prolongator block per row
-	int32_t* d_agg_ptr = nullptr;     // members of each next-level row
+	int32_t* d_agg = nullptr;         // next-level row (aggregate) of each row
+	float* d_Pt = nullptr;            // tentative prolongator: bs x 3 block per row
+	int32_t* d_agg_ptr = nullptr;     // members of each aggregate
 	int32_t* d_agg_nodes = nullptr;
+	// prolongator P = (I - w D^-1 A) P_t, CSR of bs x 3 blocks over this
+	// level's rows (unsmoothed: one entry per row, P = P_t)
+	bool sa = true;                   // smoothed (false: P = P_t)
+	int64_t nnzp = 0;
+	int32_t* d_pptr = nullptr;        // nb + 1
+	int32_t* d_pcol = nullptr;        // next-level row of each entry
+	float* d_P = nullptr;             // bs x 3 per entry
+	uint8_t* d_ppos = nullptr;        // per block slot: entry (within its row) of its column's aggregate
+	int32_t* d_rptr = nullptr;        // per next-level row: the entries of its column of P
+	int32_t* d_rent = nullptr;        //   entry index
+	int32_t* d_rrow = nullptr;        //   this level's row
 	int32_t* d_gal_ptr = nullptr;     // per next-level block slot: range of contributions
 	int32_t* d_gal_blk = nullptr;     // this level's block slot
-	int32_t* d_gal_i = nullptr;       // this level's row
-	int32_t* d_gal_j = nullptr;       // this level's column
+	int32_t* d_gal_pi = nullptr;      // entry of P in the slot's row
+	int32_t* d_gal_pj = nullptr;      // entry of P in the slot's column
 	// coarsest level: dense inverse (n x n, n = bs * nb) when small enough
 	double* d_dense = nullptr;
 	// FP32 work vectors (bs*nb each)
@@ -108,6 +119,17 @@ struct AmgLevel {
 	float* d_d = nullptr;
 	float* d_r = nullptr;
 	double* d_partial = nullptr;      // per-workgroup maxima / sums
+};
+
+// Multigrid parameters of a hierarchy, fixed when it is built (defaults,
+// or the environment's BFMX_AMG_DEG, BFMX_AMG_RATIO, BFMX_AMG_COARSEST,
+// BFMX_AMG_OMEGA).
+struct AmgParams {
+	int deg = 2;            // Chebyshev steps before and after each coarse correction
+	double ratio = 30.0;    // Chebyshev interval [lambda_max / ratio, lambda_max]
+	int coarsest = 64;      // coarsen until a level has at most this many block rows
+	double omega = 1.0;     // prolongator damping: omega * 4 / (3 lambda_max)
+	int power = 10;         // power-iteration steps estimating lambda_max(D^-1 A); 0: Gershgorin bound
 };
 
 struct Graph {
@@ -189,6 +211,7 @@ struct bfmx_system {
 	hipEvent_t ev[6] = {};
 	std::vector<bfmx::Graph> graphs;   // captured chunks, one per solver configuration
 	std::vector<bfmx::AmgLevel> amg;   // multigrid hierarchy (built on first use)
+	bfmx::AmgParams amg_par;           // its parameters
 	bfmx_model* model = nullptr;       // learned preconditioner (borrowed)
 	float* d_mp_feat = nullptr;        // its node features (8 per node)
 	float* d_mp_h[4] = {nullptr, nullptr, nullptr, nullptr};   // hidden features (FP32, 64 per node)

@@ -691,11 +691,13 @@ int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, hipGraphExe
 	return BFMX_OK;
 }
 
-// A learned run stalls when its best r.r has not halved for twice as many
-// iterations as the run has taken (and at least kLearnedStall): a window
-// that grows with the run, as CG's residual plateaus do. The network's
-// operator is SPD, so this is a safeguard, not the expected end of a run.
-constexpr int64_t kLearnedStall = 4000;
+// A flexible-CG run (learned or multigrid preconditioner) stalls when its
+// best r.r has not halved for twice as many iterations as the run has taken
+// (and at least kFcgStall): a window that grows with the run, as CG's
+// residual plateaus do. Both preconditioners are SPD, so this is a safeguard
+// (a diverging or NaN run ends instead of running to the iteration cap), not
+// the expected end of a run.
+constexpr int64_t kFcgStall = 4000;
 
 // Runs chunks of one solver from the current state (warm: x kept, else x = 0)
 // until the device flags an end, cap iterations are launched or, with
@@ -755,7 +757,7 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 			if (last.rr < 0.5 * mark) {
 				mark = last.rr;
 				mark_iter = last.iter;
-			} else if (last.iter - mark_iter >= std::max(kLearnedStall, 2 * mark_iter)) {
+			} else if (last.iter - mark_iter >= std::max(kFcgStall, 2 * mark_iter)) {
 				break;
 			}
 		}
@@ -796,9 +798,262 @@ void release_solver(bfmx_system* s) {
 	}
 }
 
+namespace {
+
+// ------------------------------------------------------------------ MINRES
+// Preconditioned MINRES (Paige and Saunders, SIAM J. Numer. Anal. 12 (1975)
+// 617-629) for symmetric indefinite systems, with M = |diag(A)|. Per
+// iteration: one SpMV pass (v = y / beta, y' = A v - (beta / beta_old) r1,
+// partial v . y'), one update pass (y' -= (alpha / beta) r2, z = M^-1 y',
+// partial y' . z), the Givens recurrences on the host, and the direction and
+// solution update. The sums are fixed-order per-workgroup partials.
+
+__global__ __launch_bounds__(kBlock) void k_mr_diag(int nb, const int32_t* __restrict__ diag_blk,
+                                                    const double* __restrict__ val, double* __restrict__ dinv) {
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= nb) {
+		return;
+	}
+	const double* top = val + 4 * static_cast<int64_t>(diag_blk[i]) - 2 * (i & 63);
+	double const a0 = fabs(top[0]), a1 = fabs(top[2 * kSlice + 1]);
+	dinv[2 * i] = a0 > 0 ? 1.0 / a0 : 1.0;
+	dinv[2 * i + 1] = a1 > 0 ? 1.0 / a1 : 1.0;
+}
+
+// r1 = b, r2 = b, y = M^-1 b, x = w = w2 = 0; partials b . y, b . b.
+__global__ __launch_bounds__(kBlock) void k_mr_start(int nb, const double* __restrict__ b,
+                                                     const double* __restrict__ dinv, double* __restrict__ r1,
+                                                     double* __restrict__ r2, double* __restrict__ y,
+                                                     double* __restrict__ x, double* __restrict__ w,
+                                                     double* __restrict__ w2, double* __restrict__ partial) {
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	double v[2] = {0, 0};
+	if (i < nb) {
+		dbl2 const bv = ld2(b + 2 * i), d = ld2(dinv + 2 * i);
+		dbl2 const yv = dbl2{d.x * bv.x, d.y * bv.y};
+		st2(r1 + 2 * i, bv);
+		st2(r2 + 2 * i, bv);
+		st2(y + 2 * i, yv);
+		st2(x + 2 * i, dbl2{0, 0});
+		st2(w + 2 * i, dbl2{0, 0});
+		st2(w2 + 2 * i, dbl2{0, 0});
+		v[0] = bv.x * yv.x + bv.y * yv.y;
+		v[1] = bv.x * bv.x + bv.y * bv.y;
+	}
+	write_partials<2>(v, partial);
+}
+
+// v = sy y, yn = sy A y - sr r1; partial v . yn.
+__global__ __launch_bounds__(kBlock) void k_mr_spmv(int nb, const int32_t* __restrict__ slice_ptr,
+                                                    const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                    const double* __restrict__ y, const double* __restrict__ r1,
+                                                    double sy, double sr, double* __restrict__ v,
+                                                    double* __restrict__ yn, double* __restrict__ partial) {
+	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	double d[1] = {0};
+	if (i < nb) {
+		dbl2 const ay = row_spmv<double>(slice_ptr, col, val, i,
+		                                 [&](int32_t c) { return ld2(y + 2 * static_cast<int64_t>(c)); });
+		dbl2 const yi = ld2(y + 2 * i), ri = ld2(r1 + 2 * i);
+		dbl2 const vi = dbl2{sy * yi.x, sy * yi.y};
+		dbl2 const ni = dbl2{sy * ay.x - sr * ri.x, sy * ay.y - sr * ri.y};
+		st2(v + 2 * i, vi);
+		st2(yn + 2 * i, ni);
+		d[0] = vi.x * ni.x + vi.y * ni.y;
+	}
+	write_partials<1>(d, partial);
+}
+
+// yn -= sa r2, z = M^-1 yn; partial yn . z.
+__global__ __launch_bounds__(kBlock) void k_mr_update(int nb, double* __restrict__ yn, const double* __restrict__ r2,
+                                                      const double* __restrict__ dinv, double sa,
+                                                      double* __restrict__ z, double* __restrict__ partial) {
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	double d[1] = {0};
+	if (i < nb) {
+		dbl2 const a = ld2(yn + 2 * i), r = ld2(r2 + 2 * i), m = ld2(dinv + 2 * i);
+		dbl2 const t = dbl2{a.x - sa * r.x, a.y - sa * r.y};
+		dbl2 const zi = dbl2{m.x * t.x, m.y * t.y};
+		st2(yn + 2 * i, t);
+		st2(z + 2 * i, zi);
+		d[0] = t.x * zi.x + t.y * zi.y;
+	}
+	write_partials<1>(d, partial);
+}
+
+// w_new = (v - oldeps w_older - delta w_old) / gamma into wout, x += phi w_new.
+__global__ __launch_bounds__(kBlock) void k_mr_x(int nb, const double* __restrict__ v,
+                                                 const double* __restrict__ w_older, const double* __restrict__ w_old,
+                                                 double oldeps, double delta, double rgamma, double phi,
+                                                 double* __restrict__ wout, double* __restrict__ x) {
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= nb) {
+		return;
+	}
+	dbl2 const vi = ld2(v + 2 * i), a = ld2(w_older + 2 * i), c = ld2(w_old + 2 * i), xi = ld2(x + 2 * i);
+	dbl2 const w = dbl2{(vi.x - oldeps * a.x - delta * c.x) * rgamma, (vi.y - oldeps * a.y - delta * c.y) * rgamma};
+	st2(wout + 2 * i, w);
+	st2(x + 2 * i, dbl2{xi.x + phi * w.x, xi.y + phi * w.y});
+}
+
+// partial |b - A x|^2.
+__global__ __launch_bounds__(kBlock) void k_mr_resid(int nb, const int32_t* __restrict__ slice_ptr,
+                                                     const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                     const double* __restrict__ x, const double* __restrict__ b,
+                                                     double* __restrict__ partial) {
+	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	double d[1] = {0};
+	if (i < nb) {
+		dbl2 const ax = row_spmv<double>(slice_ptr, col, val, i,
+		                                 [&](int32_t c) { return ld2(x + 2 * static_cast<int64_t>(c)); });
+		dbl2 const bi = ld2(b + 2 * i);
+		double const e0 = bi.x - ax.x, e1 = bi.y - ax.y;
+		d[0] = e0 * e0 + e1 * e1;
+	}
+	write_partials<1>(d, partial);
+}
+
+template <int NV>
+__global__ __launch_bounds__(kRed) void k_mr_reduce(int nwg, const double* __restrict__ partial,
+                                                    double* __restrict__ out) {
+	double tot[NV];
+	reduce_partials<NV>(partial, nwg, tot);
+	if (threadIdx.x == 0) {
+#pragma unroll
+		for (int k = 0; k < NV; k++) {
+			out[k] = tot[k];
+		}
+	}
+}
+
+// Totals of the NV partial sums of the last kernel (host copy, synchronous).
+template <int NV>
+int mr_sums(hipStream_t st, int nwg, const double* partial, double* d_out, double (&h)[NV]) {
+	k_mr_reduce<NV><<<1, kRed, 0, st>>>(nwg, partial, d_out);
+	BFMX_HIP(hipGetLastError());
+	BFMX_HIP(hipMemcpyAsync(h, d_out, NV * sizeof(double), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	return BFMX_OK;
+}
+
+struct MrResult {
+	int status;
+	int64_t iter;
+	double rr, bb;
+	int checks;
+};
+
+int solve_minres(bfmx_system* s, double rtol, int64_t cap, MrResult* res) {
+	int64_t const n = s->n_nodes;
+	int const nb = static_cast<int>(n);
+	int const g = grid(n);
+	hipStream_t const st = s->stream;
+	double* wbuf[3] = {nullptr, nullptr, nullptr};
+	double* d_sum = nullptr;
+	auto release = [&]() {
+		for (auto& p : wbuf) {
+			dfree(p);
+		}
+		dfree(d_sum);
+	};
+	int const rc = [&]() -> int {
+		for (auto& p : wbuf) {
+			BFMX_CHECK(dalloc(&p, 2 * n));
+		}
+		BFMX_CHECK(dalloc(&d_sum, 4));
+		double* const b = s->d_bsolve;
+		double* const x = s->d_x;
+		double* const dinv = s->d_dinv;
+		double* rbuf[3] = {s->d_r, s->d_z, s->d_q};   // r1, r2, and the next r2
+		double* const y = s->d_p[0];
+		double* const v = s->d_p[1];
+		double* w = wbuf[0];        // latest direction
+		double* w2 = wbuf[2];       // the one before
+		double* wfree = wbuf[1];    // scratch for the next
+		k_mr_diag<<<g, kBlock, 0, st>>>(nb, s->d_diag_blk, s->d_val, dinv);
+		k_mr_start<<<g, kBlock, 0, st>>>(nb, b, dinv, rbuf[0], rbuf[1], y, x, w, w2, s->d_partial);
+		double t2[2];
+		BFMX_CHECK(mr_sums<2>(st, g, s->d_partial, d_sum, t2));
+		res->bb = t2[1];
+		res->rr = t2[1];
+		res->iter = 0;
+		res->checks = 0;
+		res->status = 0;
+		if (!(t2[0] > 0)) {
+			res->status = t2[1] == 0 ? 1 : 2;   // b = 0: x = 0; otherwise M not positive
+			return BFMX_OK;
+		}
+		double const beta1 = std::sqrt(t2[0]);
+		double const tol2 = rtol * rtol * res->bb;
+		double oldb = 0, beta = beta1, dbar = 0, epsln = 0, phibar = beta1, cs = -1, sn = 0;
+		int64_t next_check = 0;
+		for (int64_t it = 1; it <= cap; it++) {
+			double* const r1 = rbuf[0];
+			double* const r2 = rbuf[1];
+			double* const yn = rbuf[2];
+			k_mr_spmv<<<g, kBlock, 0, st>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, y, r1, 1.0 / beta,
+			                                it >= 2 ? beta / oldb : 0.0, v, yn, s->d_partial);
+			double t1[1];
+			BFMX_CHECK(mr_sums<1>(st, g, s->d_partial, d_sum, t1));
+			double const alfa = t1[0];
+			k_mr_update<<<g, kBlock, 0, st>>>(nb, yn, r2, dinv, alfa / beta, y, s->d_partial);
+			BFMX_CHECK(mr_sums<1>(st, g, s->d_partial, d_sum, t1));
+			rbuf[0] = r2;
+			rbuf[1] = yn;
+			rbuf[2] = r1;
+			if (!(t1[0] >= 0) || !std::isfinite(alfa)) {
+				res->status = 2;   // the preconditioner is not positive definite, or overflow
+				res->iter = it;
+				return BFMX_OK;
+			}
+			oldb = beta;
+			beta = std::sqrt(t1[0]);
+			double const oldeps = epsln;
+			double const delta = cs * dbar + sn * alfa;
+			double const gbar = sn * dbar - cs * alfa;
+			epsln = sn * beta;
+			dbar = -cs * beta;
+			double const gamma = std::max(std::hypot(gbar, beta), 1e-300);
+			cs = gbar / gamma;
+			sn = beta / gamma;
+			double const phi = cs * phibar;
+			phibar = sn * phibar;
+			// w_new = (v - oldeps w2 - delta w) / gamma; then (w2, w) <- (w, w_new)
+			k_mr_x<<<g, kBlock, 0, st>>>(nb, v, w2, w, oldeps, delta, 1.0 / gamma, phi, wfree, x);
+			BFMX_HIP(hipGetLastError());
+			double* const older = w2;
+			w2 = w;
+			w = wfree;
+			wfree = older;
+			res->iter = it;
+			// the M^-1-norm estimate phibar triggers a true FP64 residual check
+			if ((phibar <= rtol * beta1 && it >= next_check) || beta == 0 || it == cap) {
+				k_mr_resid<<<g, kBlock, 0, st>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, x, b, s->d_partial);
+				BFMX_CHECK(mr_sums<1>(st, g, s->d_partial, d_sum, t1));
+				res->rr = t1[0];
+				res->checks++;
+				if (t1[0] <= tol2) {
+					res->status = 1;
+					return BFMX_OK;
+				}
+				if (beta == 0) {
+					res->status = 3;   // the Krylov space is exhausted above rtol
+					return BFMX_OK;
+				}
+				next_check = it + 10;
+			}
+		}
+		return BFMX_OK;
+	}();
+	release();
+	return rc;
+}
+
+}  // namespace
+
 int solve(bfmx_system* s, const double* b_in, int precond, int precision, double rtol, size_t max_iter,
           bfmx_stats_t* stats) {
-	if (precond < BFMX_PRECOND_JACOBI || precond > BFMX_PRECOND_LEARNED) {
+	if (precond < BFMX_PRECOND_JACOBI || precond > BFMX_PRECOND_MINRES) {
 		set_error("unknown preconditioner %d", precond);
 		return BFMX_E_UNSUPPORTED;
 	}
@@ -840,6 +1095,38 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	double const rtol2 = rtol * rtol;
 
 	auto const t0 = std::chrono::steady_clock::now();
+	if (precond == BFMX_PRECOND_MINRES) {
+		MrResult mr{};
+		BFMX_CHECK(solve_minres(s, rtol, cap, &mr));
+		auto const t1 = std::chrono::steady_clock::now();
+		int32_t const status = mr.status == 1   ? BFMX_OK
+		                       : mr.status == 2 ? BFMX_E_BREAKDOWN
+		                       : mr.status == 3 ? BFMX_E_STAGNATION
+		                                        : BFMX_E_MAXITER;
+		s->last_iters = mr.iter;
+		s->has_u = true;
+		double const rel = mr.bb > 0 ? std::sqrt(mr.rr / mr.bb) : 0.0;
+		if (stats) {
+			*stats = bfmx_stats_t{};
+			stats->status = status;
+			stats->precond = precond;
+			stats->precision = BFMX_PRECISION_FP64;
+			stats->restarts = mr.checks;
+			stats->iterations = mr.iter;
+			stats->n_dof = 2 * n;
+			stats->rtol = rtol;
+			stats->rel_residual = rel;
+			stats->solve_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+			stats->b_norm = std::sqrt(mr.bb);
+		}
+		if (status != BFMX_OK) {
+			set_error("MINRES %s after %lld iterations (relative residual %g, rtol %g)",
+			          status == BFMX_E_BREAKDOWN ? "broke down" : status == BFMX_E_STAGNATION ? "stagnated"
+			                                                                                  : "did not converge",
+			          (long long) mr.iter, rel, rtol);
+		}
+		return status;
+	}
 	if (mixed || amg || learned) {
 		BFMX_CHECK(ensure_f32(s));
 	}
@@ -850,7 +1137,9 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		BFMX_CHECK(mpnn_setup_numeric(s));
 	}
 	Scalars last{};
-	BFMX_CHECK(run_chunks(s, precond, mixed, rtol2, cap, false, learned, &last));
+	// flexible CG stops when its residual stops halving for long (a
+	// divergent or stalled preconditioner) rather than running to the cap
+	BFMX_CHECK(run_chunks(s, precond, mixed, rtol2, cap, false, learned || amg, &last));
 	int64_t iters = last.iter;
 	int32_t fallback = 0;
 	int64_t fb_iters = 0;

@@ -202,7 +202,10 @@ static double now_ms(void) {
 
 static int run_planar(bfm_sim_t* sim, int stress) {
 	bfm_state_t* const state = sim->state;
-	bfm_solver_t const solver = bfm_sim_solver(sim);
+	bfm_solver_t solver = bfm_sim_solver(sim);
+	if (stress == BFMX_AXISYMMETRIC) {
+		solver.precond = BFMX_PRECOND_MINRES;   /* the reference's axisymmetric K is indefinite: no CG */
+	}
 	bfm_solve_stats_t* stats = state->alloc((sim->n_instances ? sim->n_instances : 1) * sizeof *stats);
 	if (!stats) {
 		BFM_ERROR(state, "out of memory");
@@ -246,8 +249,7 @@ int bfm_sim_run(bfm_sim_t* sim) {
 	case BFM_SIM_KIND_PLANAR_STRESS:
 		return run_planar(sim, 1);
 	case BFM_SIM_KIND_AXISYMMETRIC_STRAIN:
-		BFM_ERROR(sim->state, "axisymmetric simulations are not implemented by this library (planar only)");
-		return -1;
+		return run_planar(sim, BFMX_AXISYMMETRIC);
 	}
 	return -1;
 }

@@ -250,6 +250,9 @@ static int create_planar(bfm_system_t* system, bfm_instance_t* instance, size_t 
 	}
 	memset(h, 0, sizeof *h);
 	h->solver = bfm_solver_default();
+	if (stress == BFMX_AXISYMMETRIC) {
+		h->solver.precond = BFMX_PRECOND_MINRES;   /* the reference's axisymmetric K is indefinite */
+	}
 	if (bfm_planar_device_system(&h->sys, instance, n_forces, forces, stress, system->b.data) < 0) {
 		state->free(h);
 		bfm_vec_destroy(&system->b);
@@ -276,13 +279,12 @@ int bfm_system_create_planar_stress(bfm_system_t* system, bfm_instance_t* instan
 	return create_planar(system, instance, n_forces, forces, 1);
 }
 
+/* libbfm/src/system.c:539-622: the device system with the axisymmetric
+ * formulation (the reference's element formulas and condition loop, kept as
+ * written), solved by MINRES. */
 int bfm_system_create_axisymmetric_strain(bfm_system_t* system, bfm_instance_t* instance, size_t n_forces,
                                           bfm_force_t** forces) {
-	(void) system;
-	(void) n_forces;
-	(void) forces;
-	BFM_ERROR(instance->state, "axisymmetric systems are not implemented by this library (planar only)");
-	return -1;
+	return create_planar(system, instance, n_forces, forces, BFMX_AXISYMMETRIC);
 }
 
 /* ------------------------------------------------------------ BSR matrices */

@@ -12,7 +12,8 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libbfmhip.so")
 
-PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_AMG, PRECOND_LEARNED = 0, 1, 2, 3
+PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_AMG, PRECOND_LEARNED, PRECOND_MINRES = 0, 1, 2, 3, 4
+PLANE_STRAIN, PLANE_STRESS, AXISYMMETRIC = 0, 1, 2
 FP64, MIXED = 0, 1
 KERNEL_CG_SPMV64, KERNEL_CG_SPMV32, KERNEL_FCG_SPMV, KERNEL_AMG_SMOOTH = 0, 1, 2, 3
 DIRICHLET_X, DIRICHLET_Y, NEUMANN_X, NEUMANN_Y = 0, 1, 2, 3
@@ -392,11 +393,12 @@ BATCH_CLASSES, BATCH_CONFIGS = 3, 5
 
 
 def planar_instance(mesh, stress, material, kinds=(), values=(), flags=(), g=(0.0, 0.0), elem_E=None):
-    """The arrays of one problem of a batch built from meshes (Batch.from_instances)."""
+    """The arrays of one problem of a batch built from meshes (Batch.from_instances);
+    stress: PLANE_STRAIN (or False), PLANE_STRESS (or True) or AXISYMMETRIC."""
     n = mesh.n_nodes
     kinds = np.ascontiguousarray(kinds, np.int32)
     return {
-        "kind": int(mesh.kind), "stress": int(bool(stress)), "n_nodes": n,
+        "kind": int(mesh.kind), "stress": int(stress), "n_nodes": n,
         "coords": np.ascontiguousarray(np.asarray(mesh.coords, np.float64).reshape(-1)),
         "elems": np.ascontiguousarray(np.asarray(mesh.elems, np.uint64).reshape(-1)),
         "edges": np.ascontiguousarray(np.asarray(mesh.edges, np.uint64).reshape(-1)),

@@ -13,7 +13,9 @@
 
 typedef struct {
 	int precond;     /* 0 point Jacobi, 1 2x2 block Jacobi, 2 FP32 multigrid V-cycle
-	                    inside FP64 flexible CG */
+	                    inside FP64 flexible CG, 3 learned network (flexible CG),
+	                    4 MINRES with |diagonal| Jacobi; axisymmetric sims and
+	                    systems always use 4 (their K is indefinite) */
 	int precision;   /* 0 FP64, 1 FP32 matrix inside FP64 refinement */
 	double rtol;     /* ||b - K u|| / ||b|| target, FP64 */
 	size_t max_iter; /* 0 = 20 x DOFs */

@@ -65,6 +65,10 @@ enum {
 	                                  bfmx_system_set_model (BF16/FP32); falls back
 	                                  to block-Jacobi PCG from the current iterate
 	                                  on breakdown, the iteration limit or a stall */
+	BFMX_PRECOND_MINRES = 4,       /* MINRES (FP64) preconditioned by 1 / |diag(A)|,
+	                                  for symmetric indefinite systems (the
+	                                  axisymmetric formulation); precision is
+	                                  ignored */
 };
 
 /* Arithmetic of the solve. */
@@ -72,6 +76,16 @@ enum {
 	BFMX_PRECISION_FP64 = 0,  /* FP64 matrix, vectors and reductions */
 	BFMX_PRECISION_MIXED = 1, /* FP32 matrix and search directions inside FP64
 	                             refinement; FP64 solution, residual and dots */
+};
+
+/* Formulations (the stress argument of bfmx_system_assemble). */
+enum {
+	BFMX_PLANE_STRAIN = 0,
+	BFMX_PLANE_STRESS = 1,
+	BFMX_AXISYMMETRIC = 2,   /* r = x; the reference's axisymmetric formulas as
+	                            written (libbfm/src/system.c:228-356, 539-622):
+	                            symmetric, indefinite for common nu, so solved by
+	                            BFMX_PRECOND_MINRES */
 };
 
 /* Condition kinds, numbered as bfm_condition_kind_t (libbfm/src/bfm/condition.h:5-14). */
@@ -124,8 +138,11 @@ int bfmx_system_destroy(bfmx_system_t* s);
 int bfmx_system_set_edges(bfmx_system_t* s, size_t n_edges, const size_t* edge_nodes,
                           const uint8_t* edge_boundary);
 
-/* Assembles K and the body load on the device: plane stress when stress != 0,
- * plane strain otherwise; Young's modulus E (or elem_E[e] per element when
+/* Assembles K and the body load on the device: stress is a formulation
+ * (BFMX_PLANE_STRAIN, BFMX_PLANE_STRESS or BFMX_AXISYMMETRIC; the latter
+ * replaces fill_axisymmetric_elem and the condition loop of
+ * bfm_system_create_axisymmetric_strain, libbfm/src/system.c:228-356,
+ * 539-622); Young's modulus E (or elem_E[e] per element when
  * elem_E is not NULL), Poisson ratio nu, density rho. The body force per unit
  * mass of node j is node_force[2j..2j+1] when node_force is not NULL, else
  * (gx, gy). Clears all boundary conditions. */

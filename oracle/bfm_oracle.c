@@ -16,6 +16,16 @@
  *   orc_band_*           band layout, unpivoted LU and substitutions
  *                        (matrix.c:196-302, 351-404).
  *   orc_solve_dense      run_elasticity (libbfm/src/sim.c:103-135).
+ *   stress == 2          the axisymmetric formulation:
+ *                        fill_axisymmetric_elem (system.c:228-356) with its
+ *                        constants (the reference multiplies by 1 - 2 nu
+ *                        where the plane-strain formula divides, line 242)
+ *                        and its f_22 (second term without r, line 339), and
+ *                        the condition loop of
+ *                        bfm_system_create_axisymmetric_strain
+ *                        (system.c:539-622: Neumann X/Y weighted by r1 + r2
+ *                        as the reference forms them, Neumann normal/tangent
+ *                        ignored).
  * and adds sparse counterparts for meshes whose dense matrix does not fit:
  *   orc_csr_*            the same element arithmetic accumulated into CSR in
  *                        the same element order (entries equal the dense
@@ -92,6 +102,12 @@ static void orc_shape(int kind, double xsi, double eta, double phi[4], double dx
 
 /* D-matrix constants, libbfm/src/system.c:454-458. */
 void orc_dconst(int stress, double E, double nu, double* a, double* b, double* c) {
+	if (stress == 2) {   /* axisymmetric, system.c:241-243 as written */
+		*a = E * (1 - nu) / (1 + nu) * (1 - 2 * nu);
+		*b = E * nu / (1 + nu) / (1 - 2 * nu);
+		*c = E / (2 * (1 + nu));
+		return;
+	}
 	*a = !stress ? E * (1 - nu) / (1 + nu) / (1 - 2 * nu) : E / (1 - nu * nu);
 	*b = !stress ? E * nu / (1 + nu) / (1 - 2 * nu) : E * nu / (1 - nu * nu);
 	*c = E / (2 * (1 + nu));
@@ -103,8 +119,76 @@ void orc_dconst(int stress, double E, double nu, double* a, double* b, double* c
  * (libbfm/src/system.c:136-224). */
 typedef void (*orc_add_fn)(void* ctx, size_t i, size_t j, double v);
 
+/* Axisymmetric element (fill_axisymmetric_elem, system.c:228-356): r is the
+ * x coordinate interpolated at the Gauss point. */
+static void orc_element_axi(int kind, const size_t* map, const double* coords, double E, double nu, double rho,
+                            size_t n_forces, const double* forces, double* b, orc_add_fn add, void* ctx) {
+	double a, bb, c;
+	orc_dconst(2, E, nu, &a, &bb, &c);
+
+	double x[4], y[4];
+	for (int j = 0; j < kind; j++) {
+		x[j] = coords[map[j] * 2 + 0];
+		y[j] = coords[map[j] * 2 + 1];
+	}
+
+	int n_points;
+	double w[4], pts[4][2];
+	orc_rule(kind, &n_points, w, pts);
+
+	for (int g = 0; g < n_points; g++) {
+		double phi[4], dxsi[4], deta[4];
+		orc_shape(kind, pts[g][0], pts[g][1], phi, dxsi, deta);
+
+		double dx_dxsi = 0, dx_deta = 0, dy_dxsi = 0, dy_deta = 0, r = 0;
+		for (int j = 0; j < kind; j++) {
+			dx_dxsi += x[j] * dxsi[j];
+			dx_deta += x[j] * deta[j];
+			dy_dxsi += y[j] * dxsi[j];
+			dy_deta += y[j] * deta[j];
+			r += x[j] * phi[j];
+		}
+
+		double const det_J = fabs(dx_dxsi * dy_deta - dx_deta * dy_dxsi);
+		double dphi_dx[4], dphi_dy[4];
+
+		for (int j = 0; j < kind; j++) {
+			dphi_dx[j] = (dxsi[j] * dy_deta - deta[j] * dy_dxsi) / det_J;
+			dphi_dy[j] = (deta[j] * dx_dxsi - dxsi[j] * dx_deta) / det_J;
+		}
+
+		for (int j = 0; j < kind; j++) {
+			size_t const ii = 2 * map[j];
+			for (size_t k = 0; k < n_forces; k++) {
+				b[ii + 0] += det_J * w[g] * forces[2 * k + 0] * rho * phi[j] * r;
+				b[ii + 1] += det_J * w[g] * forces[2 * k + 1] * rho * phi[j] * r;
+			}
+		}
+
+		for (int j = 0; j < kind; j++) {
+			size_t const ii = 2 * map[j];
+			for (int k = 0; k < kind; k++) {
+				size_t const jj = 2 * map[k];
+				double const f11 = a * dphi_dx[j] * dphi_dx[k] * r + c * dphi_dy[j] * dphi_dy[k] * r +
+				                   phi[j] * (bb * dphi_dx[k] + a * phi[k] / r) + dphi_dx[j] * bb * phi[k];
+				double const f12 = bb * dphi_dx[j] * dphi_dy[k] * r + c * dphi_dy[j] * dphi_dx[k] * r + phi[j] * bb * dphi_dy[k];
+				double const f21 = bb * dphi_dy[j] * dphi_dx[k] * r + c * dphi_dx[j] * dphi_dy[k] * r + dphi_dy[j] * bb * phi[k];
+				double const f22 = a * dphi_dy[j] * dphi_dy[k] * r + c * dphi_dx[j] * dphi_dx[k];
+				add(ctx, ii + 0, jj + 0, det_J * w[g] * f11);
+				add(ctx, ii + 0, jj + 1, det_J * w[g] * f12);
+				add(ctx, ii + 1, jj + 0, det_J * w[g] * f21);
+				add(ctx, ii + 1, jj + 1, det_J * w[g] * f22);
+			}
+		}
+	}
+}
+
 static void orc_element(int kind, const size_t* map, const double* coords, int stress, double E, double nu, double rho,
                         size_t n_forces, const double* forces, double* b, orc_add_fn add, void* ctx) {
+	if (stress == 2) {
+		orc_element_axi(kind, map, coords, E, nu, rho, n_forces, forces, b, add, ctx);
+		return;
+	}
 	double a, bb, c;
 	orc_dconst(stress, E, nu, &a, &bb, &c);
 
@@ -233,9 +317,14 @@ static void orc_node_tangent(size_t i, const double* coords, size_t n_edges, con
 	}
 }
 
-/* Neumann loads of one condition on b (system.c:477-522). */
+/* Neumann loads of one condition on b (system.c:477-522). Axisymmetric
+ * (system.c:584-617): X/Y loads carry the factor r1 + r2 as the reference
+ * forms each r (from one node's x and y), normal/tangent loads add nothing. */
 static void orc_neumann(int kind, double value, const uint8_t* flags, const double* coords, size_t n_edges,
-                        const size_t* edge_nodes, double* b) {
+                        const size_t* edge_nodes, int axi, double* b) {
+	if (axi && (kind == 4 || kind == 5)) {
+		return;
+	}
 	for (size_t j = 0; j < n_edges; j++) {
 		size_t const n1 = edge_nodes[2 * j + 0];
 		size_t const n2 = edge_nodes[2 * j + 1];
@@ -243,11 +332,22 @@ static void orc_neumann(int kind, double value, const uint8_t* flags, const doub
 			continue;
 		}
 		if (kind == 2 || kind == 3) {
+			double fac = 1;
+			if (axi) {
+				double const r1 = coords[n1 * 2 + 0] * (1 - 1 / sqrt(3)) / 2 + coords[n1 * 2 + 1] * (1 + 1 / sqrt(3)) / 2;
+				double const r2 = coords[n2 * 2 + 0] * (1 - 1 / sqrt(3)) / 2 + coords[n2 * 2 + 1] * (1 + 1 / sqrt(3)) / 2;
+				fac = r1 + r2;
+			}
 			double const c2 = pow(coords[n1 * 2 + 0] - coords[n2 * 2 + 0], 2) + pow(coords[n1 * 2 + 1] - coords[n2 * 2 + 1], 2);
 			double const jac = sqrt(c2) / 2;
 			size_t const shift = kind == 2 ? 0 : 1;
-			b[n1 * 2 + shift] += jac * value;
-			b[n2 * 2 + shift] += jac * value;
+			if (axi) {
+				b[n1 * 2 + shift] += fac * jac * value;
+				b[n2 * 2 + shift] += fac * jac * value;
+			} else {
+				b[n1 * 2 + shift] += jac * value;
+				b[n2 * 2 + shift] += jac * value;
+			}
 		} else {
 			double const dx = coords[n1 * 2 + 0] - coords[n2 * 2 + 0];
 			double const dy = coords[n1 * 2 + 1] - coords[n2 * 2 + 1];
@@ -260,11 +360,12 @@ static void orc_neumann(int kind, double value, const uint8_t* flags, const doub
 	}
 }
 
-/* The ordered condition loop of create_planar (system.c:470-526) on a dense
+/* The ordered condition loop of create_planar (system.c:470-526) or of
+ * bfm_system_create_axisymmetric_strain (axi, system.c:575-619) on a dense
  * system. flags holds n_conditions * n_nodes bytes. */
-int orc_apply_conditions_dense(size_t n_nodes, const double* coords, size_t n_edges, const size_t* edge_nodes,
-                               const uint8_t* edge_boundary, size_t n_conditions, const int* kinds,
-                               const double* values, const uint8_t* flags, double* K, double* b) {
+static int orc_conditions_dense(size_t n_nodes, const double* coords, size_t n_edges, const size_t* edge_nodes,
+                                const uint8_t* edge_boundary, size_t n_conditions, const int* kinds,
+                                const double* values, const uint8_t* flags, int axi, double* K, double* b) {
 	size_t const m = 2 * n_nodes;
 	for (size_t c = 0; c < n_conditions; c++) {
 		int const kind = kinds[c];
@@ -277,7 +378,7 @@ int orc_apply_conditions_dense(size_t n_nodes, const double* coords, size_t n_ed
 				}
 			}
 		} else if (kind >= 2 && kind <= 5) {
-			orc_neumann(kind, v, f, coords, n_edges, edge_nodes, b);
+			orc_neumann(kind, v, f, coords, n_edges, edge_nodes, axi, b);
 		} else if (kind == 6 || kind == 7) {
 			for (size_t i = 0; i < n_nodes; i++) {
 				if (!f[i]) {
@@ -293,6 +394,20 @@ int orc_apply_conditions_dense(size_t n_nodes, const double* coords, size_t n_ed
 		}
 	}
 	return 0;
+}
+
+int orc_apply_conditions_dense(size_t n_nodes, const double* coords, size_t n_edges, const size_t* edge_nodes,
+                               const uint8_t* edge_boundary, size_t n_conditions, const int* kinds,
+                               const double* values, const uint8_t* flags, double* K, double* b) {
+	return orc_conditions_dense(n_nodes, coords, n_edges, edge_nodes, edge_boundary, n_conditions, kinds, values,
+	                            flags, 0, K, b);
+}
+
+int orc_apply_conditions_dense_axi(size_t n_nodes, const double* coords, size_t n_edges, const size_t* edge_nodes,
+                                   const uint8_t* edge_boundary, size_t n_conditions, const int* kinds,
+                                   const double* values, const uint8_t* flags, double* K, double* b) {
+	return orc_conditions_dense(n_nodes, coords, n_edges, edge_nodes, edge_boundary, n_conditions, kinds, values,
+	                            flags, 1, K, b);
 }
 
 /* ------------------------------------------------------------------- RCM */
@@ -497,8 +612,8 @@ int orc_solve_dense(int kind, size_t n_nodes, const double* coords, size_t n_ele
 	if (orc_assemble_dense(kind, n_nodes, coords, n_elems, elems, stress, E, nu, rho, elem_E, n_forces, forces, K, b) < 0) {
 		goto out;
 	}
-	if (orc_apply_conditions_dense(n_nodes, coords, n_edges, edge_nodes, edge_boundary, n_conditions, kinds, values,
-	                               flags, K, b) < 0) {
+	if (orc_conditions_dense(n_nodes, coords, n_edges, edge_nodes, edge_boundary, n_conditions, kinds, values, flags,
+	                         stress == 2, K, b) < 0) {
 		goto out;
 	}
 	if (K_out) {
@@ -670,9 +785,11 @@ static void orc_csr_constrain(orc_csr_t* A, double* b, size_t d, double v) {
 	b[d] = v;
 }
 
+/* The ordered conditions on CSR; axi selects the axisymmetric loop. */
 int orc_csr_apply_conditions(size_t n_nodes, const double* coords, size_t n_edges, const size_t* edge_nodes,
                              const uint8_t* edge_boundary, size_t n_conditions, const int* kinds, const double* values,
-                             const uint8_t* flags, const int64_t* rowptr, const int32_t* col, double* val, double* b) {
+                             const uint8_t* flags, int axi, const int64_t* rowptr, const int32_t* col, double* val,
+                             double* b) {
 	orc_csr_t A = {2 * n_nodes, (int64_t*) rowptr, (int32_t*) col, val};
 	for (size_t c = 0; c < n_conditions; c++) {
 		int const kind = kinds[c];
@@ -685,7 +802,7 @@ int orc_csr_apply_conditions(size_t n_nodes, const double* coords, size_t n_edge
 				}
 			}
 		} else if (kind >= 2 && kind <= 5) {
-			orc_neumann(kind, v, f, coords, n_edges, edge_nodes, b);
+			orc_neumann(kind, v, f, coords, n_edges, edge_nodes, axi, b);
 		} else if (kind == 6 || kind == 7) {
 			for (size_t i = 0; i < n_nodes; i++) {
 				if (!f[i]) {

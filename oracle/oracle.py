@@ -69,8 +69,10 @@ def lib():
         L.orc_csr_assemble.argtypes = [C.c_int, sz, _dp, sz, _up, C.c_int, C.c_double, C.c_double,
                                        C.c_double, _Opt(_dp), sz, _Opt(_dp), _i64, _i32, _dp, _dp]
         L.orc_csr_apply_conditions.restype = C.c_int
-        L.orc_csr_apply_conditions.argtypes = [sz, _dp, sz, _up, _u8, sz, _i32, _dp, _u8, _i64, _i32,
+        L.orc_csr_apply_conditions.argtypes = [sz, _dp, sz, _up, _u8, sz, _i32, _dp, _u8, C.c_int, _i64, _i32,
                                                _dp, _dp]
+        L.orc_apply_conditions_dense_axi.restype = C.c_int
+        L.orc_apply_conditions_dense_axi.argtypes = [sz, _dp, sz, _up, _u8, sz, _i32, _dp, _u8, _dp, _dp]
         L.orc_csr_spmv.argtypes = [sz, _i64, _i32, _dp, _dp, _dp]
         L.orc_csr_pcg.restype = C.c_int64
         L.orc_csr_pcg.argtypes = [sz, _i64, _i32, _dp, _dp, _dp, C.c_double, C.c_int64,
@@ -112,22 +114,23 @@ def assemble_dense(mesh, stress, E, nu, rho, forces=None):
     return K.reshape(m, m), b
 
 
-def apply_conditions_dense(mesh, K, b, kinds, values, flags, edge_boundary=None):
+def apply_conditions_dense(mesh, K, b, kinds, values, flags, edge_boundary=None, axisymmetric=False):
     ne, en, eb = _edges(mesh)
     if edge_boundary is not None:
         eb = np.ascontiguousarray(edge_boundary.astype(np.uint8))
     K = np.ascontiguousarray(K.reshape(-1).copy())
     b = b.copy()
-    rc = lib().orc_apply_conditions_dense(mesh.n_nodes, np.ascontiguousarray(mesh.coords.reshape(-1)), ne, en, eb,
-                                          len(kinds), np.asarray(kinds, np.int32),
-                                          np.asarray(values, np.float64),
-                                          np.ascontiguousarray(np.asarray(flags, np.uint8).reshape(-1)), K, b)
+    fn = lib().orc_apply_conditions_dense_axi if axisymmetric else lib().orc_apply_conditions_dense
+    rc = fn(mesh.n_nodes, np.ascontiguousarray(mesh.coords.reshape(-1)), ne, en, eb,
+            len(kinds), np.asarray(kinds, np.int32), np.asarray(values, np.float64),
+            np.ascontiguousarray(np.asarray(flags, np.uint8).reshape(-1)), K, b)
     assert rc == 0
     return K.reshape(b.size, b.size), b
 
 
 def solve_dense(mesh, stress, E, nu, rho, kinds, values, flags, forces=None, edge_boundary=None):
-    """run_elasticity restated: returns (u, K_bc, b_bc, perm, k, rc)."""
+    """run_elasticity restated: returns (u, K_bc, b_bc, perm, k, rc). stress: 0
+    plane strain, 1 plane stress, 2 axisymmetric (the reference's formulas)."""
     m = mesh.n_dof
     u = np.zeros(m)
     K = np.zeros(m * m)
@@ -185,8 +188,8 @@ def assemble_csr(mesh, stress, E, nu, rho, forces=None, kinds=None, values=None,
         ne, en, eb = _edges(mesh)
         rc = L.orc_csr_apply_conditions(mesh.n_nodes, xy, ne, en, eb, len(kinds), np.asarray(kinds, np.int32),
                                         np.asarray(values, np.float64),
-                                        np.ascontiguousarray(np.asarray(flags, np.uint8).reshape(-1)), rowptr, col,
-                                        val, b)
+                                        np.ascontiguousarray(np.asarray(flags, np.uint8).reshape(-1)),
+                                        int(stress == 2), rowptr, col, val, b)
         assert rc == 0
     return CSR(rowptr, col, val), b
 

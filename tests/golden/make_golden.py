@@ -57,7 +57,9 @@ def node_sets(m):
 
 
 def ref_system(L, mesh, stress, E, nu, rho, conditions, gravity, solve=True):
-    """bfm_system_create_planar_* on the reference: (K dense, b) after conditions."""
+    """bfm_system_create_planar_* (stress 0 / 1) or
+    bfm_system_create_axisymmetric_strain (stress 2) on the reference: (K
+    dense, b) after conditions, and u from its sim."""
     cm = pb.Mesh_arrays.from_generated(mesh, lib=L)
     mat = pb.Material("steel", rho, E, nu, lib=L)
     rule = pb.Rule_gauss_legendre(2, mesh.kind, lib=L)
@@ -77,7 +79,8 @@ def ref_system(L, mesh, stress, E, nu, rho, conditions, gravity, solve=True):
     for i, f in enumerate(forces):
         arr[i] = C.pointer(f.c_force)
     sysm = F.bfm_system_t()
-    fn = L.bfm_system_create_planar_stress if stress else L.bfm_system_create_planar_strain
+    fn = (L.bfm_system_create_axisymmetric_strain if stress == 2 else
+          L.bfm_system_create_planar_stress if stress else L.bfm_system_create_planar_strain)
     assert fn(C.byref(sysm), C.byref(inst.c_instance), len(forces), arr) == 0
     m = mesh.n_dof
     K = np.ctypeslib.as_array(sysm.A.full.data, shape=(m * m,)).reshape(m, m).copy()
@@ -86,7 +89,8 @@ def ref_system(L, mesh, stress, E, nu, rho, conditions, gravity, solve=True):
 
     if not solve:
         return K, b, None
-    sim = pb.Sim(pb.Sim.PLANAR_STRESS if stress else pb.Sim.PLANAR_STRAIN, lib=L)
+    sim = pb.Sim(pb.Sim.AXISYMMETRIC_STRAIN if stress == 2 else
+                 pb.Sim.PLANAR_STRESS if stress else pb.Sim.PLANAR_STRAIN, lib=L)
     sim.add_instance(inst)
     for f in forces:
         sim.add_force(f)
@@ -137,6 +141,23 @@ def main():
             K0, b0, _ = ref_system(L, tiny, stress, 7.0e10, 0.33, 2700.0, [], True, solve=False)
             out[f"{name}/K_pre"] = K0
             out[f"{name}/b_pre"] = b0
+
+    # Axisymmetric (r = x + 1 > 0): tiny meshes with every condition kind and
+    # a body load, before and after the conditions, and an annulus-like beam.
+    for kind, tag in ((4, "q1"), (3, "p1")):
+        tiny = meshgen.beam(3, 2, kind=kind, length=3.0, height=2.0, jitter=0.2, seed=11)
+        tiny.coords = tiny.coords + np.array([1.0, 0.0])
+        name = f"axi_{tag}"
+        K, b, u = ref_system(L, tiny, 2, 7.0e10, 0.33, 2700.0, TINY_CONDITIONS, True)
+        add_case(out, name, K, b, u)
+        K0, b0, _ = ref_system(L, tiny, 2, 7.0e10, 0.33, 2700.0, [], True, solve=False)
+        out[f"{name}/K_pre"] = K0
+        out[f"{name}/b_pre"] = b0
+    ring = meshgen.beam(40, 10, kind=4, length=4.0, height=1.0, jitter=0.2, seed=12)
+    ring.coords = ring.coords + np.array([1.0, 0.0])
+    conds = [(1, 0.0, "bottom"), (0, 0.0, "left"), (3, -1e6, "top")]
+    K, b, u = ref_system(L, ring, 2, 2.0e11, 0.3, 7850.0, conds, True)
+    add_case(out, "axi_ring", K, b, u)
 
     # Single elements: K of a one-element mesh is the element matrix.
     rng = np.random.default_rng(5)

@@ -328,6 +328,23 @@ def test_amg_is_deterministic_and_reusable():
     assert np.abs(u3).max() < np.abs(u1).max()   # both ends clamped: stiffer
 
 
+def test_amg_smoothed_against_unsmoothed_aggregation(monkeypatch):
+    """Smoothed aggregation (the default: P = (I - w D^-1 A) P_t) and plain
+    aggregation (BFMX_AMG_SMOOTH=0, read when a system builds its hierarchy)
+    reach the same solution; smoothing the prolongator cuts the iterations."""
+    mesh = meshgen.beam(240, 60, kind=4, jitter=0.2, seed=8)
+    s_sa = D.beam_system(mesh)
+    u_sa, st_sa = s_sa.solve(precond=D.PRECOND_AMG, rtol=1e-10)
+    monkeypatch.setenv("BFMX_AMG_SMOOTH", "0")
+    s_ua = D.beam_system(mesh)
+    u_ua, st_ua = s_ua.solve(precond=D.PRECOND_AMG, rtol=1e-10)
+    assert st_sa["status"] == 0 and st_ua["status"] == 0
+    err = rel_err(u_sa, u_ua)
+    print(f"smoothed {st_sa['iterations']} vs unsmoothed {st_ua['iterations']} iterations, difference {err:.2e}")
+    assert err <= 1e-8
+    assert st_sa["iterations"] * 2 <= st_ua["iterations"]
+
+
 def test_amg_single_level_is_a_direct_solve():
     """A mesh small enough for one level: the preconditioner is the dense
     FP32 inverse, so FCG stops after very few iterations."""
