@@ -8,7 +8,7 @@ One step = one solve from u = 0 on the device-resident assembled system until
 ||b - K u||_2 / ||b||_2 <= 1e-8 (true residual recomputed in FP64), including
 the solver's per-solve set-up (FP32 matrix copy, multigrid numeric set-up).
 value = DOFs / mean step time of the headline solver, flexible CG in FP64
-preconditioned by one FP32 aggregation-multigrid V-cycle. Plain FP64
+preconditioned by one FP32 smoothed-aggregation V-cycle. Plain FP64
 Jacobi-PCG is timed the same way beside it (the speedup is reported).
 
 Also reported: the batched workload (4096 seeded holed panels of 1k-60k DOFs,
@@ -294,7 +294,7 @@ def main():
         "config": {
             "workload": f"{args.workload}: 2000x500 Q1 plane-stress cantilever beam, 20% jitter (seed 0), "
                         f"{mesh.n_dof} DOFs, solved to rtol {args.rtol:g} from u = 0",
-            "solver": "flexible CG (FP64) + FP32 aggregation-multigrid V-cycle",
+            "solver": "flexible CG (FP64) + FP32 smoothed-aggregation multigrid V-cycle",
             "n_dof": mesh.n_dof,
             "n_elems": mesh.n_elems,
             "iterations": int(st["iterations"]),
