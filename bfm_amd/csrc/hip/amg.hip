@@ -712,6 +712,54 @@ __global__ __launch_bounds__(kGj) void k_gauss_jordan(int n, double* __restrict_
 // One Chebyshev step on a level: r = b - A x_in (skipped when FIRST: x_in = 0),
 // d = c1 d + c2 D^-1 r (c1 unused at step 0), x_out = x_in + d. With DOT, also
 // the per-workgroup sums of x_out . q (q an FP64 vector over the same rows).
+// Row i of one Chebyshev step (k_cheb); xo receives the row's x_out.
+template <int BS, typename BT, bool FIRST>
+__device__ __forceinline__ void cheb_row(int i, const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ col,
+                                         const float* __restrict__ val, const BT* __restrict__ b,
+                                         const float* __restrict__ dinv, const float* __restrict__ xin,
+                                         float* __restrict__ xout, float* __restrict__ d, float c1, float c2, int step,
+                                         float (&xo)[BS]) {
+	float res[BS];
+#pragma unroll
+	for (int a = 0; a < BS; a++) {
+		res[a] = static_cast<float>(b[BS * static_cast<int64_t>(i) + a]);
+	}
+	if (!FIRST) {
+		int const r = i & 63;
+		for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
+			int32_t const c = col[blk];
+			float A[BS][BS];
+			ld_block<BS>(val, blk, r, A);
+			float xv[BS];
+#pragma unroll
+			for (int k = 0; k < BS; k++) {
+				xv[k] = xin[BS * static_cast<int64_t>(c) + k];
+			}
+#pragma unroll
+			for (int a = 0; a < BS; a++) {
+#pragma unroll
+				for (int k = 0; k < BS; k++) {
+					res[a] -= A[a][k] * xv[k];
+				}
+			}
+		}
+	}
+#pragma unroll
+	for (int a = 0; a < BS; a++) {
+		float z = 0;
+#pragma unroll
+		for (int k = 0; k < BS; k++) {
+			z += dinv[BS * BS * static_cast<int64_t>(i) + BS * a + k] * res[k];
+		}
+		int64_t const o = BS * static_cast<int64_t>(i) + a;
+		float const dn = (FIRST || step == 0) ? c2 * z : c1 * d[o] + c2 * z;
+		float const xn = FIRST ? dn : xin[o] + dn;
+		d[o] = dn;
+		xout[o] = xn;
+		xo[a] = xn;
+	}
+}
+
 template <int BS, typename BT, bool FIRST, bool DOT>
 __global__ __launch_bounds__(kBlock) void k_cheb(int nb, const int32_t* __restrict__ slice_ptr,
                                                  const int32_t* __restrict__ col, const float* __restrict__ val,
@@ -728,45 +776,12 @@ __global__ __launch_bounds__(kBlock) void k_cheb(int nb, const int32_t* __restri
 	float const c2 = static_cast<float>(scal[2 + 2 * step]);
 	double dot = 0;
 	if (i < nb) {
-		float res[BS];
+		float xo[BS];
+		cheb_row<BS, BT, FIRST>(i, slice_ptr, col, val, b, dinv, xin, xout, d, c1, c2, step, xo);
+		if (DOT) {
 #pragma unroll
-		for (int a = 0; a < BS; a++) {
-			res[a] = static_cast<float>(b[BS * static_cast<int64_t>(i) + a]);
-		}
-		if (!FIRST) {
-			int const r = i & 63;
-			for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
-				int32_t const c = col[blk];
-				float A[BS][BS];
-				ld_block<BS>(val, blk, r, A);
-				float xv[BS];
-#pragma unroll
-				for (int k = 0; k < BS; k++) {
-					xv[k] = xin[BS * static_cast<int64_t>(c) + k];
-				}
-#pragma unroll
-				for (int a = 0; a < BS; a++) {
-#pragma unroll
-					for (int k = 0; k < BS; k++) {
-						res[a] -= A[a][k] * xv[k];
-					}
-				}
-			}
-		}
-#pragma unroll
-		for (int a = 0; a < BS; a++) {
-			float z = 0;
-#pragma unroll
-			for (int k = 0; k < BS; k++) {
-				z += dinv[BS * BS * static_cast<int64_t>(i) + BS * a + k] * res[k];
-			}
-			int64_t const o = BS * static_cast<int64_t>(i) + a;
-			float const dn = (FIRST || step == 0) ? c2 * z : c1 * d[o] + c2 * z;
-			float const xn = FIRST ? dn : xin[o] + dn;
-			d[o] = dn;
-			xout[o] = xn;
-			if (DOT) {
-				dot += static_cast<double>(xn) * q[o];
+			for (int a = 0; a < BS; a++) {
+				dot += static_cast<double>(xo[a]) * q[BS * static_cast<int64_t>(i) + a];
 			}
 		}
 	}
@@ -775,20 +790,12 @@ __global__ __launch_bounds__(kBlock) void k_cheb(int nb, const int32_t* __restri
 	}
 }
 
-// r = b - A x on a level.
+// Row i of r = b - A x.
 template <int BS, typename BT>
-__global__ __launch_bounds__(kBlock) void k_level_resid(int nb, const int32_t* __restrict__ slice_ptr,
-                                                        const int32_t* __restrict__ col,
-                                                        const float* __restrict__ val, const BT* __restrict__ b,
-                                                        const float* __restrict__ x, float* __restrict__ rv,
-                                                        const Scalars* __restrict__ S) {
-	if (S->done) {
-		return;
-	}
-	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-	if (i >= nb) {
-		return;
-	}
+__device__ __forceinline__ void resid_row(int i, const int32_t* __restrict__ slice_ptr,
+                                          const int32_t* __restrict__ col, const float* __restrict__ val,
+                                          const BT* __restrict__ b, const float* __restrict__ x,
+                                          float* __restrict__ rv) {
 	int const r = i & 63;
 	float res[BS];
 #pragma unroll
@@ -813,23 +820,34 @@ __global__ __launch_bounds__(kBlock) void k_level_resid(int nb, const int32_t* _
 	}
 }
 
+// r = b - A x on a level.
+template <int BS, typename BT>
+__global__ __launch_bounds__(kBlock) void k_level_resid(int nb, const int32_t* __restrict__ slice_ptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const float* __restrict__ val, const BT* __restrict__ b,
+                                                        const float* __restrict__ x, float* __restrict__ rv,
+                                                        const Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	if (i < nb) {
+		resid_row<BS, BT>(i, slice_ptr, col, val, b, x, rv);
+	}
+}
+
 // b_c[J] = sum over the entries of column J of P (row i, entry e) of
 // P_e^T r_i. kRestrictLanes lanes share a coarse row: lane l takes entries
 // l, l + kRestrictLanes, ..., combined by a fixed butterfly.
 constexpr int kRestrictLanes = 8;
 
+// Coarse row J (live) by the lane group of `lane`; every lane of the group
+// must call it (the butterfly).
 template <int BSF>
-__global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int32_t* __restrict__ rptr,
-                                                     const int32_t* __restrict__ rent,
-                                                     const int32_t* __restrict__ rrow, const float* __restrict__ P,
-                                                     const float* __restrict__ r, float* __restrict__ bc,
-                                                     const Scalars* __restrict__ S) {
-	if (S->done) {
-		return;
-	}
-	int64_t const J = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kRestrictLanes;
-	int const lane = threadIdx.x % kRestrictLanes;
-	bool const live = J < nc;   // every lane stays for the butterfly
+__device__ __forceinline__ void restrict_row(int64_t J, bool live, int lane, const int32_t* __restrict__ rptr,
+                                             const int32_t* __restrict__ rent, const int32_t* __restrict__ rrow,
+                                             const float* __restrict__ P, const float* __restrict__ r,
+                                             float* __restrict__ bc) {
 	float acc[3] = {0, 0, 0};
 	int32_t const t0 = live ? rptr[J] : 0, t1 = live ? rptr[J + 1] : 0;
 	for (int32_t t = t0 + lane; t < t1; t += kRestrictLanes) {
@@ -859,19 +877,24 @@ __global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int32_t* __re
 	}
 }
 
-// x_i += sum over the entries e of row i of P_e x_c[pcol[e]].
 template <int BSF>
-__global__ __launch_bounds__(kBlock) void k_prolong(int nb, const int32_t* __restrict__ pptr,
-                                                    const int32_t* __restrict__ pcol, const float* __restrict__ P,
-                                                    const float* __restrict__ xc, float* __restrict__ x,
-                                                    const Scalars* __restrict__ S) {
+__global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int32_t* __restrict__ rptr,
+                                                     const int32_t* __restrict__ rent,
+                                                     const int32_t* __restrict__ rrow, const float* __restrict__ P,
+                                                     const float* __restrict__ r, float* __restrict__ bc,
+                                                     const Scalars* __restrict__ S) {
 	if (S->done) {
 		return;
 	}
-	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-	if (i >= nb) {
-		return;
-	}
+	int64_t const J = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kRestrictLanes;
+	restrict_row<BSF>(J, J < nc, threadIdx.x % kRestrictLanes, rptr, rent, rrow, P, r, bc);
+}
+
+// Row i of x += P x_c.
+template <int BSF>
+__device__ __forceinline__ void prolong_row(int i, const int32_t* __restrict__ pptr, const int32_t* __restrict__ pcol,
+                                            const float* __restrict__ P, const float* __restrict__ xc,
+                                            float* __restrict__ x) {
 	float acc[BSF];
 #pragma unroll
 	for (int k = 0; k < BSF; k++) {
@@ -889,6 +912,21 @@ __global__ __launch_bounds__(kBlock) void k_prolong(int nb, const int32_t* __res
 #pragma unroll
 	for (int k = 0; k < BSF; k++) {
 		x[BSF * static_cast<int64_t>(i) + k] = acc[k];
+	}
+}
+
+// x_i += sum over the entries e of row i of P_e x_c[pcol[e]].
+template <int BSF>
+__global__ __launch_bounds__(kBlock) void k_prolong(int nb, const int32_t* __restrict__ pptr,
+                                                    const int32_t* __restrict__ pcol, const float* __restrict__ P,
+                                                    const float* __restrict__ xc, float* __restrict__ x,
+                                                    const Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	if (i < nb) {
+		prolong_row<BSF>(i, pptr, pcol, P, xc, x);
 	}
 }
 
@@ -928,6 +966,100 @@ __global__ __launch_bounds__(kBlock) void k_dot_partials(int n, const float* __r
 	int const i = blockIdx.x * kBlock + threadIdx.x;
 	double const v = i < n ? static_cast<double>(x[i]) * q[i] : 0.0;
 	wg_partial<true>(v, partial);
+}
+
+// ------------------------------------------------- one-workgroup coarse cycle
+// The small coarse levels (at most kWgRows block rows each) run their whole
+// V-cycle in one workgroup: the same row bodies as the grid kernels,
+// separated by workgroup barriers instead of kernel boundaries (a level of
+// 1,400 rows takes a few microseconds per pass from L2, where each grid
+// launch would cost its own launch latency). Descriptor 0 is the first such
+// level; its correction ends in x1 (in x0 when it is a dense coarsest).
+constexpr int kWg = 1024;
+constexpr int kWgRows = 256;
+
+__device__ __forceinline__ void wg_cheb(const AmgLevelDesc& L, bool first, int step, const float* xin, float* xout) {
+	float const c1 = static_cast<float>(L.scal[1 + 2 * step]);
+	float const c2 = static_cast<float>(L.scal[2 + 2 * step]);
+	for (int i = threadIdx.x; i < L.nb; i += kWg) {
+		float xo[3];
+		if (first) {
+			cheb_row<3, float, true>(i, L.slice_ptr, L.col, L.val32, L.b, L.dinv, xin, xout, L.d, c1, c2, step, xo);
+		} else {
+			cheb_row<3, float, false>(i, L.slice_ptr, L.col, L.val32, L.b, L.dinv, xin, xout, L.d, c1, c2, step, xo);
+		}
+	}
+	__syncthreads();
+}
+
+__global__ __launch_bounds__(kWg) void k_coarse_cycle(const AmgLevelDesc* __restrict__ lv, int nl,
+                                                      const Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	int cur[kMaxLevels];
+	for (int l = 0; l + 1 < nl; l++) {
+		AmgLevelDesc const L = lv[l];
+		wg_cheb(L, true, 0, nullptr, L.x0);
+		int c = 0;
+		for (int k = 1; k < L.deg; k++) {
+			wg_cheb(L, false, k, c ? L.x1 : L.x0, c ? L.x0 : L.x1);
+			c ^= 1;
+		}
+		cur[l] = c;
+		float* const x = c ? L.x1 : L.x0;
+		for (int i = threadIdx.x; i < L.nb; i += kWg) {
+			resid_row<3, float>(i, L.slice_ptr, L.col, L.val32, L.b, x, L.r);
+		}
+		__syncthreads();
+		AmgLevelDesc const C = lv[l + 1];
+		for (int64_t base = 0; base < C.nb; base += kWg / kRestrictLanes) {
+			int64_t const J = base + threadIdx.x / kRestrictLanes;
+			restrict_row<3>(J, J < C.nb, threadIdx.x % kRestrictLanes, L.rptr, L.rent, L.rrow, L.P, L.r, C.b);
+		}
+		__syncthreads();
+	}
+	AmgLevelDesc const Lc = lv[nl - 1];
+	if (Lc.dense) {
+		int const n = 3 * Lc.nb;
+		int const lane = threadIdx.x & 63;
+		for (int row = threadIdx.x >> 6; row < n; row += kWg / 64) {
+			double acc = 0;
+			for (int j = lane; j < n; j += 64) {
+				acc += Lc.inv[static_cast<int64_t>(row) * n + j] * static_cast<double>(Lc.b[j]);
+			}
+			acc = wave_sum(acc);
+			if (lane == 0) {
+				Lc.x0[row] = static_cast<float>(acc);
+			}
+		}
+		__syncthreads();
+		cur[nl - 1] = 0;
+	} else {
+		wg_cheb(Lc, true, 0, nullptr, Lc.x0);
+		int c = 0;
+		for (int k = 1; k < Lc.deg; k++) {
+			wg_cheb(Lc, false, k, c ? Lc.x1 : Lc.x0, c ? Lc.x0 : Lc.x1);
+			c ^= 1;
+		}
+		cur[nl - 1] = c;
+	}
+	for (int l = nl - 2; l >= 0; l--) {
+		AmgLevelDesc const L = lv[l];
+		AmgLevelDesc const C = lv[l + 1];
+		const float* xc = cur[l + 1] ? C.x1 : C.x0;
+		int c = cur[l];
+		float* const x = c ? L.x1 : L.x0;
+		for (int i = threadIdx.x; i < L.nb; i += kWg) {
+			prolong_row<3>(i, L.pptr, L.pcol, L.P, xc, x);
+		}
+		__syncthreads();
+		for (int k = 0; k < L.deg; k++) {
+			wg_cheb(L, false, k, c ? L.x1 : L.x0, c ? L.x0 : L.x1);
+			c ^= 1;
+		}
+		cur[l] = c;
+	}
 }
 
 // ------------------------------------------------------------------ host topology
@@ -1084,6 +1216,8 @@ void amg_release(bfmx_system* s) {
 		dfree(L.d_partial);
 	}
 	s->amg.clear();
+	dfree(s->d_amg_desc);
+	s->amg_wg = -1;
 }
 
 // Needs the system's FP32 matrix buffer (s->d_val32) allocated: level 0
@@ -1381,6 +1515,45 @@ int amg_build(bfmx_system* s) {
 			size_t const n = static_cast<size_t>(lv.back().bs) * lv.back().nb;
 			BFMX_CHECK(dalloc(&lv.back().d_dense, n * n));
 		}
+		// the small coarse levels run in one workgroup
+		s->amg_wg = -1;
+		for (size_t l = 1; l < lv.size(); l++) {
+			if (lv[l].nb <= kWgRows) {
+				s->amg_wg = static_cast<int>(l);
+				break;
+			}
+		}
+		if (s->amg_wg > 0) {
+			std::vector<AmgLevelDesc> desc;
+			for (size_t l = s->amg_wg; l < lv.size(); l++) {
+				AmgLevel const& L = lv[l];
+				bool const last = l + 1 == lv.size();
+				AmgLevelDesc D{};
+				D.nb = L.nb;
+				D.deg = last ? kCoarseDeg : s->amg_par.deg;
+				D.dense = last && L.d_dense ? 1 : 0;
+				D.slice_ptr = L.d_slice_ptr;
+				D.col = L.d_col;
+				D.val32 = L.d_val32;
+				D.dinv = L.d_dinv;
+				D.scal = L.d_scal;
+				D.b = L.d_b;
+				D.x0 = L.d_x[0];
+				D.x1 = L.d_x[1];
+				D.d = L.d_d;
+				D.r = L.d_r;
+				D.rptr = L.d_rptr;
+				D.rent = L.d_rent;
+				D.rrow = L.d_rrow;
+				D.P = L.d_P;
+				D.pptr = L.d_pptr;
+				D.pcol = L.d_pcol;
+				D.inv = L.d_dense;
+				desc.push_back(D);
+			}
+			BFMX_CHECK(upload(&s->d_amg_desc, desc, st));
+			BFMX_HIP(hipStreamSynchronize(st));
+		}
 		return BFMX_OK;
 	}();
 	if (rc != BFMX_OK) {
@@ -1488,6 +1661,13 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 	hipStream_t const st = s->stream;
 	AmgLevel& L = s->amg[l];
 	bool const coarsest = l + 1 == s->amg.size();
+	if (BS == 3 && static_cast<int>(l) == s->amg_wg) {
+		// b is this level's own d_b (restricted into it); the workgroup cycle
+		// runs every level from here down
+		int const nl = static_cast<int>(s->amg.size() - l);
+		k_coarse_cycle<<<1, kWg, 0, st>>>(s->d_amg_desc, nl, S);
+		return nl == 1 && L.d_dense ? L.d_x[0] : L.d_x[1];
+	}
 	if (coarsest && L.d_dense) {
 		int const n = BS * L.nb;
 		int const g = (n + kDenseRowsPerBlock - 1) / kDenseRowsPerBlock;

@@ -165,6 +165,9 @@ __device__ __forceinline__ dbl2 row_mul(const BatchInst& S, int64_t i, LoadV ldv
 	int const len = S.row_len[i];
 	const double* v = S.val + 4 * static_cast<int64_t>(beg) - 2 * r;
 	double y0 = 0, y1 = 0;
+	// unrolled so that the column, value and gather loads of several blocks
+	// are in flight together (a row holds 3-9 blocks)
+#pragma unroll 4
 	for (int k = 0; k < len; k++, v += 4 * kSlice) {
 		int32_t const c = S.col[beg + k * kSlice];
 		dbl2 const t = ld2(v), u = ld2(v + 2 * kSlice);

@@ -66,7 +66,7 @@ struct alignas(128) Scalars {
 	double rr_ref;   // r . r at the last recomputation (reliable updates)
 	double rr_best;  // lowest true r . r seen by the tolerance checks
 	int64_t iter;
-	int32_t done;    // 0 running, 1 converged, 2 breakdown, 3 stagnated
+	int32_t done;    // 0 running, 1 converged, 2 breakdown, 3 stagnated, 4 holding for a tolerance check
 	int32_t stall;   // consecutive tolerance checks without halving the true residual
 	int32_t checking;// a tolerance check has failed: check at every chunk end
 	int32_t updates; // true-residual recomputations
@@ -119,6 +119,32 @@ struct AmgLevel {
 	float* d_d = nullptr;
 	float* d_r = nullptr;
 	double* d_partial = nullptr;      // per-workgroup maxima / sums
+};
+
+// Device view of a coarse multigrid level (three DOFs per row) for the
+// one-workgroup coarse cycle.
+struct AmgLevelDesc {
+	int nb;             // block rows
+	int deg;            // Chebyshev steps (the coarsest: its iteration count)
+	int dense;          // the coarsest level, solved by its dense inverse
+	int pad;
+	const int32_t* slice_ptr;
+	const int32_t* col;
+	const float* val32;
+	const float* dinv;
+	const double* scal;
+	float* b;
+	float* x0;
+	float* x1;
+	float* d;
+	float* r;
+	const int32_t* rptr;   // transfer to the next level (absent on the coarsest)
+	const int32_t* rent;
+	const int32_t* rrow;
+	const float* P;
+	const int32_t* pptr;
+	const int32_t* pcol;
+	const double* inv;     // dense inverse of the coarsest
 };
 
 // Multigrid parameters of a hierarchy, fixed when it is built (defaults,
@@ -212,6 +238,8 @@ struct bfmx_system {
 	std::vector<bfmx::Graph> graphs;   // captured chunks, one per solver configuration
 	std::vector<bfmx::AmgLevel> amg;   // multigrid hierarchy (built on first use)
 	bfmx::AmgParams amg_par;           // its parameters
+	int amg_wg = -1;                   // first level run by the one-workgroup coarse cycle (-1: none)
+	bfmx::AmgLevelDesc* d_amg_desc = nullptr;   // descriptors of the levels from amg_wg on
 	bfmx_model* model = nullptr;       // learned preconditioner (borrowed)
 	float* d_mp_feat = nullptr;        // its node features (8 per node)
 	float* d_mp_h[4] = {nullptr, nullptr, nullptr, nullptr};   // hidden features (FP32, 64 per node)

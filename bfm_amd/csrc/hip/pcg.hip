@@ -133,6 +133,7 @@ __device__ __forceinline__ void write_partials(double (&v)[NV], double* __restri
 }
 
 constexpr int kRed = 1024;   // threads of the one-workgroup reduce kernels
+constexpr int32_t kHold = 4;   // Scalars::done while a flexible-CG chunk waits for its tolerance check
 
 // Sums partial[k * n + i], i < n, for k < NV in a fixed order; the totals are
 // valid in thread 0.
@@ -379,7 +380,10 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(int nb, const double* __r
 	write_partials<1>(v, partial);
 }
 
-// r . r and the iteration count.
+// r . r and the iteration count. When the recursive residual meets the
+// tolerance (before any check has failed) the run holds (done = 4): the
+// chunk's remaining kernels return at once and its closing residual kernel
+// decides (converged, or running on with a check at every chunk end).
 __global__ __launch_bounds__(kRed) void k_red_fcg_rr(int nwg, const double* __restrict__ partial,
                                                      Scalars* __restrict__ S) {
 	if (S->done) {
@@ -390,6 +394,9 @@ __global__ __launch_bounds__(kRed) void k_red_fcg_rr(int nwg, const double* __re
 	if (threadIdx.x == 0) {
 		S->rr = tot[0];
 		S->iter += 1;
+		if (tot[0] <= S->tol2 && !S->checking) {
+			S->done = kHold;
+		}
 	}
 }
 
@@ -398,7 +405,7 @@ __global__ __launch_bounds__(kRed) void k_red_fcg_rr(int nwg, const double* __re
 // recomputation, or a true-residual check has already failed (from then on
 // the true residual is checked at the end of every chunk).
 __device__ __forceinline__ bool residual_due(const Scalars* S, double delta2) {
-	return !S->done && (S->rr <= S->tol2 || S->rr <= delta2 * S->rr_ref || S->checking);
+	return (S->done == 0 || S->done == kHold) && (S->rr <= S->tol2 || S->rr <= delta2 * S->rr_ref || S->checking);
 }
 
 // r = b - A x with the FP64 matrix, z = M^-1 r; partials r.z, r.r, b.b.
@@ -471,6 +478,9 @@ __global__ __launch_bounds__(kRed) void k_red_residual(int nwg, int mode, double
 			S->beta = S->rz_prev != 0 ? tot[0] / S->rz_prev : 0;
 		}
 		S->rz = tot[0];
+		if (S->done == kHold) {
+			S->done = 0;
+		}
 		if (rr <= S->tol2) {
 			S->done = 1;
 		} else if (check) {
