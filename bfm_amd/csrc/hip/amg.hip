@@ -47,7 +47,8 @@ namespace {
 constexpr int kCoarseDeg = 16;  // Chebyshev steps on a coarsest level too large for the dense inverse
 constexpr int kMaxDeg = 16;
 constexpr int kMaxLevels = 12;
-constexpr int kDenseMax = 768;  // largest coarsest operator (DOFs) inverted densely
+constexpr int kDenseMax = 768;  // largest coarsest segment (DOFs) inverted densely
+constexpr int64_t kDenseTotal = int64_t(1) << 27;   // entries of all the coarsest's dense inverses (1 GiB)
 constexpr double kCoarseRatio = 200.0;  // Chebyshev interval of an iterated coarsest level
 // A level's scalars: [0] lambda_max of its Chebyshev interval (the
 // Gershgorin bound), [1 .. 2 kMaxDeg] the Chebyshev coefficients, then the
@@ -57,7 +58,8 @@ constexpr int kScalScale = 2 + 2 * kMaxDeg;
 constexpr int kScalLambda = kScalScale + 1;
 constexpr int kScalSize = kScalLambda + 1;
 constexpr double kDropTol = 1e-10;      // relative remainder below which a mode column is dropped
-constexpr int kGj = 1024;               // threads of the Gauss-Jordan workgroup
+constexpr int kGj = 1024;               // threads of the Gauss-Jordan workgroup (one segment)
+constexpr int kGjSeg = 256;             // ... (one workgroup per segment of many)
 
 int grid(int64_t n) {
 	return static_cast<int>((n + kBlock - 1) / kBlock);
@@ -639,45 +641,55 @@ __global__ __launch_bounds__(kBlock) void k_cheb_coef(int nwg, const double* __r
 	}
 }
 
-// Dense copy of a level's operator (n = BS * nb), one block row per thread;
-// padding blocks carry zeros and add nothing.
+// Dense copy of each segment of a level's operator (a segment of m block
+// rows: BS m x BS m, row-major, at seg_off of its segment), one block row
+// per thread; padding blocks carry zeros and add nothing.
 template <int BS>
 __global__ __launch_bounds__(kBlock) void k_dense(int nb, const int32_t* __restrict__ slice_ptr,
                                                   const int32_t* __restrict__ col, const double* __restrict__ val,
-                                                  double* __restrict__ A) {
+                                                  const int32_t* __restrict__ seg_ptr,
+                                                  const int32_t* __restrict__ seg_row,
+                                                  const int64_t* __restrict__ seg_off, double* __restrict__ A) {
 	int const i = blockIdx.x * kBlock + threadIdx.x;
 	if (i >= nb) {
 		return;
 	}
-	int const n = BS * nb, r = i & 63;
+	int const sg = seg_row[i], base = seg_ptr[sg];
+	int const n = BS * (seg_ptr[sg + 1] - base), r = i & 63, li = i - base;
+	double* const As = A + seg_off[sg];
 	for (int a = 0; a < BS; a++) {
 		for (int j = 0; j < n; j++) {
-			A[static_cast<int64_t>(BS * i + a) * n + j] = 0;
+			As[static_cast<int64_t>(BS * li + a) * n + j] = 0;
 		}
 	}
 	for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
-		int32_t const c = col[blk];
+		int32_t const c = col[blk] - base;
 		double B[BS][BS];
 		ld_block<BS>(val, blk, r, B);
 #pragma unroll
 		for (int a = 0; a < BS; a++) {
 #pragma unroll
 			for (int b = 0; b < BS; b++) {
-				A[static_cast<int64_t>(BS * i + a) * n + BS * c + b] += B[a][b];
+				As[static_cast<int64_t>(BS * li + a) * n + BS * c + b] += B[a][b];
 			}
 		}
 	}
 }
 
-// In-place Gauss-Jordan inverse of the SPD n x n matrix A by one workgroup,
-// without pivoting (the pivots of an SPD matrix are positive). A zero diagonal
-// entry marks a decoupled DOF (zero row and column) and is replaced by 1.
+// In-place Gauss-Jordan inverse of the SPD n x n matrix of one segment
+// (n = bs x its rows) by one workgroup per segment, without pivoting (the
+// pivots of an SPD matrix are positive). A zero diagonal entry marks a
+// decoupled DOF (zero row and column) and is replaced by 1.
 // Step k: a_kk <- 1/p, a_kj <- a_kj/p, a_ik <- -a_ik/p,
 // a_ij <- a_ij - a_ik a_kj / p (p = a_kk), with row and column k staged in LDS
 // so every update reads the values of the step's start.
-__global__ __launch_bounds__(kGj) void k_gauss_jordan(int n, double* __restrict__ A) {
+template <int NT>
+__global__ __launch_bounds__(NT) void k_gauss_jordan(int bs, const int32_t* __restrict__ seg_ptr,
+                                                     const int64_t* __restrict__ seg_off, double* __restrict__ A) {
 	__shared__ double rowk[kDenseMax], colk[kDenseMax];
-	for (int i = threadIdx.x; i < n; i += kGj) {
+	int const n = bs * (seg_ptr[blockIdx.x + 1] - seg_ptr[blockIdx.x]);
+	A += seg_off[blockIdx.x];
+	for (int i = threadIdx.x; i < n; i += NT) {
 		if (A[static_cast<int64_t>(i) * n + i] == 0) {
 			A[static_cast<int64_t>(i) * n + i] = 1;
 		}
@@ -685,13 +697,13 @@ __global__ __launch_bounds__(kGj) void k_gauss_jordan(int n, double* __restrict_
 	__syncthreads();
 	int64_t const nn = static_cast<int64_t>(n) * n;
 	for (int k = 0; k < n; k++) {
-		for (int j = threadIdx.x; j < n; j += kGj) {
+		for (int j = threadIdx.x; j < n; j += NT) {
 			rowk[j] = A[static_cast<int64_t>(k) * n + j];
 			colk[j] = A[static_cast<int64_t>(j) * n + k];
 		}
 		__syncthreads();
 		double const p = 1.0 / rowk[k];
-		for (int64_t e = threadIdx.x; e < nn; e += kGj) {
+		for (int64_t e = threadIdx.x; e < nn; e += NT) {
 			int const i = static_cast<int>(e / n), j = static_cast<int>(e - static_cast<int64_t>(i) * n);
 			double v;
 			if (i == k) {
@@ -930,14 +942,33 @@ __global__ __launch_bounds__(kBlock) void k_prolong(int nb, const int32_t* __res
 	}
 }
 
-// x = A^-1 b with the dense inverse, one wavefront per row (the inverse is
-// symmetric, so row i is read as column i: coalesced across the lanes).
+// Row `row` (a DOF) of x = A^-1 b with the dense inverse of its segment, by
+// one wavefront (lane `lane`); every lane returns the sum. The inverse is
+// symmetric, so the row is read as a column: coalesced across the lanes.
+template <int BS, typename BT>
+__device__ __forceinline__ double dense_row(int row, int lane, const int32_t* __restrict__ seg_ptr,
+                                            const int32_t* __restrict__ seg_row, const int64_t* __restrict__ seg_off,
+                                            const double* __restrict__ Ainv, const BT* __restrict__ b) {
+	int const sg = seg_row[row / BS];
+	int const base = BS * seg_ptr[sg], n = BS * seg_ptr[sg + 1] - base;
+	const double* const Ar = Ainv + seg_off[sg] + static_cast<int64_t>(row - base) * n;
+	double acc = 0;
+	for (int j = lane; j < n; j += 64) {
+		acc += Ar[j] * static_cast<double>(b[base + j]);
+	}
+	return wave_sum(acc);
+}
+
+// x = A^-1 b with the dense inverses of the coarsest's segments, one
+// wavefront per row.
 constexpr int kDenseRowsPerBlock = kBlock / 64;
 
-template <typename BT>
-__global__ __launch_bounds__(kBlock) void k_dense_solve(int n, const double* __restrict__ Ainv,
-                                                        const BT* __restrict__ b, float* __restrict__ x,
-                                                        const Scalars* __restrict__ S) {
+template <int BS, typename BT>
+__global__ __launch_bounds__(kBlock) void k_dense_solve(int n, const int32_t* __restrict__ seg_ptr,
+                                                        const int32_t* __restrict__ seg_row,
+                                                        const int64_t* __restrict__ seg_off,
+                                                        const double* __restrict__ Ainv, const BT* __restrict__ b,
+                                                        float* __restrict__ x, const Scalars* __restrict__ S) {
 	if (S->done) {
 		return;
 	}
@@ -946,11 +977,7 @@ __global__ __launch_bounds__(kBlock) void k_dense_solve(int n, const double* __r
 	if (row >= n) {
 		return;   // whole wavefronts leave together
 	}
-	double acc = 0;
-	for (int j = lane; j < n; j += 64) {
-		acc += Ainv[static_cast<int64_t>(row) * n + j] * static_cast<double>(b[j]);
-	}
-	acc = wave_sum(acc);
+	double const acc = dense_row<BS, BT>(row, lane, seg_ptr, seg_row, seg_off, Ainv, b);
 	if (lane == 0) {
 		x[row] = static_cast<float>(acc);
 	}
@@ -1024,11 +1051,7 @@ __global__ __launch_bounds__(kWg) void k_coarse_cycle(const AmgLevelDesc* __rest
 		int const n = 3 * Lc.nb;
 		int const lane = threadIdx.x & 63;
 		for (int row = threadIdx.x >> 6; row < n; row += kWg / 64) {
-			double acc = 0;
-			for (int j = lane; j < n; j += 64) {
-				acc += Lc.inv[static_cast<int64_t>(row) * n + j] * static_cast<double>(Lc.b[j]);
-			}
-			acc = wave_sum(acc);
+			double const acc = dense_row<3, float>(row, lane, Lc.seg_ptr, Lc.seg_row, Lc.seg_off, Lc.inv, Lc.b);
 			if (lane == 0) {
 				Lc.x0[row] = static_cast<float>(acc);
 			}
@@ -1172,8 +1195,45 @@ bool amg_smoothing() {
 	return !(e && e[0] == '0');
 }
 
-bool dense_coarsest(const AmgLevel& L) {
-	return L.bs * static_cast<int64_t>(L.nb) <= kDenseMax;
+// Row ranges of the graph (ptr, adj) that no entry couples, as offsets (0,
+// ..., n): a boundary follows row i when no row up to i has a neighbour
+// after i (the adjacency is symmetric).
+std::vector<int32_t> segments(int n, const int64_t* ptr, const int32_t* adj) {
+	std::vector<int32_t> seg{0};
+	int reach = 0;
+	for (int i = 0; i < n; i++) {
+		for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+			reach = std::max(reach, adj[e]);
+		}
+		if (reach <= i) {
+			seg.push_back(i + 1);
+		}
+	}
+	if (seg.back() != n) {
+		seg.push_back(n);
+	}
+	return seg;
+}
+
+int seg_largest(const std::vector<int32_t>& seg) {
+	int m = 0;
+	for (size_t c = 0; c + 1 < seg.size(); c++) {
+		m = std::max(m, seg[c + 1] - seg[c]);
+	}
+	return m;
+}
+
+// Entries of the coarsest's dense inverses (0: too large, Chebyshev instead).
+int64_t dense_entries(int bs, const std::vector<int32_t>& seg) {
+	int64_t total = 0;
+	for (size_t c = 0; c + 1 < seg.size(); c++) {
+		int64_t const n = static_cast<int64_t>(bs) * (seg[c + 1] - seg[c]);
+		if (n > kDenseMax) {
+			return 0;
+		}
+		total += n * n;
+	}
+	return total <= kDenseTotal ? total : 0;
 }
 
 }  // namespace
@@ -1208,6 +1268,9 @@ void amg_release(bfmx_system* s) {
 		dfree(L.d_gal_pi);
 		dfree(L.d_gal_pj);
 		dfree(L.d_dense);
+		dfree(L.d_seg_ptr);
+		dfree(L.d_seg_row);
+		dfree(L.d_seg_off);
 		dfree(L.d_b);
 		dfree(L.d_x[0]);
 		dfree(L.d_x[1]);
@@ -1229,6 +1292,16 @@ int amg_build(bfmx_system* s) {
 	if (!s->d_val32) {
 		set_error("multigrid set-up before the FP32 matrix buffer");
 		return BFMX_E_ARG;
+	}
+	if (s->coords.size() != static_cast<size_t>(2 * s->n_nodes)) {
+		// the rigid-body modes need the coordinates; a system of a batch built
+		// from meshes keeps them on the device only
+		if (!s->d_coords) {
+			set_error("multigrid set-up needs the node coordinates");
+			return BFMX_E_ARG;
+		}
+		s->coords.resize(2 * s->n_nodes);
+		BFMX_HIP(hipMemcpy(s->coords.data(), s->d_coords, s->coords.size() * sizeof(double), hipMemcpyDeviceToHost));
 	}
 	int const rc = [&]() -> int {
 		hipStream_t const st = s->stream;
@@ -1301,14 +1374,37 @@ int amg_build(bfmx_system* s) {
 		if (const char* e = std::getenv("BFMX_AMG_POWER")) {
 			par.power = std::max(0, std::atoi(e));
 		}
-		while (static_cast<int>(lv.size()) < kMaxLevels && lv.back().nb > par.coarsest) {
+		// segments of the current level: aggregates never span two, and the
+		// first row of a segment is an aggregate root (its neighbours are all
+		// in the segment and not yet aggregated), so the aggregates of a
+		// segment are numbered consecutively and give the next level's segments
+		std::vector<int32_t> fseg = segments(static_cast<int>(n0), s->adj_ptr.data(), s->adj.data());
+		while (static_cast<int>(lv.size()) < kMaxLevels && seg_largest(fseg) > par.coarsest) {
 			int const lf = static_cast<int>(lv.size()) - 1;
 			const int64_t* ptr = lf == 0 ? s->adj_ptr.data() : fptr.data();
 			const int32_t* adj = lf == 0 ? s->adj.data() : fadj.data();
 			int const nf = lv[lf].nb;
 			std::vector<int32_t> agg;
 			int const na = aggregate(nf, ptr, adj, agg);
-			if (na * 10 > nf * 8) {
+			std::vector<int32_t> cseg(fseg.size());
+			for (size_t c = 0; c + 1 < fseg.size(); c++) {
+				cseg[c] = agg[fseg[c]];
+			}
+			cseg.back() = na;
+			if (!std::is_sorted(cseg.begin(), cseg.end()) || cseg[0] != 0) {
+				set_error("multigrid: aggregates do not follow the level's segments");
+				return BFMX_E_ARG;
+			}
+			// stalled when the segments still above the target size shrink by
+			// less than a fifth
+			int64_t big_f = 0, big_a = 0;
+			for (size_t c = 0; c + 1 < fseg.size(); c++) {
+				if (fseg[c + 1] - fseg[c] > par.coarsest) {
+					big_f += fseg[c + 1] - fseg[c];
+					big_a += cseg[c + 1] - cseg[c];
+				}
+			}
+			if (big_a * 10 > big_f * 8) {
 				break;   // coarsening stalled
 			}
 			// members of each aggregate
@@ -1498,6 +1594,7 @@ int amg_build(bfmx_system* s) {
 			fadj.swap(cadj);
 			fslot.swap(CS.slot_of);
 			fslice.swap(CS.slice_ptr);
+			fseg.swap(cseg);
 		}
 		for (auto& L : lv) {
 			size_t const m = static_cast<size_t>(L.bs) * L.nb;
@@ -1511,9 +1608,35 @@ int amg_build(bfmx_system* s) {
 			BFMX_CHECK(dalloc(&L.d_partial, grid(L.nb) + grid(static_cast<int64_t>(L.bs) * L.nb)));
 			BFMX_CHECK(dalloc(&L.d_B, 3 * m));
 		}
-		if (dense_coarsest(lv.back())) {
-			size_t const n = static_cast<size_t>(lv.back().bs) * lv.back().nb;
-			BFMX_CHECK(dalloc(&lv.back().d_dense, n * n));
+		{
+			AmgLevel& Lc = lv.back();
+			Lc.n_seg = static_cast<int>(fseg.size()) - 1;
+			int64_t const entries = dense_entries(Lc.bs, fseg);
+			if (entries > 0) {
+				std::vector<int32_t> seg_row(Lc.nb);
+				std::vector<int64_t> seg_off(Lc.n_seg);
+				int64_t off = 0;
+				for (int c = 0; c < Lc.n_seg; c++) {
+					std::fill(seg_row.begin() + fseg[c], seg_row.begin() + fseg[c + 1], c);
+					seg_off[c] = off;
+					int64_t const n = static_cast<int64_t>(Lc.bs) * (fseg[c + 1] - fseg[c]);
+					off += n * n;
+				}
+				BFMX_CHECK(dalloc(&Lc.d_dense, static_cast<size_t>(entries)));
+				BFMX_CHECK(upload(&Lc.d_seg_ptr, fseg, st));
+				BFMX_CHECK(upload(&Lc.d_seg_row, seg_row, st));
+				BFMX_CHECK(upload(&Lc.d_seg_off, seg_off, st));
+				BFMX_HIP(hipStreamSynchronize(st));
+			}
+		}
+		if (const char* e = std::getenv("BFMX_AMG_VERBOSE"); e && e[0] == '1') {
+			for (size_t l = 0; l < lv.size(); l++) {
+				std::fprintf(stderr, "multigrid level %zu: %d rows of %d DOFs, %lld blocks\n", l, lv[l].nb, lv[l].bs,
+				             static_cast<long long>(lv[l].nnzb));
+			}
+			AmgLevel const& Lc = lv.back();
+			std::fprintf(stderr, "multigrid coarsest: %d segments, largest %d rows, %s\n", Lc.n_seg, seg_largest(fseg),
+			             Lc.d_dense ? "dense inverses" : "Chebyshev iteration");
 		}
 		// the small coarse levels run in one workgroup
 		s->amg_wg = -1;
@@ -1549,6 +1672,9 @@ int amg_build(bfmx_system* s) {
 				D.pptr = L.d_pptr;
 				D.pcol = L.d_pcol;
 				D.inv = L.d_dense;
+				D.seg_ptr = L.d_seg_ptr;
+				D.seg_row = L.d_seg_row;
+				D.seg_off = L.d_seg_off;
 				desc.push_back(D);
 			}
 			BFMX_CHECK(upload(&s->d_amg_desc, desc, st));
@@ -1574,13 +1700,18 @@ int amg_setup_numeric(bfmx_system* s) {
 		AmgLevel& L = lv[l];
 		bool const coarsest = l + 1 == lv.size();
 		if (coarsest && L.d_dense) {
-			int const n = L.bs * L.nb;
 			if (L.bs == 2) {
-				k_dense<2><<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_dense);
+				k_dense<2><<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_seg_ptr,
+				                                          L.d_seg_row, L.d_seg_off, L.d_dense);
 			} else {
-				k_dense<3><<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_dense);
+				k_dense<3><<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_seg_ptr,
+				                                          L.d_seg_row, L.d_seg_off, L.d_dense);
 			}
-			k_gauss_jordan<<<1, kGj, 0, st>>>(n, L.d_dense);
+			if (L.n_seg == 1) {
+				k_gauss_jordan<kGj><<<1, kGj, 0, st>>>(L.bs, L.d_seg_ptr, L.d_seg_off, L.d_dense);
+			} else {
+				k_gauss_jordan<kGjSeg><<<L.n_seg, kGjSeg, 0, st>>>(L.bs, L.d_seg_ptr, L.d_seg_off, L.d_dense);
+			}
 			continue;
 		}
 		int const g = grid(L.nb);
@@ -1671,7 +1802,8 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 	if (coarsest && L.d_dense) {
 		int const n = BS * L.nb;
 		int const g = (n + kDenseRowsPerBlock - 1) / kDenseRowsPerBlock;
-		k_dense_solve<BT><<<g, kBlock, 0, st>>>(n, L.d_dense, b, L.d_x[0], S);
+		k_dense_solve<BS, BT><<<g, kBlock, 0, st>>>(n, L.d_seg_ptr, L.d_seg_row, L.d_seg_off, L.d_dense, b, L.d_x[0],
+		                                            S);
 		if (q) {
 			k_dot_partials<<<grid(n), kBlock, 0, st>>>(n, L.d_x[0], q, partial, S);
 		}

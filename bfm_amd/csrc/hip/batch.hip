@@ -419,6 +419,112 @@ void launch(int pre, int cfg, int nblocks, hipStream_t st, const BatchInst* inst
 	}
 }
 
+// ------------------------------------------------------------ multigrid mode
+// The batch as one block-diagonal system: the instances' rows one after
+// another (each in its own renumbered order), solved by flexible CG with the
+// multigrid V-cycle. No entry couples two instances, so no aggregate spans
+// two and the coarsest level is inverted instance by instance. Every
+// instance's load is scaled to unit norm: with m instances of nonzero load
+// ||b|| = sqrt(m), and every instance's residual is at most the union's, so
+// the union solved to rtol / sqrt(m) leaves each instance at rtol or below.
+constexpr int kUniBlock = 256;
+
+// Per instance: b . b (fixed summation order).
+__global__ __launch_bounds__(kUniBlock) void k_batch_bnorm(const BatchInst* __restrict__ insts,
+                                                           double* __restrict__ bb) {
+	__shared__ double lds[1][kUniBlock / 64];
+	BatchInst const S = insts[blockIdx.x];
+	double v[1] = {0};
+	for (int64_t i = threadIdx.x; i < 2 * S.n; i += kUniBlock) {
+		v[0] += S.b[i] * S.b[i];
+	}
+	wg_sum<kUniBlock, 1>(v, lds);
+	if (threadIdx.x == 0) {
+		bb[blockIdx.x] = v[0];
+	}
+}
+
+// Instance of union row R: off[k] <= R < off[k + 1].
+__device__ __forceinline__ int instance_of(int64_t R, int m, const int64_t* __restrict__ off) {
+	int lo = 0, hi = m;
+	while (hi - lo > 1) {
+		int const mid = (lo + hi) >> 1;
+		if (off[mid] <= R) {
+			lo = mid;
+		} else {
+			hi = mid;
+		}
+	}
+	return lo;
+}
+
+// Union row R from row i of its instance: the row's blocks (same column
+// order, union slots at the union's slice offsets; padding slots stay
+// zero), the load scaled by 1 / ||b_k|| and the fixed flags.
+__global__ __launch_bounds__(kBlock) void k_union_gather(int64_t n, int m, const int64_t* __restrict__ off,
+                                                         const BatchInst* __restrict__ insts,
+                                                         const double* __restrict__ bb,
+                                                         const int32_t* __restrict__ uslice, double* __restrict__ uval,
+                                                         double* __restrict__ ub, uint8_t* __restrict__ ufixed) {
+	int64_t const R = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
+	if (R >= n) {
+		return;
+	}
+	int const k = instance_of(R, m, off);
+	const BatchInst& S = insts[k];
+	int64_t const i = R - off[k];
+	const double* src = S.val + 4 * static_cast<int64_t>(S.slice_ptr[i >> 6]) + 2 * (i & 63);
+	double* dst = uval + 4 * static_cast<int64_t>(uslice[R >> 6]) + 2 * (R & 63);
+	int const len = S.row_len[i];
+	for (int t = 0; t < len; t++, src += 4 * kSlice, dst += 4 * kSlice) {
+		st2(dst, ld2(src));
+		st2(dst + 2 * kSlice, ld2(src + 2 * kSlice));
+	}
+	double const bk = bb[k];
+	double const sc = bk > 0 ? 1.0 / sqrt(bk) : 0.0;
+	st2(ub + 2 * R, sc * ld2(S.b + 2 * i));
+	ufixed[2 * R] = S.fixed[2 * i];
+	ufixed[2 * R + 1] = S.fixed[2 * i + 1];
+}
+
+// x of every instance from the union's solution, unscaled by ||b_k||.
+__global__ __launch_bounds__(kBlock) void k_union_scatter(int64_t n, int m, const int64_t* __restrict__ off,
+                                                          const BatchInst* __restrict__ insts,
+                                                          const double* __restrict__ bb,
+                                                          const double* __restrict__ ux) {
+	int64_t const R = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x;
+	if (R >= n) {
+		return;
+	}
+	int const k = instance_of(R, m, off);
+	int64_t const i = R - off[k];
+	st2(insts[k].x + 2 * i, sqrt(bb[k]) * ld2(ux + 2 * R));
+}
+
+// Per instance: the true residual ||b - A x||^2 of its own system, and the
+// report of the union's solve.
+__global__ __launch_bounds__(kUniBlock) void k_batch_resid(const BatchInst* __restrict__ insts,
+                                                           const double* __restrict__ bb, int64_t iters,
+                                                           double rtol2, BatchOut* __restrict__ out) {
+	__shared__ double lds[1][kUniBlock / 64];
+	BatchInst const S = insts[blockIdx.x];
+	double v[1] = {0};
+	for (int64_t i = threadIdx.x; i < S.n; i += kUniBlock) {
+		dbl2 const ax = row_mul(S, i, [&](int32_t c) { return ld2(S.x + 2 * static_cast<int64_t>(c)); });
+		dbl2 const rv = ld2(S.b + 2 * i) - ax;
+		v[0] += rv.x * rv.x + rv.y * rv.y;
+	}
+	wg_sum<kUniBlock, 1>(v, lds);
+	if (threadIdx.x == 0) {
+		BatchOut& o = out[blockIdx.x];
+		o.iter = iters;
+		o.rr_true = v[0];
+		o.bb = bb[blockIdx.x];
+		o.status = v[0] <= rtol2 * bb[blockIdx.x] ? 1 : 0;
+		o.checks = 0;
+	}
+}
+
 }  // namespace
 }  // namespace bfmx
 
@@ -452,6 +558,11 @@ struct bfmx_batch {
 	int64_t* d_i64 = nullptr;
 	uint8_t* d_u8 = nullptr;
 	bfmx::CondRec* d_cond = nullptr;
+	// multigrid mode: the batch as one block-diagonal system (built at the
+	// first such solve; shares the batch's stream)
+	bfmx_system* uni = nullptr;
+	int64_t* d_uoff = nullptr;     // first union row of each instance (m + 1)
+	double* d_ubb = nullptr;       // b . b of each instance
 };
 
 namespace bfmx {
@@ -481,6 +592,28 @@ int batch_n_classes() {
 	return kClasses;
 }
 
+// The union system of the multigrid mode: what union_build allocated.
+void union_release(bfmx_batch* b) {
+	bfmx_system* U = b->uni;
+	if (!U) {
+		return;
+	}
+	release_solver(U);
+	dfree(U->d_slice_ptr);
+	dfree(U->d_row_len);
+	dfree(U->d_diag_blk);
+	dfree(U->d_col);
+	dfree(U->d_val);
+	dfree(U->d_b);
+	dfree(U->d_bsolve);
+	dfree(U->d_fixed);
+	U->stream = nullptr;   // the batch's
+	delete U;
+	b->uni = nullptr;
+	dfree(b->d_uoff);
+	dfree(b->d_ubb);
+}
+
 void batch_destroy(bfmx_batch* b) {
 	if (!b) {
 		return;
@@ -489,6 +622,7 @@ void batch_destroy(bfmx_batch* b) {
 	if (b->stream) {
 		(void) hipStreamSynchronize(b->stream);
 	}
+	union_release(b);
 	for (auto& g : b->graphs) {
 		(void) hipGraphExecDestroy(g.exec);
 	}
@@ -602,6 +736,7 @@ int batch_create(bfmx_batch** out, size_t n, bfmx_system* const* systems) {
 			I.q = w + 10 * nn;
 			I.dinv = w + 12 * nn;
 			I.n = nn;
+			I.fixed = s->d_fixed;   // read by the multigrid mode
 			w += 16 * nn;
 		}
 		for (auto* s : b->systems) {
@@ -659,10 +794,170 @@ void launch_all(bfmx_batch* b, int pre, double rtol2, int64_t max_iter, const in
 
 }  // namespace
 
+namespace {
+
+int blocks_of(int64_t n) {
+	return static_cast<int>((n + kBlock - 1) / kBlock);
+}
+
+// The union system of the multigrid mode from the instances' host topology
+// (renumbered adjacency and coordinates), its rows the instances' rows one
+// after another.
+int union_build(bfmx_batch* b) {
+	if (b->uni) {
+		return BFMX_OK;
+	}
+	size_t const m = b->systems.size();
+	std::vector<int64_t> off(m + 1, 0);
+	int64_t nnz = 0;
+	for (size_t k = 0; k < m; k++) {
+		bfmx_system const* s = b->systems[k];
+		if (s->stress == BFMX_AXISYMMETRIC) {
+			set_error("the multigrid mode needs SPD systems (axisymmetric instances are indefinite)");
+			return BFMX_E_UNSUPPORTED;
+		}
+		off[k + 1] = off[k] + s->n_nodes;
+		nnz += s->adj_ptr[s->n_nodes];
+	}
+	int64_t const n = off[m];
+	if (n >= (int64_t(1) << 30)) {
+		set_error("batch too large for the multigrid mode (%lld nodes)", (long long) n);
+		return BFMX_E_ARG;
+	}
+	auto* U = new bfmx_system();
+	b->uni = U;   // released by union_release, also on failure
+	U->device = b->device;
+	U->stream = b->stream;
+	U->kind = b->systems[0]->kind;
+	U->n_nodes = n;
+	U->coords.resize(2 * n);
+	U->adj_ptr.assign(n + 1, 0);
+	U->adj.resize(nnz);
+	for (size_t k = 0; k < m; k++) {
+		bfmx_system const* s = b->systems[k];
+		int64_t const o = off[k], e0 = U->adj_ptr[o];
+		if (s->coords.size() == static_cast<size_t>(2 * s->n_nodes)) {
+			std::copy(s->coords.begin(), s->coords.end(), U->coords.begin() + 2 * o);
+		} else {
+			// a batch built from meshes keeps its coordinates on the device only
+			BFMX_HIP(hipMemcpy(U->coords.data() + 2 * o, s->d_coords, 2 * s->n_nodes * sizeof(double),
+			                   hipMemcpyDeviceToHost));
+		}
+		for (int64_t i = 0; i < s->n_nodes; i++) {
+			U->adj_ptr[o + i + 1] = e0 + s->adj_ptr[i + 1];
+		}
+		for (int64_t e = 0; e < s->adj_ptr[s->n_nodes]; e++) {
+			U->adj[e0 + e] = static_cast<int32_t>(s->adj[e] + o);
+		}
+	}
+	std::vector<int32_t> col;
+	BFMX_CHECK(sell_layout(U, col));
+	hipStream_t const st = b->stream;
+	BFMX_CHECK(dalloc(&U->d_slice_ptr, U->n_slices + 1));
+	BFMX_CHECK(dalloc(&U->d_row_len, n));
+	BFMX_CHECK(dalloc(&U->d_diag_blk, n));
+	BFMX_CHECK(dalloc(&U->d_col, U->nnzb_pad));
+	BFMX_CHECK(dalloc(&U->d_val, 4 * U->nnzb_pad));
+	BFMX_CHECK(dalloc(&U->d_b, 2 * n));
+	BFMX_CHECK(dalloc(&U->d_bsolve, 2 * n));
+	BFMX_CHECK(dalloc(&U->d_fixed, 2 * n));
+	BFMX_CHECK(dalloc(&b->d_uoff, m + 1));
+	BFMX_CHECK(dalloc(&b->d_ubb, m));
+	BFMX_HIP(hipMemcpyAsync(U->d_slice_ptr, U->slice_ptr.data(), U->slice_ptr.size() * sizeof(int32_t),
+	                        hipMemcpyHostToDevice, st));
+	BFMX_HIP(hipMemcpyAsync(U->d_row_len, U->row_len.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	BFMX_HIP(hipMemcpyAsync(U->d_diag_blk, U->diag_blk.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	BFMX_HIP(hipMemcpyAsync(U->d_col, col.data(), col.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	BFMX_HIP(hipMemcpyAsync(b->d_uoff, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+	// padding slots keep zero values; the gather writes every stored block
+	BFMX_HIP(hipMemsetAsync(U->d_val, 0, 4 * static_cast<size_t>(U->nnzb_pad) * sizeof(double), st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	U->assembled = true;
+	U->finalized = true;
+	return BFMX_OK;
+}
+
+// Multigrid mode of bfmx_batch_solve: assembly (batches built from meshes),
+// load norms, gather into the union, flexible CG + V-cycle on the union,
+// scatter of the solutions, per-instance true residuals.
+int batch_solve_amg(bfmx_batch* b, double rtol, int64_t max_iter, bfmx_stats_t* stats, double* total_ms) {
+	BFMX_CHECK(union_build(b));
+	bfmx_system* U = b->uni;
+	hipStream_t const st = b->stream;
+	int const m = static_cast<int>(b->systems.size());
+	int64_t const n = U->n_nodes;
+	BFMX_HIP(hipEventRecord(b->ev[0], st));
+	launch_assemble(b);
+	k_batch_bnorm<<<m, kUniBlock, 0, st>>>(b->d_inst, b->d_ubb);
+	k_union_gather<<<blocks_of(n), kBlock, 0, st>>>(n, m, b->d_uoff, b->d_inst, b->d_ubb, U->d_slice_ptr, U->d_val,
+	                                                U->d_b, U->d_fixed);
+	BFMX_HIP(hipGetLastError());
+	std::vector<double> bb(m);
+	BFMX_HIP(hipMemcpyAsync(bb.data(), b->d_ubb, m * sizeof(double), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	int const nz = static_cast<int>(std::count_if(bb.begin(), bb.end(), [](double v) { return v > 0; }));
+	bfmx_stats_t us{};
+	int rc = BFMX_OK;
+	if (nz > 0) {
+		rc = solve(U, nullptr, BFMX_PRECOND_AMG, BFMX_PRECISION_FP64, rtol / std::sqrt(static_cast<double>(nz)),
+		           static_cast<size_t>(max_iter), &us);
+		if (rc != BFMX_OK && rc != BFMX_E_BREAKDOWN && rc != BFMX_E_STAGNATION && rc != BFMX_E_MAXITER) {
+			return rc;
+		}
+	} else {
+		BFMX_CHECK(ensure_workspace_public(U));
+		BFMX_HIP(hipMemsetAsync(U->d_x, 0, 2 * n * sizeof(double), st));
+	}
+	k_union_scatter<<<blocks_of(n), kBlock, 0, st>>>(n, m, b->d_uoff, b->d_inst, b->d_ubb, U->d_x);
+	k_batch_resid<<<m, kUniBlock, 0, st>>>(b->d_inst, b->d_ubb, us.iterations, rtol * rtol, b->d_out);
+	BFMX_HIP(hipGetLastError());
+	BFMX_HIP(hipEventRecord(b->ev[1], st));
+	BFMX_HIP(hipMemcpyAsync(b->h_out.data(), b->d_out, b->h_out.size() * sizeof(BatchOut), hipMemcpyDeviceToHost,
+	                        st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	mark_assembled(b);
+	float ms = 0;
+	BFMX_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
+	if (total_ms) {
+		*total_ms = ms;
+	}
+	int worst = BFMX_OK;
+	for (int k = 0; k < m; k++) {
+		BatchOut const& o = b->h_out[k];
+		int const sk = o.status == 1 ? BFMX_OK : rc != BFMX_OK ? rc : BFMX_E_STAGNATION;
+		if (sk != BFMX_OK) {
+			worst = sk;
+		}
+		if (stats) {
+			bfmx_stats_t& S = stats[k];
+			S = bfmx_stats_t{};
+			S.status = sk;
+			S.precond = BFMX_PRECOND_AMG;
+			S.precision = BFMX_PRECISION_FP64;
+			S.restarts = us.restarts;
+			S.iterations = o.iter;
+			S.n_dof = 2 * b->systems[k]->n_nodes;
+			S.rtol = rtol;
+			S.rel_residual = o.bb > 0 ? std::sqrt(o.rr_true / o.bb) : 0.0;
+			S.solve_ms = ms;
+			S.b_norm = std::sqrt(o.bb);
+		}
+	}
+	if (worst != BFMX_OK) {
+		set_error("batched multigrid solve: some systems did not reach rtol (per-system status in the stats)");
+	}
+	return worst;
+}
+
+}  // namespace
+
 int batch_solve(bfmx_batch* b, int pre, double rtol, int64_t max_iter, const int32_t* picks_in, bfmx_stats_t* stats,
                 double* total_ms) {
+	if (pre == BFMX_PRECOND_AMG) {
+		return batch_solve_amg(b, rtol, max_iter, stats, total_ms);
+	}
 	if (pre != BFMX_PRECOND_JACOBI && pre != BFMX_PRECOND_BLOCK_JACOBI) {
-		set_error("batched solves use Jacobi or block-Jacobi PCG");
+		set_error("batched solves use Jacobi or block-Jacobi PCG, or the multigrid mode");
 		return BFMX_E_UNSUPPORTED;
 	}
 	int picks[kClasses];

@@ -111,8 +111,16 @@ struct AmgLevel {
 	int32_t* d_gal_blk = nullptr;     // this level's block slot
 	int32_t* d_gal_pi = nullptr;      // entry of P in the slot's row
 	int32_t* d_gal_pj = nullptr;      // entry of P in the slot's column
-	// coarsest level: dense inverse (n x n, n = bs * nb) when small enough
+	// segments: row ranges that no entry of the level couples (the connected
+	// parts of a mesh numbered one after another; a single mesh has one).
+	// The coarsest level, when small enough, holds the dense inverse of each
+	// segment: (bs m) x (bs m) FP64 for a segment of m rows, row-major, the
+	// inverses one after another.
+	int n_seg = 1;
 	double* d_dense = nullptr;
+	int32_t* d_seg_ptr = nullptr;     // n_seg + 1 row offsets
+	int32_t* d_seg_row = nullptr;     // segment of each row
+	int64_t* d_seg_off = nullptr;     // first entry of each segment's inverse
 	// FP32 work vectors (bs*nb each)
 	float* d_b = nullptr;
 	float* d_x[2] = {nullptr, nullptr};
@@ -144,7 +152,10 @@ struct AmgLevelDesc {
 	const float* P;
 	const int32_t* pptr;
 	const int32_t* pcol;
-	const double* inv;     // dense inverse of the coarsest
+	const double* inv;     // dense inverses of the coarsest's segments
+	const int32_t* seg_ptr;
+	const int32_t* seg_row;
+	const int64_t* seg_off;
 };
 
 // Multigrid parameters of a hierarchy, fixed when it is built (defaults,
@@ -153,7 +164,7 @@ struct AmgLevelDesc {
 struct AmgParams {
 	int deg = 2;            // Chebyshev steps before and after each coarse correction
 	double ratio = 30.0;    // Chebyshev interval [lambda_max / ratio, lambda_max]
-	int coarsest = 64;      // coarsen until a level has at most this many block rows
+	int coarsest = 64;      // coarsen until every segment of a level has at most this many block rows
 	double omega = 1.0;     // prolongator damping: omega * 4 / (3 lambda_max)
 	int power = 10;         // power-iteration steps estimating lambda_max(D^-1 A); 0: Gershgorin bound
 };
@@ -311,6 +322,9 @@ namespace bfmx {
 int prepare_topology(bfmx_system* s, const double* coords, const size_t* elems, int renumber,
                      std::vector<int32_t>& col, std::vector<int32_t>& el_col);
 int build_topology(bfmx_system* s, const double* coords, const size_t* elems, int renumber);
+// Sliced-ELL pattern of s->adj_ptr / s->adj (slice_ptr, row_len, diag_blk,
+// nnzb, nnzb_pad; col: the block column of every slot).
+int sell_layout(bfmx_system* s, std::vector<int32_t>& col);
 // Host half of set_edges: renumbered edges and the node -> edge incidence.
 int prepare_edges(bfmx_system* s, size_t n_edges, const size_t* edge_nodes, const uint8_t* edge_boundary);
 int set_edges(bfmx_system* s, size_t n_edges, const size_t* edge_nodes, const uint8_t* edge_boundary);

@@ -113,6 +113,55 @@ void rcm(int64_t n, const std::vector<int64_t>& ptr, const std::vector<int32_t>&
 
 }  // namespace
 
+// Sliced-ELL layout of the system's renumbered adjacency: slice pointers,
+// row lengths, diagonal blocks and the column of every block slot (padding
+// slots repeat their row).
+int sell_layout(bfmx_system* s, std::vector<int32_t>& col) {
+	int64_t const n = s->n_nodes;
+	s->n_slices = (n + kSlice - 1) / kSlice;
+	s->slice_ptr.assign(s->n_slices + 1, 0);
+	s->row_len.assign(n, 0);
+	s->diag_blk.assign(n, 0);
+	for (int64_t sl = 0; sl < s->n_slices; sl++) {
+		int64_t kmax = 0;
+		for (int64_t r = sl * kSlice; r < std::min(n, (sl + 1) * kSlice); r++) {
+			kmax = std::max(kmax, s->adj_ptr[r + 1] - s->adj_ptr[r]);
+		}
+		int64_t const next = s->slice_ptr[sl] + kmax * kSlice;
+		if (next >= (int64_t(1) << 31) / 4) {
+			set_error("matrix too large for 32-bit block indices");
+			return BFMX_E_ARG;
+		}
+		s->slice_ptr[sl + 1] = static_cast<int32_t>(next);
+	}
+	s->nnzb = s->adj_ptr[n];
+	s->nnzb_pad = s->slice_ptr[s->n_slices];
+	col.assign(s->nnzb_pad, 0);
+	for (int64_t sl = 0; sl < s->n_slices; sl++) {
+		int64_t const kmax = (s->slice_ptr[sl + 1] - s->slice_ptr[sl]) / kSlice;
+		for (int r = 0; r < kSlice; r++) {
+			int64_t const row = sl * kSlice + r;
+			int64_t const len = row < n ? s->adj_ptr[row + 1] - s->adj_ptr[row] : 0;
+			if (row < n) {
+				s->row_len[row] = static_cast<int32_t>(len);
+			}
+			for (int64_t k = 0; k < kmax; k++) {
+				int64_t const blk = s->slice_ptr[sl] + k * kSlice + r;
+				if (k < len) {
+					int32_t const c = s->adj[s->adj_ptr[row] + k];
+					col[blk] = c;
+					if (c == row) {
+						s->diag_blk[row] = static_cast<int32_t>(blk);
+					}
+				} else {
+					col[blk] = row < n ? static_cast<int32_t>(row) : 0;
+				}
+			}
+		}
+	}
+	return BFMX_OK;
+}
+
 int prepare_topology(bfmx_system* s, const double* coords, const size_t* elems, int renumber,
                      std::vector<int32_t>& col, std::vector<int32_t>& el_col) {
 	int64_t const n = s->n_nodes;
@@ -161,48 +210,7 @@ int prepare_topology(bfmx_system* s, const double* coords, const size_t* elems, 
 	}
 	node_adjacency(n, kind, ne, el.data(), s->adj_ptr, s->adj);
 
-	// sliced ELL over block rows
-	s->n_slices = (n + kSlice - 1) / kSlice;
-	s->slice_ptr.assign(s->n_slices + 1, 0);
-	s->row_len.assign(n, 0);
-	s->diag_blk.assign(n, 0);
-	for (int64_t sl = 0; sl < s->n_slices; sl++) {
-		int64_t kmax = 0;
-		for (int64_t r = sl * kSlice; r < std::min(n, (sl + 1) * kSlice); r++) {
-			kmax = std::max(kmax, s->adj_ptr[r + 1] - s->adj_ptr[r]);
-		}
-		int64_t const next = s->slice_ptr[sl] + kmax * kSlice;
-		if (next >= (int64_t(1) << 31) / 4) {
-			set_error("matrix too large for 32-bit block indices");
-			return BFMX_E_ARG;
-		}
-		s->slice_ptr[sl + 1] = static_cast<int32_t>(next);
-	}
-	s->nnzb = s->adj_ptr[n];
-	s->nnzb_pad = s->slice_ptr[s->n_slices];
-	col.assign(s->nnzb_pad, 0);
-	for (int64_t sl = 0; sl < s->n_slices; sl++) {
-		int64_t const kmax = (s->slice_ptr[sl + 1] - s->slice_ptr[sl]) / kSlice;
-		for (int r = 0; r < kSlice; r++) {
-			int64_t const row = sl * kSlice + r;
-			int64_t const len = row < n ? s->adj_ptr[row + 1] - s->adj_ptr[row] : 0;
-			if (row < n) {
-				s->row_len[row] = static_cast<int32_t>(len);
-			}
-			for (int64_t k = 0; k < kmax; k++) {
-				int64_t const blk = s->slice_ptr[sl] + k * kSlice + r;
-				if (k < len) {
-					int32_t const c = s->adj[s->adj_ptr[row] + k];
-					col[blk] = c;
-					if (c == row) {
-						s->diag_blk[row] = static_cast<int32_t>(blk);
-					}
-				} else {
-					col[blk] = row < n ? static_cast<int32_t>(row) : 0;
-				}
-			}
-		}
-	}
+	BFMX_CHECK(sell_layout(s, col));
 
 	// Greedy colouring in order of each element's lowest node, so a colour's
 	// consecutive elements touch nearby rows; 64 colours at most.

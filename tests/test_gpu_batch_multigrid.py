@@ -1,0 +1,132 @@
+"""Multigrid mode of the batched solve (bfmx_batch_solve with
+BFMX_PRECOND_AMG, bfm_amd/csrc/hip/batch.hip): the batch solved as one
+block-diagonal system by flexible CG with the FP32 multigrid V-cycle, every
+load scaled to unit norm and the union solved to rtol / sqrt(m). Every
+instance must meet rtol on its own true residual (recomputed here on the host
+from the matrix the launch assembled), small instances must match the
+oracle's direct solve, and a mesh made of several disconnected parts solves
+with one dense coarsest inverse per part."""
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+from bfm_amd import device as D
+from bfm_amd import meshgen
+
+pytestmark = pytest.mark.gpu
+
+RANGES = [(600, 1500), (1500, 4000), (5000, 12000), (12000, 20000)]
+
+
+def host_rel_residual(system, u):
+    K, b = system.get_csr_scipy(), system.get_rhs()
+    return np.linalg.norm(b - K @ u) / np.linalg.norm(b)
+
+
+def test_multigrid_batch_meets_rtol_per_instance():
+    import oracle as O
+    meshes = []
+    for k in range(12):
+        lo, hi = RANGES[k % len(RANGES)]
+        meshes.append(meshgen.panel(200 + k, lo, hi))
+    pb = D.Batch.from_instances([D.panel_instance(m, t) for m, t in meshes])
+    st, ms = pb.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+    iters = st[0]["iterations"]
+    print(f"{len(meshes)} panels, {sum(m.n_dof for m, _ in meshes)} DOFs: {iters} iterations, {ms:.1f} ms")
+    # a union without the instances' coordinates (no rotation modes) took
+    # ~100 iterations here
+    assert iters <= 60
+    mat = meshgen.PANEL_MATERIAL
+    for k, (mesh, t) in enumerate(meshes):
+        assert st[k]["status"] == 0 and st[k]["iterations"] == iters, k
+        assert st[k]["precond"] == D.PRECOND_AMG
+        u = pb.get_u(k)
+        rel = host_rel_residual(pb.system(k), u)
+        assert rel <= 1e-9 and st[k]["rel_residual"] <= 1e-9, (k, rel)
+        assert abs(rel - st[k]["rel_residual"]) <= 1e-3 * rel + 1e-16, (k, rel, st[k]["rel_residual"])
+        if mesh.n_dof <= 1600:
+            kinds, values, flags = meshgen.panel_conditions(mesh, t)
+            u_ref, *_ = O.solve_dense(mesh, False, mat["E"], mat["nu"], mat["rho"], kinds, values, flags)
+            err = rel_err(u, u_ref)
+            print(f"panel {k} ({mesh.n_dof} DOFs): error vs oracle {err:.2e}")
+            assert err <= 1e-6, k
+    # a batch-owned system solved on its own by the multigrid (its
+    # coordinates live on the device only)
+    u_solo, st_solo = pb.system(11).solve(precond=D.PRECOND_AMG, rtol=1e-9)
+    assert st_solo["status"] == 0 and st_solo["iterations"] <= 40
+    assert rel_err(u_solo, pb.get_u(11)) <= 1e-4
+    # the same launch again: deterministic
+    u1 = [pb.get_u(k) for k in range(len(meshes))]
+    pb.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+    assert all(np.array_equal(u1[k], pb.get_u(k)) for k in range(len(meshes)))
+    # and the persistent PCG launch of the same batch agrees to the tolerance
+    stp, _ = pb.solve(precond=D.PRECOND_BLOCK_JACOBI, rtol=1e-9)
+    for k in range(len(meshes)):
+        assert stp[k]["status"] == 0
+        assert rel_err(pb.get_u(k), u1[k]) <= 1e-4, k
+
+
+def test_multigrid_batch_of_systems_with_a_zero_load():
+    systems = []
+    for k in range(4):
+        m, t = meshgen.panel(300 + k, 1500, 6000)
+        systems.append(D.panel_system(m, t))
+    m, _ = meshgen.panel(310, 1500, 3000)
+    mat = meshgen.PANEL_MATERIAL
+    s0 = D.System.from_mesh(m)
+    s0.assemble(0, mat["E"], mat["nu"], mat["rho"])
+    pins = m.node_sets["pins"]
+    s0.apply_conditions([0, 1], [0.0, 0.0], [pins, pins])   # no load at all
+    systems.append(s0)
+    b = D.Batch(systems)
+    st, ms = b.solve(precond=D.PRECOND_AMG, rtol=1e-8)
+    assert all(x["status"] == 0 for x in st)
+    assert np.all(b.get_u(4) == 0) and st[4]["b_norm"] == 0
+    for k in range(4):
+        rel = host_rel_residual(systems[k], b.get_u(k))
+        assert rel <= 1e-8, (k, rel)
+
+
+def test_multigrid_batch_refuses_axisymmetric_instances():
+    mesh = meshgen.beam(8, 4, kind=4, length=2.0, height=1.0)
+    kinds, values, flags = meshgen.beam_conditions(mesh)
+    inst = D.planar_instance(mesh, D.AXISYMMETRIC, meshgen.BEAM_MATERIAL, kinds, values, flags)
+    pb = D.Batch.from_instances([inst])
+    with pytest.raises(D.BfmxError) as e:
+        pb.solve(precond=D.PRECOND_AMG, rtol=1e-8)
+    assert e.value.code == -7
+
+
+def test_disconnected_mesh_one_dense_inverse_per_part():
+    """Three panels in one mesh (one system, RCM numbers each part in one
+    range): the multigrid solve coarsens every part to at most 64 nodes and
+    inverts each densely; it matches block-Jacobi PCG."""
+    coords, elems, edges, top, pins, trac, offs = [], [], [], [], [], [], [0]
+    for k in range(3):
+        m, t = meshgen.panel(400 + k, 800, 3000)
+        o = offs[-1]
+        coords.append(m.coords + np.array([0.0, 2.0 * k]))
+        elems.append(m.elems.astype(np.int64) + o)
+        edges.append(m.edges.astype(np.int64) + o)
+        top.append(m.node_sets["top"])
+        pins.append(m.node_sets["pins"])
+        trac.append(t)
+        offs.append(o + m.n_nodes)
+    N = offs[-1]
+    s = D.System(3, np.concatenate(coords), np.concatenate(elems), True, edges=np.concatenate(edges))
+    mat = meshgen.PANEL_MATERIAL
+    s.assemble(0, mat["E"], mat["nu"], mat["rho"])
+    top, pins = np.concatenate(top), np.concatenate(pins)
+    for k in range(3):
+        f = np.zeros(N, np.uint8)
+        f[offs[k]:offs[k + 1]] = top[offs[k]:offs[k + 1]]
+        s.apply_condition(3, trac[k], f)
+    s.apply_conditions([0, 1], [0.0, 0.0], [pins, pins])
+    u, st = s.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+    print(f"{2 * N} DOFs in 3 parts: {st['iterations']} iterations")
+    assert st["status"] == 0 and st["iterations"] <= 60
+    assert host_rel_residual(s, u) <= 1e-9
+    u_ref, _ = s.solve(precond=D.PRECOND_BLOCK_JACOBI, rtol=1e-10)
+    assert rel_err(u, u_ref) <= 1e-5
