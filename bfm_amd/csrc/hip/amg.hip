@@ -36,7 +36,10 @@
 // coarsest level applies its dense inverse.
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <exception>
+#include <thread>
 
 #include "internal.h"
 
@@ -60,6 +63,7 @@ constexpr int kScalSize = kScalLambda + 1;
 constexpr double kDropTol = 1e-10;      // relative remainder below which a mode column is dropped
 constexpr int kGj = 1024;               // threads of the Gauss-Jordan workgroup (one segment)
 constexpr int kGjSeg = 256;             // ... (one workgroup per segment of many)
+constexpr int kBuildThreads = 16;       // host threads building a level's transfer operators
 
 int grid(int64_t n) {
 	return static_cast<int>((n + kBlock - 1) / kBlock);
@@ -353,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_smooth_p(int nb, const int32_t* __re
 constexpr int kGalLanes = 16;
 
 template <int BSF>
-__global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32_t* __restrict__ gal_ptr,
+__global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int64_t* __restrict__ gal_ptr,
                                                      const int32_t* __restrict__ gal_blk,
                                                      const int32_t* __restrict__ gal_pi,
                                                      const int32_t* __restrict__ gal_pj,
@@ -363,8 +367,8 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32
 	int const lane = threadIdx.x % kGalLanes;
 	bool const live = b < nslots;   // every lane stays for the butterfly
 	double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-	int32_t const e0 = live ? gal_ptr[b] : 0, e1 = live ? gal_ptr[b + 1] : 0;
-	for (int32_t e = e0 + lane; e < e1; e += kGalLanes) {
+	int64_t const e0 = live ? gal_ptr[b] : 0, e1 = live ? gal_ptr[b + 1] : 0;
+	for (int64_t e = e0 + lane; e < e1; e += kGalLanes) {
 		int32_t const blk = gal_blk[e];
 		double A[BSF][BSF];
 		ld_block<BSF>(fval, blk, blk & 63, A);
@@ -1087,11 +1091,16 @@ __global__ __launch_bounds__(kWg) void k_coarse_cycle(const AmgLevelDesc* __rest
 
 // ------------------------------------------------------------------ host topology
 
-// Aggregates of the graph (ptr, adj) (rows include themselves); returns their count.
-int aggregate(int n, const int64_t* ptr, const int32_t* adj, std::vector<int32_t>& agg) {
-	agg.assign(n, -1);
+// Aggregates of rows r0 .. r1 - 1 of the graph (ptr, adj) (rows include
+// themselves), whose entries stay in that range: agg[i] receives the
+// aggregate's number counted from 0 within the range; returns their count.
+// Pass 1 makes a row whose neighbourhood is still free the root of a new
+// aggregate of that neighbourhood; pass 2 gives every other row the
+// aggregate of its first aggregated neighbour (every such row has one).
+int aggregate(int r0, int r1, const int64_t* ptr, const int32_t* adj, int32_t* agg) {
+	std::fill(agg + r0, agg + r1, -1);
 	int na = 0;
-	for (int i = 0; i < n; i++) {
+	for (int i = r0; i < r1; i++) {
 		if (agg[i] >= 0) {
 			continue;
 		}
@@ -1109,23 +1118,74 @@ int aggregate(int n, const int64_t* ptr, const int32_t* adj, std::vector<int32_t
 			agg[i] = na++;
 		}
 	}
-	std::vector<int32_t> out = agg;
-	for (int i = 0; i < n; i++) {
+	std::vector<int32_t> out(agg + r0, agg + r1);
+	for (int i = r0; i < r1; i++) {
 		if (agg[i] >= 0) {
 			continue;
 		}
 		for (int64_t t = ptr[i]; t < ptr[i + 1]; t++) {
 			if (agg[adj[t]] >= 0) {
-				out[i] = agg[adj[t]];
+				out[i - r0] = agg[adj[t]];
 				break;
 			}
 		}
-		if (out[i] < 0) {
-			out[i] = na++;
+		if (out[i - r0] < 0) {
+			out[i - r0] = na++;
 		}
 	}
-	agg.swap(out);
+	std::copy(out.begin(), out.end(), agg + r0);
 	return na;
+}
+
+// Row offsets of up to kBuildThreads chunks of similar size (fewer when the
+// machine has fewer hardware threads or BFMX_AMG_THREADS asks for fewer), cut
+// at the boundaries of the segments seg (offsets 0 .. n); one chunk for one
+// segment.
+std::vector<int32_t> chunk_rows(const std::vector<int32_t>& seg, int n) {
+	int hw = static_cast<int>(std::thread::hardware_concurrency());
+	if (const char* e = std::getenv("BFMX_AMG_THREADS")) {
+		hw = std::atoi(e);
+	}
+	int const T = std::max(1, std::min(kBuildThreads, hw));
+	int64_t const target = (static_cast<int64_t>(n) + T - 1) / T;
+	std::vector<int32_t> rb{0};
+	for (size_t c = 1; c + 1 < seg.size(); c++) {
+		if (seg[c] - rb.back() >= target) {
+			rb.push_back(seg[c]);
+		}
+	}
+	rb.push_back(n);
+	return rb;
+}
+
+// f(c) for c = 0 .. T - 1, one thread each (T = 1: on this thread); an
+// exception of any call is rethrown here after all have ended.
+template <typename F>
+void parallel_chunks(int T, const F& f) {
+	if (T <= 1) {
+		f(0);
+		return;
+	}
+	std::vector<std::exception_ptr> err(T);
+	std::vector<std::thread> th;
+	th.reserve(T);
+	for (int c = 0; c < T; c++) {
+		th.emplace_back([&, c] {
+			try {
+				f(c);
+			} catch (...) {
+				err[c] = std::current_exception();
+			}
+		});
+	}
+	for (auto& t : th) {
+		t.join();
+	}
+	for (auto& e : err) {
+		if (e) {
+			std::rethrow_exception(e);
+		}
+	}
 }
 
 struct Sell {
@@ -1384,8 +1444,25 @@ int amg_build(bfmx_system* s) {
 			const int64_t* ptr = lf == 0 ? s->adj_ptr.data() : fptr.data();
 			const int32_t* adj = lf == 0 ? s->adj.data() : fadj.data();
 			int const nf = lv[lf].nb;
-			std::vector<int32_t> agg;
-			int const na = aggregate(nf, ptr, adj, agg);
+			// the rows split at segment boundaries into chunks of similar size,
+			// one thread each: the aggregates, prolongator rows, restriction
+			// columns and coarse rows of a chunk stay inside it, so the result
+			// does not depend on the split
+			std::vector<int32_t> const rb = chunk_rows(fseg, nf);
+			int const T = static_cast<int>(rb.size()) - 1;
+			std::atomic<int> bad{0};
+			std::vector<int32_t> agg(nf);
+			std::vector<int32_t> ab(T + 1, 0);   // first aggregate of each chunk
+			parallel_chunks(T, [&](int c) { ab[c + 1] = aggregate(rb[c], rb[c + 1], ptr, adj, agg.data()); });
+			for (int c = 0; c < T; c++) {
+				ab[c + 1] += ab[c];
+			}
+			int const na = ab[T];
+			parallel_chunks(T, [&](int c) {
+				for (int i = rb[c]; i < rb[c + 1]; i++) {
+					agg[i] += ab[c];
+				}
+			});
 			std::vector<int32_t> cseg(fseg.size());
 			for (size_t c = 0; c + 1 < fseg.size(); c++) {
 				cseg[c] = agg[fseg[c]];
@@ -1407,27 +1484,33 @@ int amg_build(bfmx_system* s) {
 			if (big_a * 10 > big_f * 8) {
 				break;   // coarsening stalled
 			}
-			// members of each aggregate
+			// members of each aggregate (a chunk's members fill its own rows' range)
 			std::vector<int32_t> aptr(na + 1, 0), anodes(nf);
-			for (int i = 0; i < nf; i++) {
-				aptr[agg[i] + 1]++;
-			}
-			for (int I = 0; I < na; I++) {
-				aptr[I + 1] += aptr[I];
-			}
-			{
-				std::vector<int32_t> fill(aptr.begin(), aptr.end() - 1);
-				for (int i = 0; i < nf; i++) {
-					anodes[fill[agg[i]]++] = i;
+			parallel_chunks(T, [&](int c) {
+				std::vector<int32_t> fill(ab[c + 1] - ab[c], 0);
+				for (int i = rb[c]; i < rb[c + 1]; i++) {
+					fill[agg[i] - ab[c]]++;
 				}
-			}
+				int32_t run = rb[c];
+				for (int I = ab[c]; I < ab[c + 1]; I++) {
+					int32_t const cnt = fill[I - ab[c]];
+					fill[I - ab[c]] = run;
+					run += cnt;
+					aptr[I + 1] = run;
+				}
+				for (int i = rb[c]; i < rb[c + 1]; i++) {
+					anodes[fill[agg[i] - ab[c]]++] = i;
+				}
+			});
 			// prolongator pattern: row i couples to the aggregates of its
 			// neighbours (smoothed) or to its own aggregate (unsmoothed)
-			std::vector<int32_t> pptr(nf + 1, 0), pcol;
-			pcol.reserve(static_cast<size_t>(smooth ? 3 : 1) * nf);
-			{
+			std::vector<int32_t> pptr(nf + 1, 0);
+			std::vector<std::vector<int32_t>> pcol_c(T);
+			parallel_chunks(T, [&](int c) {
 				std::vector<int32_t> row;
-				for (int i = 0; i < nf; i++) {
+				std::vector<int32_t>& pc = pcol_c[c];
+				pc.reserve(static_cast<size_t>(smooth ? 3 : 1) * (rb[c + 1] - rb[c]));
+				for (int i = rb[c]; i < rb[c + 1]; i++) {
 					row.clear();
 					if (smooth) {
 						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
@@ -1439,17 +1522,32 @@ int amg_build(bfmx_system* s) {
 						row.push_back(agg[i]);
 					}
 					if (row.size() > 255) {
-						set_error("multigrid: a row couples to more than 255 aggregates");
-						return BFMX_E_UNSUPPORTED;
+						bad = 1;
 					}
-					pcol.insert(pcol.end(), row.begin(), row.end());
-					pptr[i + 1] = static_cast<int32_t>(pcol.size());
+					pc.insert(pc.end(), row.begin(), row.end());
+					pptr[i + 1] = static_cast<int32_t>(pc.size());   // within the chunk, offset below
 				}
+			});
+			if (bad) {
+				set_error("multigrid: a row couples to more than 255 aggregates");
+				return BFMX_E_UNSUPPORTED;
 			}
-			if (pcol.size() >= (size_t(1) << 31) / 9) {
+			std::vector<int64_t> pb(T + 1, 0);   // first entry of each chunk
+			for (int c = 0; c < T; c++) {
+				pb[c + 1] = pb[c] + static_cast<int64_t>(pcol_c[c].size());
+			}
+			if (pb[T] >= (int64_t(1) << 31) / 9) {
 				set_error("multigrid prolongator too large for 32-bit indices");
 				return BFMX_E_ARG;
 			}
+			std::vector<int32_t> pcol(pb[T]);
+			parallel_chunks(T, [&](int c) {
+				for (int i = rb[c]; i < rb[c + 1]; i++) {
+					pptr[i + 1] += static_cast<int32_t>(pb[c]);
+				}
+				std::copy(pcol_c[c].begin(), pcol_c[c].end(), pcol.begin() + pb[c]);
+				std::vector<int32_t>().swap(pcol_c[c]);
+			});
 			auto entry_of = [&](int i, int32_t J) -> int32_t {   // entry of (i, J) in P; J must be in row i
 				auto const beg = pcol.begin() + pptr[i], end = pcol.begin() + pptr[i + 1];
 				return static_cast<int32_t>(std::lower_bound(beg, end, J) - pcol.begin());
@@ -1458,45 +1556,56 @@ int amg_build(bfmx_system* s) {
 			// aggregate; padding slots and the unsmoothed case point at the
 			// row's own aggregate
 			std::vector<uint8_t> ppos(fslice.back(), 0);
-			for (int64_t sl = 0; sl + 1 < static_cast<int64_t>(fslice.size()); sl++) {
-				for (int32_t slot = fslice[sl]; slot < fslice[sl + 1]; slot++) {
-					int64_t const row = sl * kSlice + (slot & 63);
-					if (row < nf) {
-						ppos[slot] = static_cast<uint8_t>(entry_of(static_cast<int>(row), agg[row]) - pptr[row]);
+			parallel_chunks(T, [&](int c) {
+				for (int i = rb[c]; i < rb[c + 1]; i++) {
+					auto const own = static_cast<uint8_t>(entry_of(i, agg[i]) - pptr[i]);
+					for (int32_t slot = fslice[i >> 6] + (i & 63); slot < fslice[(i >> 6) + 1]; slot += kSlice) {
+						ppos[slot] = own;
+					}
+					if (smooth) {
+						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+							ppos[fslot[e]] = static_cast<uint8_t>(entry_of(i, agg[adj[e]]) - pptr[i]);
+						}
 					}
 				}
-			}
-			if (smooth) {
-				for (int i = 0; i < nf; i++) {
-					for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
-						ppos[fslot[e]] = static_cast<uint8_t>(entry_of(i, agg[adj[e]]) - pptr[i]);
-					}
-				}
-			}
+			});
 			// columns of P (restriction), entries in row order
 			std::vector<int32_t> rptr(na + 1, 0), rent(pcol.size()), rrow(pcol.size());
-			for (int32_t J : pcol) {
-				rptr[J + 1]++;
-			}
-			for (int I = 0; I < na; I++) {
-				rptr[I + 1] += rptr[I];
-			}
-			{
-				std::vector<int32_t> fill(rptr.begin(), rptr.end() - 1);
-				for (int i = 0; i < nf; i++) {
+			parallel_chunks(T, [&](int c) {
+				std::vector<int32_t> fill(ab[c + 1] - ab[c], 0);
+				for (int32_t e = static_cast<int32_t>(pb[c]); e < pb[c + 1]; e++) {
+					if (pcol[e] < ab[c] || pcol[e] >= ab[c + 1]) {
+						bad = 1;
+						return;
+					}
+					fill[pcol[e] - ab[c]]++;
+				}
+				int32_t run = static_cast<int32_t>(pb[c]);
+				for (int I = ab[c]; I < ab[c + 1]; I++) {
+					int32_t const cnt = fill[I - ab[c]];
+					fill[I - ab[c]] = run;
+					run += cnt;
+					rptr[I + 1] = run;
+				}
+				for (int i = rb[c]; i < rb[c + 1]; i++) {
 					for (int32_t e = pptr[i]; e < pptr[i + 1]; e++) {
-						int32_t const at = fill[pcol[e]]++;
+						int32_t const at = fill[pcol[e] - ab[c]]++;
 						rent[at] = e;
 						rrow[at] = i;
 					}
 				}
+			});
+			if (bad) {
+				set_error("multigrid: a prolongator row leaves its segment");
+				return BFMX_E_ARG;
 			}
 			// coarse graph, the pattern of P^T A P (sorted rows, with the diagonal)
 			std::vector<int64_t> cptr(na + 1, 0);
-			std::vector<int32_t> cadj;
-			{
+			std::vector<std::vector<int32_t>> cadj_c(T);
+			parallel_chunks(T, [&](int c) {
 				std::vector<int32_t> row;
-				for (int I = 0; I < na; I++) {
+				std::vector<int32_t>& ca = cadj_c[c];
+				for (int I = ab[c]; I < ab[c + 1]; I++) {
 					row.clear();
 					for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
 						int const i = rrow[t];
@@ -1507,57 +1616,69 @@ int amg_build(bfmx_system* s) {
 					}
 					std::sort(row.begin(), row.end());
 					row.erase(std::unique(row.begin(), row.end()), row.end());
-					cadj.insert(cadj.end(), row.begin(), row.end());
-					cptr[I + 1] = static_cast<int64_t>(cadj.size());
+					ca.insert(ca.end(), row.begin(), row.end());
+					cptr[I + 1] = static_cast<int64_t>(ca.size());   // within the chunk, offset below
 				}
+			});
+			std::vector<int64_t> cb(T + 1, 0);
+			for (int c = 0; c < T; c++) {
+				cb[c + 1] = cb[c] + static_cast<int64_t>(cadj_c[c].size());
 			}
+			std::vector<int32_t> cadj(cb[T]);
+			parallel_chunks(T, [&](int c) {
+				for (int I = ab[c]; I < ab[c + 1]; I++) {
+					cptr[I + 1] += cb[c];
+				}
+				std::copy(cadj_c[c].begin(), cadj_c[c].end(), cadj.begin() + cb[c]);
+				std::vector<int32_t>().swap(cadj_c[c]);
+			});
 			Sell CS;
 			BFMX_CHECK(build_sell(na, cptr, cadj, CS));
 			// Galerkin gather lists: every (fine block A_ij, entry P_iI, entry
-			// P_jJ) grouped by the coarse slot (I, J), in fine row order
+			// P_jJ) grouped by the coarse slot (I, J), in fine row order (the
+			// slots of coarse row I belong to its chunk)
 			auto coarse_slot = [&](int32_t I, int32_t J) -> int32_t {
 				auto const beg = cadj.begin() + cptr[I], end = cadj.begin() + cptr[I + 1];
 				return CS.slot_of[std::lower_bound(beg, end, J) - cadj.begin()];
 			};
-			std::vector<int32_t> gptr(CS.nnzb_pad + 1, 0);
-			int64_t nterms = 0;
-			for (int I = 0; I < na; I++) {
-				for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
-					int const i = rrow[t];
-					for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
-						int const j = adj[e];
-						for (int32_t f = pptr[j]; f < pptr[j + 1]; f++) {
-							gptr[coarse_slot(I, pcol[f]) + 1]++;
-							nterms++;
-						}
-					}
-				}
-			}
-			if (nterms >= (int64_t(1) << 31)) {
-				set_error("multigrid Galerkin product too large for 32-bit indices");
-				return BFMX_E_ARG;
-			}
-			for (int64_t b = 0; b < CS.nnzb_pad; b++) {
-				gptr[b + 1] += gptr[b];
-			}
-			std::vector<int32_t> gblk(nterms), gpi(nterms), gpj(nterms);
-			{
-				std::vector<int32_t> fill(gptr.begin(), gptr.end() - 1);
-				for (int I = 0; I < na; I++) {
+			std::vector<int64_t> gptr(CS.nnzb_pad + 1, 0);
+			parallel_chunks(T, [&](int c) {
+				for (int I = ab[c]; I < ab[c + 1]; I++) {
 					for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
 						int const i = rrow[t];
-						int32_t const pi = rent[t];
 						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
 							int const j = adj[e];
 							for (int32_t f = pptr[j]; f < pptr[j + 1]; f++) {
-								int32_t const at = fill[coarse_slot(I, pcol[f])]++;
-								gblk[at] = fslot[e];
-								gpi[at] = pi;
-								gpj[at] = f;
+								gptr[coarse_slot(I, pcol[f]) + 1]++;
 							}
 						}
 					}
 				}
+			});
+			for (int64_t b = 0; b < CS.nnzb_pad; b++) {
+				gptr[b + 1] += gptr[b];
+			}
+			int64_t const nterms = gptr[CS.nnzb_pad];
+			std::vector<int32_t> gblk(nterms), gpi(nterms), gpj(nterms);
+			{
+				std::vector<int64_t> fill(gptr.begin(), gptr.end() - 1);
+				parallel_chunks(T, [&](int c) {
+					for (int I = ab[c]; I < ab[c + 1]; I++) {
+						for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
+							int const i = rrow[t];
+							int32_t const pi = rent[t];
+							for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+								int const j = adj[e];
+								for (int32_t f = pptr[j]; f < pptr[j + 1]; f++) {
+									int64_t const at = fill[coarse_slot(I, pcol[f])]++;
+									gblk[at] = fslot[e];
+									gpi[at] = pi;
+									gpj[at] = f;
+								}
+							}
+						}
+					}
+				});
 			}
 			AmgLevel& F = lv[lf];
 			F.sa = smooth;

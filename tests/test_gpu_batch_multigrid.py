@@ -130,3 +130,19 @@ def test_disconnected_mesh_one_dense_inverse_per_part():
     assert host_rel_residual(s, u) <= 1e-9
     u_ref, _ = s.solve(precond=D.PRECOND_BLOCK_JACOBI, rtol=1e-10)
     assert rel_err(u, u_ref) <= 1e-5
+
+
+def test_multigrid_set_up_threads_give_the_same_hierarchy(monkeypatch):
+    """The host build of the transfer operators runs one thread per chunk of
+    whole segments; one thread and sixteen give bitwise the same solve."""
+    meshes = [meshgen.panel(500 + k, 1000, 4000) for k in range(20)]
+    out = []
+    for threads in ("1", "16"):
+        monkeypatch.setenv("BFMX_AMG_THREADS", threads)
+        pb = D.Batch.from_instances([D.panel_instance(m, t) for m, t in meshes])
+        st, _ = pb.solve(precond=D.PRECOND_AMG, rtol=1e-8)
+        assert all(x["status"] == 0 for x in st)
+        out.append((st[0]["iterations"], [pb.get_u(k) for k in range(len(meshes))]))
+        pb.close()
+    assert out[0][0] == out[1][0]
+    assert all(np.array_equal(a, b) for a, b in zip(out[0][1], out[1][1]))
