@@ -280,7 +280,20 @@ int bfmx_batch_autotune(bfmx_batch_t* b, int precond, int sample, int iters, int
 /* Solves every system from u = 0 to rtol (true FP64 residual); picks NULL =
  * the default configurations, max_iter 0 = 20 x the largest DOF count.
  * stats (may be NULL) receives one entry per system, total_ms the launch's
- * device time. Returns BFMX_OK when every system converged. */
+ * device time. Returns BFMX_OK when every system converged.
+ * precond BFMX_PRECOND_JACOBI / BFMX_PRECOND_BLOCK_JACOBI: the persistent
+ * launch above.
+ * precond BFMX_PRECOND_AMG (multigrid mode): the batch is solved as one
+ * block-diagonal system (the systems' rows one after another, a second copy
+ * of their matrices) by flexible CG with the FP32 multigrid V-cycle, one
+ * dense coarsest inverse per system. Each load is scaled to unit norm and
+ * the union solved to rtol / sqrt(m) (m systems with a nonzero load), which
+ * bounds every system's own residual by rtol; the stats report each
+ * system's true residual and the union's iteration count. picks are
+ * ignored; the first such call builds the union and its hierarchy on the
+ * host. A system's solution depends on the batch (the Krylov scalars are
+ * shared) only below rtol. Axisymmetric systems are refused
+ * (BFMX_E_UNSUPPORTED: indefinite). */
 int bfmx_batch_solve(bfmx_batch_t* b, int precond, double rtol, size_t max_iter, const int32_t* picks,
                      bfmx_stats_t* stats, double* total_ms);
 /* Solution of system `index`, in its original DOF order. */
