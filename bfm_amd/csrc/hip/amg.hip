@@ -353,11 +353,16 @@ __global__ __launch_bounds__(kBlock) void k_smooth_p(int nb, const int32_t* __re
 // Coarse block slot b = sum over its contributions (fine block A_ij, entries
 // pi = P_iI and pj = P_jJ) of P_iI^T A_ij P_jJ, in FP64 (P as stored, FP32).
 // kGalLanes lanes share a slot: lane l takes contributions l, l + kGalLanes,
-// ..., and the lanes' sums are combined by a fixed butterfly.
+// ..., and the lanes' sums are combined by a fixed butterfly. P^T A P is
+// symmetric: only the slots (I, J) with J >= I have contributions, and each
+// also writes its transpose into the slot of (J, I) (mirror[b]; a slot with
+// J < I has mirror -2 and is left to its mirror; a padding slot has -1 and
+// receives zeros).
 constexpr int kGalLanes = 16;
 
 template <int BSF>
-__global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int64_t* __restrict__ gal_ptr,
+__global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32_t* __restrict__ mirror,
+                                                     const int64_t* __restrict__ gal_ptr,
                                                      const int32_t* __restrict__ gal_blk,
                                                      const int32_t* __restrict__ gal_pi,
                                                      const int32_t* __restrict__ gal_pj,
@@ -365,7 +370,8 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int64
                                                      double* __restrict__ cval64, float* __restrict__ cval32) {
 	int64_t const b = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGalLanes;
 	int const lane = threadIdx.x % kGalLanes;
-	bool const live = b < nslots;   // every lane stays for the butterfly
+	int32_t const mb = b < nslots ? mirror[b] : -2;
+	bool const live = mb != -2;   // every lane stays for the butterfly
 	double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 	int64_t const e0 = live ? gal_ptr[b] : 0, e1 = live ? gal_ptr[b + 1] : 0;
 	for (int64_t e = e0 + lane; e < e1; e += kGalLanes) {
@@ -416,6 +422,7 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int64
 		return;
 	}
 	int const r = static_cast<int>(b & 63);
+	int const rm = mb & 63;
 #pragma unroll
 	for (int a = 0; a < 3; a++) {
 #pragma unroll
@@ -423,6 +430,11 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int64
 			int64_t const k = Blk<3>::at(b, r, a, c);
 			cval64[k] = C[a][c];
 			cval32[k] = static_cast<float>(C[a][c]);
+			if (mb >= 0 && mb != b) {
+				int64_t const km = Blk<3>::at(mb, rm, c, a);
+				cval64[km] = C[a][c];
+				cval32[km] = static_cast<float>(C[a][c]);
+			}
 		}
 	}
 }
@@ -1324,6 +1336,7 @@ void amg_release(bfmx_system* s) {
 		dfree(L.d_rent);
 		dfree(L.d_rrow);
 		dfree(L.d_gal_ptr);
+		dfree(L.d_gal_mirror);
 		dfree(L.d_gal_blk);
 		dfree(L.d_gal_pi);
 		dfree(L.d_gal_pj);
@@ -1642,14 +1655,21 @@ int amg_build(bfmx_system* s) {
 				return CS.slot_of[std::lower_bound(beg, end, J) - cadj.begin()];
 			};
 			std::vector<int64_t> gptr(CS.nnzb_pad + 1, 0);
+			std::vector<int32_t> mirror(CS.nnzb_pad, -1);
 			parallel_chunks(T, [&](int c) {
 				for (int I = ab[c]; I < ab[c + 1]; I++) {
+					for (int64_t e = cptr[I]; e < cptr[I + 1]; e++) {
+						int32_t const J = cadj[e];
+						mirror[CS.slot_of[e]] = J < I ? -2 : coarse_slot(J, I);
+					}
 					for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
 						int const i = rrow[t];
 						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
 							int const j = adj[e];
 							for (int32_t f = pptr[j]; f < pptr[j + 1]; f++) {
-								gptr[coarse_slot(I, pcol[f]) + 1]++;
+								if (pcol[f] >= I) {
+									gptr[coarse_slot(I, pcol[f]) + 1]++;
+								}
 							}
 						}
 					}
@@ -1670,6 +1690,9 @@ int amg_build(bfmx_system* s) {
 							for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
 								int const j = adj[e];
 								for (int32_t f = pptr[j]; f < pptr[j + 1]; f++) {
+									if (pcol[f] < I) {
+										continue;
+									}
 									int64_t const at = fill[coarse_slot(I, pcol[f])]++;
 									gblk[at] = fslot[e];
 									gpi[at] = pi;
@@ -1695,6 +1718,7 @@ int amg_build(bfmx_system* s) {
 			BFMX_CHECK(upload(&F.d_rent, rent, st));
 			BFMX_CHECK(upload(&F.d_rrow, rrow, st));
 			BFMX_CHECK(upload(&F.d_gal_ptr, gptr, st));
+			BFMX_CHECK(upload(&F.d_gal_mirror, mirror, st));
 			BFMX_CHECK(upload(&F.d_gal_blk, gblk, st));
 			BFMX_CHECK(upload(&F.d_gal_pi, gpi, st));
 			BFMX_CHECK(upload(&F.d_gal_pj, gpj, st));
@@ -1873,14 +1897,16 @@ int amg_setup_numeric(bfmx_system* s) {
 			k_tentative<2><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
 			k_smooth_p<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_diag_blk, L.d_Pt,
 			                                    L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0, L.d_P);
-			k_galerkin<2><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_ptr, L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
-			                                          L.d_val64, L.d_P, C.d_val64, C.d_val32);
+			k_galerkin<2><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr, L.d_gal_blk,
+			                                                        L.d_gal_pi, L.d_gal_pj, L.d_val64, L.d_P,
+			                                                        C.d_val64, C.d_val32);
 		} else {
 			k_tentative<3><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
 			k_smooth_p<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_diag_blk, L.d_Pt,
 			                                    L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0, L.d_P);
-			k_galerkin<3><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_ptr, L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
-			                                          L.d_val64, L.d_P, C.d_val64, C.d_val32);
+			k_galerkin<3><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr, L.d_gal_blk,
+			                                                        L.d_gal_pi, L.d_gal_pj, L.d_val64, L.d_P,
+			                                                        C.d_val64, C.d_val32);
 		}
 	}
 	BFMX_HIP(hipGetLastError());

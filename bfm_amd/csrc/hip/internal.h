@@ -107,7 +107,9 @@ struct AmgLevel {
 	int32_t* d_rptr = nullptr;        // per next-level row: the entries of its column of P
 	int32_t* d_rent = nullptr;        //   entry index
 	int32_t* d_rrow = nullptr;        //   this level's row
-	int64_t* d_gal_ptr = nullptr;     // per next-level block slot: range of contributions
+	int64_t* d_gal_ptr = nullptr;     // per next-level block slot: range of contributions (slots (I, J >= I))
+	int32_t* d_gal_mirror = nullptr;  // per next-level block slot: the slot of its transpose (-2: filled by
+	                                  // it, -1: padding)
 	int32_t* d_gal_blk = nullptr;     // this level's block slot
 	int32_t* d_gal_pi = nullptr;      // entry of P in the slot's row
 	int32_t* d_gal_pj = nullptr;      // entry of P in the slot's column
