@@ -151,8 +151,8 @@ def batched_leg(D, meshgen, count, rtol):
         f"iterations mean {its.mean():.0f} max {its.max()}, {gbs:.0f} GB/s algorithmic")
     # multigrid mode: the same batch solved as one block-diagonal system by
     # flexible CG + the multigrid V-cycle (each load scaled to unit norm, the
-    # union solved to rtol / sqrt(count)); the first launch builds the union
-    # and its hierarchy on the host
+    # tolerance checks testing every panel's own residual); the first launch
+    # builds the union and its hierarchy on the host
     t = time.perf_counter()
     b.solve(precond=D.PRECOND_AMG, rtol=rtol)
     t_mg_setup = time.perf_counter() - t
@@ -178,8 +178,8 @@ def batched_leg(D, meshgen, count, rtol):
                      "algorithmic_bytes": byt},
         "multigrid": {
             "solver": "the batch as one block-diagonal system: flexible CG (FP64) + FP32 smoothed-aggregation "
-                      "V-cycle, one dense coarsest inverse per panel, loads scaled to unit norm, union rtol "
-                      f"{rtol:g}/sqrt({count})",
+                      "V-cycle, one dense coarsest inverse per panel, loads scaled to unit norm, every panel's "
+                      f"own true residual checked against rtol {rtol:g}",
             "ms_per_step": ms_mg, "wall_ms": wall_mg * 1e3, "value": dofs / (ms_mg / 1e3), "unit": "DOF/s",
             "iterations": int(st_mg[0]["iterations"]), "worst_rel_residual": worst,
             "speedup_vs_persistent_launch": ms / ms_mg, "first_launch_s": t_mg_setup},

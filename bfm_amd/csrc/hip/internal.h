@@ -253,6 +253,12 @@ struct bfmx_system {
 	bfmx::AmgParams amg_par;           // its parameters
 	int amg_wg = -1;                   // first level run by the one-workgroup coarse cycle (-1: none)
 	bfmx::AmgLevelDesc* d_amg_desc = nullptr;   // descriptors of the levels from amg_wg on
+	// convergence segments: node ranges d_conv_seg[k] .. d_conv_seg[k+1] - 1
+	// (borrowed) that must each meet the tolerance on their own (the union
+	// system of a batch); d_conv_part holds their r.r and b.b at a check
+	int n_conv_seg = 0;
+	const int64_t* d_conv_seg = nullptr;
+	double* d_conv_part = nullptr;
 	bfmx_model* model = nullptr;       // learned preconditioner (borrowed)
 	float* d_mp_feat = nullptr;        // its node features (8 per node)
 	float* d_mp_h[4] = {nullptr, nullptr, nullptr, nullptr};   // hidden features (FP32, 64 per node)

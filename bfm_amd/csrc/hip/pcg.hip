@@ -442,22 +442,70 @@ __global__ __launch_bounds__(kBlock) void k_residual(int nb, int mode, double de
 	write_partials<3>(v, partial);
 }
 
+// Convergence segments (a system made of independent parts, each to meet
+// the tolerance on its own): per segment of rows off[k] .. off[k+1] - 1, the
+// true residual's r.r and b.b, one workgroup per segment, after k_residual
+// and under its condition.
+constexpr int kSegBlock = 256;
+
+__global__ __launch_bounds__(kSegBlock) void k_seg_norms(int mode, double delta2, const int64_t* __restrict__ off,
+                                                         const double* __restrict__ r,
+                                                         const double* __restrict__ b, double* __restrict__ part,
+                                                         const Scalars* __restrict__ S) {
+	if (mode == 1 && !residual_due(S, delta2)) {
+		return;
+	}
+	__shared__ double lds[2][kSegBlock / 64];
+	int64_t const d0 = 2 * off[blockIdx.x], d1 = 2 * off[blockIdx.x + 1];
+	double rr = 0, bb = 0;
+	for (int64_t d = d0 + threadIdx.x; d < d1; d += kSegBlock) {
+		rr += r[d] * r[d];
+		bb += b[d] * b[d];
+	}
+	rr = wave_sum(rr);
+	bb = wave_sum(bb);
+	if ((threadIdx.x & 63) == 0) {
+		lds[0][threadIdx.x >> 6] = rr;
+		lds[1][threadIdx.x >> 6] = bb;
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		double a = 0, c = 0;
+		for (int w = 0; w < kSegBlock / 64; w++) {
+			a += lds[0][w];
+			c += lds[1][w];
+		}
+		part[2 * blockIdx.x] = a;
+		part[2 * blockIdx.x + 1] = c;
+	}
+}
+
 // Completes k_residual. mode 0: starts the run. mode 1 (when it ran): the
 // recomputed residual replaces the recursive one and the directions are kept
 // (beta = r'.z' / previous r.z). When the true residual meets the tolerance
-// the run ends (done = 1). Once a tolerance check has failed, every chunk ends
-// with a check; when the true residual has not halved (in norm) below its best
-// for kStallChecks checks, it sits at its rounding floor and the run ends
-// (done = 3).
+// (with convergence segments: every segment with a nonzero load meets it on
+// its own, r_k.r_k <= rtol2 b_k.b_k) the run ends (done = 1). Once a tolerance
+// check has failed, every chunk ends with a check; when the true residual has
+// not halved (in norm) below its best for kStallChecks checks, it sits at its
+// rounding floor and the run ends (done = 3).
 constexpr int kStallChecks = 8;
 
 __global__ __launch_bounds__(kRed) void k_red_residual(int nwg, int mode, double rtol2, double delta2,
-                                                       const double* __restrict__ partial,
+                                                       const double* __restrict__ partial, int nseg,
+                                                       const double* __restrict__ seg_part,
                                                        Scalars* __restrict__ S) {
 	if (mode == 1 && !residual_due(S, delta2)) {
 		return;
 	}
 	bool const check = mode == 1 && (S->rr <= S->tol2 || S->checking);
+	int seg_fail = 0;
+	for (int k = threadIdx.x; k < nseg; k += kRed) {
+		double const bk = seg_part[2 * k + 1];
+		if (bk > 0 && !(seg_part[2 * k] <= rtol2 * bk)) {
+			seg_fail = 1;
+		}
+	}
+	seg_fail = __syncthreads_or(seg_fail);
 	double tot[3];
 	reduce_partials<3>(partial, nwg, tot);
 	if (threadIdx.x == 0) {
@@ -481,7 +529,7 @@ __global__ __launch_bounds__(kRed) void k_red_residual(int nwg, int mode, double
 		if (S->done == kHold) {
 			S->done = 0;
 		}
-		if (rr <= S->tol2) {
+		if (rr <= S->tol2 && !seg_fail) {
 			S->done = 1;
 		} else if (check) {
 			S->checking = 1;
@@ -586,7 +634,12 @@ void launch_residual(bfmx_system* s, int pre, int mode, double rtol2, double del
 		    nb, mode, delta2, s->d_slice_ptr, s->d_col, s->d_val, s->d_bsolve, s->d_dinv, s->d_x, s->d_r, s->d_z,
 		    s->d_p[0], s->d_p[1], s->d_partial, s->d_scal);
 	}
-	k_red_residual<<<1, kRed, 0, s->stream>>>(g, mode, rtol2, delta2, s->d_partial, s->d_scal);
+	if (s->n_conv_seg > 0) {
+		k_seg_norms<<<s->n_conv_seg, kSegBlock, 0, s->stream>>>(mode, delta2, s->d_conv_seg, s->d_r, s->d_bsolve,
+		                                                        s->d_conv_part, s->d_scal);
+	}
+	k_red_residual<<<1, kRed, 0, s->stream>>>(g, mode, rtol2, delta2, s->d_partial, s->n_conv_seg, s->d_conv_part,
+	                                          s->d_scal);
 }
 
 void launch_spmv(bfmx_system* s, bool mixed, const double* pold, double* pnew) {

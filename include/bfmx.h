@@ -286,10 +286,10 @@ int bfmx_batch_autotune(bfmx_batch_t* b, int precond, int sample, int iters, int
  * precond BFMX_PRECOND_AMG (multigrid mode): the batch is solved as one
  * block-diagonal system (the systems' rows one after another, a second copy
  * of their matrices) by flexible CG with the FP32 multigrid V-cycle, one
- * dense coarsest inverse per system. Each load is scaled to unit norm and
- * the union solved to rtol / sqrt(m) (m systems with a nonzero load), which
- * bounds every system's own residual by rtol; the stats report each
- * system's true residual and the union's iteration count. picks are
+ * dense coarsest inverse per system. Each load is scaled to unit norm; the
+ * union's tolerance checks test every system's own true residual, and the
+ * run ends when each meets rtol. The stats report each system's true
+ * residual and the union's iteration count. picks are
  * ignored; the first such call builds the union and its hierarchy on the
  * host. A system's solution depends on the batch (the Krylov scalars are
  * shared) only below rtol. Axisymmetric systems are refused
