@@ -13,11 +13,11 @@ Jacobi-PCG is timed the same way beside it (the speedup is reported).
 
 Also reported: the batched workload (4096 seeded holed panels of 1k-60k DOFs,
 assembled and solved by block-Jacobi PCG in one graph launch, one workgroup per
-panel: DOF/s and the launch's algorithmic GB/s); the roofline of the headline
-solver's dominant kernel (the
-finest-level smoothing step: FP32 SpMV + block Jacobi) and of the FP64 CG
-SpMV (algorithmic bytes / mean launch time from HIP events on the solver's
-stream), the end-to-end time (upload + renumbering + assembly + conditions +
+panel: DOF/s and the launch's algorithmic GB/s, and the same batch in the
+multigrid mode); the roofline of the headline solver's largest kernel (the
+flexible-CG FP64 SpMV), of its finest-level post-smoothing step (FP32 SpMV +
+block-Jacobi Chebyshev step) and of the FP64 CG SpMV (algorithmic bytes /
+mean launch time from HIP events on the solver's stream), the end-to-end time (upload + renumbering + assembly + conditions +
 solve), and a CPU baseline: the reference libbfm compiled from its own C
 sources (oracle/_ref), single-threaded as libbfm is, on a bounded sample (its
 dense path cannot hold the 2M-DOF matrix: 32 TB).
@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch on the q1_2m beam measured by two rocprofv3 counter runs
 # (FETCH_SIZE x 1.9998 + WRITE_SIZE, the factors read off a 256 MiB
 # device-to-device copy in the same runs): profiles/hbm_traffic_q1_2m.md
-TRAFFIC_Q1_2M = {"k_cheb": 238.2e6, "k_cg_spmv": 389.3e6, "k_fcg_spmv": 392.0e6}
+TRAFFIC_Q1_2M = {"k_cg_spmv": 389.3e6, "k_fcg_spmv": 397.4e6, "k_cheb": 254.9e6}
 CPU_SAMPLE = (180, 45)   # Q1 beam cells for the reference CPU path (16,652 DOFs)
 
 
@@ -257,9 +257,9 @@ def main():
                 "frac": gbs / HBM_PEAK_GBS, "traffic": traffic, "kernel": label,
                 "kernel_us": k_ms * 1e3, "algorithmic_bytes": k_bytes}
 
-    roof_smooth = roof(D.KERNEL_AMG_SMOOTH, "k_cheb<2,double,false,false> (finest-level smoothing step)")
+    roof_fcg = roof(D.KERNEL_FCG_SPMV, "k_fcg_spmv (flexible-CG FP64 SpMV, the solve's largest kernel)")
+    roof_smooth = roof(D.KERNEL_AMG_SMOOTH, "k_cheb<2,double,false,true> (finest-level post-smoothing step)")
     roof_spmv = roof(D.KERNEL_CG_SPMV64, "k_cg_spmv<double> (FP64 CG SpMV)")
-    roof_fcg = roof(D.KERNEL_FCG_SPMV, "k_fcg_spmv (flexible-CG FP64 SpMV)")
 
     extra = {}
     if not args.no_pcg:
@@ -329,9 +329,9 @@ def main():
             "setup_s": t_setup,
             "end_to_end_s": t_setup + ms / 1e3,
         },
-        "roofline": roof_smooth,
+        "roofline": roof_fcg,
+        "roofline_smoother": roof_smooth,
         "roofline_spmv_fp64": roof_spmv,
-        "roofline_fcg_spmv": roof_fcg,
         "cpu_baseline": cpu,
     }
     out.update(extra)

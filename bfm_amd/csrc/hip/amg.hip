@@ -1435,6 +1435,9 @@ int amg_build(bfmx_system* s) {
 		if (const char* e = std::getenv("BFMX_AMG_DEG")) {
 			par.deg = std::min(kMaxDeg, std::max(1, std::atoi(e)));
 		}
+		if (const char* e = std::getenv("BFMX_AMG_DEG_COARSE")) {
+			par.deg_coarse = std::min(kMaxDeg, std::max(1, std::atoi(e)));
+		}
 		if (const char* e = std::getenv("BFMX_AMG_RATIO")) {
 			par.ratio = std::max(1.5, std::atof(e));
 		}
@@ -1798,7 +1801,7 @@ int amg_build(bfmx_system* s) {
 				bool const last = l + 1 == lv.size();
 				AmgLevelDesc D{};
 				D.nb = L.nb;
-				D.deg = last ? kCoarseDeg : s->amg_par.deg;
+				D.deg = last ? kCoarseDeg : s->amg_par.deg_coarse;
 				D.dense = last && L.d_dense ? 1 : 0;
 				D.slice_ptr = L.d_slice_ptr;
 				D.col = L.d_col;
@@ -1887,7 +1890,8 @@ int amg_setup_numeric(bfmx_system* s) {
 			}
 		}
 		k_cheb_coef<<<1, kBlock, 0, st>>>(g, L.d_partial, coarsest ? kCoarseRatio : s->amg_par.ratio,
-		                                  coarsest ? kCoarseDeg : s->amg_par.deg, power > 0, L.d_scal);
+		                                  coarsest ? kCoarseDeg : l == 0 ? s->amg_par.deg : s->amg_par.deg_coarse,
+		                                  power > 0, L.d_scal);
 		if (coarsest) {
 			continue;
 		}
@@ -1956,7 +1960,7 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 		}
 		return L.d_x[0];
 	}
-	int const deg = coarsest ? kCoarseDeg : s->amg_par.deg;
+	int const deg = coarsest ? kCoarseDeg : l == 0 ? s->amg_par.deg : s->amg_par.deg_coarse;
 	int cur = 0;
 	cheb<BS, BT>(st, L, b, true, 0, nullptr, L.d_x[0], nullptr, nullptr, S);
 	if (coarsest) {
@@ -1989,12 +1993,13 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 
 }  // namespace
 
-// reps launches of the finest level's Chebyshev step with its SpMV (the
-// V-cycle's step k = 1), bracketed by events e0 and e1 on the system's
-// stream; b is the solver's residual buffer. Algorithmic bytes per launch:
-// FP32 blocks and column indices, slice pointers, and per node the FP64 b,
-// the FP32 inverse diagonal block, x_in (gathered once), d read and written
-// and x_out written.
+// reps launches of the finest level's last post-smoothing step as the
+// V-cycle runs it (Chebyshev step deg - 1 with its SpMV, fused with the
+// partial sums of x . q), bracketed by events e0 and e1 on the system's
+// stream; b is the solver's residual buffer and q its q buffer. Algorithmic
+// bytes per launch: FP32 blocks and column indices, slice pointers, and per
+// node the FP64 b, the FP32 inverse diagonal block, x_in (gathered once), d
+// read and written, x_out written and the FP64 q.
 int amg_time_smoother(bfmx_system* s, int reps, hipEvent_t e0, hipEvent_t e1, double* bytes) {
 	hipStream_t const st = s->stream;
 	AmgLevel& L = s->amg[0];
@@ -2003,19 +2008,20 @@ int amg_time_smoother(bfmx_system* s, int reps, hipEvent_t e0, hipEvent_t e1, do
 		return BFMX_E_UNSUPPORTED;
 	}
 	const Scalars* S = s->d_scal;
+	int const step = s->amg_par.deg - 1;
 	cheb<2, double>(st, L, s->d_r, true, 0, nullptr, L.d_x[0], nullptr, nullptr, S);
 	for (int i = 0; i < 3; i++) {
-		cheb<2, double>(st, L, s->d_r, false, 1, L.d_x[0], L.d_x[1], nullptr, nullptr, S);
+		cheb<2, double>(st, L, s->d_r, false, step, L.d_x[0], L.d_x[1], s->d_q, s->d_partial, S);
 	}
 	BFMX_HIP(hipGetLastError());
 	BFMX_HIP(hipEventRecord(e0, st));
 	for (int i = 0; i < reps; i++) {
-		cheb<2, double>(st, L, s->d_r, false, 1, L.d_x[0], L.d_x[1], nullptr, nullptr, S);
+		cheb<2, double>(st, L, s->d_r, false, step, L.d_x[0], L.d_x[1], s->d_q, s->d_partial, S);
 	}
 	BFMX_HIP(hipEventRecord(e1, st));
 	BFMX_HIP(hipGetLastError());
 	if (bytes) {
-		*bytes = static_cast<double>(L.nnzb) * 20 + 4.0 * (L.n_slices + 1) + 64.0 * L.nb;
+		*bytes = static_cast<double>(L.nnzb) * 20 + 4.0 * (L.n_slices + 1) + 80.0 * L.nb;
 	}
 	return BFMX_OK;
 }

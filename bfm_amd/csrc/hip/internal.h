@@ -161,10 +161,11 @@ struct AmgLevelDesc {
 };
 
 // Multigrid parameters of a hierarchy, fixed when it is built (defaults,
-// or the environment's BFMX_AMG_DEG, BFMX_AMG_RATIO, BFMX_AMG_COARSEST,
+// or the environment's BFMX_AMG_DEG, BFMX_AMG_DEG_COARSE, BFMX_AMG_RATIO, BFMX_AMG_COARSEST,
 // BFMX_AMG_OMEGA).
 struct AmgParams {
-	int deg = 2;            // Chebyshev steps before and after each coarse correction
+	int deg = 1;            // Chebyshev steps before and after each coarse correction (level 0)
+	int deg_coarse = 2;     // ... on the coarser levels
 	double ratio = 30.0;    // Chebyshev interval [lambda_max / ratio, lambda_max]
 	int coarsest = 64;      // coarsen until every segment of a level has at most this many block rows
 	double omega = 1.0;     // prolongator damping: omega * 4 / (3 lambda_max)

@@ -1316,6 +1316,7 @@ int time_kernel(bfmx_system* s, int kernel, int reps, double* avg_ms, double* by
 	BFMX_HIP(hipMemcpyAsync(s->d_scal, &init, sizeof(Scalars), hipMemcpyHostToDevice, st));
 	k_fill<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 1.0, s->d_r);
 	k_fill<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 0.0, s->d_p[0]);
+	k_fill<<<grid(2 * n), kBlock, 0, st>>>(2 * n, 1.0, s->d_q);
 	float ms = 0;
 	if (kernel == BFMX_KERNEL_FCG_SPMV) {
 		int const nb = static_cast<int>(n);

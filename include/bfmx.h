@@ -197,8 +197,9 @@ enum {
 	BFMX_KERNEL_CG_SPMV64 = 0,   /* FP64 CG SpMV (as bfmx_system_time_spmv, FP64) */
 	BFMX_KERNEL_CG_SPMV32 = 1,   /* FP32-matrix CG SpMV (as bfmx_system_time_spmv, mixed) */
 	BFMX_KERNEL_FCG_SPMV = 2,    /* flexible-CG SpMV: FP32 z, FP64 matrix and directions */
-	BFMX_KERNEL_AMG_SMOOTH = 3,  /* one finest-level Chebyshev smoothing step of the
-	                                multigrid V-cycle: FP32 SpMV + block Jacobi */
+	BFMX_KERNEL_AMG_SMOOTH = 3,  /* the finest level's last post-smoothing step of the
+	                                multigrid V-cycle: FP32 SpMV + block-Jacobi
+	                                Chebyshev step + partial sums of x . q */
 };
 
 /* Times reps back-to-back launches of one solver kernel on the system's
