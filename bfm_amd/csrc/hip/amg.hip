@@ -1267,6 +1267,11 @@ bool amg_smoothing() {
 	return !(e && e[0] == '0');
 }
 
+// Chebyshev steps per side on level l (not the coarsest).
+int level_deg(const AmgParams& p, size_t l) {
+	return l == 0 ? p.deg : l == 1 ? p.deg_coarse : p.deg_deep;
+}
+
 // Row ranges of the graph (ptr, adj) that no entry couples, as offsets (0,
 // ..., n): a boundary follows row i when no row up to i has a neighbour
 // after i (the adjacency is symmetric).
@@ -1437,6 +1442,10 @@ int amg_build(bfmx_system* s) {
 		}
 		if (const char* e = std::getenv("BFMX_AMG_DEG_COARSE")) {
 			par.deg_coarse = std::min(kMaxDeg, std::max(1, std::atoi(e)));
+		}
+		par.deg_deep = par.deg_coarse;
+		if (const char* e = std::getenv("BFMX_AMG_DEG_DEEP")) {
+			par.deg_deep = std::min(kMaxDeg, std::max(1, std::atoi(e)));
 		}
 		if (const char* e = std::getenv("BFMX_AMG_RATIO")) {
 			par.ratio = std::max(1.5, std::atof(e));
@@ -1801,7 +1810,7 @@ int amg_build(bfmx_system* s) {
 				bool const last = l + 1 == lv.size();
 				AmgLevelDesc D{};
 				D.nb = L.nb;
-				D.deg = last ? kCoarseDeg : s->amg_par.deg_coarse;
+				D.deg = last ? kCoarseDeg : level_deg(s->amg_par, l);
 				D.dense = last && L.d_dense ? 1 : 0;
 				D.slice_ptr = L.d_slice_ptr;
 				D.col = L.d_col;
@@ -1890,7 +1899,7 @@ int amg_setup_numeric(bfmx_system* s) {
 			}
 		}
 		k_cheb_coef<<<1, kBlock, 0, st>>>(g, L.d_partial, coarsest ? kCoarseRatio : s->amg_par.ratio,
-		                                  coarsest ? kCoarseDeg : l == 0 ? s->amg_par.deg : s->amg_par.deg_coarse,
+		                                  coarsest ? kCoarseDeg : level_deg(s->amg_par, l),
 		                                  power > 0, L.d_scal);
 		if (coarsest) {
 			continue;
@@ -1960,7 +1969,7 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 		}
 		return L.d_x[0];
 	}
-	int const deg = coarsest ? kCoarseDeg : l == 0 ? s->amg_par.deg : s->amg_par.deg_coarse;
+	int const deg = coarsest ? kCoarseDeg : level_deg(s->amg_par, l);
 	int cur = 0;
 	cheb<BS, BT>(st, L, b, true, 0, nullptr, L.d_x[0], nullptr, nullptr, S);
 	if (coarsest) {

@@ -804,11 +804,22 @@ int blocks_of(int64_t n) {
 
 // The union system of the multigrid mode from the instances' host topology
 // (renumbered adjacency and coordinates), its rows the instances' rows one
-// after another.
+// after another. On failure nothing of it stays.
+int union_build_parts(bfmx_batch* b);
+
 int union_build(bfmx_batch* b) {
 	if (b->uni) {
 		return BFMX_OK;
 	}
+	int const rc = union_build_parts(b);
+	if (rc != BFMX_OK) {
+		(void) hipStreamSynchronize(b->stream);
+		union_release(b);
+	}
+	return rc;
+}
+
+int union_build_parts(bfmx_batch* b) {
 	size_t const m = b->systems.size();
 	std::vector<int64_t> off(m + 1, 0);
 	int64_t nnz = 0;

@@ -161,11 +161,12 @@ struct AmgLevelDesc {
 };
 
 // Multigrid parameters of a hierarchy, fixed when it is built (defaults,
-// or the environment's BFMX_AMG_DEG, BFMX_AMG_DEG_COARSE, BFMX_AMG_RATIO, BFMX_AMG_COARSEST,
+// or the environment's BFMX_AMG_DEG, BFMX_AMG_DEG_COARSE, BFMX_AMG_DEG_DEEP, BFMX_AMG_RATIO, BFMX_AMG_COARSEST,
 // BFMX_AMG_OMEGA).
 struct AmgParams {
 	int deg = 1;            // Chebyshev steps before and after each coarse correction (level 0)
-	int deg_coarse = 2;     // ... on the coarser levels
+	int deg_coarse = 2;     // ... on level 1
+	int deg_deep = 2;       // ... on the levels below it
 	double ratio = 30.0;    // Chebyshev interval [lambda_max / ratio, lambda_max]
 	int coarsest = 64;      // coarsen until every segment of a level has at most this many block rows
 	double omega = 1.0;     // prolongator damping: omega * 4 / (3 lambda_max)

@@ -183,7 +183,14 @@ __device__ __forceinline__ dbl2 apply_precond(const double* __restrict__ dinv, i
 }
 
 // y = A x for block row `row`; MT is the stored value type (double or float),
-// the accumulation is FP64. ldx(c) returns x at node c.
+// the accumulation is FP64. ldx(c) returns x at node c. The block loop is
+// unrolled kSpmvUnroll times, so the loads of that many blocks are in flight
+// together.
+#ifndef BFMX_SPMV_UNROLL
+#define BFMX_SPMV_UNROLL 3
+#endif
+constexpr int kSpmvUnroll = BFMX_SPMV_UNROLL;
+
 template <typename MT, typename LoadX>
 __device__ __forceinline__ dbl2 row_spmv(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ col,
                                          const MT* __restrict__ val, int row, LoadX ldx) {
@@ -191,7 +198,7 @@ __device__ __forceinline__ dbl2 row_spmv(const int32_t* __restrict__ slice_ptr, 
 	int32_t const beg = slice_ptr[s], end = slice_ptr[s + 1];
 	const MT* v = val + 4 * static_cast<int64_t>(beg) + 2 * r;
 	double y0 = 0, y1 = 0;
-#pragma unroll 3
+#pragma unroll kSpmvUnroll
 	for (int32_t blk = beg + r; blk < end; blk += kSlice, v += 4 * kSlice) {
 		int32_t const c = __builtin_nontemporal_load(col + blk);
 		dbl2 const t = ldv(v), u = ldv(v + 2 * kSlice);
