@@ -795,11 +795,12 @@ __global__ __launch_bounds__(kBlock) void k_cheb(int nb, const int32_t* __restri
                                                  const float* __restrict__ xin, float* __restrict__ xout,
                                                  float* __restrict__ d, const double* __restrict__ scal, int step,
                                                  const double* __restrict__ q, double* __restrict__ partial,
-                                                 const Scalars* __restrict__ S) {
+                                                 SegView sv, const Scalars* __restrict__ S) {
 	if (S->done) {
 		return;
 	}
-	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	int const blk = xcd_remap(blockIdx.x, gridDim.x);
+	int const i = blk * kBlock + threadIdx.x;
 	float const c1 = static_cast<float>(scal[1 + 2 * step]);
 	float const c2 = static_cast<float>(scal[2 + 2 * step]);
 	double dot = 0;
@@ -814,7 +815,14 @@ __global__ __launch_bounds__(kBlock) void k_cheb(int nb, const int32_t* __restri
 		}
 	}
 	if (DOT) {
-		wg_partial<true>(dot, partial);
+		if (sv.seg) {
+			// the two segments of the block apart (segmented flexible CG)
+			bool const second = static_cast<int>(threadIdx.x) >= sv.split[blk];
+			double lo[1] = {second ? 0.0 : dot}, hi[1] = {second ? dot : 0.0};
+			seg_block_partials<1>(lo, hi, blk, gridDim.x, partial);
+		} else {
+			wg_partial<true>(dot, partial);
+		}
 	}
 }
 
@@ -1930,17 +1938,18 @@ namespace {
 
 template <int BS, typename BT>
 void cheb(hipStream_t st, AmgLevel& L, const BT* b, bool first, int step, const float* xin, float* xout,
-          const double* q, double* partial, const Scalars* S) {
+          const double* q, double* partial, const Scalars* S, SegView sv = SegView{}) {
 	int const g = grid(L.nb);
 	if (first) {
 		k_cheb<BS, BT, true, false><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_dinv, xin,
-		                                                  xout, L.d_d, L.d_scal, step, nullptr, nullptr, S);
+		                                                  xout, L.d_d, L.d_scal, step, nullptr, nullptr, sv, S);
 	} else if (q) {
 		k_cheb<BS, BT, false, true><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_dinv, xin,
-		                                                  xout, L.d_d, L.d_scal, step, q, partial, S);
+		                                                  xout, L.d_d, L.d_scal, step, q, partial, sv, S);
 	} else {
 		k_cheb<BS, BT, false, false><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_dinv,
-		                                                   xin, xout, L.d_d, L.d_scal, step, nullptr, nullptr, S);
+		                                                   xin, xout, L.d_d, L.d_scal, step, nullptr, nullptr, sv,
+		                                                   S);
 	}
 }
 
@@ -1992,9 +2001,13 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 	    C.nb, L.d_rptr, L.d_rent, L.d_rrow, L.d_P, L.d_r, C.d_b, S);
 	float* xc = vcycle<3, float>(s, l + 1, C.d_b, nullptr, nullptr, S);
 	k_prolong<BS><<<g, kBlock, 0, st>>>(L.nb, L.d_pptr, L.d_pcol, L.d_P, xc, L.d_x[cur], S);
+	// the finest level's last step also writes the partial sums of x . q,
+	// split by segment in a segmented flexible CG
+	SegView const sv = l == 0 ? s->fcg_sv : SegView{};
 	for (int k = 0; k < deg; k++) {
 		bool const last = k + 1 == deg;
-		cheb<BS, BT>(st, L, b, false, k, L.d_x[cur], L.d_x[cur ^ 1], last ? q : nullptr, last ? partial : nullptr, S);
+		cheb<BS, BT>(st, L, b, false, k, L.d_x[cur], L.d_x[cur ^ 1], last ? q : nullptr, last ? partial : nullptr, S,
+		             last ? sv : SegView{});
 		cur ^= 1;
 	}
 	return L.d_x[cur];

@@ -72,6 +72,52 @@ struct alignas(128) Scalars {
 	int32_t updates; // true-residual recomputations
 };
 
+// Segmented flexible CG: a system whose convergence segments (at least kBlock
+// rows each) run their own Krylov scalars. Every kBlock-row block then holds
+// rows of at most two segments: seg[2 b] from its first row on and
+// seg[2 b + 1] from its row split[b] on (split[b] = kBlock when the block lies
+// in one segment). A block's partial sums are kept apart for the two, value
+// k of block b of nblk at partial[(k nblk + b) 2 + h], h = 0 (first) or 1.
+struct SegView {
+	const int32_t* seg = nullptr;     // 2 per block (nullptr: not segmented)
+	const int32_t* split = nullptr;   // 1 per block
+};
+
+template <int NV>
+__device__ __forceinline__ void seg_block_partials(double (&lo)[NV], double (&hi)[NV], int64_t blk, int64_t nblk,
+                                                   double* __restrict__ partial) {
+	__shared__ double lds[2 * NV][kBlock / 64];
+#pragma unroll
+	for (int k = 0; k < NV; k++) {
+#pragma unroll
+		for (int off = 32; off > 0; off >>= 1) {
+			lo[k] += __shfl_xor(lo[k], off, 64);
+			hi[k] += __shfl_xor(hi[k], off, 64);
+		}
+	}
+	if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+		for (int k = 0; k < NV; k++) {
+			lds[k][threadIdx.x >> 6] = lo[k];
+			lds[NV + k][threadIdx.x >> 6] = hi[k];
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x == 0) {
+#pragma unroll
+		for (int k = 0; k < NV; k++) {
+			double a = 0, c = 0;
+#pragma unroll
+			for (int w = 0; w < kBlock / 64; w++) {
+				a += lds[k][w];
+				c += lds[NV + k][w];
+			}
+			partial[(k * nblk + blk) * 2] = a;
+			partial[(k * nblk + blk) * 2 + 1] = c;
+		}
+	}
+}
+
 // One level of the aggregation multigrid used as the FP32 preconditioner of
 // the flexible-CG path. Level 0 is the system's own matrix (2x2 blocks);
 // coarser levels have three DOFs per node (the coefficients of the three
@@ -261,6 +307,14 @@ struct bfmx_system {
 	int n_conv_seg = 0;
 	const int64_t* d_conv_seg = nullptr;
 	double* d_conv_part = nullptr;
+	// segmented flexible CG over the convergence segments (when each has at
+	// least kBlock rows): the per-block view, per segment alpha, beta, p.q,
+	// and the two-slot partial sums (4 per block)
+	bfmx::SegView fcg_sv;
+	int32_t* d_fcg_seg = nullptr;
+	int32_t* d_fcg_split = nullptr;
+	double* d_fcg_scal = nullptr;
+	double* d_fcg_part = nullptr;
 	bfmx_model* model = nullptr;       // learned preconditioner (borrowed)
 	float* d_mp_feat = nullptr;        // its node features (8 per node)
 	float* d_mp_h[4] = {nullptr, nullptr, nullptr, nullptr};   // hidden features (FP32, 64 per node)

@@ -314,18 +314,23 @@ __global__ __launch_bounds__(kRed) void k_red_fcg_beta(int nwg, const double* __
 // p_new = z + beta p_old (z the preconditioner's output: FP32 from the
 // V-cycle, FP64 from the learned preconditioner), q = A p_new with the FP64
 // matrix, partials p_new . q and p_new . r.
-template <typename ZT>
+// SEG: a segmented flexible CG (beta of the row's segment, partial sums kept
+// apart per segment).
+template <typename ZT, bool SEG = false>
 __global__ __launch_bounds__(kBlock) void k_fcg_spmv(int nb, const int32_t* __restrict__ slice_ptr,
                                                      const int32_t* __restrict__ col,
                                                      const double* __restrict__ val, const ZT* __restrict__ z,
                                                      const double* __restrict__ pold, double* __restrict__ pnew,
                                                      double* __restrict__ q, const double* __restrict__ r,
-                                                     double* __restrict__ partial, const Scalars* __restrict__ S) {
+                                                     double* __restrict__ partial, const Scalars* __restrict__ S,
+                                                     SegView sv = SegView{}, const double* __restrict__ seg_scal = nullptr) {
 	if (S->done) {
 		return;
 	}
-	double const beta = S->beta;
-	int const row = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	int const blk = xcd_remap(blockIdx.x, gridDim.x);
+	bool const second = SEG && static_cast<int>(threadIdx.x) >= sv.split[blk];
+	double const beta = SEG ? seg_scal[3 * sv.seg[2 * blk + second] + 1] : S->beta;
+	int const row = blk * kBlock + threadIdx.x;
 	auto ldz = [&](int c) {
 		if constexpr (sizeof(ZT) == 4) {
 			flt2 const f = *reinterpret_cast<const flt2*>(z + 2 * c);
@@ -344,7 +349,56 @@ __global__ __launch_bounds__(kBlock) void k_fcg_spmv(int nb, const int32_t* __re
 		v[0] = pn.x * y.x + pn.y * y.y;
 		v[1] = pn.x * rv.x + pn.y * rv.y;
 	}
-	write_partials<2>(v, partial);
+	if (SEG) {
+		double lo[2] = {second ? 0.0 : v[0], second ? 0.0 : v[1]};
+		double hi[2] = {second ? v[0] : 0.0, second ? v[1] : 0.0};
+		seg_block_partials<2>(lo, hi, blk, gridDim.x, partial);
+	} else {
+		write_partials<2>(v, partial);
+	}
+}
+
+// Segmented flexible CG: per segment (one wavefront each), the sums of its
+// blocks' partials in block order, then MODE 0: beta_k = -(z . q_old)_k /
+// (p . q)_k of the previous step; MODE 1: (p . q)_k and alpha_k =
+// (p . r)_k / (p . q)_k. A segment whose direction vanishes (no load, or
+// solved) gets alpha_k = 0; p . q < 0 or not finite is a breakdown.
+// seg_scal holds alpha, beta, p . q per segment.
+template <int MODE>
+__global__ __launch_bounds__(64) void k_seg_red_fcg(const int64_t* __restrict__ off, SegView sv, int64_t nblk,
+                                                    const double* __restrict__ partial,
+                                                    double* __restrict__ seg_scal, Scalars* __restrict__ S) {
+	if (S->done) {
+		return;
+	}
+	int const k = blockIdx.x;
+	int64_t const b0 = off[k] / kBlock, b1 = (off[k + 1] - 1) / kBlock;
+	double a = 0, c = 0;
+	for (int64_t b = b0 + threadIdx.x; b <= b1; b += 64) {
+		int const h = sv.seg[2 * b] == k ? 0 : 1;
+		a += partial[2 * b + h];
+		if (MODE == 1) {
+			c += partial[2 * (nblk + b) + h];
+		}
+	}
+	a = wave_sum(a);
+	c = wave_sum(c);
+	if (threadIdx.x == 0) {
+		double* sc = seg_scal + 3 * k;
+		if (MODE == 0) {
+			sc[1] = sc[2] > 0 ? -a / sc[2] : 0.0;
+		} else {
+			sc[2] = a;
+			if (a > 0 && isfinite(a)) {
+				sc[0] = c / a;
+			} else {
+				sc[0] = 0;
+				if (!(a == 0 && c == 0)) {
+					S->done = 2;
+				}
+			}
+		}
+	}
 }
 
 // alpha = (p . r) / (p . q), or a breakdown.
@@ -367,14 +421,17 @@ __global__ __launch_bounds__(kRed) void k_red_fcg_alpha(int nwg, const double* _
 }
 
 // x += alpha p, r -= alpha q, partials r . r.
+template <bool SEG = false>
 __global__ __launch_bounds__(kBlock) void k_fcg_update(int nb, const double* __restrict__ p,
                                                        const double* __restrict__ q, double* __restrict__ x,
                                                        double* __restrict__ r, double* __restrict__ partial,
-                                                       const Scalars* __restrict__ S) {
+                                                       const Scalars* __restrict__ S, SegView sv = SegView{},
+                                                       const double* __restrict__ seg_scal = nullptr) {
 	if (S->done) {
 		return;
 	}
-	double const alpha = S->alpha;
+	bool const second = SEG && static_cast<int>(threadIdx.x) >= sv.split[blockIdx.x];
+	double const alpha = SEG ? seg_scal[3 * sv.seg[2 * blockIdx.x + second]] : S->alpha;
 	int const i = blockIdx.x * kBlock + threadIdx.x;
 	double v[1] = {0};
 	if (i < nb) {
@@ -686,6 +743,22 @@ void launch_fcg_iteration(bfmx_system* s, int pre, int it) {
 	int const g = grid(nb);
 	int const a = it & 1, b = a ^ 1;
 	int nwz = 0;
+	if (pre == BFMX_PRECOND_AMG && s->fcg_sv.seg) {
+		// segmented: every convergence segment runs its own alpha and beta
+		int const m = s->n_conv_seg;
+		float* z = amg_vcycle(s, s->d_r, s->d_q, s->d_fcg_part, s->d_scal, &nwz);
+		k_seg_red_fcg<0><<<m, 64, 0, s->stream>>>(s->d_conv_seg, s->fcg_sv, g, s->d_fcg_part, s->d_fcg_scal,
+		                                          s->d_scal);
+		k_fcg_spmv<float, true><<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[a],
+		                                                     s->d_p[b], s->d_q, s->d_r, s->d_fcg_part, s->d_scal,
+		                                                     s->fcg_sv, s->d_fcg_scal);
+		k_seg_red_fcg<1><<<m, 64, 0, s->stream>>>(s->d_conv_seg, s->fcg_sv, g, s->d_fcg_part, s->d_fcg_scal,
+		                                          s->d_scal);
+		k_fcg_update<true><<<g, kBlock, 0, s->stream>>>(nb, s->d_p[b], s->d_q, s->d_x, s->d_r, s->d_partial,
+		                                                s->d_scal, s->fcg_sv, s->d_fcg_scal);
+		k_red_fcg_rr<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
+		return;
+	}
 	if (pre == BFMX_PRECOND_AMG) {
 		float* z = amg_vcycle(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nwz);
 		k_red_fcg_beta<<<1, kRed, 0, s->stream>>>(nwz, s->d_partial, s->d_scal);
@@ -698,7 +771,7 @@ void launch_fcg_iteration(bfmx_system* s, int pre, int it) {
 		                                                s->d_p[b], s->d_q, s->d_r, s->d_partial, s->d_scal);
 	}
 	k_red_fcg_alpha<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
-	k_fcg_update<<<g, kBlock, 0, s->stream>>>(nb, s->d_p[b], s->d_q, s->d_x, s->d_r, s->d_partial, s->d_scal);
+	k_fcg_update<false><<<g, kBlock, 0, s->stream>>>(nb, s->d_p[b], s->d_q, s->d_x, s->d_r, s->d_partial, s->d_scal);
 	k_red_fcg_rr<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
 }
 
@@ -792,6 +865,9 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 	}
 	BFMX_HIP(hipMemsetAsync(s->d_q, 0, 2 * n * sizeof(double), st));
 	BFMX_HIP(hipMemsetAsync(s->d_scal, 0, sizeof(Scalars), st));
+	if (s->d_fcg_scal) {
+		BFMX_HIP(hipMemsetAsync(s->d_fcg_scal, 0, 3 * static_cast<size_t>(s->n_conv_seg) * sizeof(double), st));
+	}
 	launch_residual(s, pre_res, 0, rtol2, 0.0);
 	BFMX_HIP(hipGetLastError());
 	BFMX_HIP(hipMemcpyAsync(&s->h_scal[0], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
@@ -841,6 +917,11 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 
 void release_solver(bfmx_system* s) {
 	amg_release(s);
+	dfree(s->d_fcg_seg);
+	dfree(s->d_fcg_split);
+	dfree(s->d_fcg_scal);
+	dfree(s->d_fcg_part);
+	s->fcg_sv = SegView{};
 	mpnn_release(s);
 	for (auto& gr : s->graphs) {
 		(void) hipGraphExecDestroy(gr.exec);
@@ -1121,6 +1202,54 @@ int solve_minres(bfmx_system* s, double rtol, int64_t cap, MrResult* res) {
 
 }  // namespace
 
+namespace {
+
+// Segmented flexible CG for a system with convergence segments of at least
+// kBlock rows each and a multigrid hierarchy of two or more levels (else the
+// segments share one alpha and beta): the per-block segment view, per-segment
+// scalars and the two-slot partial sums.
+int fcg_segments(bfmx_system* s) {
+	if (s->fcg_sv.seg || s->n_conv_seg == 0 || s->amg.size() < 2) {
+		return BFMX_OK;
+	}
+	int const m = s->n_conv_seg;
+	int64_t const n = s->n_nodes, nblk = grid(n);
+	std::vector<int64_t> off(m + 1);
+	BFMX_HIP(hipMemcpy(off.data(), s->d_conv_seg, off.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+	for (int k = 0; k < m; k++) {
+		if (off[k + 1] - off[k] < kBlock) {
+			return BFMX_OK;
+		}
+	}
+	std::vector<int32_t> seg(2 * nblk), split(nblk);
+	int k = 0;
+	for (int64_t b = 0; b < nblk; b++) {
+		int64_t const r0 = b * kBlock, r1 = std::min(n, r0 + kBlock);
+		while (off[k + 1] <= r0) {
+			k++;
+		}
+		seg[2 * b] = k;
+		if (off[k + 1] < r1) {   // the next segment starts inside the block (at most one does)
+			seg[2 * b + 1] = k + 1;
+			split[b] = static_cast<int32_t>(off[k + 1] - r0);
+		} else {
+			seg[2 * b + 1] = k;
+			split[b] = kBlock;
+		}
+	}
+	BFMX_CHECK(dalloc(&s->d_fcg_seg, seg.size()));
+	BFMX_CHECK(dalloc(&s->d_fcg_split, split.size()));
+	BFMX_CHECK(dalloc(&s->d_fcg_scal, 3 * static_cast<size_t>(m)));
+	BFMX_CHECK(dalloc(&s->d_fcg_part, 4 * static_cast<size_t>(nblk)));
+	BFMX_HIP(hipMemcpy(s->d_fcg_seg, seg.data(), seg.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+	BFMX_HIP(hipMemcpy(s->d_fcg_split, split.data(), split.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+	s->fcg_sv.seg = s->d_fcg_seg;
+	s->fcg_sv.split = s->d_fcg_split;
+	return BFMX_OK;
+}
+
+}  // namespace
+
 int solve(bfmx_system* s, const double* b_in, int precond, int precision, double rtol, size_t max_iter,
           bfmx_stats_t* stats) {
 	if (precond < BFMX_PRECOND_JACOBI || precond > BFMX_PRECOND_MINRES) {
@@ -1147,6 +1276,7 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	}
 	if (amg) {
 		BFMX_CHECK(amg_build(s));   // topology, once per system
+		BFMX_CHECK(fcg_segments(s));
 	}
 	if (learned) {
 		BFMX_CHECK(mpnn_prepare(s));

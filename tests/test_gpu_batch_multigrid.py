@@ -46,8 +46,9 @@ def test_multigrid_batch_meets_rtol_per_instance():
         u = pb.get_u(k)
         rel = host_rel_residual(pb.system(k), u)
         assert rel <= 1e-9 and st[k]["rel_residual"] <= 1e-9, (k, rel)
-        # the two evaluations of b - K u round differently: ~1e-13 of ||b|| apart
-        assert abs(rel - st[k]["rel_residual"]) <= 1e-3 * rel + 1e-13, (k, rel, st[k]["rel_residual"])
+        # the two evaluations of b - K u round differently (up to ~2e-13 of ||b||
+        # apart for panels that end far below rtol)
+        assert abs(rel - st[k]["rel_residual"]) <= 1e-3 * rel + 1e-12, (k, rel, st[k]["rel_residual"])
         if mesh.n_dof <= 1600:
             kinds, values, flags = meshgen.panel_conditions(mesh, t)
             u_ref, *_ = O.solve_dense(mesh, False, mat["E"], mat["nu"], mat["rho"], kinds, values, flags)
