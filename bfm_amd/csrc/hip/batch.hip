@@ -424,10 +424,10 @@ void launch(int pre, int cfg, int nblocks, hipStream_t st, const BatchInst* inst
 // another (each in its own renumbered order), solved by flexible CG with the
 // multigrid V-cycle. No entry couples two instances, so no aggregate spans
 // two and the coarsest level is inverted instance by instance. Every
-// instance's load is scaled to unit norm (the Krylov scalars are shared, so
-// no instance's residual should dominate the union's), and the union's
-// tolerance checks test every instance's own true residual (its convergence
-// segments): the run ends when each meets rtol.
+// instance's load is scaled to unit norm, and the instances are the union's
+// convergence segments: each runs its own CG scalars (when each has at least
+// kBlock rows) and the union's tolerance checks test every instance's own
+// true residual, so the run ends when each meets rtol.
 constexpr int kUniBlock = 256;
 
 // Per instance: b . b (fixed summation order).

@@ -287,13 +287,16 @@ int bfmx_batch_autotune(bfmx_batch_t* b, int precond, int sample, int iters, int
  * precond BFMX_PRECOND_AMG (multigrid mode): the batch is solved as one
  * block-diagonal system (the systems' rows one after another, a second copy
  * of their matrices) by flexible CG with the FP32 multigrid V-cycle, one
- * dense coarsest inverse per system. Each load is scaled to unit norm; the
- * union's tolerance checks test every system's own true residual, and the
- * run ends when each meets rtol. The stats report each system's true
- * residual and the union's iteration count. picks are
- * ignored; the first such call builds the union and its hierarchy on the
- * host. A system's solution depends on the batch (the Krylov scalars are
- * shared) only below rtol. Axisymmetric systems are refused
+ * dense coarsest inverse per system. Each load is scaled to unit norm; every
+ * system runs its own CG scalars (alpha, beta) when each has at least 256
+ * nodes, else they are shared; the union's tolerance checks test every
+ * system's own true residual, and the run ends when each meets rtol. The
+ * stats report each system's true residual and the union's iteration count.
+ * picks are ignored; the first such call builds the union and its hierarchy
+ * on the host. A system's iterates depend on the rest of the batch through
+ * the multigrid's smoother and prolongator parameters (one estimate of the
+ * largest eigenvalue for the union), so results match a solo solve to the
+ * tolerance, not bitwise. Axisymmetric systems are refused
  * (BFMX_E_UNSUPPORTED: indefinite). */
 int bfmx_batch_solve(bfmx_batch_t* b, int precond, double rtol, size_t max_iter, const int32_t* picks,
                      bfmx_stats_t* stats, double* total_ms);
