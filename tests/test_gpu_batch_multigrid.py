@@ -149,3 +149,36 @@ def test_multigrid_set_up_threads_give_the_same_hierarchy(monkeypatch):
         pb.close()
     assert out[0][0] == out[1][0]
     assert all(np.array_equal(a, b) for a, b in zip(out[0][1], out[1][1]))
+
+
+def test_segmented_cg_takes_the_iterations_of_the_hardest_instance():
+    """Every instance runs its own CG scalars: the batch needs about the
+    iterations of its hardest instance solved alone (shared scalars needed
+    more, growing with the batch: 30 at 16 panels, 51 at 4096)."""
+    meshes = [meshgen.panel(600 + k, 2000, 12000) for k in range(8)]
+    solo = []
+    for m, t in meshes:
+        b1 = D.Batch.from_instances([D.panel_instance(m, t)])
+        st, _ = b1.solve(precond=D.PRECOND_AMG, rtol=1e-8)
+        assert st[0]["status"] == 0
+        solo.append(st[0]["iterations"])
+        b1.close()
+    pb = D.Batch.from_instances([D.panel_instance(m, t) for m, t in meshes])
+    st, _ = pb.solve(precond=D.PRECOND_AMG, rtol=1e-8)
+    print(f"alone {solo}, together {st[0]['iterations']}")
+    assert all(x["status"] == 0 and x["rel_residual"] <= 1e-8 for x in st)
+    # one eigenvalue estimate for the whole batch and 4-iteration chunks between
+    # tolerance checks allow a few more
+    assert st[0]["iterations"] <= max(solo) + 4
+
+
+def test_small_instance_falls_back_to_shared_scalars():
+    """An instance of fewer than 256 nodes: the batch shares one pair of CG
+    scalars, and every instance still meets rtol on its own residual."""
+    meshes = [meshgen.panel(700, 200, 400)] + [meshgen.panel(701 + k, 2000, 6000) for k in range(3)]
+    assert meshes[0][0].n_nodes < 256
+    pb = D.Batch.from_instances([D.panel_instance(m, t) for m, t in meshes])
+    st, _ = pb.solve(precond=D.PRECOND_AMG, rtol=1e-8)
+    for k, (m, t) in enumerate(meshes):
+        assert st[k]["status"] == 0
+        assert host_rel_residual(pb.system(k), pb.get_u(k)) <= 1e-8, k
