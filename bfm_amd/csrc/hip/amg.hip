@@ -79,9 +79,9 @@ struct Blk<2> {   // the system's split layout: top rows, then bottom rows of 64
 	}
 };
 template <>
-struct Blk<3> {   // nine entries, each contiguous over the 64 blocks of a chunk
+struct Blk<3> {   // per block row a, the three entries of the 64 blocks of a chunk lane by lane
 	__device__ __forceinline__ static int64_t at(int64_t blk, int r, int a, int b) {
-		return (blk - r) * 9 + (a * 3 + b) * kSlice + r;
+		return (blk - r) * 9 + a * 3 * kSlice + 3 * r + b;
 	}
 };
 
@@ -360,13 +360,13 @@ __global__ __launch_bounds__(kBlock) void k_smooth_p(int nb, const int32_t* __re
 // receives zeros).
 constexpr int kGalLanes = 16;
 
-template <int BSF>
+template <int BSF, typename FT>
 __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32_t* __restrict__ mirror,
                                                      const int64_t* __restrict__ gal_ptr,
                                                      const int32_t* __restrict__ gal_blk,
                                                      const int32_t* __restrict__ gal_pi,
                                                      const int32_t* __restrict__ gal_pj,
-                                                     const double* __restrict__ fval, const float* __restrict__ P,
+                                                     const FT* __restrict__ fval, const float* __restrict__ P,
                                                      double* __restrict__ cval64, float* __restrict__ cval32) {
 	int64_t const b = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kGalLanes;
 	int const lane = threadIdx.x % kGalLanes;
@@ -376,8 +376,16 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32
 	int64_t const e0 = live ? gal_ptr[b] : 0, e1 = live ? gal_ptr[b + 1] : 0;
 	for (int64_t e = e0 + lane; e < e1; e += kGalLanes) {
 		int32_t const blk = gal_blk[e];
+		FT Af[BSF][BSF];
+		ld_block<BSF>(fval, blk, blk & 63, Af);
 		double A[BSF][BSF];
-		ld_block<BSF>(fval, blk, blk & 63, A);
+#pragma unroll
+		for (int a = 0; a < BSF; a++) {
+#pragma unroll
+			for (int c = 0; c < BSF; c++) {
+				A[a][c] = static_cast<double>(Af[a][c]);
+			}
+		}
 		const float* pi = P + BSF * 3 * static_cast<int64_t>(gal_pi[e]);
 		const float* pj = P + BSF * 3 * static_cast<int64_t>(gal_pj[e]);
 		double T[BSF][3];   // A_ij P_jJ
@@ -1451,7 +1459,6 @@ int amg_build(bfmx_system* s) {
 		if (const char* e = std::getenv("BFMX_AMG_DEG_COARSE")) {
 			par.deg_coarse = std::min(kMaxDeg, std::max(1, std::atoi(e)));
 		}
-		par.deg_deep = par.deg_coarse;
 		if (const char* e = std::getenv("BFMX_AMG_DEG_DEEP")) {
 			par.deg_deep = std::min(kMaxDeg, std::max(1, std::atoi(e)));
 		}
@@ -1918,16 +1925,18 @@ int amg_setup_numeric(bfmx_system* s) {
 			k_tentative<2><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
 			k_smooth_p<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_diag_blk, L.d_Pt,
 			                                    L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0, L.d_P);
-			k_galerkin<2><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr, L.d_gal_blk,
-			                                                        L.d_gal_pi, L.d_gal_pj, L.d_val64, L.d_P,
-			                                                        C.d_val64, C.d_val32);
+			k_galerkin<2, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
+			                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
+			                                                               L.d_val32, L.d_P, C.d_val64,
+			                                                               C.d_val32);
 		} else {
 			k_tentative<3><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
 			k_smooth_p<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_diag_blk, L.d_Pt,
 			                                    L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0, L.d_P);
-			k_galerkin<3><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr, L.d_gal_blk,
-			                                                        L.d_gal_pi, L.d_gal_pj, L.d_val64, L.d_P,
-			                                                        C.d_val64, C.d_val32);
+			k_galerkin<3, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
+			                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
+			                                                               L.d_val32, L.d_P, C.d_val64,
+			                                                               C.d_val32);
 		}
 	}
 	BFMX_HIP(hipGetLastError());

@@ -122,8 +122,9 @@ __device__ __forceinline__ void seg_block_partials(double (&lo)[NV], double (&hi
 // the flexible-CG path. Level 0 is the system's own matrix (2x2 blocks);
 // coarser levels have three DOFs per node (the coefficients of the three
 // rigid-body modes of the aggregate). Block storage is sliced ELL as on
-// level 0; 3x3 blocks keep each of their nine entries contiguous across a
-// slice's 64 rows.
+// level 0; the 3x3 blocks of a chunk of 64 (one per row of a slice) are
+// stored block row by block row, each lane's three entries of a row next to
+// each other (one 12-byte load per block row, contiguous across the lanes).
 struct AmgLevel {
 	int nb = 0;                 // block rows
 	int bs = 2;                 // DOFs per block row
