@@ -1803,8 +1803,9 @@ int amg_build(bfmx_system* s) {
 		}
 		if (const char* e = std::getenv("BFMX_AMG_VERBOSE"); e && e[0] == '1') {
 			for (size_t l = 0; l < lv.size(); l++) {
-				std::fprintf(stderr, "multigrid level %zu: %d rows of %d DOFs, %lld blocks\n", l, lv[l].nb, lv[l].bs,
-				             static_cast<long long>(lv[l].nnzb));
+				std::fprintf(stderr, "multigrid level %zu: %d rows of %d DOFs, %lld blocks (%lld with the slice padding)\n",
+				             l, lv[l].nb, lv[l].bs, static_cast<long long>(lv[l].nnzb),
+				             static_cast<long long>(lv[l].nnzb_pad));
 			}
 			AmgLevel const& Lc = lv.back();
 			std::fprintf(stderr, "multigrid coarsest: %d segments, largest %d rows, %s\n", Lc.n_seg, seg_largest(fseg),

@@ -182,3 +182,29 @@ def test_small_instance_falls_back_to_shared_scalars():
     for k, (m, t) in enumerate(meshes):
         assert st[k]["status"] == 0
         assert host_rel_residual(pb.system(k), pb.get_u(k)) <= 1e-8, k
+
+
+def test_multigrid_batch_mixes_element_kinds_and_materials():
+    """Q1 beams with per-element E and a body load beside P1 panels, one
+    multigrid launch: each instance against its own single-system solve."""
+    inst, systems = [], []
+    for k, (nx, ny) in enumerate([(48, 12), (96, 24)]):
+        mesh = meshgen.beam(nx, ny, kind=4, jitter=0.2, seed=40 + k)
+        eE = meshgen.loguniform_E(mesh.n_elems, lo=5e10, hi=2e11, seed=40 + k)
+        kinds, values, flags = meshgen.beam_conditions(mesh)
+        mat = meshgen.BEAM_MATERIAL
+        inst.append(D.planar_instance(mesh, True, mat, kinds, values, flags, g=(0.0, -9.81), elem_E=eE))
+        s = D.System.from_mesh(mesh)
+        s.assemble(1, mat["E"], mat["nu"], mat["rho"], elem_E=eE, g=(0.0, -9.81))
+        s.apply_conditions(kinds, values, flags)
+        systems.append(s)
+    for k in range(2):
+        m, t = meshgen.panel(800 + k, 3000, 9000)
+        inst.append(D.panel_instance(m, t))
+        systems.append(D.panel_system(m, t))
+    pb = D.Batch.from_instances(inst)
+    st, _ = pb.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+    for k, s in enumerate(systems):
+        assert st[k]["status"] == 0 and st[k]["rel_residual"] <= 1e-9, k
+        u_ref, _ = s.solve(precond=D.PRECOND_AMG, rtol=1e-10)
+        assert rel_err(pb.get_u(k), u_ref) <= 1e-5, k
