@@ -64,6 +64,19 @@ constexpr double kDropTol = 1e-10;      // relative remainder below which a mode
 constexpr int kGj = 1024;               // threads of the Gauss-Jordan workgroup (one segment)
 constexpr int kGjSeg = 256;             // ... (one workgroup per segment of many)
 constexpr int kBuildThreads = 16;       // host threads building a level's transfer operators
+// Blocks of a row whose loads the smoothing and residual row loops keep in
+// flight together, for 2x2 (finest level) and 3x3 blocks. Measured (2M-DOF
+// Q1 beam solve / 4096-panel P1 batch): no unrolling 12.20 ms / 797 ms,
+// 2 and 2: 11.82 / 773, 3 and 3: 11.52 / 799, 4 and 4: 11.72 / 783,
+// 2 and 3: 11.63 / 782.
+#ifndef BFMX_ROW_UNROLL2
+#define BFMX_ROW_UNROLL2 3
+#endif
+#ifndef BFMX_ROW_UNROLL3
+#define BFMX_ROW_UNROLL3 3
+#endif
+template <int BS>
+constexpr int kRowUnroll = BS == 2 ? BFMX_ROW_UNROLL2 : BFMX_ROW_UNROLL3;
 
 int grid(int64_t n) {
 	return static_cast<int>((n + kBlock - 1) / kBlock);
@@ -762,6 +775,7 @@ __device__ __forceinline__ void cheb_row(int i, const int32_t* __restrict__ slic
 	}
 	if (!FIRST) {
 		int const r = i & 63;
+#pragma unroll kRowUnroll<BS>
 		for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
 			int32_t const c = col[blk];
 			float A[BS][BS];
@@ -846,6 +860,7 @@ __device__ __forceinline__ void resid_row(int i, const int32_t* __restrict__ sli
 	for (int a = 0; a < BS; a++) {
 		res[a] = static_cast<float>(b[BS * static_cast<int64_t>(i) + a]);
 	}
+#pragma unroll kRowUnroll<BS>
 	for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
 		int32_t const c = col[blk];
 		float A[BS][BS];
