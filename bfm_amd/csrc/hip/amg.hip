@@ -514,6 +514,7 @@ __global__ __launch_bounds__(kBlock) void k_diag_gersh(int nb, const int32_t* __
 		for (int a = 0; a < BS; a++) {
 			rs[a] = 0;
 		}
+#pragma unroll kRowUnroll<BS>
 		for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
 			double A[BS][BS];
 			ld_block<BS>(val, blk, r, A);
@@ -578,6 +579,7 @@ __global__ __launch_bounds__(kBlock) void k_power_step(int nb, const int32_t* __
 		for (int a = 0; a < BS; a++) {
 			ax[a] = 0;
 		}
+#pragma unroll kRowUnroll<BS>
 		for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
 			int32_t const c = col[blk];
 			float A[BS][BS];
