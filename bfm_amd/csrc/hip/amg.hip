@@ -1491,6 +1491,10 @@ int amg_build(bfmx_system* s) {
 		if (const char* e = std::getenv("BFMX_AMG_POWER")) {
 			par.power = std::max(0, std::atoi(e));
 		}
+		par.power_coarse = par.power;
+		if (const char* e = std::getenv("BFMX_AMG_POWER_COARSE")) {
+			par.power_coarse = std::max(0, std::atoi(e));
+		}
 		// segments of the current level: aggregates never span two, and the
 		// first row of a segment is an aggregate root (its neighbours are all
 		// in the segment and not yet aggregated), so the aggregates of a
@@ -1910,7 +1914,7 @@ int amg_setup_numeric(bfmx_system* s) {
 		} else {
 			k_diag_gersh<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_diag_blk, L.d_val64, L.d_dinv, L.d_partial);
 		}
-		int const power = coarsest ? 0 : s->amg_par.power;
+		int const power = coarsest ? 0 : l == 0 ? s->amg_par.power : s->amg_par.power_coarse;
 		if (power > 0) {
 			// the Gershgorin maxima stay in d_partial's first g entries; the
 			// power sums use the entries after them

@@ -218,6 +218,7 @@ struct AmgParams {
 	int coarsest = 64;      // coarsen until every segment of a level has at most this many block rows
 	double omega = 1.0;     // prolongator damping: omega * 4 / (3 lambda_max)
 	int power = 10;         // power-iteration steps estimating lambda_max(D^-1 A); 0: Gershgorin bound
+	int power_coarse = 10;  // ... on the coarser levels
 };
 
 struct Graph {
