@@ -1,7 +1,12 @@
 // Smoothed aggregation multigrid: the FP32 preconditioner of the
 // flexible-CG path.
 //
-// Hierarchy topology (built once per system on the host from the node graph):
+// Hierarchy topology (built once per system on the host from the node graph,
+// by up to kBuildThreads threads over chunks of whole segments):
+//   * segments: row ranges that no entry couples (the parts of a mesh, or the
+//     systems of a batch solved as one); aggregates never span two, so the
+//     segments follow the hierarchy down, and coarsening stops when every
+//     segment has at most AmgParams::coarsest rows (or it stalls);
 //   * aggregates: a node whose neighbours are all free starts an aggregate
 //     with them (about 3 x 3 nodes on a quad mesh); a leftover node joins the
 //     aggregate of its first aggregated neighbour;
@@ -10,30 +15,34 @@
 //     as CSR of bs x 3 blocks, with its transpose as per-aggregate lists for
 //     the restriction;
 //   * the coarse block pattern (that of P^T A P) in sliced ELL, and for every
-//     coarse block slot the list of its contributions (fine block A_ij, entry
-//     P_iI, entry P_jJ), so the Galerkin product is formed without atomics
-//     and in a fixed order; the coarse pattern is the next level's graph.
+//     coarse block slot on or above the diagonal the list of its
+//     contributions (fine block A_ij, entry P_iI, entry P_jJ), so the
+//     Galerkin product is formed without atomics and in a fixed order, each
+//     block also stored as the transpose of its mirror; the coarse pattern is
+//     the next level's graph.
 // Numeric setup (every solve, on the GPU, because the Dirichlet set enters),
 // level by level:
 //   * level-0 modes B: the three rigid-body motions (two translations and the
 //     rotation, in coordinates centred on the mesh and scaled to its size),
 //     with the rows of constrained DOFs zeroed;
-//   * inverse diagonal blocks, the Gershgorin bound lambda_max of D^-1 A and
-//     the Chebyshev coefficients of the level;
+//   * inverse diagonal blocks, the Gershgorin bound of D^-1 A (the upper end
+//     of the level's Chebyshev interval) and a power-iteration estimate
+//     lambda of its largest eigenvalue (which sets the prolongator damping);
 //   * per aggregate, a Cholesky QR of its rows of B: P_t = B R^-1 (orthonormal
 //     columns) and the coarse modes B_c = R. A column whose remainder is below
 //     kDropTol of its norm (a mode that the constrained DOFs make dependent) is
 //     dropped: its coarse DOF is decoupled;
-//   * the smoothed prolongator P = (I - w D^-1 A) P_t, w = 4 / (3 lambda_max),
+//   * the smoothed prolongator P = (I - w D^-1 A) P_t, w = 4 / (3 lambda),
 //     stored FP32 (Vanek, Mandel and Brezina, Computing 56 (1996) 179-196);
-//   * the Galerkin operator P^T A P in FP64 with an FP32 copy, and a dense
-//     Gauss-Jordan inverse of the coarsest operator.
+//   * the Galerkin operator P^T A P in FP64 with an FP32 copy, and the dense
+//     Gauss-Jordan inverse of every segment of the coarsest level.
 // BFMX_AMG_SMOOTH=0 in the environment (read when a hierarchy is built)
 // keeps P = P_t: plain aggregation, for comparisons.
 // V-cycle (FP32 vectors and matrices, FP32 arithmetic): Chebyshev smoothing
 // with block Jacobi (AmgParams::deg steps before and after the coarse
-// correction), residual, restriction P^T r, recursion, prolongation; the
-// coarsest level applies its dense inverse.
+// correction on the finest level, deg_coarse / deg_deep below), residual,
+// restriction P^T r, recursion, prolongation; the levels of at most kWgRows
+// rows run in one workgroup; the coarsest level applies its dense inverses.
 
 #include <algorithm>
 #include <atomic>

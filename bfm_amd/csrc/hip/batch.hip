@@ -19,6 +19,10 @@
 //
 // Workspace: one packed FP64 allocation, per system x, r, z, p0, p1, q (2 per
 // node each) and the Jacobi inverses (4 per node).
+//
+// Multigrid mode (BFMX_PRECOND_AMG): the batch solved as one block-diagonal
+// system by the multigrid flexible CG of pcg.hip / amg.hip, every system a
+// convergence segment with its own CG scalars and tolerance check (below).
 
 #include <algorithm>
 #include <chrono>
