@@ -947,17 +947,38 @@ __device__ __forceinline__ void restrict_row(int64_t J, bool live, int lane, con
 	}
 }
 
+// With dinv: also the next level's first Chebyshev step on the restricted
+// row (it reads only that row: x0 = d = c2 D^-1 b_c, c2 from cscal), which
+// saves that step's launch.
 template <int BSF>
 __global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int32_t* __restrict__ rptr,
                                                      const int32_t* __restrict__ rent,
                                                      const int32_t* __restrict__ rrow, const float* __restrict__ P,
                                                      const float* __restrict__ r, float* __restrict__ bc,
+                                                     const float* __restrict__ dinv, const double* __restrict__ cscal,
+                                                     float* __restrict__ x0, float* __restrict__ d,
                                                      const Scalars* __restrict__ S) {
 	if (S->done) {
 		return;
 	}
 	int64_t const J = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kRestrictLanes;
-	restrict_row<BSF>(J, J < nc, threadIdx.x % kRestrictLanes, rptr, rent, rrow, P, r, bc);
+	int const lane = threadIdx.x % kRestrictLanes;
+	restrict_row<BSF>(J, J < nc, lane, rptr, rent, rrow, P, r, bc);
+	if (dinv && J < nc && lane == 0) {
+		float const c2 = static_cast<float>(cscal[2]);
+		float const bj[3] = {bc[3 * J], bc[3 * J + 1], bc[3 * J + 2]};
+#pragma unroll
+		for (int a = 0; a < 3; a++) {
+			float z = 0;
+#pragma unroll
+			for (int k = 0; k < 3; k++) {
+				z += dinv[9 * J + 3 * a + k] * bj[k];
+			}
+			float const dn = c2 * z;
+			d[3 * J + a] = dn;
+			x0[3 * J + a] = dn;
+		}
+	}
 }
 
 // Row i of x += P x_c.
@@ -1997,7 +2018,8 @@ void cheb(hipStream_t st, AmgLevel& L, const BT* b, bool first, int step, const 
 // below); returns the buffer that holds the level's correction. With q, the
 // last kernel also writes the per-workgroup partial sums of x . q.
 template <int BS, typename BT>
-float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* partial, const Scalars* S) {
+float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* partial, const Scalars* S,
+              bool first_done = false) {
 	hipStream_t const st = s->stream;
 	AmgLevel& L = s->amg[l];
 	bool const coarsest = l + 1 == s->amg.size();
@@ -2020,7 +2042,9 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 	}
 	int const deg = coarsest ? kCoarseDeg : level_deg(s->amg_par, l);
 	int cur = 0;
-	cheb<BS, BT>(st, L, b, true, 0, nullptr, L.d_x[0], nullptr, nullptr, S);
+	if (!first_done) {
+		cheb<BS, BT>(st, L, b, true, 0, nullptr, L.d_x[0], nullptr, nullptr, S);
+	}
 	if (coarsest) {
 		for (int k = 1; k < deg; k++) {
 			bool const last = k + 1 == deg;
@@ -2037,9 +2061,14 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 	AmgLevel& C = s->amg[l + 1];
 	int const g = grid(L.nb);
 	k_level_resid<BS, BT><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_x[cur], L.d_r, S);
+	// the restriction also takes the next level's first Chebyshev step when
+	// that level starts with one on the grid (not the one-workgroup cycle, not
+	// a dense coarsest)
+	bool const fuse = static_cast<int>(l + 1) != s->amg_wg && !(l + 2 == s->amg.size() && C.d_dense);
 	k_restrict<BS><<<grid(static_cast<int64_t>(C.nb) * kRestrictLanes), kBlock, 0, st>>>(
-	    C.nb, L.d_rptr, L.d_rent, L.d_rrow, L.d_P, L.d_r, C.d_b, S);
-	float* xc = vcycle<3, float>(s, l + 1, C.d_b, nullptr, nullptr, S);
+	    C.nb, L.d_rptr, L.d_rent, L.d_rrow, L.d_P, L.d_r, C.d_b, fuse ? C.d_dinv : nullptr, C.d_scal, C.d_x[0],
+	    C.d_d, S);
+	float* xc = vcycle<3, float>(s, l + 1, C.d_b, nullptr, nullptr, S, fuse);
 	k_prolong<BS><<<g, kBlock, 0, st>>>(L.nb, L.d_pptr, L.d_pcol, L.d_P, xc, L.d_x[cur], S);
 	// the finest level's last step also writes the partial sums of x . q,
 	// split by segment in a segmented flexible CG
