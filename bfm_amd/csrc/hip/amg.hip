@@ -301,10 +301,13 @@ __device__ __forceinline__ void diag_inverse(const double* __restrict__ val, int
 // blocks A_ij with agg(j) = J of A_ij Pt_j, w = omega 4 / (3 lambda_max(D^-1 A))
 // with lambda_max the level's power-iteration estimate (or Gershgorin bound);
 // omega = 0 keeps the tentative prolongator. Padding blocks are zero and add
-// nothing.
+// nothing. The off-diagonal blocks are read from the FP32 copy (P is stored
+// FP32), the diagonal inverse from the FP64 values.
+constexpr int kSmoothRegs = 6;   // entries per row summed in registers in one pass
 template <int BS>
 __global__ __launch_bounds__(kBlock) void k_smooth_p(int nb, const int32_t* __restrict__ slice_ptr,
                                                      const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                     const float* __restrict__ val32,
                                                      const int32_t* __restrict__ diag_blk,
                                                      const float* __restrict__ Pt, const int32_t* __restrict__ pptr,
                                                      const uint8_t* __restrict__ ppos, const double* __restrict__ scal,
@@ -321,37 +324,8 @@ __global__ __launch_bounds__(kBlock) void k_smooth_p(int nb, const int32_t* __re
 	int const self = ppos[dblk];
 	int32_t const p0 = pptr[i], np = pptr[i + 1] - p0;
 	int32_t const sbeg = slice_ptr[i >> 6] + r, send = slice_ptr[(i >> 6) + 1];
-	for (int k = 0; k < np; k++) {
-		double acc[BS][3];
-#pragma unroll
-		for (int a = 0; a < BS; a++) {
-#pragma unroll
-			for (int c = 0; c < 3; c++) {
-				acc[a][c] = 0;
-			}
-		}
-		if (w != 0) {
-			for (int32_t blk = sbeg; blk < send; blk += kSlice) {
-				if (ppos[blk] != k) {
-					continue;
-				}
-				int64_t const j = col[blk];
-				double A[BS][BS];
-				ld_block<BS>(val, blk, r, A);
-#pragma unroll
-				for (int a = 0; a < BS; a++) {
-#pragma unroll
-					for (int c = 0; c < 3; c++) {
-						double t = 0;
-#pragma unroll
-						for (int m = 0; m < BS; m++) {
-							t += A[a][m] * static_cast<double>(Pt[BS * 3 * j + 3 * m + c]);
-						}
-						acc[a][c] += t;
-					}
-				}
-			}
-		}
+	// writes entry k of the row from its sum acc = sum A_ij Pt_j
+	auto store = [&](int k, const double (&acc)[BS][3]) {
 		float* out = P + BS * 3 * static_cast<int64_t>(p0 + k);
 #pragma unroll
 		for (int a = 0; a < BS; a++) {
@@ -369,6 +343,75 @@ __global__ __launch_bounds__(kBlock) void k_smooth_p(int nb, const int32_t* __re
 				out[3 * a + c] = static_cast<float>(v);
 			}
 		}
+	};
+	// A_ij Pt_j for block blk (FP32 values, FP64 products)
+	auto term = [&](int32_t blk, double (&T)[BS][3]) {
+		int64_t const j = col[blk];
+		float A[BS][BS];
+		ld_block<BS>(val32, blk, r, A);
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				double t = 0;
+#pragma unroll
+				for (int m = 0; m < BS; m++) {
+					t += static_cast<double>(A[a][m]) * static_cast<double>(Pt[BS * 3 * j + 3 * m + c]);
+				}
+				T[a][c] = t;
+			}
+		}
+	};
+	if (np <= kSmoothRegs) {
+		// one pass over the row, each block's term added to its entry's sum
+		// (the entry is selected, not indexed, so the sums stay in registers)
+		double acc[kSmoothRegs][BS][3] = {};
+		if (w != 0) {
+#pragma unroll 2
+			for (int32_t blk = sbeg; blk < send; blk += kSlice) {
+				int const p = ppos[blk];
+				double T[BS][3];
+				term(blk, T);
+#pragma unroll
+				for (int k = 0; k < kSmoothRegs; k++) {
+#pragma unroll
+					for (int a = 0; a < BS; a++) {
+#pragma unroll
+						for (int c = 0; c < 3; c++) {
+							acc[k][a][c] += p == k ? T[a][c] : 0.0;
+						}
+					}
+				}
+			}
+		}
+#pragma unroll
+		for (int k = 0; k < kSmoothRegs; k++) {
+			if (k < np) {
+				store(k, acc[k]);
+			}
+		}
+		return;
+	}
+	// rows coupled to more aggregates: one pass per entry
+	for (int k = 0; k < np; k++) {
+		double acc[BS][3] = {};
+		if (w != 0) {
+			for (int32_t blk = sbeg; blk < send; blk += kSlice) {
+				if (ppos[blk] != k) {
+					continue;
+				}
+				double T[BS][3];
+				term(blk, T);
+#pragma unroll
+				for (int a = 0; a < BS; a++) {
+#pragma unroll
+					for (int c = 0; c < 3; c++) {
+						acc[a][c] += T[a][c];
+					}
+				}
+			}
+		}
+		store(k, acc);
 	}
 }
 
@@ -1975,16 +2018,18 @@ int amg_setup_numeric(bfmx_system* s) {
 		int64_t const ns = C.nnzb_pad;
 		if (L.bs == 2) {
 			k_tentative<2><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
-			k_smooth_p<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_diag_blk, L.d_Pt,
-			                                    L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0, L.d_P);
+			k_smooth_p<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_val32, L.d_diag_blk,
+			                                    L.d_Pt, L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0,
+			                                    L.d_P);
 			k_galerkin<2, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
 			                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
 			                                                               L.d_val32, L.d_P, C.d_val64,
 			                                                               C.d_val32);
 		} else {
 			k_tentative<3><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
-			k_smooth_p<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_diag_blk, L.d_Pt,
-			                                    L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0, L.d_P);
+			k_smooth_p<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_val32, L.d_diag_blk,
+			                                    L.d_Pt, L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0,
+			                                    L.d_P);
 			k_galerkin<3, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
 			                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
 			                                                               L.d_val32, L.d_P, C.d_val64,
