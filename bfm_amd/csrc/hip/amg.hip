@@ -1995,6 +1995,9 @@ int amg_setup_numeric(bfmx_system* s) {
 			int64_t const m = static_cast<int64_t>(L.bs) * L.nb;
 			k_power_init<<<grid(m), kBlock, 0, st>>>(m, L.d_x[0], pp);
 			k_power_norm<<<1, kBlock, 0, st>>>(grid(m), pp, L.d_scal);
+			// the iterates are normalised only for the last step, whose input is
+			// then a unit vector: lambda_max(D^-1 A) lies between 1 and a few
+			// units, so ten unnormalised FP32 steps neither overflow nor underflow
 			for (int k = 0; k < power; k++) {
 				float* const xin = L.d_x[k & 1];
 				float* const yout = L.d_x[(k & 1) ^ 1];
@@ -2005,7 +2008,9 @@ int amg_setup_numeric(bfmx_system* s) {
 					k_power_step<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, L.d_dinv, xin,
 					                                      L.d_scal, yout, pp);
 				}
-				k_power_norm<<<1, kBlock, 0, st>>>(g, pp, L.d_scal);
+				if (k + 2 >= power) {
+					k_power_norm<<<1, kBlock, 0, st>>>(g, pp, L.d_scal);
+				}
 			}
 		}
 		k_cheb_coef<<<1, kBlock, 0, st>>>(g, L.d_partial, coarsest ? kCoarseRatio : s->amg_par.ratio,
