@@ -423,7 +423,17 @@ __global__ __launch_bounds__(kBlock) void k_smooth_p(int nb, const int32_t* __re
 // also writes its transpose into the slot of (J, I) (mirror[b]; a slot with
 // J < I has mirror -2 and is left to its mirror; a padding slot has -1 and
 // receives zeros).
-constexpr int kGalLanes = 16;
+// Lanes per coarse slot and unrolling of their term loop: 16 and none
+// measured best (2M beam solve 11.15 ms; 8 lanes 11.50, 16 unrolled by 2
+// 11.15, 32 lanes 11.23, 32 unrolled by 2 11.35).
+#ifndef BFMX_GAL_LANES
+#define BFMX_GAL_LANES 16
+#endif
+#ifndef BFMX_GAL_UNROLL
+#define BFMX_GAL_UNROLL 1
+#endif
+constexpr int kGalLanes = BFMX_GAL_LANES;
+constexpr int kGalUnroll = BFMX_GAL_UNROLL;
 
 template <int BSF, typename FT>
 __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32_t* __restrict__ mirror,
@@ -439,6 +449,7 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32
 	bool const live = mb != -2;   // every lane stays for the butterfly
 	double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 	int64_t const e0 = live ? gal_ptr[b] : 0, e1 = live ? gal_ptr[b + 1] : 0;
+#pragma unroll kGalUnroll
 	for (int64_t e = e0 + lane; e < e1; e += kGalLanes) {
 		int32_t const blk = gal_blk[e];
 		FT Af[BSF][BSF];
