@@ -826,22 +826,39 @@ __global__ __launch_bounds__(NT) void k_gauss_jordan(int bs, const int32_t* __re
 // One Chebyshev step on a level: r = b - A x_in (skipped when FIRST: x_in = 0),
 // d = c1 d + c2 D^-1 r (c1 unused at step 0), x_out = x_in + d. With DOT, also
 // the per-workgroup sums of x_out . q (q an FP64 vector over the same rows).
-// Row i of one Chebyshev step (k_cheb); xo receives the row's x_out.
-template <int BS, typename BT, bool FIRST>
-__device__ __forceinline__ void cheb_row(int i, const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ col,
-                                         const float* __restrict__ val, const BT* __restrict__ b,
-                                         const float* __restrict__ dinv, const float* __restrict__ xin,
-                                         float* __restrict__ xout, float* __restrict__ d, float c1, float c2, int step,
-                                         float (&xo)[BS]) {
+// The first loads of row i of a level: its slice's block range and b_i. A
+// grid kernel issues them before it reads the done flag (done_after).
+template <int BS>
+struct RowHead {
+	int32_t beg, end;
 	float res[BS];
+};
+
+template <int BS, typename BT>
+__device__ __forceinline__ RowHead<BS> row_head(int i, const int32_t* __restrict__ slice_ptr,
+                                                const BT* __restrict__ b) {
+	RowHead<BS> h;
+	h.beg = slice_ptr[i >> 6];
+	h.end = slice_ptr[(i >> 6) + 1];
 #pragma unroll
 	for (int a = 0; a < BS; a++) {
-		res[a] = static_cast<float>(b[BS * static_cast<int64_t>(i) + a]);
+		h.res[a] = static_cast<float>(b[BS * static_cast<int64_t>(i) + a]);
 	}
+	return h;
+}
+
+// Row i of one Chebyshev step (k_cheb) from its head; xo receives the row's
+// x_out.
+template <int BS, bool FIRST>
+__device__ __forceinline__ void cheb_row(int i, RowHead<BS> h, const int32_t* __restrict__ col,
+                                         const float* __restrict__ val, const float* __restrict__ dinv,
+                                         const float* __restrict__ xin, float* __restrict__ xout,
+                                         float* __restrict__ d, float c1, float c2, int step, float (&xo)[BS]) {
+	float (&res)[BS] = h.res;
 	if (!FIRST) {
 		int const r = i & 63;
 #pragma unroll kRowUnroll<BS>
-		for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
+		for (int32_t blk = h.beg + r; blk < h.end; blk += kSlice) {
 			int32_t const c = col[blk];
 			float A[BS][BS];
 			ld_block<BS>(val, blk, r, A);
@@ -883,17 +900,18 @@ __global__ __launch_bounds__(kBlock) void k_cheb(int nb, const int32_t* __restri
                                                  float* __restrict__ d, const double* __restrict__ scal, int step,
                                                  const double* __restrict__ q, double* __restrict__ partial,
                                                  SegView sv, const Scalars* __restrict__ S) {
-	if (S->done) {
-		return;
-	}
 	int const blk = xcd_remap(blockIdx.x, gridDim.x);
 	int const i = blk * kBlock + threadIdx.x;
+	RowHead<BS> const h = row_head<BS>(min(i, nb - 1), slice_ptr, b);
+	if (FIRST ? done_after(S, h.res[0]) : done_after(S, h.beg, h.end, h.res[0])) {
+		return;
+	}
 	float const c1 = static_cast<float>(scal[1 + 2 * step]);
 	float const c2 = static_cast<float>(scal[2 + 2 * step]);
 	double dot = 0;
 	if (i < nb) {
 		float xo[BS];
-		cheb_row<BS, BT, FIRST>(i, slice_ptr, col, val, b, dinv, xin, xout, d, c1, c2, step, xo);
+		cheb_row<BS, FIRST>(i, h, col, val, dinv, xin, xout, d, c1, c2, step, xo);
 		if (DOT) {
 #pragma unroll
 			for (int a = 0; a < BS; a++) {
@@ -913,20 +931,15 @@ __global__ __launch_bounds__(kBlock) void k_cheb(int nb, const int32_t* __restri
 	}
 }
 
-// Row i of r = b - A x.
-template <int BS, typename BT>
-__device__ __forceinline__ void resid_row(int i, const int32_t* __restrict__ slice_ptr,
-                                          const int32_t* __restrict__ col, const float* __restrict__ val,
-                                          const BT* __restrict__ b, const float* __restrict__ x,
+// Row i of r = b - A x from its head.
+template <int BS>
+__device__ __forceinline__ void resid_row(int i, RowHead<BS> h, const int32_t* __restrict__ col,
+                                          const float* __restrict__ val, const float* __restrict__ x,
                                           float* __restrict__ rv) {
 	int const r = i & 63;
-	float res[BS];
-#pragma unroll
-	for (int a = 0; a < BS; a++) {
-		res[a] = static_cast<float>(b[BS * static_cast<int64_t>(i) + a]);
-	}
+	float (&res)[BS] = h.res;
 #pragma unroll kRowUnroll<BS>
-	for (int32_t blk = slice_ptr[i >> 6] + r; blk < slice_ptr[(i >> 6) + 1]; blk += kSlice) {
+	for (int32_t blk = h.beg + r; blk < h.end; blk += kSlice) {
 		int32_t const c = col[blk];
 		float A[BS][BS];
 		ld_block<BS>(val, blk, r, A);
@@ -951,12 +964,13 @@ __global__ __launch_bounds__(kBlock) void k_level_resid(int nb, const int32_t* _
                                                         const float* __restrict__ val, const BT* __restrict__ b,
                                                         const float* __restrict__ x, float* __restrict__ rv,
                                                         const Scalars* __restrict__ S) {
-	if (S->done) {
+	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	RowHead<BS> const h = row_head<BS>(min(i, nb - 1), slice_ptr, b);
+	if (done_after(S, h.beg, h.end, h.res[0])) {
 		return;
 	}
-	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
 	if (i < nb) {
-		resid_row<BS, BT>(i, slice_ptr, col, val, b, x, rv);
+		resid_row<BS>(i, h, col, val, x, rv);
 	}
 }
 
@@ -968,12 +982,12 @@ constexpr int kRestrictLanes = 8;
 // Coarse row J (live) by the lane group of `lane`; every lane of the group
 // must call it (the butterfly).
 template <int BSF>
-__device__ __forceinline__ void restrict_row(int64_t J, bool live, int lane, const int32_t* __restrict__ rptr,
+// [t0, t1) is the row's range of rptr (empty when not live).
+__device__ __forceinline__ void restrict_row(int64_t J, bool live, int lane, int32_t t0, int32_t t1,
                                              const int32_t* __restrict__ rent, const int32_t* __restrict__ rrow,
                                              const float* __restrict__ P, const float* __restrict__ r,
                                              float* __restrict__ bc) {
 	float acc[3] = {0, 0, 0};
-	int32_t const t0 = live ? rptr[J] : 0, t1 = live ? rptr[J + 1] : 0;
 	for (int32_t t = t0 + lane; t < t1; t += kRestrictLanes) {
 		int64_t const i = rrow[t];
 		const float* p = P + BSF * 3 * static_cast<int64_t>(rent[t]);
@@ -1012,12 +1026,15 @@ __global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int32_t* __re
                                                      const float* __restrict__ dinv, const double* __restrict__ cscal,
                                                      float* __restrict__ x0, float* __restrict__ d,
                                                      const Scalars* __restrict__ S) {
-	if (S->done) {
-		return;
-	}
 	int64_t const J = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) / kRestrictLanes;
 	int const lane = threadIdx.x % kRestrictLanes;
-	restrict_row<BSF>(J, J < nc, lane, rptr, rent, rrow, P, r, bc);
+	bool const live = J < nc;
+	int64_t const Jc = live ? J : nc - 1;
+	int32_t const t0 = rptr[Jc], t1 = rptr[Jc + 1];
+	if (done_after(S, t0, t1)) {
+		return;
+	}
+	restrict_row<BSF>(J, live, lane, live ? t0 : 0, live ? t1 : 0, rent, rrow, P, r, bc);
 	if (dinv && J < nc && lane == 0) {
 		float const c2 = static_cast<float>(cscal[2]);
 		float const bj[3] = {bc[3 * J], bc[3 * J + 1], bc[3 * J + 2]};
@@ -1035,17 +1052,33 @@ __global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int32_t* __re
 	}
 }
 
-// Row i of x += P x_c.
+// The first loads of row i of x += P x_c: x_i and the row's entry range.
 template <int BSF>
-__device__ __forceinline__ void prolong_row(int i, const int32_t* __restrict__ pptr, const int32_t* __restrict__ pcol,
-                                            const float* __restrict__ P, const float* __restrict__ xc,
-                                            float* __restrict__ x) {
+struct ProlongHead {
+	int32_t e0, e1;
 	float acc[BSF];
+};
+
+template <int BSF>
+__device__ __forceinline__ ProlongHead<BSF> prolong_head(int i, const int32_t* __restrict__ pptr,
+                                                         const float* __restrict__ x) {
+	ProlongHead<BSF> h;
+	h.e0 = pptr[i];
+	h.e1 = pptr[i + 1];
 #pragma unroll
 	for (int k = 0; k < BSF; k++) {
-		acc[k] = x[BSF * static_cast<int64_t>(i) + k];
+		h.acc[k] = x[BSF * static_cast<int64_t>(i) + k];
 	}
-	for (int32_t e = pptr[i]; e < pptr[i + 1]; e++) {
+	return h;
+}
+
+// Row i of x += P x_c from its head.
+template <int BSF>
+__device__ __forceinline__ void prolong_row(int i, ProlongHead<BSF> h, const int32_t* __restrict__ pcol,
+                                            const float* __restrict__ P, const float* __restrict__ xc,
+                                            float* __restrict__ x) {
+	float (&acc)[BSF] = h.acc;
+	for (int32_t e = h.e0; e < h.e1; e++) {
 		int64_t const J = pcol[e];
 		float const e0 = xc[3 * J], e1 = xc[3 * J + 1], e2 = xc[3 * J + 2];
 		const float* p = P + BSF * 3 * static_cast<int64_t>(e);
@@ -1066,12 +1099,13 @@ __global__ __launch_bounds__(kBlock) void k_prolong(int nb, const int32_t* __res
                                                     const int32_t* __restrict__ pcol, const float* __restrict__ P,
                                                     const float* __restrict__ xc, float* __restrict__ x,
                                                     const Scalars* __restrict__ S) {
-	if (S->done) {
+	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	ProlongHead<BSF> const h = prolong_head<BSF>(min(i, nb - 1), pptr, x);
+	if (done_after(S, h.e0, h.e1, h.acc[0])) {
 		return;
 	}
-	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
 	if (i < nb) {
-		prolong_row<BSF>(i, pptr, pcol, P, xc, x);
+		prolong_row<BSF>(i, h, pcol, P, xc, x);
 	}
 }
 
@@ -1144,9 +1178,9 @@ __device__ __forceinline__ void wg_cheb(const AmgLevelDesc& L, bool first, int s
 	for (int i = threadIdx.x; i < L.nb; i += kWg) {
 		float xo[3];
 		if (first) {
-			cheb_row<3, float, true>(i, L.slice_ptr, L.col, L.val32, L.b, L.dinv, xin, xout, L.d, c1, c2, step, xo);
+			cheb_row<3, true>(i, row_head<3>(i, L.slice_ptr, L.b), L.col, L.val32, L.dinv, xin, xout, L.d, c1, c2, step, xo);
 		} else {
-			cheb_row<3, float, false>(i, L.slice_ptr, L.col, L.val32, L.b, L.dinv, xin, xout, L.d, c1, c2, step, xo);
+			cheb_row<3, false>(i, row_head<3>(i, L.slice_ptr, L.b), L.col, L.val32, L.dinv, xin, xout, L.d, c1, c2, step, xo);
 		}
 	}
 	__syncthreads();
@@ -1169,13 +1203,15 @@ __global__ __launch_bounds__(kWg) void k_coarse_cycle(const AmgLevelDesc* __rest
 		cur[l] = c;
 		float* const x = c ? L.x1 : L.x0;
 		for (int i = threadIdx.x; i < L.nb; i += kWg) {
-			resid_row<3, float>(i, L.slice_ptr, L.col, L.val32, L.b, x, L.r);
+			resid_row<3>(i, row_head<3>(i, L.slice_ptr, L.b), L.col, L.val32, x, L.r);
 		}
 		__syncthreads();
 		AmgLevelDesc const C = lv[l + 1];
 		for (int64_t base = 0; base < C.nb; base += kWg / kRestrictLanes) {
 			int64_t const J = base + threadIdx.x / kRestrictLanes;
-			restrict_row<3>(J, J < C.nb, threadIdx.x % kRestrictLanes, L.rptr, L.rent, L.rrow, L.P, L.r, C.b);
+			bool const live = J < C.nb;
+			restrict_row<3>(J, live, threadIdx.x % kRestrictLanes, live ? L.rptr[J] : 0, live ? L.rptr[J + 1] : 0,
+			                L.rent, L.rrow, L.P, L.r, C.b);
 		}
 		__syncthreads();
 	}
@@ -1207,7 +1243,7 @@ __global__ __launch_bounds__(kWg) void k_coarse_cycle(const AmgLevelDesc* __rest
 		int c = cur[l];
 		float* const x = c ? L.x1 : L.x0;
 		for (int i = threadIdx.x; i < L.nb; i += kWg) {
-			prolong_row<3>(i, L.pptr, L.pcol, L.P, xc, x);
+			prolong_row<3>(i, prolong_head<3>(i, L.pptr, x), L.pcol, L.P, xc, x);
 		}
 		__syncthreads();
 		for (int k = 0; k < L.deg; k++) {

@@ -501,4 +501,25 @@ __device__ __forceinline__ int xcd_remap(int wg, int nwg) {
 	return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
 }
 
+// The done flag, read so that the loads the caller issued before this call
+// are in flight together with it, one memory round trip instead of two at the
+// start of a latency-bound kernel. The flag is read by a vector load (its
+// index is a zero the compiler cannot see through): a scalar load would be
+// waited for together with the kernel arguments that those loads' addresses
+// need. The empty asm consumes the flag and the values, so the compiler can
+// neither sink the loads below the caller's early return nor issue the
+// flag's load after waiting for them.
+template <typename T>
+__device__ __forceinline__ void issued_with(int done, T const& v) {
+	asm volatile("" ::"v"(done), "v"(v));
+}
+template <typename... T>
+__device__ __forceinline__ int done_after(const Scalars* __restrict__ S, T const&... v) {
+	int zero;
+	asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+	int const done = (&S->done)[zero];
+	(issued_with(done, v), ...);
+	return done;
+}
+
 }  // namespace bfmx

@@ -138,13 +138,14 @@ constexpr int32_t kHold = 4;   // Scalars::done while a flexible-CG chunk waits 
 // Sums partial[k * n + i], i < n, for k < NV in a fixed order; the totals are
 // valid in thread 0.
 template <int NV>
-__device__ __forceinline__ void reduce_partials(const double* __restrict__ partial, int n, double (&tot)[NV]) {
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ partial, int n, double (&tot)[NV],
+                                                const double (&first)[NV]) {
 	__shared__ double lds[NV][kRed / 64];
 	double acc[NV];
 #pragma unroll
 	for (int k = 0; k < NV; k++) {
-		acc[k] = 0;
-		for (int i = threadIdx.x; i < n; i += kRed) {
+		acc[k] = first[k];
+		for (int i = threadIdx.x + kRed; i < n; i += kRed) {
 			acc[k] += partial[k * n + i];
 		}
 		acc[k] = wave_sum(acc[k]);
@@ -168,6 +169,41 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ parti
 			tot[k] = s;
 		}
 	}
+}
+
+// This thread's first partial sums (index threadIdx.x; 0 past the end).
+template <int NV>
+__device__ __forceinline__ void first_partials(const double* __restrict__ partial, int n, double (&first)[NV]) {
+#pragma unroll
+	for (int k = 0; k < NV; k++) {
+		first[k] = static_cast<int>(threadIdx.x) < n ? partial[k * n + threadIdx.x] : 0.0;
+	}
+}
+
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ partial, int n, double (&tot)[NV]) {
+	double first[NV];
+	first_partials<NV>(partial, n, first);
+	reduce_partials<NV>(partial, n, tot, first);
+}
+
+// reduce_partials of a run that is not done, with this thread's first loads
+// in flight together with the done flag (done_after); false when it is done.
+template <int NV>
+__device__ __forceinline__ bool reduce_live(const double* __restrict__ partial, int n, double (&tot)[NV],
+                                            const Scalars* __restrict__ S) {
+	double first[NV];
+	first_partials<NV>(partial, n, first);
+	int const done = done_after(S, first[0]);
+#pragma unroll
+	for (int k = 1; k < NV; k++) {
+		issued_with(done, first[k]);
+	}
+	if (done) {
+		return false;
+	}
+	reduce_partials<NV>(partial, n, tot, first);
+	return true;
 }
 
 // z = M^-1 r for one node.
@@ -237,11 +273,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_spmv(int nb, const int32_t* __res
 // alpha = (r . z) / (p . q), or a breakdown.
 __global__ __launch_bounds__(kRed) void k_red_alpha(int nwg, const double* __restrict__ partial,
                                                     Scalars* __restrict__ S) {
-	if (S->done) {
+	double tot[1];
+	if (!reduce_live<1>(partial, nwg, tot, S)) {
 		return;
 	}
-	double tot[1];
-	reduce_partials<1>(partial, nwg, tot);
 	if (threadIdx.x == 0) {
 		double const pq = tot[0];
 		S->pq = pq;
@@ -282,11 +317,10 @@ __global__ __launch_bounds__(kBlock) void k_cg_update(int nb, const double* __re
 // beta = r'.z' / r.z, iteration count.
 __global__ __launch_bounds__(kRed) void k_red_beta(int nwg, const double* __restrict__ partial,
                                                    Scalars* __restrict__ S) {
-	if (S->done) {
+	double tot[2];
+	if (!reduce_live<2>(partial, nwg, tot, S)) {
 		return;
 	}
-	double tot[2];
-	reduce_partials<2>(partial, nwg, tot);
 	if (threadIdx.x == 0) {
 		S->beta = tot[0] / S->rz;
 		S->rz_prev = S->rz;
@@ -301,11 +335,10 @@ __global__ __launch_bounds__(kRed) void k_red_beta(int nwg, const double* __rest
 // beta = -(z . q_prev) / (p_prev . q_prev), 0 before the first direction.
 __global__ __launch_bounds__(kRed) void k_red_fcg_beta(int nwg, const double* __restrict__ partial,
                                                        Scalars* __restrict__ S) {
-	if (S->done) {
+	double tot[1];
+	if (!reduce_live<1>(partial, nwg, tot, S)) {
 		return;
 	}
-	double tot[1];
-	reduce_partials<1>(partial, nwg, tot);
 	if (threadIdx.x == 0) {
 		S->beta = S->pq > 0 ? -tot[0] / S->pq : 0.0;
 	}
@@ -404,11 +437,10 @@ __global__ __launch_bounds__(64) void k_seg_red_fcg(const int64_t* __restrict__ 
 // alpha = (p . r) / (p . q), or a breakdown.
 __global__ __launch_bounds__(kRed) void k_red_fcg_alpha(int nwg, const double* __restrict__ partial,
                                                         Scalars* __restrict__ S) {
-	if (S->done) {
+	double tot[2];
+	if (!reduce_live<2>(partial, nwg, tot, S)) {
 		return;
 	}
-	double tot[2];
-	reduce_partials<2>(partial, nwg, tot);
 	if (threadIdx.x == 0) {
 		double const pq = tot[0];
 		S->pq = pq;
@@ -450,11 +482,10 @@ __global__ __launch_bounds__(kBlock) void k_fcg_update(int nb, const double* __r
 // decides (converged, or running on with a check at every chunk end).
 __global__ __launch_bounds__(kRed) void k_red_fcg_rr(int nwg, const double* __restrict__ partial,
                                                      Scalars* __restrict__ S) {
-	if (S->done) {
+	double tot[1];
+	if (!reduce_live<1>(partial, nwg, tot, S)) {
 		return;
 	}
-	double tot[1];
-	reduce_partials<1>(partial, nwg, tot);
 	if (threadIdx.x == 0) {
 		S->rr = tot[0];
 		S->iter += 1;
