@@ -498,7 +498,10 @@ __device__ __forceinline__ int xcd_remap(int wg, int nwg) {
 	int const rem = nwg % kXcd;
 	int const xcd = wg % kXcd;
 	int const local = wg / kXcd;
-	return xcd < rem ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+	// XCD x takes per + 1 workgroups when x < rem, per otherwise; its range
+	// starts at x per + min(x, rem) (no branch, so the kernel's argument loads
+	// are issued together).
+	return xcd * per + min(xcd, rem) + local;
 }
 
 // The done flag, read so that the loads the caller issued before this call
