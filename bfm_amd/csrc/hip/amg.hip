@@ -1171,6 +1171,10 @@ __global__ __launch_bounds__(kBlock) void k_dot_partials(int n, const float* __r
 // level; its correction ends in x1 (in x0 when it is a dense coarsest).
 constexpr int kWg = 1024;
 constexpr int kWgRows = 256;
+constexpr int kPfLine = 128;                 // bytes per prefetched line
+constexpr int kPfPerThread = 4;
+constexpr size_t kPfMax = kWg * kPfPerThread;   // lines prefetched at most
+static_assert(kPfPerThread == 4, "k_coarse_cycle passes its four line addresses to done_after");
 
 __device__ __forceinline__ void wg_cheb(const AmgLevelDesc& L, bool first, int step, const float* xin, float* xout) {
 	float const c1 = static_cast<float>(L.scal[1 + 2 * step]);
@@ -1186,10 +1190,28 @@ __device__ __forceinline__ void wg_cheb(const AmgLevelDesc& L, bool first, int s
 	__syncthreads();
 }
 
+// pf: addresses of the npf (at most kPfMax) lines of the arrays the cycle
+// reads. They are all loaded at once at its start, so that its passes find
+// them in this XCD's L2: between two cycles the finer levels' streams evict
+// them, and each pass would otherwise wait on HBM for every step of its chain
+// of dependent loads.
 __global__ __launch_bounds__(kWg) void k_coarse_cycle(const AmgLevelDesc* __restrict__ lv, int nl,
+                                                      const uint64_t* __restrict__ pf, int npf,
                                                       const Scalars* __restrict__ S) {
-	if (S->done) {
+	uint64_t line[kPfPerThread];
+#pragma unroll
+	for (int j = 0; j < kPfPerThread; j++) {
+		int const k = static_cast<int>(threadIdx.x) + j * kWg;
+		line[j] = k < npf ? pf[k] : 0;
+	}
+	if (done_after(S, line[0], line[1], line[2], line[3])) {
 		return;
+	}
+	uint32_t touched[kPfPerThread];
+#pragma unroll
+	for (int j = 0; j < kPfPerThread; j++) {
+		// a global (not flat) load: the scalar loads' waits do not cover it
+		touched[j] = line[j] ? *reinterpret_cast<const __attribute__((address_space(1))) uint32_t*>(line[j]) : 0;
 	}
 	int cur[kMaxLevels];
 	for (int l = 0; l + 1 < nl; l++) {
@@ -1251,6 +1273,11 @@ __global__ __launch_bounds__(kWg) void k_coarse_cycle(const AmgLevelDesc* __rest
 			c ^= 1;
 		}
 		cur[l] = c;
+	}
+	// the prefetched values themselves are not needed: this keeps their loads
+#pragma unroll
+	for (int j = 0; j < kPfPerThread; j++) {
+		asm volatile("" ::"v"(touched[j]));
 	}
 }
 
@@ -1511,6 +1538,8 @@ void amg_release(bfmx_system* s) {
 	}
 	s->amg.clear();
 	dfree(s->d_amg_desc);
+	dfree(s->d_amg_pf);
+	s->amg_npf = 0;
 	s->amg_wg = -1;
 }
 
@@ -1936,6 +1965,7 @@ int amg_build(bfmx_system* s) {
 					off += n * n;
 				}
 				BFMX_CHECK(dalloc(&Lc.d_dense, static_cast<size_t>(entries)));
+				Lc.n_dense = entries;
 				BFMX_CHECK(upload(&Lc.d_seg_ptr, fseg, st));
 				BFMX_CHECK(upload(&Lc.d_seg_row, seg_row, st));
 				BFMX_CHECK(upload(&Lc.d_seg_off, seg_off, st));
@@ -1992,6 +2022,42 @@ int amg_build(bfmx_system* s) {
 				desc.push_back(D);
 			}
 			BFMX_CHECK(upload(&s->d_amg_desc, desc, st));
+			// the lines of the arrays the cycle reads before writing (its
+			// scratch vectors aside), prefetched at its start
+			std::vector<uint64_t> pf;
+			auto lines = [&](const void* p, size_t bytes) {
+				for (size_t off = 0; off < bytes && pf.size() < kPfMax; off += kPfLine) {
+					pf.push_back(reinterpret_cast<uint64_t>(p) + off);
+				}
+			};
+			for (size_t l = s->amg_wg; l < lv.size(); l++) {
+				AmgLevel const& L = lv[l];
+				size_t const f4 = sizeof(float), i4 = sizeof(int32_t), nb = L.nb, nnzp = L.nnzp;
+				size_t const pad = L.nnzb_pad;
+				lines(L.d_slice_ptr, (L.n_slices + 1) * i4);
+				lines(L.d_col, pad * i4);
+				lines(L.d_val32, 9 * pad * f4);
+				lines(L.d_dinv, 9 * nb * f4);
+				lines(L.d_scal, kScalSize * sizeof(double));
+				if (static_cast<int>(l) == s->amg_wg) {
+					lines(L.d_b, 3 * nb * f4);
+				}
+				if (l + 1 < lv.size()) {
+					lines(L.d_pptr, (nb + 1) * i4);
+					lines(L.d_pcol, nnzp * i4);
+					lines(L.d_P, 9 * nnzp * f4);
+					lines(L.d_rptr, (static_cast<size_t>(lv[l + 1].nb) + 1) * i4);
+					lines(L.d_rent, nnzp * i4);
+					lines(L.d_rrow, nnzp * i4);
+				} else if (L.d_dense) {
+					lines(L.d_dense, static_cast<size_t>(L.n_dense) * sizeof(double));
+					lines(L.d_seg_ptr, (static_cast<size_t>(L.n_seg) + 1) * i4);
+					lines(L.d_seg_row, nb * i4);
+					lines(L.d_seg_off, static_cast<size_t>(L.n_seg) * sizeof(int64_t));
+				}
+			}
+			BFMX_CHECK(upload(&s->d_amg_pf, pf, st));
+			s->amg_npf = static_cast<int>(pf.size());
 			BFMX_HIP(hipStreamSynchronize(st));
 		}
 		return BFMX_OK;
@@ -2124,7 +2190,7 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 		// b is this level's own d_b (restricted into it); the workgroup cycle
 		// runs every level from here down
 		int const nl = static_cast<int>(s->amg.size() - l);
-		k_coarse_cycle<<<1, kWg, 0, st>>>(s->d_amg_desc, nl, S);
+		k_coarse_cycle<<<1, kWg, 0, st>>>(s->d_amg_desc, nl, s->d_amg_pf, s->amg_npf, S);
 		return nl == 1 && L.d_dense ? L.d_x[0] : L.d_x[1];
 	}
 	if (coarsest && L.d_dense) {

@@ -167,6 +167,7 @@ struct AmgLevel {
 	// inverses one after another.
 	int n_seg = 1;
 	double* d_dense = nullptr;
+	int64_t n_dense = 0;              // its entries
 	int32_t* d_seg_ptr = nullptr;     // n_seg + 1 row offsets
 	int32_t* d_seg_row = nullptr;     // segment of each row
 	int64_t* d_seg_off = nullptr;     // first entry of each segment's inverse
@@ -303,6 +304,8 @@ struct bfmx_system {
 	bfmx::AmgParams amg_par;           // its parameters
 	int amg_wg = -1;                   // first level run by the one-workgroup coarse cycle (-1: none)
 	bfmx::AmgLevelDesc* d_amg_desc = nullptr;   // descriptors of the levels from amg_wg on
+	uint64_t* d_amg_pf = nullptr;      // addresses of the 128-byte lines those levels' arrays span
+	int amg_npf = 0;
 	// convergence segments: node ranges d_conv_seg[k] .. d_conv_seg[k+1] - 1
 	// (borrowed) that must each meet the tolerance on their own (the union
 	// system of a batch); d_conv_part holds their r.r and b.b at a check
