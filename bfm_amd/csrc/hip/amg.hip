@@ -12,8 +12,8 @@
 //     aggregate of its first aggregated neighbour;
 //   * the prolongator pattern: row i couples to the aggregates of its
 //     neighbours (P = (I - w D^-1 A) P_t has the sparsity of A P_t), stored
-//     as CSR of bs x 3 blocks, with its transpose as per-aggregate lists for
-//     the restriction;
+//     as CSR of bs x 3 blocks, with its transpose's pattern for the
+//     restriction (P's entries column by column, with their rows);
 //   * the coarse block pattern (that of P^T A P) in sliced ELL, and for every
 //     coarse block slot on or above the diagonal the list of its
 //     contributions (fine block A_ij, entry P_iI, entry P_jJ), so the
@@ -33,7 +33,9 @@
 //     kDropTol of its norm (a mode that the constrained DOFs make dependent) is
 //     dropped: its coarse DOF is decoupled;
 //   * the smoothed prolongator P = (I - w D^-1 A) P_t, w = 4 / (3 lambda),
-//     stored FP32 (Vanek, Mandel and Brezina, Computing 56 (1996) 179-196);
+//     stored FP32 (Vanek, Mandel and Brezina, Computing 56 (1996) 179-196),
+//     also in column order (R = P^T row by row), which the restriction reads
+//     contiguously;
 //   * the Galerkin operator P^T A P in FP64 with an FP32 copy, and the dense
 //     Gauss-Jordan inverse of every segment of the coarsest level.
 // BFMX_AMG_SMOOTH=0 in the environment (read when a hierarchy is built)
@@ -412,6 +414,28 @@ __global__ __launch_bounds__(kBlock) void k_smooth_p(int nb, const int32_t* __re
 			}
 		}
 		store(k, acc);
+	}
+}
+
+// R[t] = P[rent[t]]: P's entries in column order, so that the restriction
+// reads a coarse row's entries contiguously (written here, contiguously,
+// rather than by k_smooth_p, whose stores in that order would scatter).
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_transpose_p(int64_t nnzp, const int32_t* __restrict__ rent,
+                                                        const float* __restrict__ P, float* __restrict__ R) {
+	int64_t const t = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+	if (t >= nnzp) {
+		return;
+	}
+	const float* src = P + BS * 3 * static_cast<int64_t>(rent[t]);
+	float v[BS * 3];
+#pragma unroll
+	for (int k = 0; k < BS * 3; k++) {
+		v[k] = src[k];
+	}
+#pragma unroll
+	for (int k = 0; k < BS * 3; k++) {
+		R[BS * 3 * t + k] = v[k];
 	}
 }
 
@@ -975,8 +999,9 @@ __global__ __launch_bounds__(kBlock) void k_level_resid(int nb, const int32_t* _
 }
 
 // b_c[J] = sum over the entries of column J of P (row i, entry e) of
-// P_e^T r_i. kRestrictLanes lanes share a coarse row: lane l takes entries
-// l, l + kRestrictLanes, ..., combined by a fixed butterfly.
+// P_e^T r_i, the entries read from R (P's entries column by column, so a
+// coarse row's are contiguous). kRestrictLanes lanes share a coarse row: lane
+// l takes entries l, l + kRestrictLanes, ..., combined by a fixed butterfly.
 constexpr int kRestrictLanes = 8;
 
 // Coarse row J (live) by the lane group of `lane`; every lane of the group
@@ -984,13 +1009,12 @@ constexpr int kRestrictLanes = 8;
 template <int BSF>
 // [t0, t1) is the row's range of rptr (empty when not live).
 __device__ __forceinline__ void restrict_row(int64_t J, bool live, int lane, int32_t t0, int32_t t1,
-                                             const int32_t* __restrict__ rent, const int32_t* __restrict__ rrow,
-                                             const float* __restrict__ P, const float* __restrict__ r,
-                                             float* __restrict__ bc) {
+                                             const int32_t* __restrict__ rrow, const float* __restrict__ R,
+                                             const float* __restrict__ r, float* __restrict__ bc) {
 	float acc[3] = {0, 0, 0};
 	for (int32_t t = t0 + lane; t < t1; t += kRestrictLanes) {
 		int64_t const i = rrow[t];
-		const float* p = P + BSF * 3 * static_cast<int64_t>(rent[t]);
+		const float* p = R + BSF * 3 * static_cast<int64_t>(t);
 #pragma unroll
 		for (int k = 0; k < BSF; k++) {
 			float const rk = r[BSF * i + k];
@@ -1020,8 +1044,7 @@ __device__ __forceinline__ void restrict_row(int64_t J, bool live, int lane, int
 // saves that step's launch.
 template <int BSF>
 __global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int32_t* __restrict__ rptr,
-                                                     const int32_t* __restrict__ rent,
-                                                     const int32_t* __restrict__ rrow, const float* __restrict__ P,
+                                                     const int32_t* __restrict__ rrow, const float* __restrict__ R,
                                                      const float* __restrict__ r, float* __restrict__ bc,
                                                      const float* __restrict__ dinv, const double* __restrict__ cscal,
                                                      float* __restrict__ x0, float* __restrict__ d,
@@ -1034,7 +1057,7 @@ __global__ __launch_bounds__(kBlock) void k_restrict(int nc, const int32_t* __re
 	if (done_after(S, t0, t1)) {
 		return;
 	}
-	restrict_row<BSF>(J, live, lane, live ? t0 : 0, live ? t1 : 0, rent, rrow, P, r, bc);
+	restrict_row<BSF>(J, live, lane, live ? t0 : 0, live ? t1 : 0, rrow, R, r, bc);
 	if (dinv && J < nc && lane == 0) {
 		float const c2 = static_cast<float>(cscal[2]);
 		float const bj[3] = {bc[3 * J], bc[3 * J + 1], bc[3 * J + 2]};
@@ -1233,7 +1256,7 @@ __global__ __launch_bounds__(kWg) void k_coarse_cycle(const AmgLevelDesc* __rest
 			int64_t const J = base + threadIdx.x / kRestrictLanes;
 			bool const live = J < C.nb;
 			restrict_row<3>(J, live, threadIdx.x % kRestrictLanes, live ? L.rptr[J] : 0, live ? L.rptr[J + 1] : 0,
-			                L.rent, L.rrow, L.P, L.r, C.b);
+			                L.rrow, L.R, L.r, C.b);
 		}
 		__syncthreads();
 	}
@@ -1519,6 +1542,7 @@ void amg_release(bfmx_system* s) {
 		dfree(L.d_ppos);
 		dfree(L.d_rptr);
 		dfree(L.d_rent);
+		dfree(L.d_R);
 		dfree(L.d_rrow);
 		dfree(L.d_gal_ptr);
 		dfree(L.d_gal_mirror);
@@ -1913,6 +1937,7 @@ int amg_build(bfmx_system* s) {
 			BFMX_CHECK(upload(&F.d_ppos, ppos, st));
 			BFMX_CHECK(upload(&F.d_rptr, rptr, st));
 			BFMX_CHECK(upload(&F.d_rent, rent, st));
+			BFMX_CHECK(dalloc(&F.d_R, static_cast<size_t>(F.bs) * 3 * pcol.size()));
 			BFMX_CHECK(upload(&F.d_rrow, rrow, st));
 			BFMX_CHECK(upload(&F.d_gal_ptr, gptr, st));
 			BFMX_CHECK(upload(&F.d_gal_mirror, mirror, st));
@@ -2010,7 +2035,7 @@ int amg_build(bfmx_system* s) {
 				D.d = L.d_d;
 				D.r = L.d_r;
 				D.rptr = L.d_rptr;
-				D.rent = L.d_rent;
+				D.R = L.d_R;
 				D.rrow = L.d_rrow;
 				D.P = L.d_P;
 				D.pptr = L.d_pptr;
@@ -2047,7 +2072,7 @@ int amg_build(bfmx_system* s) {
 					lines(L.d_pcol, nnzp * i4);
 					lines(L.d_P, 9 * nnzp * f4);
 					lines(L.d_rptr, (static_cast<size_t>(lv[l + 1].nb) + 1) * i4);
-					lines(L.d_rent, nnzp * i4);
+					lines(L.d_R, 9 * nnzp * f4);
 					lines(L.d_rrow, nnzp * i4);
 				} else if (L.d_dense) {
 					lines(L.d_dense, static_cast<size_t>(L.n_dense) * sizeof(double));
@@ -2139,6 +2164,7 @@ int amg_setup_numeric(bfmx_system* s) {
 			k_smooth_p<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_val32, L.d_diag_blk,
 			                                    L.d_Pt, L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0,
 			                                    L.d_P);
+			k_transpose_p<2><<<grid(L.nnzp), kBlock, 0, st>>>(L.nnzp, L.d_rent, L.d_P, L.d_R);
 			k_galerkin<2, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
 			                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
 			                                                               L.d_val32, L.d_P, C.d_val64,
@@ -2148,6 +2174,7 @@ int amg_setup_numeric(bfmx_system* s) {
 			k_smooth_p<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_val32, L.d_diag_blk,
 			                                    L.d_Pt, L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0,
 			                                    L.d_P);
+			k_transpose_p<3><<<grid(L.nnzp), kBlock, 0, st>>>(L.nnzp, L.d_rent, L.d_P, L.d_R);
 			k_galerkin<3, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
 			                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
 			                                                               L.d_val32, L.d_P, C.d_val64,
@@ -2229,7 +2256,7 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 	// a dense coarsest)
 	bool const fuse = static_cast<int>(l + 1) != s->amg_wg && !(l + 2 == s->amg.size() && C.d_dense);
 	k_restrict<BS><<<grid(static_cast<int64_t>(C.nb) * kRestrictLanes), kBlock, 0, st>>>(
-	    C.nb, L.d_rptr, L.d_rent, L.d_rrow, L.d_P, L.d_r, C.d_b, fuse ? C.d_dinv : nullptr, C.d_scal, C.d_x[0],
+	    C.nb, L.d_rptr, L.d_rrow, L.d_R, L.d_r, C.d_b, fuse ? C.d_dinv : nullptr, C.d_scal, C.d_x[0],
 	    C.d_d, S);
 	float* xc = vcycle<3, float>(s, l + 1, C.d_b, nullptr, nullptr, S, fuse);
 	k_prolong<BS><<<g, kBlock, 0, st>>>(L.nb, L.d_pptr, L.d_pcol, L.d_P, xc, L.d_x[cur], S);

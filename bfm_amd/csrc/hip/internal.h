@@ -151,9 +151,10 @@ struct AmgLevel {
 	int32_t* d_pcol = nullptr;        // next-level row of each entry
 	float* d_P = nullptr;             // bs x 3 per entry
 	uint8_t* d_ppos = nullptr;        // per block slot: entry (within its row) of its column's aggregate
-	int32_t* d_rptr = nullptr;        // per next-level row: the entries of its column of P
-	int32_t* d_rent = nullptr;        //   entry index
-	int32_t* d_rrow = nullptr;        //   this level's row
+	int32_t* d_rptr = nullptr;        // per next-level row J: range of P's column J in the order below
+	int32_t* d_rent = nullptr;        //   P's entries column by column (entry index)
+	int32_t* d_rrow = nullptr;        //   and their rows on this level
+	float* d_R = nullptr;             // those entries' values in that order (R = P^T read row by row)
 	int64_t* d_gal_ptr = nullptr;     // per next-level block slot: range of contributions (slots (I, J >= I))
 	int32_t* d_gal_mirror = nullptr;  // per next-level block slot: the slot of its transpose (-2: filled by
 	                                  // it, -1: padding)
@@ -197,7 +198,7 @@ struct AmgLevelDesc {
 	float* d;
 	float* r;
 	const int32_t* rptr;   // transfer to the next level (absent on the coarsest)
-	const int32_t* rent;
+	const float* R;
 	const int32_t* rrow;
 	const float* P;
 	const int32_t* pptr;
