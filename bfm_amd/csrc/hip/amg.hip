@@ -998,6 +998,73 @@ __global__ __launch_bounds__(kBlock) void k_level_resid(int nb, const int32_t* _
 	}
 }
 
+// The first Chebyshev step and the residual after it in one pass, for a
+// level smoothed by one step per side: x0 = c2 D^-1 b is row-local, so each
+// neighbour's x0 is formed here from its b and inverse diagonal block instead
+// of being written by one launch and read back by the next; each row writes
+// its own x0 (d is not written: with one step per side no later step reads
+// it), and r = b - A x0. The same FP32 operations as the first step of k_cheb.
+template <int BS, typename BT>
+__global__ __launch_bounds__(kBlock) void k_first_resid(int nb, const int32_t* __restrict__ slice_ptr,
+                                                        const int32_t* __restrict__ col,
+                                                        const float* __restrict__ val, const BT* __restrict__ b,
+                                                        const float* __restrict__ dinv,
+                                                        const double* __restrict__ scal, float* __restrict__ x0,
+                                                        float* __restrict__ rv, const Scalars* __restrict__ S) {
+	int const i = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+	RowHead<BS> h = row_head<BS>(min(i, nb - 1), slice_ptr, b);
+	if (done_after(S, h.beg, h.end, h.res[0])) {
+		return;
+	}
+	if (i >= nb) {
+		return;
+	}
+	float const c2 = static_cast<float>(scal[2]);
+	auto first_x = [&](int64_t j, float (&xv)[BS]) {
+		float bj[BS];
+#pragma unroll
+		for (int k = 0; k < BS; k++) {
+			bj[k] = static_cast<float>(b[BS * j + k]);
+		}
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+			float z = 0;
+#pragma unroll
+			for (int k = 0; k < BS; k++) {
+				z += dinv[BS * BS * j + BS * a + k] * bj[k];
+			}
+			xv[a] = c2 * z;
+		}
+	};
+	float xi[BS];
+	first_x(i, xi);
+#pragma unroll
+	for (int a = 0; a < BS; a++) {
+		x0[BS * static_cast<int64_t>(i) + a] = xi[a];
+	}
+	int const r = i & 63;
+	float (&res)[BS] = h.res;
+#pragma unroll kRowUnroll<BS>
+	for (int32_t blk = h.beg + r; blk < h.end; blk += kSlice) {
+		int32_t const c = col[blk];
+		float A[BS][BS];
+		ld_block<BS>(val, blk, r, A);
+		float xv[BS];
+		first_x(c, xv);
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+#pragma unroll
+			for (int k = 0; k < BS; k++) {
+				res[a] -= A[a][k] * xv[k];
+			}
+		}
+	}
+#pragma unroll
+	for (int a = 0; a < BS; a++) {
+		rv[BS * static_cast<int64_t>(i) + a] = res[a];
+	}
+}
+
 // b_c[J] = sum over the entries of column J of P (row i, entry e) of
 // P_e^T r_i, the entries read from R (P's entries column by column, so a
 // coarse row's are contiguous). kRestrictLanes lanes share a coarse row: lane
@@ -2232,7 +2299,9 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 	}
 	int const deg = coarsest ? kCoarseDeg : level_deg(s->amg_par, l);
 	int cur = 0;
-	if (!first_done) {
+	// one step per side: the first step goes with the residual (k_first_resid)
+	bool const first_resid = !coarsest && !first_done && deg == 1;
+	if (!first_done && !first_resid) {
 		cheb<BS, BT>(st, L, b, true, 0, nullptr, L.d_x[0], nullptr, nullptr, S);
 	}
 	if (coarsest) {
@@ -2250,7 +2319,13 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 	}
 	AmgLevel& C = s->amg[l + 1];
 	int const g = grid(L.nb);
-	k_level_resid<BS, BT><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_x[cur], L.d_r, S);
+	if (first_resid) {
+		k_first_resid<BS, BT><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_dinv, L.d_scal,
+		                                            L.d_x[0], L.d_r, S);
+	} else {
+		k_level_resid<BS, BT><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, b, L.d_x[cur], L.d_r,
+		                                            S);
+	}
 	// the restriction also takes the next level's first Chebyshev step when
 	// that level starts with one on the grid (not the one-workgroup cycle, not
 	// a dense coarsest)
