@@ -1069,7 +1069,12 @@ __global__ __launch_bounds__(kBlock) void k_first_resid(int nb, const int32_t* _
 // P_e^T r_i, the entries read from R (P's entries column by column, so a
 // coarse row's are contiguous). kRestrictLanes lanes share a coarse row: lane
 // l takes entries l, l + kRestrictLanes, ..., combined by a fixed butterfly.
-constexpr int kRestrictLanes = 8;
+// 16 measured best on the 2M beam's solve (10.62 ms; 8 lanes 10.66, 4 lanes
+// 10.96).
+#ifndef BFMX_RESTRICT_LANES
+#define BFMX_RESTRICT_LANES 16
+#endif
+constexpr int kRestrictLanes = BFMX_RESTRICT_LANES;
 
 // Coarse row J (live) by the lane group of `lane`; every lane of the group
 // must call it (the butterfly).
