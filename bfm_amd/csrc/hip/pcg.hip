@@ -907,11 +907,20 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 	double mark = last.rr;
 	int64_t mark_iter = 0;
 
-	// Replay chunks, keeping one in flight while the one before it is inspected.
-	int64_t launched = 0, chunks = 0;
+	// Replay chunks, keeping one in flight while the one before it is
+	// inspected, except when the chunk in flight is expected to end the run
+	// (r.r at the last inspection times its fall over the chunk before, at or
+	// below the tolerance): the next chunk then waits for that one's result
+	// instead of running after the end, where each of its launches returns
+	// at once but still costs a dispatch. A wrong guess costs one wait on the
+	// host with the GPU idle.
+	int64_t launched = 0, chunks = 0, seen = -1;
+	double rr_before = -1;   // r.r at the inspection before `last` (when consecutive)
 	while (last.done == 0) {
 		bool const more = launched < cap;
-		if (more) {
+		bool const ending = seen < chunks - 1 && rr_before > 0 && last.rr * (last.rr / rr_before) <= last.tol2;
+		bool const launch = more && !ending;
+		if (launch) {
 			if (use_graph) {
 				BFMX_HIP(hipGraphLaunch(exec, st));
 			} else {
@@ -924,9 +933,11 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 			BFMX_HIP(hipEventRecord(s->ev[slot], st));
 			chunks++;
 		}
-		int64_t const look = (more && chunks >= 2) ? chunks - 2 : chunks - 1;
+		int64_t const look = (launch && chunks >= 2) ? chunks - 2 : chunks - 1;
 		BFMX_HIP(hipEventSynchronize(s->ev[look & 3]));
+		rr_before = look == seen + 1 ? last.rr : -1;
 		last = s->h_scal[look & 3];
+		seen = look;
 		if (!more && look == chunks - 1) {
 			break;
 		}
