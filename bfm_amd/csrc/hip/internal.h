@@ -229,6 +229,8 @@ struct Graph {
 	int precision = -1;
 	double rtol2 = -1;
 	const void* model = nullptr;
+	int iters = 0;    // iterations in the chunk
+	int parity = 0;   // flexible CG: the direction buffer its first iteration reads
 };
 
 }  // namespace bfmx

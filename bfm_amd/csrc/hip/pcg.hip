@@ -818,11 +818,13 @@ int chunk_iters(int pre) {
 
 // One chunk of iterations and the conditional residual recomputation. The
 // multigrid path recomputes the residual only for the tolerance checks (its
-// recursive residual is FP64 throughout).
-void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
+// recursive residual is FP64 throughout). Flexible CG runs n iterations
+// (1 .. kFcgChunk), the first reading direction buffer `parity`; the other
+// solvers run kChunk.
+void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n, int parity) {
 	if (is_fcg(pre)) {
-		for (int it = 0; it < kFcgChunk; it++) {
-			launch_fcg_iteration(s, pre, it);
+		for (int it = 0; it < n; it++) {
+			launch_fcg_iteration(s, pre, parity + it);
 		}
 		launch_residual(s, pre == BFMX_PRECOND_LEARNED ? BFMX_PRECOND_BLOCK_JACOBI : BFMX_PRECOND_JACOBI, 1, rtol2,
 		                0.0);
@@ -836,18 +838,19 @@ void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2) {
 
 
 // The captured chunk for this configuration (captured on first use).
-int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, hipGraphExec_t* out) {
+int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n, int parity, hipGraphExec_t* out) {
 	int const prec = mixed ? 1 : 0;
 	const void* model = pre == BFMX_PRECOND_LEARNED ? static_cast<const void*>(s->model) : nullptr;
 	for (auto const& gr : s->graphs) {
-		if (gr.precond == pre && gr.precision == prec && gr.rtol2 == rtol2 && gr.model == model) {
+		if (gr.precond == pre && gr.precision == prec && gr.rtol2 == rtol2 && gr.model == model && gr.iters == n &&
+		    gr.parity == parity) {
 			*out = gr.exec;
 			return BFMX_OK;
 		}
 	}
 	hipGraph_t g = nullptr;
 	BFMX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-	launch_chunk(s, pre, mixed, rtol2);
+	launch_chunk(s, pre, mixed, rtol2, n, parity);
 	BFMX_HIP(hipStreamEndCapture(s->stream, &g));
 	Graph gr;
 	hipError_t const e = hipGraphInstantiate(&gr.exec, g, nullptr, nullptr, 0);
@@ -860,6 +863,8 @@ int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, hipGraphExe
 	gr.precision = prec;
 	gr.rtol2 = rtol2;
 	gr.model = model;
+	gr.iters = n;
+	gr.parity = parity;
 	s->graphs.push_back(gr);
 	*out = gr.exec;
 	return BFMX_OK;
@@ -881,10 +886,6 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 	int64_t const n = s->n_nodes;
 	hipStream_t const st = s->stream;
 	bool const use_graph = graphs_enabled();
-	hipGraphExec_t exec = nullptr;
-	if (use_graph) {
-		BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2, &exec));
-	}
 	// the residual kernel's z (unused by flexible CG); the learned decoder reads
 	// the 2x2 block inverses
 	int const pre_res = precond == BFMX_PRECOND_AMG      ? BFMX_PRECOND_JACOBI
@@ -908,26 +909,46 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 	int64_t mark_iter = 0;
 
 	// Replay chunks, keeping one in flight while the one before it is
-	// inspected, except when the chunk in flight is expected to end the run
-	// (r.r at the last inspection times its fall over the chunk before, at or
-	// below the tolerance): the next chunk then waits for that one's result
-	// instead of running after the end, where each of its launches returns
-	// at once but still costs a dispatch. A wrong guess costs one wait on the
-	// host with the GPU idle.
+	// inspected. Launches after the end return at once but still cost a
+	// dispatch each, so the fall of r.r per iteration over the chunk inspected
+	// last (q) sizes the end: no chunk is launched ahead when the one in
+	// flight is expected to reach the tolerance (it is waited for instead; a
+	// wrong guess leaves the GPU idle for one wait on the host), and a
+	// flexible-CG chunk runs only the iterations expected to be needed (at
+	// least one, at most kFcgChunk).
 	int64_t launched = 0, chunks = 0, seen = -1;
-	double rr_before = -1;   // r.r at the inspection before `last` (when consecutive)
+	int its[4] = {};          // iterations of the chunks in the ring
+	double rr_before = -1;    // r.r and iterations at the inspection before `last` (when consecutive)
+	int64_t iter_before = 0;
+	int parity = 0;           // flexible CG: the direction buffer the next iteration reads
 	while (last.done == 0) {
 		bool const more = launched < cap;
-		bool const ending = seen < chunks - 1 && rr_before > 0 && last.rr * (last.rr / rr_before) <= last.tol2;
+		bool const in_flight = seen < chunks - 1;
+		double q = -1;
+		if (rr_before > 0 && last.iter > iter_before && last.rr > 0) {
+			q = std::pow(last.rr / rr_before, 1.0 / static_cast<double>(last.iter - iter_before));
+		}
+		double const ahead = in_flight && q > 0 ? last.rr * std::pow(q, its[(chunks - 1) & 3]) : last.rr;
+		bool const ending = in_flight && q > 0 && ahead <= last.tol2;
 		bool const launch = more && !ending;
 		if (launch) {
+			int n = chunk_iters(precond);
+			if (is_fcg(precond) && q > 0 && q < 1 && ahead > last.tol2) {
+				double const need = std::ceil(std::log(last.tol2 / ahead) / std::log(q));
+				n = static_cast<int>(std::min<double>(std::max(need, 1.0), kFcgChunk));
+			}
+			int const par = is_fcg(precond) ? parity : 0;
 			if (use_graph) {
+				hipGraphExec_t exec = nullptr;
+				BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2, n, par, &exec));
 				BFMX_HIP(hipGraphLaunch(exec, st));
 			} else {
-				launch_chunk(s, precond, mixed, rtol2);
+				launch_chunk(s, precond, mixed, rtol2, n, par);
 				BFMX_HIP(hipGetLastError());
 			}
-			launched += chunk_iters(precond);
+			launched += n;
+			parity = (parity + n) & 1;
+			its[chunks & 3] = n;
 			int const slot = static_cast<int>(chunks & 3);
 			BFMX_HIP(hipMemcpyAsync(&s->h_scal[slot], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
 			BFMX_HIP(hipEventRecord(s->ev[slot], st));
@@ -936,6 +957,7 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 		int64_t const look = (launch && chunks >= 2) ? chunks - 2 : chunks - 1;
 		BFMX_HIP(hipEventSynchronize(s->ev[look & 3]));
 		rr_before = look == seen + 1 ? last.rr : -1;
+		iter_before = last.iter;
 		last = s->h_scal[look & 3];
 		seen = look;
 		if (!more && look == chunks - 1) {
