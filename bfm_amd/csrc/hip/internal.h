@@ -230,7 +230,6 @@ struct Graph {
 	double rtol2 = -1;
 	const void* model = nullptr;
 	int iters = 0;    // iterations in the chunk
-	int parity = 0;   // flexible CG: the direction buffer its first iteration reads
 };
 
 }  // namespace bfmx

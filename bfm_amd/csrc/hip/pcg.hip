@@ -353,13 +353,18 @@ template <typename ZT, bool SEG = false>
 __global__ __launch_bounds__(kBlock) void k_fcg_spmv(int nb, const int32_t* __restrict__ slice_ptr,
                                                      const int32_t* __restrict__ col,
                                                      const double* __restrict__ val, const ZT* __restrict__ z,
-                                                     const double* __restrict__ pold, double* __restrict__ pnew,
+                                                     double* __restrict__ p0, double* __restrict__ p1,
                                                      double* __restrict__ q, const double* __restrict__ r,
                                                      double* __restrict__ partial, const Scalars* __restrict__ S,
                                                      SegView sv = SegView{}, const double* __restrict__ seg_scal = nullptr) {
 	if (S->done) {
 		return;
 	}
+	// the two direction buffers alternate with the count of iterations run
+	// (the iterations a hold skips do not switch them)
+	bool const odd = S->iter & 1;
+	const double* __restrict__ pold = odd ? p1 : p0;
+	double* __restrict__ pnew = odd ? p0 : p1;
 	int const blk = xcd_remap(blockIdx.x, gridDim.x);
 	bool const second = SEG && static_cast<int>(threadIdx.x) >= sv.split[blk];
 	double const beta = SEG ? seg_scal[3 * sv.seg[2 * blk + second] + 1] : S->beta;
@@ -454,14 +459,16 @@ __global__ __launch_bounds__(kRed) void k_red_fcg_alpha(int nwg, const double* _
 
 // x += alpha p, r -= alpha q, partials r . r.
 template <bool SEG = false>
-__global__ __launch_bounds__(kBlock) void k_fcg_update(int nb, const double* __restrict__ p,
-                                                       const double* __restrict__ q, double* __restrict__ x,
-                                                       double* __restrict__ r, double* __restrict__ partial,
-                                                       const Scalars* __restrict__ S, SegView sv = SegView{},
+__global__ __launch_bounds__(kBlock) void k_fcg_update(int nb, const double* __restrict__ p0,
+                                                       const double* __restrict__ p1, const double* __restrict__ q,
+                                                       double* __restrict__ x, double* __restrict__ r,
+                                                       double* __restrict__ partial, const Scalars* __restrict__ S,
+                                                       SegView sv = SegView{},
                                                        const double* __restrict__ seg_scal = nullptr) {
 	if (S->done) {
 		return;
 	}
+	const double* __restrict__ p = (S->iter & 1) ? p0 : p1;   // the direction k_fcg_spmv wrote
 	bool const second = SEG && static_cast<int>(threadIdx.x) >= sv.split[blockIdx.x];
 	double const alpha = SEG ? seg_scal[3 * sv.seg[2 * blockIdx.x + second]] : S->alpha;
 	int const i = blockIdx.x * kBlock + threadIdx.x;
@@ -769,10 +776,9 @@ void launch_iteration(bfmx_system* s, int pre, bool mixed, int it) {
 
 // One flexible-CG iteration with the multigrid V-cycle or the learned
 // preconditioner.
-void launch_fcg_iteration(bfmx_system* s, int pre, int it) {
+void launch_fcg_iteration(bfmx_system* s, int pre) {
 	int const nb = static_cast<int>(s->n_nodes);
 	int const g = grid(nb);
-	int const a = it & 1, b = a ^ 1;
 	int nwz = 0;
 	if (pre == BFMX_PRECOND_AMG && s->fcg_sv.seg) {
 		// segmented: every convergence segment runs its own alpha and beta
@@ -780,29 +786,30 @@ void launch_fcg_iteration(bfmx_system* s, int pre, int it) {
 		float* z = amg_vcycle(s, s->d_r, s->d_q, s->d_fcg_part, s->d_scal, &nwz);
 		k_seg_red_fcg<0><<<m, 64, 0, s->stream>>>(s->d_conv_seg, s->fcg_sv, g, s->d_fcg_part, s->d_fcg_scal,
 		                                          s->d_scal);
-		k_fcg_spmv<float, true><<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[a],
-		                                                     s->d_p[b], s->d_q, s->d_r, s->d_fcg_part, s->d_scal,
+		k_fcg_spmv<float, true><<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[0],
+		                                                     s->d_p[1], s->d_q, s->d_r, s->d_fcg_part, s->d_scal,
 		                                                     s->fcg_sv, s->d_fcg_scal);
 		k_seg_red_fcg<1><<<m, 64, 0, s->stream>>>(s->d_conv_seg, s->fcg_sv, g, s->d_fcg_part, s->d_fcg_scal,
 		                                          s->d_scal);
-		k_fcg_update<true><<<g, kBlock, 0, s->stream>>>(nb, s->d_p[b], s->d_q, s->d_x, s->d_r, s->d_partial,
-		                                                s->d_scal, s->fcg_sv, s->d_fcg_scal);
+		k_fcg_update<true><<<g, kBlock, 0, s->stream>>>(nb, s->d_p[0], s->d_p[1], s->d_q, s->d_x, s->d_r,
+		                                                s->d_partial, s->d_scal, s->fcg_sv, s->d_fcg_scal);
 		k_red_fcg_rr<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
 		return;
 	}
 	if (pre == BFMX_PRECOND_AMG) {
 		float* z = amg_vcycle(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nwz);
 		k_red_fcg_beta<<<1, kRed, 0, s->stream>>>(nwz, s->d_partial, s->d_scal);
-		k_fcg_spmv<float><<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[a],
-		                                               s->d_p[b], s->d_q, s->d_r, s->d_partial, s->d_scal);
+		k_fcg_spmv<float><<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[0],
+		                                               s->d_p[1], s->d_q, s->d_r, s->d_partial, s->d_scal);
 	} else {
 		double* z = mpnn_apply(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nwz);
 		k_red_fcg_beta<<<1, kRed, 0, s->stream>>>(nwz, s->d_partial, s->d_scal);
-		k_fcg_spmv<double><<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[a],
-		                                                s->d_p[b], s->d_q, s->d_r, s->d_partial, s->d_scal);
+		k_fcg_spmv<double><<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[0],
+		                                                s->d_p[1], s->d_q, s->d_r, s->d_partial, s->d_scal);
 	}
 	k_red_fcg_alpha<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
-	k_fcg_update<false><<<g, kBlock, 0, s->stream>>>(nb, s->d_p[b], s->d_q, s->d_x, s->d_r, s->d_partial, s->d_scal);
+	k_fcg_update<false><<<g, kBlock, 0, s->stream>>>(nb, s->d_p[0], s->d_p[1], s->d_q, s->d_x, s->d_r, s->d_partial,
+	                                                 s->d_scal);
 	k_red_fcg_rr<<<1, kRed, 0, s->stream>>>(g, s->d_partial, s->d_scal);
 }
 
@@ -819,12 +826,11 @@ int chunk_iters(int pre) {
 // One chunk of iterations and the conditional residual recomputation. The
 // multigrid path recomputes the residual only for the tolerance checks (its
 // recursive residual is FP64 throughout). Flexible CG runs n iterations
-// (1 .. kFcgChunk), the first reading direction buffer `parity`; the other
-// solvers run kChunk.
-void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n, int parity) {
+// (1 .. kFcgChunk); the other solvers run kChunk.
+void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n) {
 	if (is_fcg(pre)) {
 		for (int it = 0; it < n; it++) {
-			launch_fcg_iteration(s, pre, parity + it);
+			launch_fcg_iteration(s, pre);
 		}
 		launch_residual(s, pre == BFMX_PRECOND_LEARNED ? BFMX_PRECOND_BLOCK_JACOBI : BFMX_PRECOND_JACOBI, 1, rtol2,
 		                0.0);
@@ -838,19 +844,18 @@ void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n, int 
 
 
 // The captured chunk for this configuration (captured on first use).
-int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n, int parity, hipGraphExec_t* out) {
+int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n, hipGraphExec_t* out) {
 	int const prec = mixed ? 1 : 0;
 	const void* model = pre == BFMX_PRECOND_LEARNED ? static_cast<const void*>(s->model) : nullptr;
 	for (auto const& gr : s->graphs) {
-		if (gr.precond == pre && gr.precision == prec && gr.rtol2 == rtol2 && gr.model == model && gr.iters == n &&
-		    gr.parity == parity) {
+		if (gr.precond == pre && gr.precision == prec && gr.rtol2 == rtol2 && gr.model == model && gr.iters == n) {
 			*out = gr.exec;
 			return BFMX_OK;
 		}
 	}
 	hipGraph_t g = nullptr;
 	BFMX_HIP(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-	launch_chunk(s, pre, mixed, rtol2, n, parity);
+	launch_chunk(s, pre, mixed, rtol2, n);
 	BFMX_HIP(hipStreamEndCapture(s->stream, &g));
 	Graph gr;
 	hipError_t const e = hipGraphInstantiate(&gr.exec, g, nullptr, nullptr, 0);
@@ -864,7 +869,6 @@ int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n, int 
 	gr.rtol2 = rtol2;
 	gr.model = model;
 	gr.iters = n;
-	gr.parity = parity;
 	s->graphs.push_back(gr);
 	*out = gr.exec;
 	return BFMX_OK;
@@ -920,7 +924,6 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 	int its[4] = {};          // iterations of the chunks in the ring
 	double rr_before = -1;    // r.r and iterations at the inspection before `last` (when consecutive)
 	int64_t iter_before = 0;
-	int parity = 0;           // flexible CG: the direction buffer the next iteration reads
 	while (last.done == 0) {
 		bool const more = launched < cap;
 		bool const in_flight = seen < chunks - 1;
@@ -937,17 +940,15 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 				double const need = std::ceil(std::log(last.tol2 / ahead) / std::log(q));
 				n = static_cast<int>(std::min<double>(std::max(need, 1.0), kFcgChunk));
 			}
-			int const par = is_fcg(precond) ? parity : 0;
 			if (use_graph) {
 				hipGraphExec_t exec = nullptr;
-				BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2, n, par, &exec));
+				BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2, n, &exec));
 				BFMX_HIP(hipGraphLaunch(exec, st));
 			} else {
-				launch_chunk(s, precond, mixed, rtol2, n, par);
+				launch_chunk(s, precond, mixed, rtol2, n);
 				BFMX_HIP(hipGetLastError());
 			}
 			launched += n;
-			parity = (parity + n) & 1;
 			its[chunks & 3] = n;
 			int const slot = static_cast<int>(chunks & 3);
 			BFMX_HIP(hipMemcpyAsync(&s->h_scal[slot], s->d_scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
