@@ -141,13 +141,35 @@ template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ partial, int n, double (&tot)[NV],
                                                 const double (&first)[NV]) {
 	__shared__ double lds[NV][kRed / 64];
+	// each thread sums its partials in index order; they are loaded kLoads at
+	// a time, all in flight together (a rolled loop waits for each in turn).
+	// Slots past the end add 0.0, which changes no sum.
+	constexpr int kLoads = 4;
 	double acc[NV];
 #pragma unroll
 	for (int k = 0; k < NV; k++) {
 		acc[k] = first[k];
-		for (int i = threadIdx.x + kRed; i < n; i += kRed) {
-			acc[k] += partial[k * n + i];
+	}
+	for (int base = threadIdx.x + kRed; base < n; base += kLoads * kRed) {
+		double v[NV][kLoads];
+#pragma unroll
+		for (int k = 0; k < NV; k++) {
+#pragma unroll
+			for (int u = 0; u < kLoads; u++) {
+				int const i = base + u * kRed;
+				v[k][u] = i < n ? partial[k * n + i] : 0.0;
+			}
 		}
+#pragma unroll
+		for (int k = 0; k < NV; k++) {
+#pragma unroll
+			for (int u = 0; u < kLoads; u++) {
+				acc[k] += v[k][u];
+			}
+		}
+	}
+#pragma unroll
+	for (int k = 0; k < NV; k++) {
 		acc[k] = wave_sum(acc[k]);
 	}
 	int const lane = threadIdx.x & 63, w = threadIdx.x >> 6;
