@@ -1075,6 +1075,8 @@ __global__ __launch_bounds__(kBlock) void k_first_resid(int nb, const int32_t* _
 #define BFMX_RESTRICT_LANES 16
 #endif
 constexpr int kRestrictLanes = BFMX_RESTRICT_LANES;
+static_assert(kRestrictLanes >= 1 && kRestrictLanes <= 64 && (kRestrictLanes & (kRestrictLanes - 1)) == 0,
+              "a coarse row's lanes: a power of two within one wavefront (the butterfly)");
 
 // Coarse row J (live) by the lane group of `lane`; every lane of the group
 // must call it (the butterfly).
