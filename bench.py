@@ -206,8 +206,8 @@ def main():
                     help="also run CG on the FP32 matrix with FP64 reliable updates")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); multi-GPU sharding is "
-              "not built (see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the solver has no "
+              "multi-GPU sharding; value is one GPU's throughput", file=sys.stderr)
         sys.exit(2)
 
     from bfm_amd import device as D
