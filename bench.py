@@ -4,12 +4,14 @@ Workload (default): the 2000 x 500 Q1 plane-stress cantilever beam, nodes
 jittered by +-20% of the cell size (seed 0), 2,005,002 DOFs, E = 211 GPa,
 nu = 0.3, left edge clamped, traction -1e6 N/m on the top edge, 2x2 Gauss.
 
-One step = one solve from u = 0 on the device-resident assembled system until
-||b - K u||_2 / ||b||_2 <= 1e-8 (true residual recomputed in FP64), including
-the solver's per-solve set-up (FP32 matrix copy, multigrid numeric set-up).
-value = DOFs / mean step time of the headline solver, flexible CG in FP64
-preconditioned by one FP32 smoothed-aggregation V-cycle. Plain FP64
-Jacobi-PCG is timed the same way beside it (the speedup is reported).
+One step = the first solve, from u = 0, of a freshly assembled device-resident
+system until ||b - K u||_2 / ||b||_2 <= 1e-8 (true residual recomputed in
+FP64): every per-system cost of the solver is inside it (FP32 matrix copy,
+multigrid topology and numeric set-up, solver buffers, captured chunks), as
+in each bfm_sim_run. value = DOFs / mean step time of the headline solver,
+flexible CG in FP64 preconditioned by one FP32 smoothed-aggregation V-cycle.
+A re-solve on the same system (hierarchy cached) is reported beside it, and
+plain FP64 Jacobi-PCG is timed on the system (the speedups are reported).
 
 Also reported: the batched workload (4096 seeded holed panels of 1k-60k DOFs,
 assembled and solved by block-Jacobi PCG in one graph launch, one workgroup per
@@ -221,10 +223,36 @@ def main():
     mesh, stress = meshgen.workload(args.workload)
     log(f"workload {args.workload}: {mesh.n_nodes} nodes, {mesh.n_dof} DOFs, {mesh.n_elems} elements")
 
-    t0 = time.perf_counter()
-    s = D.beam_system(mesh, stress=stress)
+    def fresh():
+        t = time.perf_counter()
+        sys_ = D.beam_system(mesh, stress=stress)
+        return sys_, time.perf_counter() - t
+
+    # headline: the first solve of a freshly assembled system, so every
+    # per-system cost of the solver is inside the step (the FP32 matrix copy,
+    # the multigrid hierarchy's topology and numeric set-up, the solver's
+    # buffers and its captured chunks), as each bfm_sim_run pays it; the
+    # system itself (upload, renumbering, pattern, assembly, conditions) is
+    # built before the step's clock starts
+    first_ms, first_st, setups = [], None, []
+    s = None
+    for k in range(args.warmup + args.steps):
+        if s is not None:
+            s.close()
+        s, t_setup = fresh()
+        setups.append(t_setup)
+        t = time.perf_counter()
+        _, st0 = s.solve(precond=D.PRECOND_AMG, rtol=args.rtol, want_u=False)
+        dt = time.perf_counter() - t
+        assert st0["status"] == 0 and st0["rel_residual"] <= args.rtol, st0
+        tag = "warmup" if k < args.warmup else f"step {k - args.warmup}"
+        log(f"fcg_amg first solve of a fresh system, {tag}: {dt * 1e3:.1f} ms, {st0['iterations']} iterations, "
+            f"rel residual {st0['rel_residual']:.3e} (system set-up {t_setup:.2f} s)")
+        if k >= args.warmup:
+            first_ms.append(dt * 1e3)
+            first_st = st0
     info = s.info()
-    t_setup = time.perf_counter() - t0
+    t_setup = float(np.mean(setups))
     log(f"setup {t_setup:.2f} s: {info}")
 
     def leg(name, precond, precision=D.FP64, steps=args.steps):
@@ -245,8 +273,10 @@ def main():
         ms = 1e3 * float(np.mean(times))
         return ms, stats[-1]
 
-    ms, st = leg("fcg_amg", D.PRECOND_AMG)
-    value = mesh.n_dof / (ms / 1e3)
+    ms_first = float(np.mean(first_ms))
+    value = mesh.n_dof / (ms_first / 1e3)
+    # the same solver re-solving on a system whose hierarchy exists
+    ms, st = leg("fcg_amg (cached hierarchy)", D.PRECOND_AMG)
 
     def roof(kernel, label):
         k_ms, k_bytes = s.time_kernel(kernel, reps=100)
@@ -268,8 +298,10 @@ def main():
             "ms_per_step": p_ms, "value": mesh.n_dof / (p_ms / 1e3), "iterations": int(p_st["iterations"]),
             "rel_residual": p_st["rel_residual"],
         }
-        extra["speedup_vs_pcg"] = p_ms / ms
-        log(f"speedup of multigrid FCG over Jacobi-PCG: {p_ms / ms:.2f}x")
+        extra["speedup_vs_pcg"] = p_ms / ms_first
+        extra["speedup_vs_pcg_cached_hierarchy"] = p_ms / ms
+        log(f"speedup of multigrid FCG over Jacobi-PCG: {p_ms / ms_first:.2f}x first solve, "
+            f"{p_ms / ms:.2f}x re-solve")
     weights = os.path.join(ROOT, "bfm_amd", "weights", "mpnn_elasticity.f32")
     if not args.no_learned and os.path.exists(weights):
         # the learned preconditioner beside the method without it (block-Jacobi PCG)
@@ -312,7 +344,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms,
+        "ms_per_step": ms_first,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -324,10 +356,14 @@ def main():
             "solver": "flexible CG (FP64) + FP32 smoothed-aggregation multigrid V-cycle",
             "n_dof": mesh.n_dof,
             "n_elems": mesh.n_elems,
-            "iterations": int(st["iterations"]),
-            "rel_residual": st["rel_residual"],
+            "iterations": int(first_st["iterations"]),
+            "rel_residual": first_st["rel_residual"],
+            "step": "first solve of a freshly assembled system: FP32 matrix copy, multigrid topology and "
+                    "numeric set-up, solver buffers and captured chunks, and the iterations to the tolerance",
             "setup_s": t_setup,
-            "end_to_end_s": t_setup + ms / 1e3,
+            "end_to_end_s": t_setup + ms_first / 1e3,
+            "resolve_cached_hierarchy": {"ms_per_step": ms, "value": mesh.n_dof / (ms / 1e3),
+                                         "iterations": int(st["iterations"]), "rel_residual": st["rel_residual"]},
         },
         "roofline": roof_fcg,
         "roofline_smoother": roof_smooth,

@@ -257,10 +257,35 @@ SIGNATURES = {
 }
 
 
+class bfm_solver_t(C.Structure):
+    """include/bfm/gpu.h (this implementation's extension; no reference counterpart)."""
+    _fields_ = [("precond", C.c_int), ("precision", C.c_int), ("rtol", C.c_double), ("max_iter", c_size_t)]
+
+
+class bfm_solve_stats_t(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", c_size_t), ("rel_residual", C.c_double),
+                ("solve_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+# include/bfm/gpu.h: bound only where the library exports them (a libbfm
+# built from the reference sources does not)
+EXTENSIONS = {
+    "bfm_sim_set_solver": (_I, [P(bfm_sim_t), P(bfm_solver_t)]),
+    "bfm_sim_get_solver": (_I, [P(bfm_sim_t), P(bfm_solver_t)]),
+    "bfm_sim_set_model": (_I, [P(bfm_sim_t), P(C.c_float), c_size_t]),
+    "bfm_sim_get_stats": (_I, [P(bfm_sim_t), c_size_t, P(bfm_solve_stats_t)]),
+}
+
+
 def bind(lib):
     """Sets restype/argtypes of every libbfm entry point on a loaded CDLL."""
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
+    for name, (res, args) in EXTENSIONS.items():
+        fn = getattr(lib, name, None)
+        if fn is not None:
+            fn.restype = res
+            fn.argtypes = args
     return lib

@@ -536,7 +536,15 @@ int bfmx_model_train(bfmx_model_t* m, bfmx_trainset_t* ts, int steps, int batch,
 
 int bfmx_system_set_model(bfmx_system_t* s, bfmx_model_t* m) {
 	BFMX_CHECK(check(s));
-	s->model = m;
+	if (m != s->model) {
+		// the captured learned chunks read the old model's parameters, which
+		// its owner may free once it is detached
+		return guarded([&] {
+			drop_graphs(s, BFMX_PRECOND_LEARNED);
+			s->model = m;
+			return BFMX_OK;
+		});
+	}
 	return BFMX_OK;
 }
 

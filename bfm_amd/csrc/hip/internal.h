@@ -228,7 +228,11 @@ struct Graph {
 	int precond = -1;
 	int precision = -1;
 	double rtol2 = -1;
-	const void* model = nullptr;
+	// the learned preconditioner's model a chunk was captured with: its
+	// creation number (never reused, unlike a freed model's address) and the
+	// parameter buffer the captured kernels read
+	uint64_t model_gen = 0;
+	const float* model_params = nullptr;
 	int iters = 0;    // iterations in the chunk
 };
 
@@ -339,6 +343,7 @@ struct bfmx_system {
 // Learned preconditioner: parameters, optimiser state and training workspace.
 struct bfmx_model {
 	int device = 0;
+	uint64_t gen = 0;           // creation number, unique in the process (keys captured graphs)
 	hipStream_t stream = nullptr;
 	float* d_params = nullptr;
 	float* d_grad = nullptr;
@@ -467,6 +472,9 @@ int time_kernel(bfmx_system* s, int kernel, int reps, double* avg_ms, double* by
 // amg.hip: reps launches of one finest-level smoothing step (set-up done)
 int amg_time_smoother(bfmx_system* s, int reps, hipEvent_t e0, hipEvent_t e1, double* bytes);
 void release_solver(bfmx_system* s);
+// Destroys the system's captured chunks of one preconditioner (all with -1),
+// after the system's stream has drained.
+void drop_graphs(bfmx_system* s, int precond);
 // false when the environment sets BFMX_GRAPHS=0: launches go straight to the
 // stream instead of through captured graphs (for tools that do not follow
 // graph launches)

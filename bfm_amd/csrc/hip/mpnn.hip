@@ -36,6 +36,7 @@
 // orders K inside a lane group.
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <random>
 
@@ -1026,7 +1027,9 @@ int mpnn_n_params() {
 }
 
 int model_create(bfmx_model** out, uint64_t seed) {
+	static std::atomic<uint64_t> created{0};
 	auto* m = new bfmx_model();
+	m->gen = ++created;
 	int const rc = [&]() -> int {
 		BFMX_HIP(hipGetDevice(&m->device));
 		BFMX_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));

@@ -868,9 +868,12 @@ void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n) {
 // The captured chunk for this configuration (captured on first use).
 int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n, hipGraphExec_t* out) {
 	int const prec = mixed ? 1 : 0;
-	const void* model = pre == BFMX_PRECOND_LEARNED ? static_cast<const void*>(s->model) : nullptr;
+	bool const learned = pre == BFMX_PRECOND_LEARNED && s->model;
+	uint64_t const gen = learned ? s->model->gen : 0;
+	const float* params = learned ? s->model->d_params : nullptr;
 	for (auto const& gr : s->graphs) {
-		if (gr.precond == pre && gr.precision == prec && gr.rtol2 == rtol2 && gr.model == model && gr.iters == n) {
+		if (gr.precond == pre && gr.precision == prec && gr.rtol2 == rtol2 && gr.model_gen == gen &&
+		    gr.model_params == params && gr.iters == n) {
 			*out = gr.exec;
 			return BFMX_OK;
 		}
@@ -889,7 +892,8 @@ int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n, hipG
 	gr.precond = pre;
 	gr.precision = prec;
 	gr.rtol2 = rtol2;
-	gr.model = model;
+	gr.model_gen = gen;
+	gr.model_params = params;
 	gr.iters = n;
 	s->graphs.push_back(gr);
 	*out = gr.exec;
@@ -933,6 +937,18 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 	Scalars last = s->h_scal[0];
 	double mark = last.rr;
 	int64_t mark_iter = 0;
+	// Every chunk length the loop may launch is captured here, while the
+	// stream is idle: a capture begun on the stream while an earlier chunk's
+	// graph launch is still running on it is avoided (capture mode is
+	// per-thread, but the runtime and tools that intercept launches then see
+	// a capture interleaved with in-flight work on the same stream).
+	hipGraphExec_t execs[kFcgChunk > kChunk ? kFcgChunk + 1 : kChunk + 1] = {};
+	if (use_graph) {
+		int const lo = is_fcg(precond) ? 1 : kChunk;
+		for (int len = lo; len <= chunk_iters(precond); len++) {
+			BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2, len, &execs[len]));
+		}
+	}
 
 	// Replay chunks, keeping one in flight while the one before it is
 	// inspected. Launches after the end return at once but still cost a
@@ -963,9 +979,7 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 				n = static_cast<int>(std::min<double>(std::max(need, 1.0), kFcgChunk));
 			}
 			if (use_graph) {
-				hipGraphExec_t exec = nullptr;
-				BFMX_CHECK(capture_chunk(s, precond, mixed, rtol2, n, &exec));
-				BFMX_HIP(hipGraphLaunch(exec, st));
+				BFMX_HIP(hipGraphLaunch(execs[n], st));
 			} else {
 				launch_chunk(s, precond, mixed, rtol2, n);
 				BFMX_HIP(hipGetLastError());
@@ -1001,6 +1015,19 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 }
 
 }  // namespace
+
+void drop_graphs(bfmx_system* s, int precond) {
+	(void) hipStreamSynchronize(s->stream);
+	auto keep = s->graphs.begin();
+	for (auto& gr : s->graphs) {
+		if (precond < 0 || gr.precond == precond) {
+			(void) hipGraphExecDestroy(gr.exec);
+		} else {
+			*keep++ = gr;
+		}
+	}
+	s->graphs.erase(keep, s->graphs.end());
+}
 
 void release_solver(bfmx_system* s) {
 	amg_release(s);

@@ -31,6 +31,13 @@ WEIGHTS = os.path.join(HERE, "weights", "mpnn_elasticity.f32")
 TRAIN_SHAPES = [(60, 15), (100, 25), (160, 40), (200, 50), (300, 75)]
 
 
+def default_params(path=WEIGHTS):
+    """The packaged trained parameters (FP32, raw little-endian)."""
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path}: no trained weights; run python -m bfm_amd.learned train")
+    return np.fromfile(path, dtype=np.float32)
+
+
 def training_problems(n_problems, seed0=100):
     from . import meshgen
     for k in range(n_problems):

@@ -6,6 +6,9 @@
  * reference's dense band LU.
  *
  * Defaults: point Jacobi, FP64, rtol 1e-10, max_iter 0 (= 20 x DOFs).
+ * bfm_sim_run solves several instances with constant body forces in one
+ * batched launch when the solver is point Jacobi or block Jacobi (instances
+ * of at most 32768 nodes) or multigrid (any size), in FP64.
  * A run that stops at the rounding floor of the residual (above rtol but no
  * longer decreasing) is accepted: it is the attainable accuracy. */
 
@@ -31,6 +34,11 @@ typedef struct {
 
 int bfm_sim_set_solver(bfm_sim_t* sim, bfm_solver_t const* solver);
 int bfm_sim_get_solver(bfm_sim_t* sim, bfm_solver_t* solver);
+
+/* Parameters of the learned preconditioner (precond 3), copied: the FP32
+ * vector of bfmx_model_get_params (n_params must equal bfmx_model_n_params).
+ * NULL clears them. */
+int bfm_sim_set_model(bfm_sim_t* sim, float const* params, size_t n_params);
 
 /* Statistics of instance `index` of the last bfm_sim_run. */
 int bfm_sim_get_stats(bfm_sim_t* sim, size_t index, bfm_solve_stats_t* stats);

@@ -206,3 +206,91 @@ def test_wavefront_bridge_against_reference_library(L):
     err = rel_err(u, u_ref)
     print(f"bridge-nothing.obj: error vs libbfm {err:.2e}")
     assert err <= U_TOL
+
+
+def test_lepl1110_command_line(tmp_path):
+    """python -m bfm_amd.lepl1110 MESH PROBLEM writes U.txt and V.txt as the
+    reference's lepl1110.py:9-14 does (libbfm/src/ez.c:211-229 format); they
+    match the reference's own data/U.txt and data/V.txt to their 8 digits."""
+    import sys
+    out = tmp_path / "data"
+    r = subprocess.run([sys.executable, "-m", "bfm_amd.lepl1110", LEPL, PROBLEM, "--out", str(out)],
+                       cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    print(r.stdout.strip())
+    for name in ("U.txt", "V.txt"):
+        mine = pb.read_effects_file(str(out / name))
+        ref = pb.read_effects_file(os.path.join(REF_DATA, name))
+        assert np.abs(mine - ref).max() <= 5e-8 * np.abs(ref).max(), name
+        head_mine = open(out / name).readline()
+        assert head_mine == open(os.path.join(REF_DATA, name)).readline()
+
+
+def beam_sim(L, meshes, solver=None, params=None):
+    """One sim holding one instance per generated beam mesh (plane stress,
+    traction on top, left edge clamped)."""
+    mat = meshgen.BEAM_MATERIAL
+    keep = []
+    sim = pb.Sim(pb.Sim.PLANAR_STRESS, lib=L)
+    insts = []
+    for mesh in meshes:
+        cm = pb.Mesh_arrays.from_generated(mesh, lib=L)
+        m = pb.Material("steel", mat["rho"], mat["E"], mat["nu"], lib=L)
+        rule = pb.Rule_gauss_legendre(2, mesh.kind, lib=L)
+        obj = pb.Obj(cm, m, rule, lib=L)
+        inst = pb.Instance(obj, lib=L)
+        kinds, values, flags = meshgen.beam_conditions(mesh)
+        for k, v, f in zip(kinds, values, flags):
+            c = pb.Condition(cm, int(k), float(v), lib=L)
+            c.set_nodes(f)
+            inst.add_condition(c)
+            keep.append(c)
+        keep += [cm, m, rule, obj]
+        sim.add_instance(inst)
+        insts.append(inst)
+    if params is not None:
+        assert L.bfm_sim_set_model(C.byref(sim.c_sim), params.ctypes.data_as(C.POINTER(C.c_float)),
+                                   params.size) == 0
+    if solver is not None:
+        assert L.bfm_sim_set_solver(C.byref(sim.c_sim), C.byref(solver)) == 0
+    sim.keep = keep
+    return sim, insts
+
+
+@pytest.mark.parametrize("precond", [0, 1, 2])
+def test_sim_solves_several_instances_in_one_batch(L, precond):
+    """Several instances of uneven sizes (P1 and Q1) in one sim run as one
+    batch (the instance loop of libbfm/src/sim.c:103-135 in one launch); each
+    matches the same instance run alone (a single-instance sim, one device
+    system) within 1e-8."""
+    meshes = [meshgen.beam(40, 10, kind=3), meshgen.beam(30, 8, kind=4, jitter=0.2, seed=1),
+              meshgen.beam(60, 15, kind=3, jitter=0.2, seed=2), meshgen.beam(12, 3, kind=4)]
+    solver = F.bfm_solver_t(precond, 0, 1e-11, 0)
+    sim, insts = beam_sim(L, meshes, solver)
+    sim.run()
+    for k, mesh in enumerate(meshes):
+        solo, (one,) = beam_sim(L, [mesh], F.bfm_solver_t(1, 0, 1e-11, 0))
+        solo.run()
+        st = F.bfm_solve_stats_t()
+        assert L.bfm_sim_get_stats(C.byref(sim.c_sim), k, C.byref(st)) == 0
+        err = rel_err(insts[k].effects, one.effects)
+        print(f"precond {precond}, instance {k} ({mesh.n_dof} DOFs): {st.iterations} iterations, "
+              f"residual {st.rel_residual:.2e}, error vs solo {err:.2e}")
+        assert st.rel_residual <= 1e-11 and err <= U_TOL
+
+
+def test_sim_learned_solver(L, golden):
+    """precond 3 (learned flexible CG) through the libbfm API with the
+    packaged parameters; refused without parameters."""
+    from bfm_amd.learned import default_params
+    mesh = meshgen.beam(40, 10, kind=3)
+    sim, (inst,) = beam_sim(L, [mesh], F.bfm_solver_t(3, 0, 1e-10, 0))
+    with pytest.raises(AssertionError, match="bfm_sim_set_model"):
+        sim.run()
+    sim2, (inst2,) = beam_sim(L, [mesh], F.bfm_solver_t(3, 0, 1e-10, 0), params=default_params())
+    sim2.run()
+    st = F.bfm_solve_stats_t()
+    assert L.bfm_sim_get_stats(C.byref(sim2.c_sim), 0, C.byref(st)) == 0
+    err = rel_err(inst2.effects, golden["c1/u"])
+    print(f"learned through bfm_sim_run: {st.iterations} iterations, error vs libbfm {err:.2e}")
+    assert err <= U_TOL

@@ -203,3 +203,39 @@ def test_learned_solve_partial_sums_fit():
     print(f"10k nodes: learned {st['iterations']} iterations (fallback {st['fallback']})")
     assert st["status"] == 0
     assert rel_err(u, u_ref) <= 1e-6
+
+
+def test_model_swap_replays_no_stale_graph():
+    """Regression: captured learned chunks were looked up by the model's host
+    address, so a model created at the address of a destroyed one replayed
+    graphs that read the destroyed model's freed parameters. Models A, B, C
+    are attached in turn with a learned solve each, A is destroyed before C
+    exists; C's solve must equal C's solve on a fresh system (no cached
+    chunks) bitwise, and its preconditioner the oracle's with C's weights."""
+    mesh, s = beam(40, 10, seed=7, hetero=True)
+    models = {}
+    for name, seed in (("A", 11), ("B", 12)):
+        models[name], _ = model_with_encoder(seed)
+        s.set_model(models[name])
+        _, st = s.solve(precond=D.PRECOND_LEARNED, rtol=1e-9)
+        assert st["status"] == 0
+    models["A"].close()
+    models["C"], p_c = model_with_encoder(13)
+    s.set_model(models["C"])
+    u, st = s.solve(precond=D.PRECOND_LEARNED, rtol=1e-9)
+    _, fresh = beam(40, 10, seed=7, hetero=True)
+    fresh.set_model(models["C"])
+    u_fresh, st_fresh = fresh.solve(precond=D.PRECOND_LEARNED, rtol=1e-9)
+    print(f"model C: {st['iterations']} iterations, fresh system {st_fresh['iterations']}")
+    assert st["status"] == 0 and st["fallback"] == 0
+    assert st["iterations"] == st_fresh["iterations"]
+    assert np.array_equal(u, u_fresh)
+    feat = M.features(*s.get_bsr(), internal_fixed(mesh, s))
+    G = M.Graph(*s.get_bsr(), feat)
+    rng = np.random.default_rng(3)
+    rv = rng.standard_normal((G.n, 2))
+    rv[internal_fixed(mesh, s)] = 0.0
+    z = s.apply_model(rv).reshape(-1, 2)
+    assert rel_err(z, M.apply(p_c, G, rv, bf=True)) <= 1e-3
+    s.set_model(None)
+    fresh.set_model(None)

@@ -192,6 +192,44 @@ def test_renumbering_invariance():
     assert err <= U_TOL
 
 
+@pytest.mark.parametrize("kind", [4, 3])
+def test_heterogeneous_E_against_oracle(kind):
+    """Per-element E drawn log-uniform over [1e9, 1e11] (the mixed-precision
+    and learned workloads' material) on a jittered 120 x 30 beam with a body
+    load: every K entry and b against the oracle's CSR assembly with the same
+    elem_E (reference: the single material E of libbfm/src/system.c:454-458,
+    taken per element), u against a direct solve of the oracle's system; then
+    the same through a batch built from the mesh (its own assembly kernel)."""
+    import scipy.sparse.linalg as spla
+    mesh = meshgen.beam(120, 30, kind=kind, jitter=0.2, seed=5)
+    mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=5)
+    assert mesh.elem_E.min() >= 1e9 and mesh.elem_E.max() <= 1e11
+    g = (0.0, -9.81)
+    s = D.beam_system(mesh, gravity=g)
+    A, b = oracle_csr(mesh, True, forces=[g])
+    Ko = A.to_scipy()
+    Kg = s.get_csr_scipy()
+    assert abs(Kg - Ko).max() <= K_TOL * abs(Ko).max()
+    close_entries(s.get_rhs(), b)
+    u_ref = spla.spsolve(Ko.tocsc(), b)
+    u, st = s.solve(precond=D.PRECOND_BLOCK_JACOBI, rtol=1e-9)
+    assert st["status"] == 0
+    err = rel_err(u, u_ref)
+    mat = meshgen.BEAM_MATERIAL
+    kinds, values, flags = meshgen.beam_conditions(mesh)
+    pb = D.Batch.from_instances([D.planar_instance(mesh, True, mat, kinds, values, flags, g=g,
+                                                   elem_E=mesh.elem_E)])
+    stb, _ = pb.solve(precond=D.PRECOND_BLOCK_JACOBI, rtol=1e-9)
+    own = pb.system(0)
+    assert abs(own.get_csr_scipy() - Ko).max() <= K_TOL * abs(Ko).max()
+    close_entries(own.get_rhs(), b)
+    err_b = rel_err(pb.get_u(0), u_ref)
+    print(f"heterogeneous E, kind {kind}: E in [{mesh.elem_E.min():.3g}, {mesh.elem_E.max():.3g}], "
+          f"system error {err:.2e} ({st['iterations']} it), batch error {err_b:.2e}")
+    assert stb[0]["status"] == 0
+    assert err <= U_TOL and err_b <= U_TOL
+
+
 def test_mixed_precision_reaches_fp64_residual():
     mesh = meshgen.beam(200, 50, kind=4, jitter=0.2, seed=2)
     mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=2)

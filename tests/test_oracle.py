@@ -319,3 +319,38 @@ def test_csr_pcg_matches_direct():
     assert rel_err(x, u_d) <= 1e-8
     r = b - A.matvec(x)
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
+
+
+@pytest.mark.parametrize("kind", [3, 4])
+def test_per_element_E_is_linear_sum_of_unit_elements(kind):
+    """The oracle's per-element-E branch (elem_E[e] in place of the material
+    E of libbfm/src/system.c:454-458) pinned by linearity: at fixed nu the D
+    constants are proportional to E, so K(elem_E) = sum_e E_e K_e(E = 1),
+    each K_e assembled from a one-element mesh by the constant-E path (the
+    path checked bit for bit against the reference above). The body load
+    does not depend on E. An element-index slip in the elem_E branch
+    (elem_E[e + 1], or E taken from the wrong element) breaks the sum."""
+    mesh = meshgen.beam(6, 3, kind=kind, jitter=0.2, seed=8)
+    rng = np.random.default_rng(8)
+    mesh.elem_E = 10.0 ** rng.uniform(9, 11, mesh.n_elems)
+    nu, rho, g = 0.3, 7850.0, [(0.0, -9.81)]
+    for stress in (0, 1):
+        K, b = O.assemble_dense(mesh, stress, 0.0, nu, rho, forces=g)
+        K_sum = np.zeros_like(K)
+        for e in range(mesh.n_elems):
+            nodes = mesh.elems[e].astype(np.int64)
+            one = meshgen.Mesh(kind, mesh.coords[nodes].copy(), np.arange(kind, dtype=np.uint64)[None, :],
+                               np.zeros((0, 2), np.uint64), np.zeros(0, np.int64))
+            Ke, _ = O.assemble_dense(one, stress, 1.0, nu, rho)
+            dof = np.stack([2 * nodes, 2 * nodes + 1], axis=1).reshape(-1)
+            K_sum[np.ix_(dof, dof)] += mesh.elem_E[e] * Ke
+        scale = np.abs(K).max()
+        assert np.abs(K - K_sum).max() <= 1e-13 * scale
+        # the per-element values are the ones used: a shifted elem_E differs
+        shifted = meshgen.Mesh(kind, mesh.coords, mesh.elems, mesh.edges, mesh.edge_elem,
+                               elem_E=np.roll(mesh.elem_E, 1))
+        K_shift, _ = O.assemble_dense(shifted, stress, 0.0, nu, rho, forces=g)
+        assert np.abs(K_shift - K_sum).max() > 1e-3 * scale
+        mesh_c = meshgen.Mesh(kind, mesh.coords, mesh.elems, mesh.edges, mesh.edge_elem)
+        _, b_c = O.assemble_dense(mesh_c, stress, 2.0e11, nu, rho, forces=g)
+        assert np.array_equal(b, b_c)
