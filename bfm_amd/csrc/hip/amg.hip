@@ -50,8 +50,10 @@
 #include <atomic>
 #include <cmath>
 #include <exception>
+#include <string>
 #include <thread>
 
+#include "amg_topo.h"
 #include "internal.h"
 
 namespace bfmx {
@@ -182,6 +184,79 @@ __global__ __launch_bounds__(kBlock) void k_fine_modes(int nb, const double* __r
 			B[6 * i + 3 * a + c] = f ? 0.0 : B0[6 * i + 3 * a + c];
 		}
 	}
+}
+
+// Per-workgroup extents of the node coordinates: part[4 b + {0, 1, 2, 3}] =
+// min x, min y, max x, max y over the workgroup's grid-stride rows.
+constexpr int kBboxGrid = 256;
+__global__ __launch_bounds__(kBlock) void k_bbox(int64_t n, const double* __restrict__ xy, double* __restrict__ part) {
+	__shared__ double lds[4][kBlock / 64];
+	double v[4] = {INFINITY, INFINITY, -INFINITY, -INFINITY};
+	for (int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+	     i += static_cast<int64_t>(gridDim.x) * kBlock) {
+		double const x = xy[2 * i], y = xy[2 * i + 1];
+		v[0] = fmin(v[0], x);
+		v[1] = fmin(v[1], y);
+		v[2] = fmax(v[2], x);
+		v[3] = fmax(v[3], y);
+	}
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		v[0] = fmin(v[0], __shfl_xor(v[0], off, 64));
+		v[1] = fmin(v[1], __shfl_xor(v[1], off, 64));
+		v[2] = fmax(v[2], __shfl_xor(v[2], off, 64));
+		v[3] = fmax(v[3], __shfl_xor(v[3], off, 64));
+	}
+	if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+		for (int k = 0; k < 4; k++) {
+			lds[k][threadIdx.x >> 6] = v[k];
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x < 4) {
+		int const k = threadIdx.x;
+		double m = lds[k][0];
+		for (int w = 1; w < kBlock / 64; w++) {
+			m = k < 2 ? fmin(m, lds[k][w]) : fmax(m, lds[k][w]);
+		}
+		part[4 * blockIdx.x + k] = m;
+	}
+}
+
+// Level-0 rigid-body modes (two translations, the rotation) in coordinates
+// centred on the mesh and scaled to its half-size, from the workgroup
+// extents of k_bbox (each thread folds the kBboxGrid partials: they sit in
+// L2). Rows: (1, 0, -y), (0, 1, x).
+__global__ __launch_bounds__(kBlock) void k_modes0(int64_t n, const double* __restrict__ part,
+                                                   const double* __restrict__ xy, double* __restrict__ B0) {
+	__shared__ double ext[4];
+	if (threadIdx.x < 4) {
+		int const k = threadIdx.x;
+		double m = part[k];
+		for (int b = 1; b < kBboxGrid; b++) {
+			m = k < 2 ? fmin(m, part[4 * b + k]) : fmax(m, part[4 * b + k]);
+		}
+		ext[k] = m;
+	}
+	__syncthreads();
+	int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+	if (i >= n) {
+		return;
+	}
+	double const cx = 0.5 * (ext[0] + ext[2]), cy = 0.5 * (ext[1] + ext[3]);
+	double L = 0.5 * fmax(ext[2] - ext[0], ext[3] - ext[1]);
+	if (!(L > 0)) {
+		L = 1;
+	}
+	double const x = (xy[2 * i] - cx) / L, y = (xy[2 * i + 1] - cy) / L;
+	double* b = B0 + 6 * i;
+	b[0] = 1;
+	b[1] = 0;
+	b[2] = -y;
+	b[3] = 0;
+	b[4] = 1;
+	b[5] = x;
 }
 
 // One aggregate K per thread: G = sum_i B_i^T B_i, Cholesky G = R^T R with
@@ -542,6 +617,181 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32
 				int64_t const km = Blk<3>::at(mb, rm, c, a);
 				cval64[km] = C[a][c];
 				cval32[km] = static_cast<float>(C[a][c]);
+			}
+		}
+	}
+}
+
+// The Galerkin product in two steps (levels built on the device; the
+// patterns come from amg_topo.hip). Step 1, A P: entry (i, J) of row i =
+// sum over the row's blocks A_ij (FP32 copy) with J in P's row j of
+// A_ij P_jJ, summed in FP64 (one thread per row, one entry at a time, so the
+// sums stay in registers), stored FP32.
+template <int BS>
+__global__ __launch_bounds__(kBlock) void k_ap(int nb, const int32_t* __restrict__ slice_ptr,
+                                               const int32_t* __restrict__ col, const int32_t* __restrict__ row_len,
+                                               const float* __restrict__ val32, const int32_t* __restrict__ pptr,
+                                               const int32_t* __restrict__ pcol, const float* __restrict__ P,
+                                               const int32_t* __restrict__ apptr, const int32_t* __restrict__ apcol,
+                                               float* __restrict__ apval) {
+	int const i = blockIdx.x * kBlock + threadIdx.x;
+	if (i >= nb) {
+		return;
+	}
+	int const r = i & 63;
+	int32_t const sbeg = slice_ptr[i >> 6] + r;
+	int const len = row_len[i];
+	for (int32_t q = apptr[i]; q < apptr[i + 1]; q++) {
+		int32_t const J = apcol[q];
+		double acc[BS][3] = {};
+		for (int k = 0; k < len; k++) {
+			int32_t const blk = sbeg + k * kSlice;
+			int const j = col[blk];
+			int32_t f = -1;
+			for (int32_t g = pptr[j]; g < pptr[j + 1]; g++) {
+				int32_t const c = pcol[g];
+				if (c >= J) {
+					f = c == J ? g : -1;
+					break;
+				}
+			}
+			if (f < 0) {
+				continue;
+			}
+			float A[BS][BS];
+			ld_block<BS>(val32, blk, r, A);
+			const float* pj = P + BS * 3 * static_cast<int64_t>(f);
+#pragma unroll
+			for (int a = 0; a < BS; a++) {
+#pragma unroll
+				for (int c = 0; c < 3; c++) {
+					double t = 0;
+#pragma unroll
+					for (int m = 0; m < BS; m++) {
+						t += static_cast<double>(A[a][m]) * static_cast<double>(pj[3 * m + c]);
+					}
+					acc[a][c] += t;
+				}
+			}
+		}
+		float* out = apval + BS * 3 * static_cast<int64_t>(q);
+#pragma unroll
+		for (int a = 0; a < BS; a++) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				out[3 * a + c] = static_cast<float>(acc[a][c]);
+			}
+		}
+	}
+}
+
+// Step 2, P^T (A P): one wavefront per coarse row I (the rows past the
+// level's end in its last slice included), in four groups of 16 lanes; a
+// group takes the row's slots (I, J >= I) in turn, its lanes P's column I
+// (R, entries in column order) with a stride of 16: C_IJ = sum over the
+// column's rows i of P_iI^T (A P)_iJ (binary search of J in row i of A P),
+// in FP64, combined by a fixed butterfly; the block is stored with its
+// transpose in the mirror slot (FP64 and the FP32 copy). Padding slots are
+// written with zeros.
+template <int BSF>
+__global__ __launch_bounds__(kBlock) void k_rap(int64_t nrows_pad, int nc, const int32_t* __restrict__ c_slice_ptr,
+                                                const int32_t* __restrict__ c_col,
+                                                const int32_t* __restrict__ c_row_len,
+                                                const int32_t* __restrict__ c_diag_blk,
+                                                const int32_t* __restrict__ mirror, const int32_t* __restrict__ rptr,
+                                                const int32_t* __restrict__ rrow, const float* __restrict__ R,
+                                                const int32_t* __restrict__ apptr, const int32_t* __restrict__ apcol,
+                                                const float* __restrict__ apval, double* __restrict__ cval64,
+                                                float* __restrict__ cval32) {
+	int64_t const I = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 6;
+	if (I >= nrows_pad) {
+		return;
+	}
+	int const lane = threadIdx.x & 63;
+	int const grp = lane >> 4, l = lane & 15;
+	int64_t const s = I >> 6;
+	int const r = static_cast<int>(I & 63);
+	int32_t const sp = c_slice_ptr[s];
+	int const width = (c_slice_ptr[s + 1] - sp) / kSlice;
+	int const len = I < nc ? c_row_len[I] : 0;
+	for (int q = len + lane; q < width; q += 64) {   // padding slots
+		int64_t const slot = sp + static_cast<int64_t>(q) * kSlice + r;
+#pragma unroll
+		for (int a = 0; a < 3; a++) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				int64_t const k = Blk<3>::at(slot, r, a, c);
+				cval64[k] = 0;
+				cval32[k] = 0;
+			}
+		}
+	}
+	if (I >= nc) {
+		return;
+	}
+	int const q0 = (c_diag_blk[I] - sp - r) / kSlice;
+	int32_t const t0 = rptr[I], t1 = rptr[I + 1];
+	for (int q = q0 + grp; q < len; q += 4) {
+		int64_t const slot = sp + static_cast<int64_t>(q) * kSlice + r;
+		int32_t const J = c_col[slot];
+		double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+		for (int32_t t = t0 + l; t < t1; t += 16) {
+			int const i = rrow[t];
+			int32_t lo = apptr[i], hi = apptr[i + 1] - 1;
+			while (lo < hi) {
+				int32_t const mid = (lo + hi) >> 1;
+				if (apcol[mid] < J) {
+					lo = mid + 1;
+				} else {
+					hi = mid;
+				}
+			}
+			if (lo > hi || apcol[lo] != J) {
+				continue;
+			}
+			const float* rt = R + BSF * 3 * static_cast<int64_t>(t);
+			const float* ap = apval + BSF * 3 * static_cast<int64_t>(lo);
+#pragma unroll
+			for (int a = 0; a < 3; a++) {
+#pragma unroll
+				for (int c = 0; c < 3; c++) {
+					double v = 0;
+#pragma unroll
+					for (int k = 0; k < BSF; k++) {
+						v += static_cast<double>(rt[3 * k + a]) * static_cast<double>(ap[3 * k + c]);
+					}
+					C[a][c] += v;
+				}
+			}
+		}
+#pragma unroll
+		for (int a = 0; a < 3; a++) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				double v = C[a][c];
+#pragma unroll
+				for (int off = 8; off > 0; off >>= 1) {
+					v += __shfl_xor(v, off, 64);
+				}
+				C[a][c] = v;
+			}
+		}
+		if (l == 0) {
+			int32_t const mb = mirror[slot];
+			int const rm = mb & 63;
+#pragma unroll
+			for (int a = 0; a < 3; a++) {
+#pragma unroll
+				for (int c = 0; c < 3; c++) {
+					int64_t const k = Blk<3>::at(slot, r, a, c);
+					cval64[k] = C[a][c];
+					cval32[k] = static_cast<float>(C[a][c]);
+					if (mb >= 0 && mb != slot) {
+						int64_t const km = Blk<3>::at(mb, rm, c, a);
+						cval64[km] = C[a][c];
+						cval32[km] = static_cast<float>(C[a][c]);
+					}
+				}
 			}
 		}
 	}
@@ -1597,6 +1847,7 @@ void amg_release(bfmx_system* s) {
 		AmgLevel& L = s->amg[l];
 		if (l > 0) {
 			dfree(L.d_slice_ptr);
+			dfree(L.d_row_len);
 			dfree(L.d_col);
 			dfree(L.d_diag_blk);
 			dfree(L.d_val64);
@@ -1623,6 +1874,9 @@ void amg_release(bfmx_system* s) {
 		dfree(L.d_gal_blk);
 		dfree(L.d_gal_pi);
 		dfree(L.d_gal_pj);
+		dfree(L.d_apptr);
+		dfree(L.d_apcol);
+		dfree(L.d_apval);
 		dfree(L.d_dense);
 		dfree(L.d_seg_ptr);
 		dfree(L.d_seg_row);
@@ -1641,9 +1895,433 @@ void amg_release(bfmx_system* s) {
 	s->amg_wg = -1;
 }
 
+namespace {
+
+// The levels below level 0, built on the host from the node adjacency
+// (s->adj_ptr, s->adj), by up to kBuildThreads threads over chunks of whole
+// segments. fseg receives the coarsest level's segments.
+int build_levels_host(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg) {
+	hipStream_t const st = s->stream;
+	std::vector<AmgLevel>& lv = s->amg;
+	int64_t const n0 = s->n_nodes;
+	AmgParams const& par = s->amg_par;
+	// graph of the current finest-so-far level, the block slot of each
+	// entry and the level's slice pointers
+	std::vector<int64_t> fptr;
+	std::vector<int32_t> fadj;
+	std::vector<int32_t> fslot(s->nnzb);
+	std::vector<int32_t> fslice(s->slice_ptr);
+	for (int64_t i = 0; i < n0; i++) {
+		for (int64_t k = 0; k < s->adj_ptr[i + 1] - s->adj_ptr[i]; k++) {
+			fslot[s->adj_ptr[i] + k] = static_cast<int32_t>(s->slice_ptr[i >> 6] + k * kSlice + (i & 63));
+		}
+	}
+	// coarsen to at most par.coarsest block rows (the one-workgroup
+	// dense inverse of the coarsest operator costs O(n^3) per set-up:
+	// ~0.1 ms at 192 DOFs, ~40 ms at 500)
+	// segments of the current level: aggregates never span two, and the
+	// first row of a segment is an aggregate root (its neighbours are all
+	// in the segment and not yet aggregated), so the aggregates of a
+	// segment are numbered consecutively and give the next level's segments
+	fseg = segments(static_cast<int>(n0), s->adj_ptr.data(), s->adj.data());
+	while (static_cast<int>(lv.size()) < kMaxLevels && seg_largest(fseg) > par.coarsest) {
+		int const lf = static_cast<int>(lv.size()) - 1;
+		const int64_t* ptr = lf == 0 ? s->adj_ptr.data() : fptr.data();
+		const int32_t* adj = lf == 0 ? s->adj.data() : fadj.data();
+		int const nf = lv[lf].nb;
+		// the rows split at segment boundaries into chunks of similar size,
+		// one thread each: the aggregates, prolongator rows, restriction
+		// columns and coarse rows of a chunk stay inside it, so the result
+		// does not depend on the split
+		std::vector<int32_t> const rb = chunk_rows(fseg, nf);
+		int const T = static_cast<int>(rb.size()) - 1;
+		std::atomic<int> bad{0};
+		std::vector<int32_t> agg(nf);
+		std::vector<int32_t> ab(T + 1, 0);   // first aggregate of each chunk
+		parallel_chunks(T, [&](int c) { ab[c + 1] = aggregate(rb[c], rb[c + 1], ptr, adj, agg.data()); });
+		for (int c = 0; c < T; c++) {
+			ab[c + 1] += ab[c];
+		}
+		int const na = ab[T];
+		parallel_chunks(T, [&](int c) {
+			for (int i = rb[c]; i < rb[c + 1]; i++) {
+				agg[i] += ab[c];
+			}
+		});
+		std::vector<int32_t> cseg(fseg.size());
+		for (size_t c = 0; c + 1 < fseg.size(); c++) {
+			cseg[c] = agg[fseg[c]];
+		}
+		cseg.back() = na;
+		if (!std::is_sorted(cseg.begin(), cseg.end()) || cseg[0] != 0) {
+			set_error("multigrid: aggregates do not follow the level's segments");
+			return BFMX_E_ARG;
+		}
+		// stalled when the segments still above the target size shrink by
+		// less than a fifth
+		int64_t big_f = 0, big_a = 0;
+		for (size_t c = 0; c + 1 < fseg.size(); c++) {
+			if (fseg[c + 1] - fseg[c] > par.coarsest) {
+				big_f += fseg[c + 1] - fseg[c];
+				big_a += cseg[c + 1] - cseg[c];
+			}
+		}
+		if (big_a * 10 > big_f * 8) {
+			break;   // coarsening stalled
+		}
+		// members of each aggregate (a chunk's members fill its own rows' range)
+		std::vector<int32_t> aptr(na + 1, 0), anodes(nf);
+		parallel_chunks(T, [&](int c) {
+			std::vector<int32_t> fill(ab[c + 1] - ab[c], 0);
+			for (int i = rb[c]; i < rb[c + 1]; i++) {
+				fill[agg[i] - ab[c]]++;
+			}
+			int32_t run = rb[c];
+			for (int I = ab[c]; I < ab[c + 1]; I++) {
+				int32_t const cnt = fill[I - ab[c]];
+				fill[I - ab[c]] = run;
+				run += cnt;
+				aptr[I + 1] = run;
+			}
+			for (int i = rb[c]; i < rb[c + 1]; i++) {
+				anodes[fill[agg[i] - ab[c]]++] = i;
+			}
+		});
+		// prolongator pattern: row i couples to the aggregates of its
+		// neighbours (smoothed) or to its own aggregate (unsmoothed)
+		std::vector<int32_t> pptr(nf + 1, 0);
+		std::vector<std::vector<int32_t>> pcol_c(T);
+		parallel_chunks(T, [&](int c) {
+			std::vector<int32_t> row;
+			std::vector<int32_t>& pc = pcol_c[c];
+			pc.reserve(static_cast<size_t>(smooth ? 3 : 1) * (rb[c + 1] - rb[c]));
+			for (int i = rb[c]; i < rb[c + 1]; i++) {
+				row.clear();
+				if (smooth) {
+					for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+						row.push_back(agg[adj[e]]);
+					}
+					std::sort(row.begin(), row.end());
+					row.erase(std::unique(row.begin(), row.end()), row.end());
+				} else {
+					row.push_back(agg[i]);
+				}
+				if (row.size() > 255) {
+					bad = 1;
+				}
+				pc.insert(pc.end(), row.begin(), row.end());
+				pptr[i + 1] = static_cast<int32_t>(pc.size());   // within the chunk, offset below
+			}
+		});
+		if (bad) {
+			set_error("multigrid: a row couples to more than 255 aggregates");
+			return BFMX_E_UNSUPPORTED;
+		}
+		std::vector<int64_t> pb(T + 1, 0);   // first entry of each chunk
+		for (int c = 0; c < T; c++) {
+			pb[c + 1] = pb[c] + static_cast<int64_t>(pcol_c[c].size());
+		}
+		if (pb[T] >= (int64_t(1) << 31) / 9) {
+			set_error("multigrid prolongator too large for 32-bit indices");
+			return BFMX_E_ARG;
+		}
+		std::vector<int32_t> pcol(pb[T]);
+		parallel_chunks(T, [&](int c) {
+			for (int i = rb[c]; i < rb[c + 1]; i++) {
+				pptr[i + 1] += static_cast<int32_t>(pb[c]);
+			}
+			std::copy(pcol_c[c].begin(), pcol_c[c].end(), pcol.begin() + pb[c]);
+			std::vector<int32_t>().swap(pcol_c[c]);
+		});
+		auto entry_of = [&](int i, int32_t J) -> int32_t {   // entry of (i, J) in P; J must be in row i
+			auto const beg = pcol.begin() + pptr[i], end = pcol.begin() + pptr[i + 1];
+			return static_cast<int32_t>(std::lower_bound(beg, end, J) - pcol.begin());
+		};
+		// per fine block slot, the entry (within its row) of its column's
+		// aggregate; padding slots and the unsmoothed case point at the
+		// row's own aggregate
+		std::vector<uint8_t> ppos(fslice.back(), 0);
+		parallel_chunks(T, [&](int c) {
+			for (int i = rb[c]; i < rb[c + 1]; i++) {
+				auto const own = static_cast<uint8_t>(entry_of(i, agg[i]) - pptr[i]);
+				for (int32_t slot = fslice[i >> 6] + (i & 63); slot < fslice[(i >> 6) + 1]; slot += kSlice) {
+					ppos[slot] = own;
+				}
+				if (smooth) {
+					for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+						ppos[fslot[e]] = static_cast<uint8_t>(entry_of(i, agg[adj[e]]) - pptr[i]);
+					}
+				}
+			}
+		});
+		// columns of P (restriction), entries in row order
+		std::vector<int32_t> rptr(na + 1, 0), rent(pcol.size()), rrow(pcol.size());
+		parallel_chunks(T, [&](int c) {
+			std::vector<int32_t> fill(ab[c + 1] - ab[c], 0);
+			for (int32_t e = static_cast<int32_t>(pb[c]); e < pb[c + 1]; e++) {
+				if (pcol[e] < ab[c] || pcol[e] >= ab[c + 1]) {
+					bad = 1;
+					return;
+				}
+				fill[pcol[e] - ab[c]]++;
+			}
+			int32_t run = static_cast<int32_t>(pb[c]);
+			for (int I = ab[c]; I < ab[c + 1]; I++) {
+				int32_t const cnt = fill[I - ab[c]];
+				fill[I - ab[c]] = run;
+				run += cnt;
+				rptr[I + 1] = run;
+			}
+			for (int i = rb[c]; i < rb[c + 1]; i++) {
+				for (int32_t e = pptr[i]; e < pptr[i + 1]; e++) {
+					int32_t const at = fill[pcol[e] - ab[c]]++;
+					rent[at] = e;
+					rrow[at] = i;
+				}
+			}
+		});
+		if (bad) {
+			set_error("multigrid: a prolongator row leaves its segment");
+			return BFMX_E_ARG;
+		}
+		// coarse graph, the pattern of P^T A P (sorted rows, with the diagonal)
+		std::vector<int64_t> cptr(na + 1, 0);
+		std::vector<std::vector<int32_t>> cadj_c(T);
+		parallel_chunks(T, [&](int c) {
+			std::vector<int32_t> row;
+			std::vector<int32_t>& ca = cadj_c[c];
+			for (int I = ab[c]; I < ab[c + 1]; I++) {
+				row.clear();
+				for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
+					int const i = rrow[t];
+					for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+						int const j = adj[e];
+						row.insert(row.end(), pcol.begin() + pptr[j], pcol.begin() + pptr[j + 1]);
+					}
+				}
+				std::sort(row.begin(), row.end());
+				row.erase(std::unique(row.begin(), row.end()), row.end());
+				ca.insert(ca.end(), row.begin(), row.end());
+				cptr[I + 1] = static_cast<int64_t>(ca.size());   // within the chunk, offset below
+			}
+		});
+		std::vector<int64_t> cb(T + 1, 0);
+		for (int c = 0; c < T; c++) {
+			cb[c + 1] = cb[c] + static_cast<int64_t>(cadj_c[c].size());
+		}
+		std::vector<int32_t> cadj(cb[T]);
+		parallel_chunks(T, [&](int c) {
+			for (int I = ab[c]; I < ab[c + 1]; I++) {
+				cptr[I + 1] += cb[c];
+			}
+			std::copy(cadj_c[c].begin(), cadj_c[c].end(), cadj.begin() + cb[c]);
+			std::vector<int32_t>().swap(cadj_c[c]);
+		});
+		Sell CS;
+		BFMX_CHECK(build_sell(na, cptr, cadj, CS));
+		// Galerkin gather lists: every (fine block A_ij, entry P_iI, entry
+		// P_jJ) grouped by the coarse slot (I, J), in fine row order (the
+		// slots of coarse row I belong to its chunk)
+		auto coarse_slot = [&](int32_t I, int32_t J) -> int32_t {
+			auto const beg = cadj.begin() + cptr[I], end = cadj.begin() + cptr[I + 1];
+			return CS.slot_of[std::lower_bound(beg, end, J) - cadj.begin()];
+		};
+		std::vector<int64_t> gptr(CS.nnzb_pad + 1, 0);
+		std::vector<int32_t> mirror(CS.nnzb_pad, -1);
+		parallel_chunks(T, [&](int c) {
+			for (int I = ab[c]; I < ab[c + 1]; I++) {
+				for (int64_t e = cptr[I]; e < cptr[I + 1]; e++) {
+					int32_t const J = cadj[e];
+					mirror[CS.slot_of[e]] = J < I ? -2 : coarse_slot(J, I);
+				}
+				for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
+					int const i = rrow[t];
+					for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+						int const j = adj[e];
+						for (int32_t f = pptr[j]; f < pptr[j + 1]; f++) {
+							if (pcol[f] >= I) {
+								gptr[coarse_slot(I, pcol[f]) + 1]++;
+							}
+						}
+					}
+				}
+			}
+		});
+		for (int64_t b = 0; b < CS.nnzb_pad; b++) {
+			gptr[b + 1] += gptr[b];
+		}
+		int64_t const nterms = gptr[CS.nnzb_pad];
+		std::vector<int32_t> gblk(nterms), gpi(nterms), gpj(nterms);
+		{
+			std::vector<int64_t> fill(gptr.begin(), gptr.end() - 1);
+			parallel_chunks(T, [&](int c) {
+				for (int I = ab[c]; I < ab[c + 1]; I++) {
+					for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
+						int const i = rrow[t];
+						int32_t const pi = rent[t];
+						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
+							int const j = adj[e];
+							for (int32_t f = pptr[j]; f < pptr[j + 1]; f++) {
+								if (pcol[f] < I) {
+									continue;
+								}
+								int64_t const at = fill[coarse_slot(I, pcol[f])]++;
+								gblk[at] = fslot[e];
+								gpi[at] = pi;
+								gpj[at] = f;
+							}
+						}
+					}
+				}
+			});
+		}
+		AmgLevel& F = lv[lf];
+		F.sa = smooth;
+		F.nnzp = static_cast<int64_t>(pcol.size());
+		BFMX_CHECK(upload(&F.d_agg, agg, st));
+		BFMX_CHECK(dalloc(&F.d_Pt, static_cast<size_t>(F.bs) * 3 * nf));
+		BFMX_CHECK(upload(&F.d_agg_ptr, aptr, st));
+		BFMX_CHECK(upload(&F.d_agg_nodes, anodes, st));
+		BFMX_CHECK(upload(&F.d_pptr, pptr, st));
+		BFMX_CHECK(upload(&F.d_pcol, pcol, st));
+		BFMX_CHECK(dalloc(&F.d_P, static_cast<size_t>(F.bs) * 3 * pcol.size()));
+		BFMX_CHECK(upload(&F.d_ppos, ppos, st));
+		BFMX_CHECK(upload(&F.d_rptr, rptr, st));
+		BFMX_CHECK(upload(&F.d_rent, rent, st));
+		BFMX_CHECK(dalloc(&F.d_R, static_cast<size_t>(F.bs) * 3 * pcol.size()));
+		BFMX_CHECK(upload(&F.d_rrow, rrow, st));
+		BFMX_CHECK(upload(&F.d_gal_ptr, gptr, st));
+		BFMX_CHECK(upload(&F.d_gal_mirror, mirror, st));
+		BFMX_CHECK(upload(&F.d_gal_blk, gblk, st));
+		BFMX_CHECK(upload(&F.d_gal_pi, gpi, st));
+		BFMX_CHECK(upload(&F.d_gal_pj, gpj, st));
+		lv.emplace_back();   // its buffers are released with the hierarchy, also on failure
+		AmgLevel& C = lv.back();
+		C.nb = na;
+		C.bs = 3;
+		C.n_slices = CS.n_slices;
+		C.nnzb = CS.nnzb;
+		C.nnzb_pad = CS.nnzb_pad;
+		BFMX_CHECK(upload(&C.d_slice_ptr, CS.slice_ptr, st));
+		BFMX_CHECK(upload(&C.d_col, CS.col, st));
+		BFMX_CHECK(upload(&C.d_diag_blk, CS.diag_blk, st));
+		BFMX_CHECK(dalloc(&C.d_val64, 9 * static_cast<size_t>(CS.nnzb_pad)));
+		BFMX_CHECK(dalloc(&C.d_val32, 9 * static_cast<size_t>(CS.nnzb_pad)));
+		BFMX_HIP(hipStreamSynchronize(st));   // the host sources above are temporaries
+		fptr.swap(cptr);
+		fadj.swap(cadj);
+		fslot.swap(CS.slot_of);
+		fslice.swap(CS.slice_ptr);
+		fseg.swap(cseg);
+	}
+	return BFMX_OK;
+}
+
+constexpr uint32_t kAggSalt = 0x2545F491u;   // seeds the aggregation's row priorities (per level)
+
+// The levels below level 0, built on the GPU from the level graphs' sliced
+// ELL (amg_topo.hip): the same arrays as build_levels_host, from MIS-2
+// aggregates. BFMX_E_UNSUPPORTED when a row exceeds the device build's
+// bounds (the caller then builds on the host).
+int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg) {
+	hipStream_t const st = s->stream;
+	std::vector<AmgLevel>& lv = s->amg;
+	AmgParams const& par = s->amg_par;
+	TopoGraph G;
+	G.n = static_cast<int>(s->n_nodes);
+	G.n_slices = s->n_slices;
+	G.nnzb_pad = s->nnzb_pad;
+	G.slice_ptr = s->d_slice_ptr;
+	G.col = s->d_col;
+	G.row_len = s->d_row_len;
+	BFMX_CHECK(topo_segments(st, G, fseg));
+	while (static_cast<int>(lv.size()) < kMaxLevels && seg_largest(fseg) > par.coarsest) {
+		int const lf = static_cast<int>(lv.size()) - 1;
+		int const nf = lv[lf].nb;
+		int na = 0;
+		std::vector<int32_t> cseg;
+		int32_t* agg = nullptr;
+		int const ra = topo_aggregate(st, G, kAggSalt + static_cast<uint32_t>(lf), par.seed_frac, par.seed_growth,
+		                              fseg, &agg, &na, cseg);
+		lv[lf].d_agg = agg;   // released with the hierarchy
+		BFMX_CHECK(ra);
+		if (!std::is_sorted(cseg.begin(), cseg.end()) || cseg[0] != 0 || cseg.back() != na) {
+			set_error("multigrid: aggregates do not follow the level's segments");
+			return BFMX_E_ARG;
+		}
+		// stalled when the segments still above the target size shrink by
+		// less than a fifth
+		int64_t big_f = 0, big_a = 0;
+		for (size_t c = 0; c + 1 < fseg.size(); c++) {
+			if (fseg[c + 1] - fseg[c] > par.coarsest) {
+				big_f += fseg[c + 1] - fseg[c];
+				big_a += cseg[c + 1] - cseg[c];
+			}
+		}
+		if (big_a * 10 > big_f * 8) {
+			dfree(lv[lf].d_agg);
+			break;   // coarsening stalled
+		}
+		TopoLevel T;
+		int const rt = topo_transfer(st, G, agg, na, smooth, T);
+		if (rt != BFMX_OK) {
+			topo_release(T);
+			return rt;
+		}
+		{
+			AmgLevel& F = lv[lf];
+			F.sa = smooth;
+			F.nnzp = T.nnzp;
+			F.d_agg_ptr = T.agg_ptr;
+			F.d_agg_nodes = T.agg_nodes;
+			F.d_pptr = T.pptr;
+			F.d_pcol = T.pcol;
+			F.d_ppos = T.ppos;
+			F.d_rptr = T.rptr;
+			F.d_rent = T.rent;
+			F.d_rrow = T.rrow;
+			F.d_gal_mirror = T.mirror;
+			F.nnz_ap = T.nnz_ap;
+			F.d_apptr = T.apptr;
+			F.d_apcol = T.apcol;
+		}
+		lv.emplace_back();   // its buffers are released with the hierarchy, also on failure
+		{
+			AmgLevel& C = lv.back();
+			C.nb = na;
+			C.bs = 3;
+			C.n_slices = T.c_slices;
+			C.nnzb = T.c_nnzb;
+			C.nnzb_pad = T.c_nnzb_pad;
+			C.d_slice_ptr = T.c_slice_ptr;
+			C.d_col = T.c_col;
+			C.d_diag_blk = T.c_diag_blk;
+			C.d_row_len = T.c_row_len;
+			BFMX_CHECK(dalloc(&C.d_val64, 9 * static_cast<size_t>(C.nnzb_pad)));
+			BFMX_CHECK(dalloc(&C.d_val32, 9 * static_cast<size_t>(C.nnzb_pad)));
+			G.n = na;
+			G.n_slices = C.n_slices;
+			G.nnzb_pad = C.nnzb_pad;
+			G.slice_ptr = C.d_slice_ptr;
+			G.col = C.d_col;
+			G.row_len = C.d_row_len;
+		}
+		AmgLevel& F = lv[lf];
+		BFMX_CHECK(dalloc(&F.d_Pt, static_cast<size_t>(F.bs) * 3 * nf));
+		BFMX_CHECK(dalloc(&F.d_P, static_cast<size_t>(F.bs) * 3 * F.nnzp));
+		BFMX_CHECK(dalloc(&F.d_R, static_cast<size_t>(F.bs) * 3 * F.nnzp));
+		BFMX_CHECK(dalloc(&F.d_apval, static_cast<size_t>(F.bs) * 3 * F.nnz_ap));
+		fseg.swap(cseg);
+	}
+	return BFMX_OK;
+}
+
+}  // namespace
+
 // Needs the system's FP32 matrix buffer (s->d_val32) allocated: level 0
-// reads it.
-int amg_build(bfmx_system* s) {
+// reads it. device: the levels' topology is built on the GPU (amg_topo.hip),
+// else on the host.
+int amg_build_impl(bfmx_system* s, bool device) {
 	if (!s->amg.empty()) {
 		return BFMX_OK;
 	}
@@ -1651,15 +2329,9 @@ int amg_build(bfmx_system* s) {
 		set_error("multigrid set-up before the FP32 matrix buffer");
 		return BFMX_E_ARG;
 	}
-	if (s->coords.size() != static_cast<size_t>(2 * s->n_nodes)) {
-		// the rigid-body modes need the coordinates; a system of a batch built
-		// from meshes keeps them on the device only
-		if (!s->d_coords) {
-			set_error("multigrid set-up needs the node coordinates");
-			return BFMX_E_ARG;
-		}
-		s->coords.resize(2 * s->n_nodes);
-		BFMX_HIP(hipMemcpy(s->coords.data(), s->d_coords, s->coords.size() * sizeof(double), hipMemcpyDeviceToHost));
+	if (!s->d_coords && s->coords.size() != static_cast<size_t>(2 * s->n_nodes)) {
+		set_error("multigrid set-up needs the node coordinates");
+		return BFMX_E_ARG;
 	}
 	int const rc = [&]() -> int {
 		hipStream_t const st = s->stream;
@@ -1674,47 +2346,48 @@ int amg_build(bfmx_system* s) {
 			L0.nnzb = s->nnzb;
 			L0.nnzb_pad = s->nnzb_pad;
 			L0.d_slice_ptr = s->d_slice_ptr;
+			L0.d_row_len = s->d_row_len;
 			L0.d_col = s->d_col;
 			L0.d_diag_blk = s->d_diag_blk;
 			L0.d_val64 = s->d_val;
 			L0.d_val32 = s->d_val32;
-			// rigid-body modes in coordinates centred on the mesh, scaled to its half-size
-			double lo[2] = {s->coords[0], s->coords[1]}, hi[2] = {s->coords[0], s->coords[1]};
-			for (int64_t i = 0; i < n0; i++) {
-				for (int a = 0; a < 2; a++) {
-					lo[a] = std::min(lo[a], s->coords[2 * i + a]);
-					hi[a] = std::max(hi[a], s->coords[2 * i + a]);
+			// rigid-body modes in coordinates centred on the mesh, scaled to its
+			// half-size: on the device from its coordinates, else (the union of a
+			// batch, whose coordinates are gathered on the host) on the host
+			if (s->d_coords) {
+				double* part = nullptr;
+				BFMX_CHECK(dalloc(&L0.d_B0, 6 * n0));
+				BFMX_CHECK(dalloc(&part, 4 * kBboxGrid));
+				k_bbox<<<kBboxGrid, kBlock, 0, st>>>(n0, s->d_coords, part);
+				k_modes0<<<grid(n0), kBlock, 0, st>>>(n0, part, s->d_coords, L0.d_B0);
+				hipError_t const e = hipStreamSynchronize(st);
+				dfree(part);
+				BFMX_HIP(e);
+				BFMX_HIP(hipGetLastError());
+			} else {
+				double lo[2] = {s->coords[0], s->coords[1]}, hi[2] = {s->coords[0], s->coords[1]};
+				for (int64_t i = 0; i < n0; i++) {
+					for (int a = 0; a < 2; a++) {
+						lo[a] = std::min(lo[a], s->coords[2 * i + a]);
+						hi[a] = std::max(hi[a], s->coords[2 * i + a]);
+					}
 				}
+				double const cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]);
+				double L = 0.5 * std::max(hi[0] - lo[0], hi[1] - lo[1]);
+				if (!(L > 0)) {
+					L = 1;
+				}
+				std::vector<double> B0(6 * n0);
+				for (int64_t i = 0; i < n0; i++) {
+					double const x = (s->coords[2 * i] - cx) / L, y = (s->coords[2 * i + 1] - cy) / L;
+					double const m[6] = {1, 0, -y, 0, 1, x};
+					std::copy(m, m + 6, &B0[6 * i]);
+				}
+				BFMX_CHECK(upload(&L0.d_B0, B0, st));
+				BFMX_HIP(hipStreamSynchronize(st));
 			}
-			double const cx = 0.5 * (lo[0] + hi[0]), cy = 0.5 * (lo[1] + hi[1]);
-			double L = 0.5 * std::max(hi[0] - lo[0], hi[1] - lo[1]);
-			if (!(L > 0)) {
-				L = 1;
-			}
-			std::vector<double> B0(6 * n0);
-			for (int64_t i = 0; i < n0; i++) {
-				double const x = (s->coords[2 * i] - cx) / L, y = (s->coords[2 * i + 1] - cy) / L;
-				double const m[6] = {1, 0, -y, 0, 1, x};
-				std::copy(m, m + 6, &B0[6 * i]);
-			}
-			BFMX_CHECK(upload(&L0.d_B0, B0, st));
-			BFMX_HIP(hipStreamSynchronize(st));
 		}
-		// graph of the current finest-so-far level, the block slot of each
-		// entry and the level's slice pointers
 		bool const smooth = amg_smoothing();
-		std::vector<int64_t> fptr;
-		std::vector<int32_t> fadj;
-		std::vector<int32_t> fslot(s->nnzb);
-		std::vector<int32_t> fslice(s->slice_ptr);
-		for (int64_t i = 0; i < n0; i++) {
-			for (int64_t k = 0; k < s->adj_ptr[i + 1] - s->adj_ptr[i]; k++) {
-				fslot[s->adj_ptr[i] + k] = static_cast<int32_t>(s->slice_ptr[i >> 6] + k * kSlice + (i & 63));
-			}
-		}
-		// coarsen to at most par.coarsest block rows (the one-workgroup
-		// dense inverse of the coarsest operator costs O(n^3) per set-up:
-		// ~0.1 ms at 192 DOFs, ~40 ms at 500)
 		AmgParams& par = s->amg_par;
 		par = AmgParams{};
 		if (const char* e = std::getenv("BFMX_AMG_DEG")) {
@@ -1735,6 +2408,12 @@ int amg_build(bfmx_system* s) {
 		if (const char* e = std::getenv("BFMX_AMG_OMEGA")) {
 			par.omega = std::atof(e);
 		}
+		if (const char* e = std::getenv("BFMX_AMG_SEED")) {
+			par.seed_frac = std::min(1.0, std::max(1e-6, std::atof(e)));
+		}
+		if (const char* e = std::getenv("BFMX_AMG_SEED_GROWTH")) {
+			par.seed_growth = std::max(1.01, std::atof(e));
+		}
 		if (const char* e = std::getenv("BFMX_AMG_POWER")) {
 			par.power = std::max(0, std::atoi(e));
 		}
@@ -1742,301 +2421,8 @@ int amg_build(bfmx_system* s) {
 		if (const char* e = std::getenv("BFMX_AMG_POWER_COARSE")) {
 			par.power_coarse = std::max(0, std::atoi(e));
 		}
-		// segments of the current level: aggregates never span two, and the
-		// first row of a segment is an aggregate root (its neighbours are all
-		// in the segment and not yet aggregated), so the aggregates of a
-		// segment are numbered consecutively and give the next level's segments
-		std::vector<int32_t> fseg = segments(static_cast<int>(n0), s->adj_ptr.data(), s->adj.data());
-		while (static_cast<int>(lv.size()) < kMaxLevels && seg_largest(fseg) > par.coarsest) {
-			int const lf = static_cast<int>(lv.size()) - 1;
-			const int64_t* ptr = lf == 0 ? s->adj_ptr.data() : fptr.data();
-			const int32_t* adj = lf == 0 ? s->adj.data() : fadj.data();
-			int const nf = lv[lf].nb;
-			// the rows split at segment boundaries into chunks of similar size,
-			// one thread each: the aggregates, prolongator rows, restriction
-			// columns and coarse rows of a chunk stay inside it, so the result
-			// does not depend on the split
-			std::vector<int32_t> const rb = chunk_rows(fseg, nf);
-			int const T = static_cast<int>(rb.size()) - 1;
-			std::atomic<int> bad{0};
-			std::vector<int32_t> agg(nf);
-			std::vector<int32_t> ab(T + 1, 0);   // first aggregate of each chunk
-			parallel_chunks(T, [&](int c) { ab[c + 1] = aggregate(rb[c], rb[c + 1], ptr, adj, agg.data()); });
-			for (int c = 0; c < T; c++) {
-				ab[c + 1] += ab[c];
-			}
-			int const na = ab[T];
-			parallel_chunks(T, [&](int c) {
-				for (int i = rb[c]; i < rb[c + 1]; i++) {
-					agg[i] += ab[c];
-				}
-			});
-			std::vector<int32_t> cseg(fseg.size());
-			for (size_t c = 0; c + 1 < fseg.size(); c++) {
-				cseg[c] = agg[fseg[c]];
-			}
-			cseg.back() = na;
-			if (!std::is_sorted(cseg.begin(), cseg.end()) || cseg[0] != 0) {
-				set_error("multigrid: aggregates do not follow the level's segments");
-				return BFMX_E_ARG;
-			}
-			// stalled when the segments still above the target size shrink by
-			// less than a fifth
-			int64_t big_f = 0, big_a = 0;
-			for (size_t c = 0; c + 1 < fseg.size(); c++) {
-				if (fseg[c + 1] - fseg[c] > par.coarsest) {
-					big_f += fseg[c + 1] - fseg[c];
-					big_a += cseg[c + 1] - cseg[c];
-				}
-			}
-			if (big_a * 10 > big_f * 8) {
-				break;   // coarsening stalled
-			}
-			// members of each aggregate (a chunk's members fill its own rows' range)
-			std::vector<int32_t> aptr(na + 1, 0), anodes(nf);
-			parallel_chunks(T, [&](int c) {
-				std::vector<int32_t> fill(ab[c + 1] - ab[c], 0);
-				for (int i = rb[c]; i < rb[c + 1]; i++) {
-					fill[agg[i] - ab[c]]++;
-				}
-				int32_t run = rb[c];
-				for (int I = ab[c]; I < ab[c + 1]; I++) {
-					int32_t const cnt = fill[I - ab[c]];
-					fill[I - ab[c]] = run;
-					run += cnt;
-					aptr[I + 1] = run;
-				}
-				for (int i = rb[c]; i < rb[c + 1]; i++) {
-					anodes[fill[agg[i] - ab[c]]++] = i;
-				}
-			});
-			// prolongator pattern: row i couples to the aggregates of its
-			// neighbours (smoothed) or to its own aggregate (unsmoothed)
-			std::vector<int32_t> pptr(nf + 1, 0);
-			std::vector<std::vector<int32_t>> pcol_c(T);
-			parallel_chunks(T, [&](int c) {
-				std::vector<int32_t> row;
-				std::vector<int32_t>& pc = pcol_c[c];
-				pc.reserve(static_cast<size_t>(smooth ? 3 : 1) * (rb[c + 1] - rb[c]));
-				for (int i = rb[c]; i < rb[c + 1]; i++) {
-					row.clear();
-					if (smooth) {
-						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
-							row.push_back(agg[adj[e]]);
-						}
-						std::sort(row.begin(), row.end());
-						row.erase(std::unique(row.begin(), row.end()), row.end());
-					} else {
-						row.push_back(agg[i]);
-					}
-					if (row.size() > 255) {
-						bad = 1;
-					}
-					pc.insert(pc.end(), row.begin(), row.end());
-					pptr[i + 1] = static_cast<int32_t>(pc.size());   // within the chunk, offset below
-				}
-			});
-			if (bad) {
-				set_error("multigrid: a row couples to more than 255 aggregates");
-				return BFMX_E_UNSUPPORTED;
-			}
-			std::vector<int64_t> pb(T + 1, 0);   // first entry of each chunk
-			for (int c = 0; c < T; c++) {
-				pb[c + 1] = pb[c] + static_cast<int64_t>(pcol_c[c].size());
-			}
-			if (pb[T] >= (int64_t(1) << 31) / 9) {
-				set_error("multigrid prolongator too large for 32-bit indices");
-				return BFMX_E_ARG;
-			}
-			std::vector<int32_t> pcol(pb[T]);
-			parallel_chunks(T, [&](int c) {
-				for (int i = rb[c]; i < rb[c + 1]; i++) {
-					pptr[i + 1] += static_cast<int32_t>(pb[c]);
-				}
-				std::copy(pcol_c[c].begin(), pcol_c[c].end(), pcol.begin() + pb[c]);
-				std::vector<int32_t>().swap(pcol_c[c]);
-			});
-			auto entry_of = [&](int i, int32_t J) -> int32_t {   // entry of (i, J) in P; J must be in row i
-				auto const beg = pcol.begin() + pptr[i], end = pcol.begin() + pptr[i + 1];
-				return static_cast<int32_t>(std::lower_bound(beg, end, J) - pcol.begin());
-			};
-			// per fine block slot, the entry (within its row) of its column's
-			// aggregate; padding slots and the unsmoothed case point at the
-			// row's own aggregate
-			std::vector<uint8_t> ppos(fslice.back(), 0);
-			parallel_chunks(T, [&](int c) {
-				for (int i = rb[c]; i < rb[c + 1]; i++) {
-					auto const own = static_cast<uint8_t>(entry_of(i, agg[i]) - pptr[i]);
-					for (int32_t slot = fslice[i >> 6] + (i & 63); slot < fslice[(i >> 6) + 1]; slot += kSlice) {
-						ppos[slot] = own;
-					}
-					if (smooth) {
-						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
-							ppos[fslot[e]] = static_cast<uint8_t>(entry_of(i, agg[adj[e]]) - pptr[i]);
-						}
-					}
-				}
-			});
-			// columns of P (restriction), entries in row order
-			std::vector<int32_t> rptr(na + 1, 0), rent(pcol.size()), rrow(pcol.size());
-			parallel_chunks(T, [&](int c) {
-				std::vector<int32_t> fill(ab[c + 1] - ab[c], 0);
-				for (int32_t e = static_cast<int32_t>(pb[c]); e < pb[c + 1]; e++) {
-					if (pcol[e] < ab[c] || pcol[e] >= ab[c + 1]) {
-						bad = 1;
-						return;
-					}
-					fill[pcol[e] - ab[c]]++;
-				}
-				int32_t run = static_cast<int32_t>(pb[c]);
-				for (int I = ab[c]; I < ab[c + 1]; I++) {
-					int32_t const cnt = fill[I - ab[c]];
-					fill[I - ab[c]] = run;
-					run += cnt;
-					rptr[I + 1] = run;
-				}
-				for (int i = rb[c]; i < rb[c + 1]; i++) {
-					for (int32_t e = pptr[i]; e < pptr[i + 1]; e++) {
-						int32_t const at = fill[pcol[e] - ab[c]]++;
-						rent[at] = e;
-						rrow[at] = i;
-					}
-				}
-			});
-			if (bad) {
-				set_error("multigrid: a prolongator row leaves its segment");
-				return BFMX_E_ARG;
-			}
-			// coarse graph, the pattern of P^T A P (sorted rows, with the diagonal)
-			std::vector<int64_t> cptr(na + 1, 0);
-			std::vector<std::vector<int32_t>> cadj_c(T);
-			parallel_chunks(T, [&](int c) {
-				std::vector<int32_t> row;
-				std::vector<int32_t>& ca = cadj_c[c];
-				for (int I = ab[c]; I < ab[c + 1]; I++) {
-					row.clear();
-					for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
-						int const i = rrow[t];
-						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
-							int const j = adj[e];
-							row.insert(row.end(), pcol.begin() + pptr[j], pcol.begin() + pptr[j + 1]);
-						}
-					}
-					std::sort(row.begin(), row.end());
-					row.erase(std::unique(row.begin(), row.end()), row.end());
-					ca.insert(ca.end(), row.begin(), row.end());
-					cptr[I + 1] = static_cast<int64_t>(ca.size());   // within the chunk, offset below
-				}
-			});
-			std::vector<int64_t> cb(T + 1, 0);
-			for (int c = 0; c < T; c++) {
-				cb[c + 1] = cb[c] + static_cast<int64_t>(cadj_c[c].size());
-			}
-			std::vector<int32_t> cadj(cb[T]);
-			parallel_chunks(T, [&](int c) {
-				for (int I = ab[c]; I < ab[c + 1]; I++) {
-					cptr[I + 1] += cb[c];
-				}
-				std::copy(cadj_c[c].begin(), cadj_c[c].end(), cadj.begin() + cb[c]);
-				std::vector<int32_t>().swap(cadj_c[c]);
-			});
-			Sell CS;
-			BFMX_CHECK(build_sell(na, cptr, cadj, CS));
-			// Galerkin gather lists: every (fine block A_ij, entry P_iI, entry
-			// P_jJ) grouped by the coarse slot (I, J), in fine row order (the
-			// slots of coarse row I belong to its chunk)
-			auto coarse_slot = [&](int32_t I, int32_t J) -> int32_t {
-				auto const beg = cadj.begin() + cptr[I], end = cadj.begin() + cptr[I + 1];
-				return CS.slot_of[std::lower_bound(beg, end, J) - cadj.begin()];
-			};
-			std::vector<int64_t> gptr(CS.nnzb_pad + 1, 0);
-			std::vector<int32_t> mirror(CS.nnzb_pad, -1);
-			parallel_chunks(T, [&](int c) {
-				for (int I = ab[c]; I < ab[c + 1]; I++) {
-					for (int64_t e = cptr[I]; e < cptr[I + 1]; e++) {
-						int32_t const J = cadj[e];
-						mirror[CS.slot_of[e]] = J < I ? -2 : coarse_slot(J, I);
-					}
-					for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
-						int const i = rrow[t];
-						for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
-							int const j = adj[e];
-							for (int32_t f = pptr[j]; f < pptr[j + 1]; f++) {
-								if (pcol[f] >= I) {
-									gptr[coarse_slot(I, pcol[f]) + 1]++;
-								}
-							}
-						}
-					}
-				}
-			});
-			for (int64_t b = 0; b < CS.nnzb_pad; b++) {
-				gptr[b + 1] += gptr[b];
-			}
-			int64_t const nterms = gptr[CS.nnzb_pad];
-			std::vector<int32_t> gblk(nterms), gpi(nterms), gpj(nterms);
-			{
-				std::vector<int64_t> fill(gptr.begin(), gptr.end() - 1);
-				parallel_chunks(T, [&](int c) {
-					for (int I = ab[c]; I < ab[c + 1]; I++) {
-						for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
-							int const i = rrow[t];
-							int32_t const pi = rent[t];
-							for (int64_t e = ptr[i]; e < ptr[i + 1]; e++) {
-								int const j = adj[e];
-								for (int32_t f = pptr[j]; f < pptr[j + 1]; f++) {
-									if (pcol[f] < I) {
-										continue;
-									}
-									int64_t const at = fill[coarse_slot(I, pcol[f])]++;
-									gblk[at] = fslot[e];
-									gpi[at] = pi;
-									gpj[at] = f;
-								}
-							}
-						}
-					}
-				});
-			}
-			AmgLevel& F = lv[lf];
-			F.sa = smooth;
-			F.nnzp = static_cast<int64_t>(pcol.size());
-			BFMX_CHECK(upload(&F.d_agg, agg, st));
-			BFMX_CHECK(dalloc(&F.d_Pt, static_cast<size_t>(F.bs) * 3 * nf));
-			BFMX_CHECK(upload(&F.d_agg_ptr, aptr, st));
-			BFMX_CHECK(upload(&F.d_agg_nodes, anodes, st));
-			BFMX_CHECK(upload(&F.d_pptr, pptr, st));
-			BFMX_CHECK(upload(&F.d_pcol, pcol, st));
-			BFMX_CHECK(dalloc(&F.d_P, static_cast<size_t>(F.bs) * 3 * pcol.size()));
-			BFMX_CHECK(upload(&F.d_ppos, ppos, st));
-			BFMX_CHECK(upload(&F.d_rptr, rptr, st));
-			BFMX_CHECK(upload(&F.d_rent, rent, st));
-			BFMX_CHECK(dalloc(&F.d_R, static_cast<size_t>(F.bs) * 3 * pcol.size()));
-			BFMX_CHECK(upload(&F.d_rrow, rrow, st));
-			BFMX_CHECK(upload(&F.d_gal_ptr, gptr, st));
-			BFMX_CHECK(upload(&F.d_gal_mirror, mirror, st));
-			BFMX_CHECK(upload(&F.d_gal_blk, gblk, st));
-			BFMX_CHECK(upload(&F.d_gal_pi, gpi, st));
-			BFMX_CHECK(upload(&F.d_gal_pj, gpj, st));
-			lv.emplace_back();   // its buffers are released with the hierarchy, also on failure
-			AmgLevel& C = lv.back();
-			C.nb = na;
-			C.bs = 3;
-			C.n_slices = CS.n_slices;
-			C.nnzb = CS.nnzb;
-			C.nnzb_pad = CS.nnzb_pad;
-			BFMX_CHECK(upload(&C.d_slice_ptr, CS.slice_ptr, st));
-			BFMX_CHECK(upload(&C.d_col, CS.col, st));
-			BFMX_CHECK(upload(&C.d_diag_blk, CS.diag_blk, st));
-			BFMX_CHECK(dalloc(&C.d_val64, 9 * static_cast<size_t>(CS.nnzb_pad)));
-			BFMX_CHECK(dalloc(&C.d_val32, 9 * static_cast<size_t>(CS.nnzb_pad)));
-			BFMX_HIP(hipStreamSynchronize(st));   // the host sources above are temporaries
-			fptr.swap(cptr);
-			fadj.swap(cadj);
-			fslot.swap(CS.slot_of);
-			fslice.swap(CS.slice_ptr);
-			fseg.swap(cseg);
-		}
+		std::vector<int32_t> fseg;
+		BFMX_CHECK(device ? build_levels_device(s, smooth, fseg) : build_levels_host(s, smooth, fseg));
 		for (auto& L : lv) {
 			size_t const m = static_cast<size_t>(L.bs) * L.nb;
 			BFMX_CHECK(dalloc(&L.d_b, m));
@@ -2167,6 +2553,22 @@ int amg_build(bfmx_system* s) {
 	return rc;
 }
 
+// The hierarchy's topology is built on the GPU unless the environment sets
+// BFMX_AMG_BUILD=host, or a level exceeds the device build's bounds (then on
+// the host, from the node adjacency).
+int amg_build(bfmx_system* s) {
+	if (!s->amg.empty()) {
+		return BFMX_OK;
+	}
+	const char* e = std::getenv("BFMX_AMG_BUILD");
+	bool const device = !(e && std::string(e) == "host") && s->d_row_len;
+	int rc = amg_build_impl(s, device);
+	if (rc == BFMX_E_UNSUPPORTED && device) {
+		rc = amg_build_impl(s, false);
+	}
+	return rc;
+}
+
 // Modes, prolongators, Galerkin operators, smoother data and the coarsest
 // inverse, from the current FP64 matrix and Dirichlet set, level by level:
 // a level's smoother data (inverse diagonal blocks, lambda_max) come first,
@@ -2239,20 +2641,36 @@ int amg_setup_numeric(bfmx_system* s) {
 			                                    L.d_Pt, L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0,
 			                                    L.d_P);
 			k_transpose_p<2><<<grid(L.nnzp), kBlock, 0, st>>>(L.nnzp, L.d_rent, L.d_P, L.d_R);
-			k_galerkin<2, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
-			                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
-			                                                               L.d_val32, L.d_P, C.d_val64,
-			                                                               C.d_val32);
+			if (L.d_apptr) {
+				k_ap<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_row_len, L.d_val32, L.d_pptr,
+				                              L.d_pcol, L.d_P, L.d_apptr, L.d_apcol, L.d_apval);
+				k_rap<2><<<grid(C.n_slices * kSlice * 64), kBlock, 0, st>>>(
+				    C.n_slices * kSlice, C.nb, C.d_slice_ptr, C.d_col, C.d_row_len, C.d_diag_blk, L.d_gal_mirror,
+				    L.d_rptr, L.d_rrow, L.d_R, L.d_apptr, L.d_apcol, L.d_apval, C.d_val64, C.d_val32);
+			} else {
+				k_galerkin<2, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
+				                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
+				                                                               L.d_val32, L.d_P, C.d_val64,
+				                                                               C.d_val32);
+			}
 		} else {
 			k_tentative<3><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
 			k_smooth_p<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_val32, L.d_diag_blk,
 			                                    L.d_Pt, L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0,
 			                                    L.d_P);
 			k_transpose_p<3><<<grid(L.nnzp), kBlock, 0, st>>>(L.nnzp, L.d_rent, L.d_P, L.d_R);
-			k_galerkin<3, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
-			                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
-			                                                               L.d_val32, L.d_P, C.d_val64,
-			                                                               C.d_val32);
+			if (L.d_apptr) {
+				k_ap<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_row_len, L.d_val32, L.d_pptr,
+				                              L.d_pcol, L.d_P, L.d_apptr, L.d_apcol, L.d_apval);
+				k_rap<3><<<grid(C.n_slices * kSlice * 64), kBlock, 0, st>>>(
+				    C.n_slices * kSlice, C.nb, C.d_slice_ptr, C.d_col, C.d_row_len, C.d_diag_blk, L.d_gal_mirror,
+				    L.d_rptr, L.d_rrow, L.d_R, L.d_apptr, L.d_apcol, L.d_apval, C.d_val64, C.d_val32);
+			} else {
+				k_galerkin<3, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
+				                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
+				                                                               L.d_val32, L.d_P, C.d_val64,
+				                                                               C.d_val32);
+			}
 		}
 	}
 	BFMX_HIP(hipGetLastError());

@@ -130,6 +130,7 @@ struct AmgLevel {
 	int bs = 2;                 // DOFs per block row
 	int64_t n_slices = 0, nnzb = 0, nnzb_pad = 0;
 	int32_t* d_slice_ptr = nullptr;   // level 0: the system's arrays (not owned)
+	int32_t* d_row_len = nullptr;     // coarse levels built on the device: stored blocks per row
 	int32_t* d_col = nullptr;
 	int32_t* d_diag_blk = nullptr;
 	double* d_val64 = nullptr;        // level 0: the system's FP64 values (not owned)
@@ -157,10 +158,17 @@ struct AmgLevel {
 	float* d_R = nullptr;             // those entries' values in that order (R = P^T read row by row)
 	int64_t* d_gal_ptr = nullptr;     // per next-level block slot: range of contributions (slots (I, J >= I))
 	int32_t* d_gal_mirror = nullptr;  // per next-level block slot: the slot of its transpose (-2: filled by
-	                                  // it, -1: padding)
+	                                  // it, -1: padding); both build paths
 	int32_t* d_gal_blk = nullptr;     // this level's block slot
 	int32_t* d_gal_pi = nullptr;      // entry of P in the slot's row
 	int32_t* d_gal_pj = nullptr;      // entry of P in the slot's column
+	// levels built on the device form the Galerkin product in two steps
+	// instead of from contribution lists: A P as CSR of bs x 3 FP32 blocks
+	// over this level's rows (absent for levels built on the host)
+	int64_t nnz_ap = 0;
+	int32_t* d_apptr = nullptr;
+	int32_t* d_apcol = nullptr;
+	float* d_apval = nullptr;
 	// segments: row ranges that no entry of the level couples (the connected
 	// parts of a mesh numbered one after another; a single mesh has one).
 	// The coarsest level, when small enough, holds the dense inverse of each
@@ -221,6 +229,8 @@ struct AmgParams {
 	double omega = 1.0;     // prolongator damping: omega * 4 / (3 lambda_max)
 	int power = 10;         // power-iteration steps estimating lambda_max(D^-1 A); 0: Gershgorin bound
 	int power_coarse = 10;  // ... on the coarser levels
+	double seed_frac = 0.002;   // device aggregation: rows that may seed a region in the first round
+	double seed_growth = 1.25;  // ... and that fraction's growth per round
 };
 
 struct Graph {

@@ -136,9 +136,22 @@ def test_disconnected_mesh_one_dense_inverse_per_part():
 
 
 def test_multigrid_set_up_threads_give_the_same_hierarchy(monkeypatch):
-    """The host build of the transfer operators runs one thread per chunk of
-    whole segments; one thread and sixteen give bitwise the same solve."""
+    """The host build of the transfer operators (BFMX_AMG_BUILD=host) runs one
+    thread per chunk of whole segments; one thread and sixteen give bitwise
+    the same solve. The device build of the same batch (the default) also
+    meets the tolerance on every panel, and twice gives bitwise the same
+    solve."""
     meshes = [meshgen.panel(500 + k, 1000, 4000) for k in range(20)]
+    dev = []
+    for _ in range(2):
+        pb = D.Batch.from_instances([D.panel_instance(m, t) for m, t in meshes])
+        st, _ = pb.solve(precond=D.PRECOND_AMG, rtol=1e-8)
+        assert all(x["status"] == 0 for x in st)
+        dev.append((st[0]["iterations"], [pb.get_u(k) for k in range(len(meshes))]))
+        pb.close()
+    assert dev[0][0] == dev[1][0]
+    assert all(np.array_equal(a, b) for a, b in zip(dev[0][1], dev[1][1]))
+    monkeypatch.setenv("BFMX_AMG_BUILD", "host")
     out = []
     for threads in ("1", "16"):
         monkeypatch.setenv("BFMX_AMG_THREADS", threads)

@@ -265,18 +265,33 @@ def test_sim_solves_several_instances_in_one_batch(L, precond):
     system) within 1e-8."""
     meshes = [meshgen.beam(40, 10, kind=3), meshgen.beam(30, 8, kind=4, jitter=0.2, seed=1),
               meshgen.beam(60, 15, kind=3, jitter=0.2, seed=2), meshgen.beam(12, 3, kind=4)]
-    solver = F.bfm_solver_t(precond, 0, 1e-11, 0)
+    solver = F.bfm_solver_t(precond, 0, 1e-10, 0)
     sim, insts = beam_sim(L, meshes, solver)
     sim.run()
     for k, mesh in enumerate(meshes):
-        solo, (one,) = beam_sim(L, [mesh], F.bfm_solver_t(1, 0, 1e-11, 0))
+        solo, (one,) = beam_sim(L, [mesh], F.bfm_solver_t(1, 0, 1e-10, 0))
         solo.run()
         st = F.bfm_solve_stats_t()
         assert L.bfm_sim_get_stats(C.byref(sim.c_sim), k, C.byref(st)) == 0
         err = rel_err(insts[k].effects, one.effects)
         print(f"precond {precond}, instance {k} ({mesh.n_dof} DOFs): {st.iterations} iterations, "
               f"residual {st.rel_residual:.2e}, error vs solo {err:.2e}")
-        assert st.rel_residual <= 1e-11 and err <= U_TOL
+        assert st.status == 0 and st.rel_residual <= 1e-10 and err <= U_TOL
+
+
+def test_sim_batch_below_the_rounding_floor(L):
+    """A tolerance below what FP64 can reach on some instances (the 902-DOF
+    beam's floor is ~2e-11): the batch leaves them above rtol, each is solved
+    again on its own, and the run is accepted at the floor (status 1), as a
+    single-instance run is."""
+    meshes = [meshgen.beam(40, 10, kind=3), meshgen.beam(12, 3, kind=4)]
+    sim, insts = beam_sim(L, meshes, F.bfm_solver_t(1, 0, 1e-14, 20000))
+    sim.run()
+    for k in range(len(meshes)):
+        st = F.bfm_solve_stats_t()
+        assert L.bfm_sim_get_stats(C.byref(sim.c_sim), k, C.byref(st)) == 0
+        print(f"instance {k}: status {st.status}, residual {st.rel_residual:.2e}")
+        assert st.status in (0, 1) and st.rel_residual <= 1e-9
 
 
 def test_sim_learned_solver(L, golden):

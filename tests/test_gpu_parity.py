@@ -383,6 +383,34 @@ def test_amg_smoothed_against_unsmoothed_aggregation(monkeypatch):
     assert st_sa["iterations"] * 2 <= st_ua["iterations"]
 
 
+@pytest.mark.parametrize("kind", [4, 3])
+def test_amg_device_and_host_hierarchies(monkeypatch, kind):
+    """The hierarchy topology built on the GPU (the default: MIS-2
+    aggregates, the two-step Galerkin product) and on the host
+    (BFMX_AMG_BUILD=host: sequential aggregation, contribution lists) both
+    lead flexible CG to the oracle's solution; two fresh systems built on the
+    GPU give bitwise the same solution (the device build does not depend on
+    scheduling)."""
+    import scipy.sparse.linalg as spla
+    mesh = meshgen.beam(240, 60, kind=kind, jitter=0.2, seed=13)
+    mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=13)
+    A, b = oracle_csr(mesh, True)
+    u_ref = spla.spsolve(A.to_scipy().tocsc(), b)
+    dev1 = D.beam_system(mesh)
+    u1, st1 = dev1.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+    dev1.close()
+    dev2 = D.beam_system(mesh)
+    u2, st2 = dev2.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+    monkeypatch.setenv("BFMX_AMG_BUILD", "host")
+    host = D.beam_system(mesh)
+    u3, st3 = host.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+    print(f"kind {kind}: device build {st1['iterations']} iterations (error {rel_err(u1, u_ref):.2e}), "
+          f"host build {st3['iterations']} (error {rel_err(u3, u_ref):.2e})")
+    assert st1["status"] == 0 and st3["status"] == 0
+    assert st1["iterations"] == st2["iterations"] and np.array_equal(u1, u2)
+    assert rel_err(u1, u_ref) <= U_TOL and rel_err(u3, u_ref) <= U_TOL
+
+
 def test_amg_single_level_is_a_direct_solve():
     """A mesh small enough for one level: the preconditioner is the dense
     FP32 inverse, so FCG stops after very few iterations."""
