@@ -241,6 +241,16 @@ int bfmx_system_solve(bfmx_system_t* s, const double* b_in, double* u_out, int p
 	});
 }
 
+int bfmx_system_solve_noalloc(bfmx_system_t* s, const double* b_in, double* u_out, int precond, double rtol,
+                              size_t max_iter, bfmx_stats_t* stats) {
+	BFMX_CHECK(check(s));
+	if (!s->assembled) {
+		set_error("solve before assemble");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return solve_noalloc(s, b_in, u_out, precond, rtol, max_iter, stats); });
+}
+
 int bfmx_system_info(bfmx_system_t* s, size_t* n_nodes, size_t* nnzb, size_t* nnzb_padded, size_t* n_colors) {
 	if (!s) {
 		set_error("null system");

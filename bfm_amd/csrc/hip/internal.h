@@ -482,6 +482,23 @@ int time_kernel(bfmx_system* s, int kernel, int reps, double* avg_ms, double* by
 // amg.hip: reps launches of one finest-level smoothing step (set-up done)
 int amg_time_smoother(bfmx_system* s, int reps, hipEvent_t e0, hipEvent_t e1, double* bytes);
 void release_solver(bfmx_system* s);
+// Solver buffers of a PCG run held by the caller (2 n doubles each for
+// x, r, z, q, p0, p1, bsolve; 4 n for dinv; partial_cap partial sums).
+struct PcgBuffers {
+	double *x, *r, *z, *q, *p0, *p1, *dinv, *bsolve, *partial;
+	int64_t partial_cap;
+	Scalars* scal;
+};
+// Point or block Jacobi PCG (FP64) from u = 0 for the right-hand side in
+// w.bsolve, in the buffers w, with kernel launches only (no allocation, no
+// graph capture); *out receives the final scalars, the solution stays in
+// w.x (internal node order). The system's own solver buffers are untouched.
+int pcg_in_buffers(bfmx_system* s, PcgBuffers const& w, int precond, double rtol, int64_t cap, Scalars* out);
+// noalloc.hip: bfmx_system_solve_noalloc, and the largest system (nodes) its
+// fixed scratch array holds.
+int solve_noalloc(bfmx_system* s, const double* b_in, double* u_out, int precond, double rtol, size_t max_iter,
+                  bfmx_stats_t* stats);
+int64_t noalloc_max_nodes();
 // Destroys the system's captured chunks of one preconditioner (all with -1),
 // after the system's stream has drained.
 void drop_graphs(bfmx_system* s, int precond);

@@ -1502,6 +1502,88 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	return status;
 }
 
+namespace {
+
+__global__ __launch_bounds__(kBlock) void k_zero(int64_t n, double* __restrict__ p) {
+	int64_t const i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+	if (i < n) {
+		p[i] = 0;
+	}
+}
+
+}  // namespace
+
+int pcg_in_buffers(bfmx_system* s, PcgBuffers const& w, int precond, double rtol, int64_t cap, Scalars* out) {
+	if (precond != BFMX_PRECOND_JACOBI && precond != BFMX_PRECOND_BLOCK_JACOBI) {
+		set_error("this solve runs point or block Jacobi PCG only (precond %d)", precond);
+		return BFMX_E_UNSUPPORTED;
+	}
+	if (s->n_conv_seg > 0) {
+		set_error("this solve does not take systems with convergence segments");
+		return BFMX_E_UNSUPPORTED;
+	}
+	int64_t const n = s->n_nodes;
+	hipStream_t const st = s->stream;
+	// the launch helpers read the system's solver buffers: point them at the
+	// caller's for the duration of the call
+	struct Saved {
+		bfmx_system* s;
+		double *x, *r, *z, *q, *p0, *p1, *dinv, *partial, *bsolve;
+		int64_t cap;
+		Scalars* scal;
+		~Saved() {
+			s->d_x = x;
+			s->d_r = r;
+			s->d_z = z;
+			s->d_q = q;
+			s->d_p[0] = p0;
+			s->d_p[1] = p1;
+			s->d_dinv = dinv;
+			s->d_partial = partial;
+			s->d_bsolve = bsolve;
+			s->partial_cap = cap;
+			s->d_scal = scal;
+		}
+	} const saved{s, s->d_x, s->d_r, s->d_z, s->d_q, s->d_p[0], s->d_p[1], s->d_dinv, s->d_partial, s->d_bsolve,
+	              s->partial_cap, s->d_scal};
+	s->d_x = w.x;
+	s->d_r = w.r;
+	s->d_z = w.z;
+	s->d_q = w.q;
+	s->d_p[0] = w.p0;
+	s->d_p[1] = w.p1;
+	s->d_dinv = w.dinv;
+	s->d_partial = w.partial;
+	s->d_bsolve = w.bsolve;
+	s->partial_cap = w.partial_cap;
+	s->d_scal = w.scal;
+	double const rtol2 = rtol * rtol;
+	k_precond<<<grid(n), kBlock, 0, st>>>(static_cast<int>(n), precond, s->d_diag_blk, s->d_val, s->d_dinv);
+	for (double* v : {w.x, w.q, w.p0, w.p1}) {
+		k_zero<<<grid(2 * n), kBlock, 0, st>>>(2 * n, v);
+	}
+	static_assert(sizeof(Scalars) % sizeof(double) == 0, "Scalars holds whole doubles");
+	k_zero<<<1, kBlock, 0, st>>>(sizeof(Scalars) / sizeof(double), reinterpret_cast<double*>(w.scal));
+	launch_residual(s, precond, 0, rtol2, 0.0);
+	BFMX_HIP(hipGetLastError());
+	Scalars last{};
+	BFMX_HIP(hipMemcpyAsync(&last, w.scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	// chunks of kChunk iterations and a residual check, launched directly (no
+	// graph: capturing one allocates), each inspected before the next
+	for (int64_t launched = 0; last.done == 0 && launched < cap; launched += kChunk) {
+		for (int it = 0; it < kChunk; it++) {
+			launch_iteration(s, precond, false, it);
+		}
+		launch_residual(s, precond, 1, rtol2, 0.0);
+		BFMX_HIP(hipGetLastError());
+		BFMX_HIP(hipMemcpyAsync(&last, w.scal, sizeof(Scalars), hipMemcpyDeviceToHost, st));
+		BFMX_HIP(hipStreamSynchronize(st));
+	}
+	*out = last;
+	return BFMX_OK;
+}
+
 int record_pcg(bfmx_system* s, int P, int every, const std::function<int(int)>& rec) {
 	int64_t const n = s->n_nodes;
 	hipStream_t const st = s->stream;

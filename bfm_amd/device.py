@@ -26,7 +26,7 @@ ERRORS = {-1: "BFMX_E_ARG", -2: "BFMX_E_HIP", -3: "BFMX_E_NOMEM", -4: "BFMX_E_BR
 EXPORTS = [
     "bfmx_last_error", "bfmx_device_count", "bfmx_system_create", "bfmx_system_destroy",
     "bfmx_system_set_edges", "bfmx_system_assemble", "bfmx_system_apply_condition",
-    "bfmx_system_solve", "bfmx_system_info", "bfmx_system_node_perm", "bfmx_system_get_bsr",
+    "bfmx_system_solve", "bfmx_system_solve_noalloc", "bfmx_system_info", "bfmx_system_node_perm", "bfmx_system_get_bsr",
     "bfmx_system_get_rhs", "bfmx_system_get_u", "bfmx_system_get_entry", "bfmx_system_time_spmv",
     "bfmx_system_time_kernel", "bfmx_model_n_params", "bfmx_model_create", "bfmx_model_destroy",
     "bfmx_model_get_params", "bfmx_model_set_params", "bfmx_trainset_create", "bfmx_trainset_destroy",
@@ -98,6 +98,7 @@ def lib():
         L.bfmx_system_apply_condition.argtypes = [vp, C.c_int, C.c_double, _u8]
         L.bfmx_system_solve.argtypes = [vp, _Opt(_dp), _Opt(_dp), C.c_int, C.c_int, C.c_double, sz,
                                         C.POINTER(Stats)]
+        L.bfmx_system_solve_noalloc.argtypes = [vp, _Opt(_dp), _Opt(_dp), C.c_int, C.c_double, sz, C.POINTER(Stats)]
         L.bfmx_system_info.argtypes = [vp, C.POINTER(sz), C.POINTER(sz), C.POINTER(sz), C.POINTER(sz)]
         L.bfmx_system_node_perm.argtypes = [vp, _i64]
         L.bfmx_system_get_bsr.argtypes = [vp, _Opt(_i64), _Opt(_i32), _Opt(_dp)]
@@ -199,6 +200,15 @@ class System:
         bb = None if b is None else np.ascontiguousarray(b, np.float64)
         rc = lib().bfmx_system_solve(self.h, bb, u, precond, precision, rtol, max_iter, C.byref(st))
         _check(rc)
+        return u, st.as_dict()
+
+    def solve_noalloc(self, b=None, precond=PRECOND_JACOBI, rtol=1e-8, max_iter=0):
+        """bfmx_system_solve_noalloc: point / block Jacobi PCG in the library's
+        fixed scratch array (no device allocation)."""
+        u = np.empty(2 * self.n_nodes)
+        st = Stats()
+        bb = None if b is None else np.ascontiguousarray(b, np.float64)
+        _check(lib().bfmx_system_solve_noalloc(self.h, bb, u, precond, rtol, max_iter, C.byref(st)))
         return u, st.as_dict()
 
     def info(self):

@@ -163,6 +163,21 @@ int bfmx_system_apply_condition(bfmx_system_t* s, int kind, double value, const 
 int bfmx_system_solve(bfmx_system_t* s, const double* b_in, double* u_out, int precond,
                       int precision, double rtol, size_t max_iter, bfmx_stats_t* stats);
 
+/* bfmx_system_solve without allocating: no reference counterpart; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). Point or block
+ * Jacobi PCG in FP64 (precond BFMX_PRECOND_JACOBI or BFMX_PRECOND_BLOCK_JACOBI)
+ * from u = 0, with the same kernels as bfmx_system_solve, launched directly
+ * (no graph). Its solver buffers live in one fixed 64 MiB device array per
+ * GPU, held until the process ends, so it takes systems of at most 408,880
+ * nodes (larger ones are refused with BFMX_E_ARG and a message stating the
+ * limit); calls that overlap on one GPU share that array and race. b_in,
+ * u_out and stats as in bfmx_system_solve; the solution is not kept for
+ * bfmx_system_get_u, and the system's own solver buffers are left as they
+ * were. Pending boundary conditions are finalized first (host memory only). */
+int bfmx_system_solve_noalloc(bfmx_system_t* s, const double* b_in, double* u_out, int precond, double rtol,
+                              size_t max_iter, bfmx_stats_t* stats);
+
 /* Sizes: nodes, stored 2x2 blocks, blocks including the sliced-ELL padding,
  * element colours. Any pointer may be NULL. */
 int bfmx_system_info(bfmx_system_t* s, size_t* n_nodes, size_t* nnzb, size_t* nnzb_padded,
