@@ -267,6 +267,46 @@ __device__ __forceinline__ dbl2 row_spmv(const int32_t* __restrict__ slice_ptr, 
 	return dbl2{y0, y1};
 }
 
+// s - a x accumulated without losing its rounding errors: s carries the
+// running sum and c the sum of the errors, so that s + c is the row's
+// b - sum A_ij x_j as accurately as if it were summed in twice the working
+// precision and then rounded (the product's error exactly from an FMA,
+// Knuth's TwoSum for the difference; Ogita, Rump and Oishi, SIAM J. Sci.
+// Comput. 26 (2005) 1955-1988, Algorithm Dot2). Contraction is off here: a
+// fused s - a x would not be the rounded difference the error terms assume.
+__device__ __forceinline__ void sub_exact(double& s, double& c, double a, double x) {
+#pragma clang fp contract(off)
+	double const p = a * x;
+	double const pe = fma(a, x, -p);   // a x = p + pe exactly
+	double const t = s - p;
+	double const z = t - s;
+	double const e = (s - (t - z)) + (-p - z);   // s - p = t + e exactly
+	s = t;
+	c += e - pe;
+}
+
+// b - A x for one block row, compensated (sub_exact).
+template <typename MT>
+__device__ __forceinline__ dbl2 row_residual(const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ col,
+                                             const MT* __restrict__ val, int row, dbl2 bv,
+                                             const double* __restrict__ x) {
+	int const sl = row >> 6, r = row & 63;
+	int32_t const beg = slice_ptr[sl], end = slice_ptr[sl + 1];
+	const MT* v = val + 4 * static_cast<int64_t>(beg) + 2 * r;
+	double s0 = bv.x, s1 = bv.y, c0 = 0, c1 = 0;
+#pragma unroll kSpmvUnroll
+	for (int32_t blk = beg + r; blk < end; blk += kSlice, v += 4 * kSlice) {
+		int32_t const c = __builtin_nontemporal_load(col + blk);
+		dbl2 const t = ldv(v), u = ldv(v + 2 * kSlice);
+		dbl2 const xv = ld2(x + 2 * c);
+		sub_exact(s0, c0, t.x, xv.x);
+		sub_exact(s0, c0, t.y, xv.y);
+		sub_exact(s1, c1, u.x, xv.x);
+		sub_exact(s1, c1, u.y, xv.y);
+	}
+	return dbl2{s0 + c0, s1 + c1};
+}
+
 // p_new = z + beta p_old, q = A p_new, partial p_new . q.
 template <typename MT>
 __global__ __launch_bounds__(kBlock) void k_cg_spmv(int nb, const int32_t* __restrict__ slice_ptr,
@@ -532,8 +572,11 @@ __device__ __forceinline__ bool residual_due(const Scalars* S, double delta2) {
 	return (S->done == 0 || S->done == kHold) && (S->rr <= S->tol2 || S->rr <= delta2 * S->rr_ref || S->checking);
 }
 
-// r = b - A x with the FP64 matrix, z = M^-1 r; partials r.z, r.r, b.b.
-// mode 0 starts a run (directions zeroed); mode 1 runs when residual_due.
+// r = b - A x with the FP64 matrix, compensated (row_residual: the true
+// residual of the FP64 iterate, free of the rounding of its own evaluation,
+// which on a strongly heterogeneous matrix is of the order of 1e-8 of ||b||),
+// z = M^-1 r; partials r.z, r.r, b.b. mode 0 starts a run (directions
+// zeroed); mode 1 runs when residual_due.
 template <int PRE>
 __global__ __launch_bounds__(kBlock) void k_residual(int nb, int mode, double delta2,
                                                      const int32_t* __restrict__ slice_ptr,
@@ -549,9 +592,8 @@ __global__ __launch_bounds__(kBlock) void k_residual(int nb, int mode, double de
 	int const row = xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
 	double v[3] = {0, 0, 0};
 	if (row < nb) {
-		dbl2 const ax = row_spmv(slice_ptr, col, val, row, [&](int c) { return ld2(x + 2 * c); });
 		dbl2 const bv = ld2(b + 2 * row);
-		dbl2 const rv = bv - ax;
+		dbl2 const rv = row_residual(slice_ptr, col, val, row, bv, x);
 		dbl2 const zv = apply_precond<PRE>(dinv, row, rv);
 		st2(r + 2 * row, rv);
 		st2(z + 2 * row, zv);
