@@ -99,6 +99,36 @@ def cpu_baseline_reference():
     }
 
 
+def cpu_pcg_sample(mesh, stress, iterations_needed, iters=300):
+    """The sparse FP64 Jacobi-PCG of the oracle (CSR, OpenMP) on the full
+    workload's own system, timed for a bounded number of iterations on all
+    host threads (at most 16) and on one, then scaled to the iterations the
+    same algorithm needs to reach the tolerance on the GPU."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from bfm_amd import meshgen
+    mat = meshgen.BEAM_MATERIAL
+    kinds, values, flags = meshgen.beam_conditions(mesh)
+    A, b = O.assemble_csr(mesh, stress, mat["E"], mat["nu"], mat["rho"], kinds=kinds, values=values, flags=flags)
+    out = {}
+    for threads, n_it in ((min(16, os.cpu_count() or 1), iters), (1, max(20, iters // 6))):
+        t0 = time.perf_counter()
+        _, it, _ = O.pcg_csr(A, b, rtol=1e-300, max_iter=n_it, threads=threads)
+        dt = time.perf_counter() - t0
+        per = dt / max(1, it)
+        out[threads] = {"value": mesh.n_dof / (per * iterations_needed), "s_per_iteration": per, "timed_iterations": it}
+    all_t = max(out)
+    return {
+        "value": out[all_t]["value"], "unit": "DOF/s", "cores": all_t, "kind": "port",
+        "sample": (f"the oracle's sparse FP64 CSR Jacobi-PCG (OpenMP) on the full {mesh.n_dof}-DOF system: "
+                   f"{out[all_t]['timed_iterations']} iterations timed on {all_t} threads "
+                   f"({out[all_t]['s_per_iteration'] * 1e3:.1f} ms each) and {out[1]['timed_iterations']} on one "
+                   f"({out[1]['s_per_iteration'] * 1e3:.1f} ms each), scaled to the {iterations_needed} iterations "
+                   f"Jacobi PCG needs for rtol 1e-8"),
+        "single_core_value": out[1]["value"],
+    }
+
+
 def copy_calibration(nbytes):
     """One hipMemcpy of nbytes between two device buffers (the runtime's copy
     kernel reads and writes nbytes): the reference that scales the
@@ -188,6 +218,104 @@ def batched_leg(D, meshgen, count, rtol):
     }
 
 
+def timed_solves(D, s, name, precond, rtol, warmup, steps, accept_floor=False):
+    """warmup untimed then steps timed solves of s: (mean ms, last stats,
+    reached rtol). accept_floor: a stop at the rounding floor (the solver's
+    stagnation status) is timed and reported instead of raised."""
+    times, st = [], None
+    for k in range(warmup + steps):
+        t = time.perf_counter()
+        try:
+            _, st = s.solve(precond=precond, rtol=rtol, want_u=False)
+        except D.BfmxError as e:
+            if not (accept_floor and e.code == -8):
+                raise
+            st = {"iterations": -1, "rel_residual": float(str(e).split("residual ")[1].split()[0]),
+                  "fallback": 0, "status": e.code}
+        dt = (time.perf_counter() - t) * 1e3
+        if k >= warmup:
+            times.append(dt)
+        log(f"{name} {'warmup' if k < warmup else 'step'}: {dt:.1f} ms, {st['iterations']} iterations, "
+            f"rel residual {st['rel_residual']:.3e}")
+    return float(np.mean(times)), st, st["status"] == 0
+
+
+def heterogeneous_leg(D, meshgen, args):
+    """The mixed-precision workload: the 2M-DOF beam with E per element
+    log-uniform over [1e9, 1e11] (seed 0), solved to 1e-8 by flexible CG in
+    FP64 with the FP32 multigrid V-cycle (every SpMV of the V-cycle FP32;
+    the true residual FP64, evaluated compensated): the first solve of a
+    fresh system, a re-solve, the error against the same solver run down to
+    its rounding floor, and Jacobi PCG beside it."""
+    mesh, stress = meshgen.workload("q1_2m_het")
+    t = time.perf_counter()
+    s = D.beam_system(mesh, stress=stress)
+    t_setup = time.perf_counter() - t
+    first_ms, st_first, _ = timed_solves(D, s, "het fcg_amg first solve", D.PRECOND_AMG, args.rtol, 0, 1)
+    ms, st, _ = timed_solves(D, s, "het fcg_amg", D.PRECOND_AMG, args.rtol, 1, max(1, args.steps))
+    u, _ = s.solve(precond=D.PRECOND_AMG, rtol=args.rtol)
+    try:
+        u_t, st_t = s.solve(precond=D.PRECOND_AMG, rtol=1e-12)
+        floor = st_t["rel_residual"]
+    except D.BfmxError as e:
+        u_t = s.get_u()
+        floor = float(str(e).split("residual ")[1].split()[0])
+    err = float(np.linalg.norm(u - u_t) / np.linalg.norm(u_t))
+    out = {
+        "workload": "q1_2m_het: 2000x500 Q1 beam, 20% jitter, E log-uniform [1e9, 1e11] per element (seed 0), "
+                    f"{mesh.n_dof} DOFs, rtol {args.rtol:g}",
+        "solver": "flexible CG (FP64, compensated true residual) + FP32 smoothed-aggregation V-cycle (FP32 SpMVs)",
+        "setup_s": t_setup, "first_solve_ms": first_ms, "value": mesh.n_dof / (first_ms / 1e3),
+        "resolve_ms": ms, "iterations": int(st["iterations"]), "rel_residual": st["rel_residual"],
+        "floor_rel_residual": floor, "error_vs_floor_solution": err,
+    }
+    if not args.no_pcg:
+        p_ms, p_st, reached = timed_solves(D, s, "het pcg_jacobi", D.PRECOND_JACOBI, args.rtol, 0, 1,
+                                           accept_floor=True)
+        out["pcg_jacobi_fp64"] = {"ms": p_ms, "rel_residual": p_st["rel_residual"], "reached_rtol": reached,
+                                  "iterations": int(p_st["iterations"])}
+        out["speedup_vs_pcg"] = p_ms / first_ms
+    log(f"heterogeneous beam: first solve {first_ms:.1f} ms, re-solve {ms:.1f} ms, {st['iterations']} iterations, "
+        f"residual {st['rel_residual']:.2e}, floor {floor:.2e}, error {err:.2e}")
+    return out, s
+
+
+def learned_leg(D, meshgen, args, seed=7):
+    """The learned preconditioner on its evaluation workload: a held-out
+    seed of the 2M-DOF heterogeneous beam (training used beams of seeds 100
+    and up and panels of seeds 100000 and up): flexible CG preconditioned by
+    the FP32 multigrid V-cycle plus the trained network's correction, after a
+    warm-up, beside the method without the network (the multigrid FCG),
+    block-Jacobi and Jacobi PCG."""
+    from bfm_amd.learned import WEIGHTS_MULTIGRID
+    mesh, stress = meshgen.workload("q1_2m_het", seed=seed)
+    s = D.beam_system(mesh, stress=stress)
+    s.set_model(D.Model.load(WEIGHTS_MULTIGRID))
+    w = max(1, args.warmup)
+    l_ms, l_st, _ = timed_solves(D, s, "learned_fcg", D.PRECOND_AMG_LEARNED, args.rtol, w, 1)
+    a_ms, a_st, _ = timed_solves(D, s, "learned leg: fcg_amg", D.PRECOND_AMG, args.rtol, w, 1)
+    out = {
+        "workload": f"q1_2m_het seed {seed} (held out), {mesh.n_dof} DOFs, rtol {args.rtol:g}",
+        "solver": "flexible CG (FP64) + FP32 multigrid V-cycle + the trained network's correction (BF16 weights, "
+                  "FP32 activations), block-Jacobi PCG fallback",
+        "ms_per_step": l_ms, "value": mesh.n_dof / (l_ms / 1e3), "iterations": int(l_st["iterations"]),
+        "rel_residual": l_st["rel_residual"], "fallback": int(l_st["fallback"]),
+        "multigrid_fcg_without_network": {"ms": a_ms, "iterations": int(a_st["iterations"])},
+    }
+    if not args.no_pcg:
+        b_ms, b_st, b_ok = timed_solves(D, s, "learned leg: pcg_block_jacobi", D.PRECOND_BLOCK_JACOBI, args.rtol, 0,
+                                        1, accept_floor=True)
+        p_ms, p_st, p_ok = timed_solves(D, s, "learned leg: pcg_jacobi", D.PRECOND_JACOBI, args.rtol, 0, 1,
+                                        accept_floor=True)
+        out["block_jacobi_pcg"] = {"ms": b_ms, "rel_residual": b_st["rel_residual"], "reached_rtol": b_ok}
+        out["jacobi_pcg"] = {"ms": p_ms, "rel_residual": p_st["rel_residual"], "reached_rtol": p_ok}
+        # the PCG runs stop at their floor above rtol: their time to that stop
+        out["speedup_vs_pcg"] = p_ms / l_ms
+    log(f"learned FCG: {l_ms:.1f} ms, {l_st['iterations']} iterations, fallback {l_st['fallback']}; multigrid "
+        f"FCG alone {a_ms:.1f} ms, {a_st['iterations']} iterations")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +326,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-pcg", action="store_true", help="skip the plain Jacobi-PCG leg")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner leg")
+    ap.add_argument("--no-het", action="store_true", help="skip the heterogeneous-E (mixed-precision) leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched holed-panel leg")
     ap.add_argument("--batch-count", type=int, default=4096, help="panels in the batched leg")
     ap.add_argument("--batch-only", action="store_true",
@@ -302,24 +431,13 @@ def main():
         extra["speedup_vs_pcg_cached_hierarchy"] = p_ms / ms
         log(f"speedup of multigrid FCG over Jacobi-PCG: {p_ms / ms_first:.2f}x first solve, "
             f"{p_ms / ms:.2f}x re-solve")
-    weights = os.path.join(ROOT, "bfm_amd", "weights", "mpnn_elasticity.f32")
-    if not args.no_learned and os.path.exists(weights):
-        # the learned preconditioner beside the method without it (block-Jacobi PCG)
-        s.set_model(D.Model.load(weights))
-        w_save, args.warmup = args.warmup, 0
-        b_ms, b_st = leg("pcg_block_jacobi", D.PRECOND_BLOCK_JACOBI, steps=1)
-        l_ms, l_st = leg("learned_fcg", D.PRECOND_LEARNED, steps=1)
-        args.warmup = w_save
-        extra["learned_fcg"] = {
-            "ms_per_step": l_ms, "value": mesh.n_dof / (l_ms / 1e3), "iterations": int(l_st["iterations"]),
-            "rel_residual": l_st["rel_residual"], "fallback": int(l_st["fallback"]),
-            "block_jacobi_pcg": {"ms_per_step": b_ms, "iterations": int(b_st["iterations"])},
-            "iteration_ratio_vs_block_jacobi": b_st["iterations"] / max(1, l_st["iterations"]),
-        }
-        if "pcg_jacobi_fp64" in extra:
-            extra["learned_fcg"]["speedup_vs_pcg"] = extra["pcg_jacobi_fp64"]["ms_per_step"] / l_ms
-        log(f"learned FCG: {l_ms:.1f} ms, {l_st['iterations']} iterations (block-Jacobi PCG "
-            f"{b_st['iterations']}, {b_ms:.1f} ms)")
+    if not args.no_het:
+        extra["mixed_het"], het_sys = heterogeneous_leg(D, meshgen, args)
+    else:
+        het_sys = None
+    if not args.no_learned:
+        extra["learned_fcg"] = learned_leg(D, meshgen, args)
+    del het_sys
     if args.mixed_cg:
         try:
             m_ms, m_st = leg("cg_fp32_matrix", D.PRECOND_JACOBI, D.MIXED, steps=1)
@@ -336,6 +454,10 @@ def main():
         log("cpu baseline ...")
         cpu = cpu_baseline_reference()
         log(f"cpu baseline {cpu['value']:.0f} DOF/s")
+        if "pcg_jacobi_fp64" in extra:
+            extra["cpu_baseline_sparse_pcg"] = cpu_pcg_sample(mesh, stress, extra["pcg_jacobi_fp64"]["iterations"])
+            log(f"cpu sparse PCG sample: {extra['cpu_baseline_sparse_pcg']['value']:.3e} DOF/s "
+                f"({extra['cpu_baseline_sparse_pcg']['cores']} threads)")
 
     out = {
         "metric": "DOF/s solved to 1e-8 FP64 rel. residual; learned-FCG vs plain PCG speedup",

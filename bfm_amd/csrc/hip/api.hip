@@ -472,7 +472,7 @@ int bfmx_trainset_destroy(bfmx_trainset_t* ts) {
 
 int bfmx_trainset_add(bfmx_trainset_t* ts, bfmx_system_t* s, int max_pairs, int every, double rtol, int* n_added) {
 	BFMX_CHECK(check(s));
-	if (!ts || max_pairs < 1 || every < 1 || !(rtol > 0)) {
+	if (!ts || max_pairs < 1 || every < 0 || !(rtol > 0)) {
 		set_error("bad arguments to trainset_add");
 		return BFMX_E_ARG;
 	}
@@ -510,6 +510,14 @@ int bfmx_trainset_get(bfmx_trainset_t* ts, int entry, float* feat, float* r, flo
 		return BFMX_E_ARG;
 	}
 	return guarded([&] { return trainset_get(ts, entry, feat, r, e, s_den); });
+}
+
+int bfmx_trainset_rebase(bfmx_trainset_t* ts) {
+	if (!ts) {
+		set_error("null training set");
+		return BFMX_E_ARG;
+	}
+	return guarded([&] { return trainset_rebase(ts); });
 }
 
 int bfmx_model_loss_grad(bfmx_model_t* m, bfmx_trainset_t* ts, int entry, int p0, int batch, double* loss,
@@ -551,6 +559,7 @@ int bfmx_system_set_model(bfmx_system_t* s, bfmx_model_t* m) {
 		// its owner may free once it is detached
 		return guarded([&] {
 			drop_graphs(s, BFMX_PRECOND_LEARNED);
+			drop_graphs(s, BFMX_PRECOND_AMG_LEARNED);
 			s->model = m;
 			return BFMX_OK;
 		});

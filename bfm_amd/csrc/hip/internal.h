@@ -444,12 +444,18 @@ int trainset_get(bfmx_trainset* ts, int entry, float* feat, float* r, float* e, 
 int mpnn_prepare(bfmx_system* s);
 int mpnn_setup_numeric(bfmx_system* s);
 // One application of the learned preconditioner to the FP64 residual r;
-// returns z (FP64, 2 per node: the FP64 block-Jacobi step of the solver's
-// inverses in s->d_dinv plus the network's FP32 correction); the last kernel
-// writes *n_partials partial sums of z . q. Every kernel returns at once when
-// S->done.
+// returns z (FP64, 2 per node: the base step plus the network's FP32
+// correction; the base is the FP64 block-Jacobi step of the solver's inverses
+// in s->d_dinv, or, given, the FP32 vector base, the multigrid V-cycle's
+// output); the last kernel writes *n_partials partial sums of z . q. Every
+// kernel returns at once when S->done.
 double* mpnn_apply(bfmx_system* s, const double* r, const double* q, double* partial, const Scalars* S,
-                   int* n_partials);
+                   int* n_partials, const float* base = nullptr);
+// Rebases the pairs of every entry for the multigrid-based preconditioner
+// (bfmx_trainset_rebase).
+int trainset_rebase(bfmx_trainset* ts);
+// The multigrid hierarchy of a system set up as a solve sets it up.
+int prepare_amg(bfmx_system* s);
 void mpnn_release(bfmx_system* s);
 // batch.hip
 int batch_n_configs();

@@ -493,7 +493,8 @@ __global__ __launch_bounds__(kWG) void k_adjoint(int64_t R, int B, const int32_t
 
 // Decoder (64 rows per workgroup, four threads per row): a0 = S0 + agg^T(T0)
 // (FP32), u = a0 W0^T, z = y + P Delta^-1/2 u. In a solve (!TRAIN) y = D^-1 r
-// with the solver's FP64 2x2 inverses (dinv64) and z is FP64; DOT adds
+// with the solver's FP64 2x2 inverses (dinv64), or y = base when given (the
+// multigrid V-cycle's output), and z is FP64; DOT adds
 // per-workgroup partial sums of z . q. Training: y from the FP32 features,
 // z FP32.
 template <bool TRAIN, bool DOT, typename RT, typename ZT, typename AT>
@@ -503,9 +504,9 @@ __global__ __launch_bounds__(kWG) void k_decode(int64_t R, int B, const RT* __re
                                                 const int32_t* __restrict__ col, const float* __restrict__ val32,
                                                 const float* __restrict__ feat, const float* __restrict__ params,
                                                 const AT* __restrict__ S0, const AT* __restrict__ T0,
-                                                const double* __restrict__ dinv64, ZT* __restrict__ z,
-                                                const double* __restrict__ q, double* __restrict__ partial,
-                                                const Scalars* __restrict__ S) {
+                                                const double* __restrict__ dinv64, const float* __restrict__ base,
+                                                ZT* __restrict__ z, const double* __restrict__ q,
+                                                double* __restrict__ partial, const Scalars* __restrict__ S) {
 	if (S && S->done) {
 		return;
 	}
@@ -549,6 +550,9 @@ __global__ __launch_bounds__(kWG) void k_decode(int64_t R, int B, const RT* __re
 			float const r1 = static_cast<float>(r[b * pstride + 2 * i + 1]);
 			z0 = static_cast<ZT>(f[0] * r0 + f[1] * r1 + c0);
 			z1 = static_cast<ZT>(f[2] * r0 + f[3] * r1 + c1);
+		} else if (base) {
+			z0 = static_cast<ZT>(static_cast<double>(base[2 * i]) + static_cast<double>(c0));
+			z1 = static_cast<ZT>(static_cast<double>(base[2 * i + 1]) + static_cast<double>(c1));
 		} else {
 			double const r0 = static_cast<double>(r[2 * i]), r1 = static_cast<double>(r[2 * i + 1]);
 			const double* d = dinv64 + 4 * i;
@@ -1137,7 +1141,7 @@ int model_loss_grad(bfmx_model* m, bfmx_trainset* ts, int entry, int p0, int B) 
 	net_adjoint<u16>(st, g, R, B, m, m->h[3], m->adj[0], m->adj[1], m->adj[2], m->adj[3], nullptr);
 	k_decode<true, false, float, float, u16><<<gt, kWG, 0, st>>>(R, B, r, ps, g.slice_ptr, g.row_len, g.col, g.val32,
 	                                                        g.feat, m->d_params, m->adj[0], m->adj[1], nullptr,
-	                                                        m->z, nullptr, nullptr, nullptr);
+	                                                        nullptr, m->z, nullptr, nullptr, nullptr);
 	// loss and g = P Delta^-1/2 dL/dz (into xg)
 	k_loss<<<gr, kWG, 0, st>>>(R, B, g.slice_ptr, g.row_len, g.col, g.val32, g.feat, m->z, e, ps, E.d_den + p0,
 	                           m->xg, m->lpart);
@@ -1217,6 +1221,9 @@ int trainset_add(bfmx_trainset* ts, bfmx_system* s, int max_pairs, int every, do
 		return rc;
 	}
 	int64_t const iters = stats.iterations;
+	if (every == 0) {   // spread the pairs over the whole run
+		every = static_cast<int>(std::max<int64_t>(1, iters / std::max(1, max_pairs)));
+	}
 	int const P = static_cast<int>(std::min<int64_t>(max_pairs, std::max<int64_t>(1, iters / every)));
 	bfmx_trainset::Entry E;
 	E.sys = s;
@@ -1278,6 +1285,52 @@ int trainset_add(bfmx_trainset* ts, bfmx_system* s, int max_pairs, int every, do
 	return BFMX_OK;
 }
 
+namespace {
+
+__global__ __launch_bounds__(kWG) void k_f32_to_f64(int64_t n, const float* __restrict__ a, double* __restrict__ b) {
+	int64_t const i = static_cast<int64_t>(blockIdx.x) * kWG + threadIdx.x;
+	if (i < n) {
+		b[i] = static_cast<double>(a[i]);
+	}
+}
+
+// e = e - v + D^-1 r for one pair (node i: D^-1 from its FP32 features).
+__global__ __launch_bounds__(kWG) void k_rebase(int64_t n, const float* __restrict__ feat, const float* __restrict__ r,
+                                                const float* __restrict__ v, float* __restrict__ e) {
+	int64_t const i = static_cast<int64_t>(blockIdx.x) * kWG + threadIdx.x;
+	if (i >= n) {
+		return;
+	}
+	const float* f = feat + 8 * i;
+	float const r0 = r[2 * i], r1 = r[2 * i + 1];
+	e[2 * i] = e[2 * i] - v[2 * i] + (f[0] * r0 + f[1] * r1);
+	e[2 * i + 1] = e[2 * i + 1] - v[2 * i + 1] + (f[2] * r0 + f[3] * r1);
+}
+
+}  // namespace
+
+int trainset_rebase(bfmx_trainset* ts) {
+	for (auto& E : ts->entries) {
+		bfmx_system* s = E.sys;
+		hipStream_t const st = s->stream;
+		int64_t const n = E.n;
+		BFMX_CHECK(ensure_workspace_public(s));
+		BFMX_CHECK(prepare_amg(s));
+		// the V-cycle's kernels run while the scalars say running (done = 0)
+		BFMX_HIP(hipMemsetAsync(s->d_scal, 0, sizeof(Scalars), st));
+		for (int p = 0; p < E.P; p++) {
+			float* rp = E.d_r + 2 * n * p;
+			k_f32_to_f64<<<grid_of(2 * n, kWG), kWG, 0, st>>>(2 * n, rp, s->d_r);
+			int nw = 0;
+			float* v = amg_vcycle(s, s->d_r, nullptr, nullptr, s->d_scal, &nw);
+			k_rebase<<<grid_of(n, kWG), kWG, 0, st>>>(n, E.d_feat, rp, v, E.d_e + 2 * n * p);
+		}
+		BFMX_HIP(hipGetLastError());
+		BFMX_HIP(hipStreamSynchronize(st));
+	}
+	return BFMX_OK;
+}
+
 int trainset_get(bfmx_trainset* ts, int entry, float* feat, float* r, float* e, float* s_den) {
 	if (entry < 0 || entry >= static_cast<int>(ts->entries.size())) {
 		set_error("no training entry %d", entry);
@@ -1332,7 +1385,7 @@ int mpnn_setup_numeric(bfmx_system* s) {
 }
 
 double* mpnn_apply(bfmx_system* s, const double* r, const double* q, double* partial, const Scalars* S,
-                   int* n_partials) {
+                   int* n_partials, const float* base) {
 	hipStream_t const st = s->stream;
 	bfmx_model* m = s->model;
 	int64_t const R = s->n_nodes;
@@ -1344,7 +1397,7 @@ double* mpnn_apply(bfmx_system* s, const double* r, const double* q, double* par
 	net_forward<float>(st, g, R, 1, m, hf, nullptr, S);
 	net_adjoint<float>(st, g, R, 1, m, h[1], h[2], h[3], h[0], h[1], S);
 	k_decode<false, true, double, double, float><<<gt, kWG, 0, st>>>(R, 1, r, 0, g.slice_ptr, g.row_len, g.col, g.val32,
-	                                                          g.feat, m->d_params, h[2], h[3], s->d_dinv,
+	                                                          g.feat, m->d_params, h[2], h[3], s->d_dinv, base,
 	                                                          s->d_mp_z, q, partial, S);
 	*n_partials = gt;
 	return s->d_mp_z;

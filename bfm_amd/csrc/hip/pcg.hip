@@ -866,7 +866,14 @@ void launch_fcg_iteration(bfmx_system* s, int pre) {
 		k_fcg_spmv<float><<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[0],
 		                                               s->d_p[1], s->d_q, s->d_r, s->d_partial, s->d_scal);
 	} else {
-		double* z = mpnn_apply(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nwz);
+		// the learned network's correction added to block Jacobi, or to the
+		// multigrid V-cycle (its dot partials are not needed: z . q comes
+		// from the decoder)
+		int nwv = 0;
+		const float* base = pre == BFMX_PRECOND_AMG_LEARNED
+		                        ? amg_vcycle(s, s->d_r, nullptr, nullptr, s->d_scal, &nwv)
+		                        : nullptr;
+		double* z = mpnn_apply(s, s->d_r, s->d_q, s->d_partial, s->d_scal, &nwz, base);
 		k_red_fcg_beta<<<1, kRed, 0, s->stream>>>(nwz, s->d_partial, s->d_scal);
 		k_fcg_spmv<double><<<g, kBlock, 0, s->stream>>>(nb, s->d_slice_ptr, s->d_col, s->d_val, z, s->d_p[0],
 		                                                s->d_p[1], s->d_q, s->d_r, s->d_partial, s->d_scal);
@@ -879,8 +886,12 @@ void launch_fcg_iteration(bfmx_system* s, int pre) {
 
 constexpr int kFcgChunk = 4;   // flexible-CG iterations per captured graph
 
+bool is_learned(int pre) {
+	return pre == BFMX_PRECOND_LEARNED || pre == BFMX_PRECOND_AMG_LEARNED;
+}
+
 bool is_fcg(int pre) {
-	return pre == BFMX_PRECOND_AMG || pre == BFMX_PRECOND_LEARNED;
+	return pre == BFMX_PRECOND_AMG || is_learned(pre);
 }
 
 int chunk_iters(int pre) {
@@ -896,7 +907,7 @@ void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n) {
 		for (int it = 0; it < n; it++) {
 			launch_fcg_iteration(s, pre);
 		}
-		launch_residual(s, pre == BFMX_PRECOND_LEARNED ? BFMX_PRECOND_BLOCK_JACOBI : BFMX_PRECOND_JACOBI, 1, rtol2,
+		launch_residual(s, is_learned(pre) ? BFMX_PRECOND_BLOCK_JACOBI : BFMX_PRECOND_JACOBI, 1, rtol2,
 		                0.0);
 		return;
 	}
@@ -910,7 +921,7 @@ void launch_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n) {
 // The captured chunk for this configuration (captured on first use).
 int capture_chunk(bfmx_system* s, int pre, bool mixed, double rtol2, int n, hipGraphExec_t* out) {
 	int const prec = mixed ? 1 : 0;
-	bool const learned = pre == BFMX_PRECOND_LEARNED && s->model;
+	bool const learned = is_learned(pre) && s->model;
 	uint64_t const gen = learned ? s->model->gen : 0;
 	const float* params = learned ? s->model->d_params : nullptr;
 	for (auto const& gr : s->graphs) {
@@ -961,7 +972,7 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
 	// the residual kernel's z (unused by flexible CG); the learned decoder reads
 	// the 2x2 block inverses
 	int const pre_res = precond == BFMX_PRECOND_AMG      ? BFMX_PRECOND_JACOBI
-	                    : precond == BFMX_PRECOND_LEARNED ? BFMX_PRECOND_BLOCK_JACOBI
+	                    : is_learned(precond)             ? BFMX_PRECOND_BLOCK_JACOBI
 	                                                      : precond;
 	k_precond<<<grid(n), kBlock, 0, st>>>(static_cast<int>(n), pre_res, s->d_diag_blk, s->d_val, s->d_dinv);
 	if (!warm) {
@@ -1408,7 +1419,7 @@ int fcg_segments(bfmx_system* s) {
 
 int solve(bfmx_system* s, const double* b_in, int precond, int precision, double rtol, size_t max_iter,
           bfmx_stats_t* stats) {
-	if (precond < BFMX_PRECOND_JACOBI || precond > BFMX_PRECOND_MINRES) {
+	if (precond < BFMX_PRECOND_JACOBI || precond > BFMX_PRECOND_AMG_LEARNED) {
 		set_error("unknown preconditioner %d", precond);
 		return BFMX_E_UNSUPPORTED;
 	}
@@ -1420,8 +1431,12 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		set_error("rtol must be positive");
 		return BFMX_E_ARG;
 	}
-	bool const amg = precond == BFMX_PRECOND_AMG;
-	bool const learned = precond == BFMX_PRECOND_LEARNED;
+	bool const amg = precond == BFMX_PRECOND_AMG || precond == BFMX_PRECOND_AMG_LEARNED;
+	bool const learned = is_learned(precond);
+	if (precond == BFMX_PRECOND_AMG_LEARNED && s->n_conv_seg > 0) {
+		set_error("the multigrid-based learned preconditioner does not take systems with convergence segments");
+		return BFMX_E_UNSUPPORTED;
+	}
 	bool const mixed = !is_fcg(precond) && precision == BFMX_PRECISION_MIXED;
 	int64_t const n = s->n_nodes;
 	int64_t const cap = max_iter ? static_cast<int64_t>(max_iter) : 20 * 2 * n;

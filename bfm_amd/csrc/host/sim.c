@@ -102,7 +102,7 @@ int bfm_sim_store_stats(bfm_sim_t const* sim, size_t n, bfm_solve_stats_t const*
 }
 
 int bfm_sim_set_solver(bfm_sim_t* sim, bfm_solver_t const* solver) {
-	if (!(solver->rtol > 0) || solver->precond < 0 || solver->precond > BFMX_PRECOND_MINRES ||
+	if (!(solver->rtol > 0) || solver->precond < 0 || solver->precond > BFMX_PRECOND_AMG_LEARNED ||
 	    solver->precision < 0 || solver->precision > 1) {
 		BFM_ERROR(sim->state, "invalid solver settings");
 		return -1;
@@ -453,7 +453,8 @@ static int run_planar(bfm_sim_t* sim, int stress) {
 		return r;
 	}
 	bfmx_model_t* model = NULL;
-	if (solver.precond == BFMX_PRECOND_LEARNED && !(model = sim_model(sim))) {
+	if ((solver.precond == BFMX_PRECOND_LEARNED || solver.precond == BFMX_PRECOND_AMG_LEARNED) &&
+	    !(model = sim_model(sim))) {
 		state->free(stats);
 		return -1;
 	}

@@ -12,7 +12,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "lib", "libbfmhip.so")
 
-PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_AMG, PRECOND_LEARNED, PRECOND_MINRES = 0, 1, 2, 3, 4
+PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_AMG, PRECOND_LEARNED, PRECOND_MINRES, PRECOND_AMG_LEARNED = 0, 1, 2, 3, 4, 5
 PLANE_STRAIN, PLANE_STRESS, AXISYMMETRIC = 0, 1, 2
 FP64, MIXED = 0, 1
 KERNEL_CG_SPMV64, KERNEL_CG_SPMV32, KERNEL_FCG_SPMV, KERNEL_AMG_SMOOTH = 0, 1, 2, 3
@@ -30,7 +30,7 @@ EXPORTS = [
     "bfmx_system_get_rhs", "bfmx_system_get_u", "bfmx_system_get_entry", "bfmx_system_time_spmv",
     "bfmx_system_time_kernel", "bfmx_model_n_params", "bfmx_model_create", "bfmx_model_destroy",
     "bfmx_model_get_params", "bfmx_model_set_params", "bfmx_trainset_create", "bfmx_trainset_destroy",
-    "bfmx_trainset_add", "bfmx_trainset_info", "bfmx_trainset_get", "bfmx_model_loss_grad", "bfmx_model_train",
+    "bfmx_trainset_add", "bfmx_trainset_rebase", "bfmx_trainset_info", "bfmx_trainset_get", "bfmx_model_loss_grad", "bfmx_model_train",
     "bfmx_system_set_model", "bfmx_system_apply_model", "bfmx_batch_create", "bfmx_batch_destroy",
     "bfmx_batch_autotune", "bfmx_batch_solve", "bfmx_batch_get_u", "bfmx_batch_create_planar",
     "bfmx_batch_system",
@@ -116,6 +116,7 @@ def lib():
         L.bfmx_trainset_create.argtypes = [C.POINTER(vp)]
         L.bfmx_trainset_destroy.argtypes = [vp]
         L.bfmx_trainset_add.argtypes = [vp, vp, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_int)]
+        L.bfmx_trainset_rebase.argtypes = [vp]
         L.bfmx_trainset_info.argtypes = [vp, C.c_int, C.POINTER(sz), C.POINTER(C.c_int)]
         L.bfmx_trainset_get.argtypes = [vp, C.c_int, _Opt(f32p), _Opt(f32p), _Opt(f32p), _Opt(f32p)]
         L.bfmx_model_loss_grad.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double), _Opt(f32p)]
@@ -368,6 +369,11 @@ class TrainSet:
         _check(lib().bfmx_trainset_add(self.h, system.h, max_pairs, every, rtol, C.byref(n)))
         self.systems.append(system)
         return n.value
+
+    def rebase(self):
+        """bfmx_trainset_rebase: the pairs scored for the multigrid-based
+        preconditioner (PRECOND_AMG_LEARNED); once, after the last add."""
+        _check(lib().bfmx_trainset_rebase(self.h))
 
     def __len__(self):
         return len(self.systems)

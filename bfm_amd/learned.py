@@ -1,19 +1,24 @@
 """Training and evaluation of the learned preconditioner (GPU).
 
-    python -m bfm_amd.learned train [--pairs 20000] [--steps 4000] [--out PATH]
-    python -m bfm_amd.learned eval  [--weights PATH] [--workload q1_2m_het] [--seed 7]
+    python -m bfm_amd.learned train [--base multigrid|jacobi] [--pairs 20000] [--steps 4000] [--out PATH]
+    python -m bfm_amd.learned eval  [--base multigrid|jacobi] [--weights PATH] [--workload q1_2m_het] [--seed 7]
 
-train: builds seeded planar problems from the beam distributions (20%
-jittered Q1 beams, homogeneous and with per-element E log-uniform in
-[1e9, 1e11], plane stress, sizes from 60x15 to 300x75 cells), records
-(residual, error) pairs from plain Jacobi-PCG runs on each
-(bfmx_trainset_add), and trains the network with Adam on random batches of
-one problem's pairs (bfmx_model_train; forward and backward run on the GPU).
-Writes the FP32 parameters (raw little-endian) and a JSON log beside them.
+train: builds seeded planar problems from the workloads' distributions
+(20% jittered Q1 beams, homogeneous and with per-element E log-uniform in
+[1e9, 1e11], plane stress, sizes from 60x15 to 1000x250 cells; holed P1
+panels of the batched workload, plane strain), records (residual, error)
+pairs from plain Jacobi-PCG runs on each (bfmx_trainset_add), and trains the
+network with Adam on random batches of one problem's pairs
+(bfmx_model_train; forward and backward run on the GPU). --base multigrid
+(the default) trains the correction added to the FP32 multigrid V-cycle
+(PRECOND_AMG_LEARNED; the pairs are rebased with bfmx_trainset_rebase),
+--base jacobi the one added to block Jacobi (PRECOND_LEARNED). Writes the
+FP32 parameters (raw little-endian) and a JSON log beside them.
 
 eval: solves a held-out problem (default: the 2M-DOF heterogeneous beam with
-a seed not used in training) with the learned flexible CG, block-Jacobi PCG
-(the method without the network) and Jacobi PCG; prints iterations and times.
+a seed not used in training) with the learned flexible CG, the method
+without the network (the multigrid FCG, or block-Jacobi PCG), block-Jacobi
+and Jacobi PCG; prints iterations and times.
 """
 
 import argparse
@@ -25,20 +30,25 @@ import time
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-WEIGHTS = os.path.join(HERE, "weights", "mpnn_elasticity.f32")
+WEIGHTS = os.path.join(HERE, "weights", "mpnn_elasticity.f32")            # block-Jacobi base
+WEIGHTS_MULTIGRID = os.path.join(HERE, "weights", "mpnn_multigrid.f32")   # multigrid base
 
 # training problems: (cells x, cells y, heterogeneous E), seeds 100 + k
-TRAIN_SHAPES = [(60, 15), (100, 25), (160, 40), (200, 50), (300, 75)]
+TRAIN_SHAPES = [(60, 15), (100, 25), (160, 40), (200, 50), (300, 75), (500, 125), (1000, 250)]
 
 
-def default_params(path=WEIGHTS):
+def default_params(path=WEIGHTS_MULTIGRID):
     """The packaged trained parameters (FP32, raw little-endian)."""
     if not os.path.exists(path):
         raise FileNotFoundError(f"{path}: no trained weights; run python -m bfm_amd.learned train")
     return np.fromfile(path, dtype=np.float32)
 
 
-def training_problems(n_problems, seed0=100):
+def training_problems(n_problems, seed0=100, panels=0):
+    """(name, system) of the training problems: n_problems beams, then
+    `panels` holed panels (seeds 100000 + k, not among the batched
+    workload's 0-4095)."""
+    from . import device as D
     from . import meshgen
     for k in range(n_problems):
         nx, ny = TRAIN_SHAPES[k % len(TRAIN_SHAPES)]
@@ -47,7 +57,10 @@ def training_problems(n_problems, seed0=100):
         mesh = meshgen.beam(nx, ny, kind=4, jitter=0.2, seed=seed)
         if hetero:
             mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=seed)
-        yield f"beam{nx}x{ny}{'_het' if hetero else ''}_s{seed}", mesh
+        yield f"beam{nx}x{ny}{'_het' if hetero else ''}_s{seed}", mesh, D.beam_system(mesh)
+    for k in range(panels):
+        mesh, traction = meshgen.panel(100000 + k, 2000, 20000)
+        yield f"panel_s{100000 + k}", mesh, D.panel_system(mesh, traction)
 
 
 def train(args):
@@ -55,18 +68,19 @@ def train(args):
     t0 = time.perf_counter()
     ts = D.TrainSet()
     log = {"problems": [], "pairs": 0}
-    per = max(1, args.pairs // args.problems)
+    per = max(1, args.pairs // (args.problems + args.panels))
     systems = []
-    for name, mesh in training_problems(args.problems):
-        s = D.beam_system(mesh)
-        # Jacobi-PCG takes about 6 iterations per cell along a 4:1 beam
-        # (about 12 sqrt(nodes)): spread the recorded iterates over the run
-        every = max(1, int(12 * np.sqrt(mesh.n_nodes) / per))
+    for name, mesh, s in training_problems(args.problems, panels=args.panels):
+        # the recorded iterates are spread over the whole Jacobi-PCG run
+        every = 0
         n = ts.add(s, max_pairs=per, every=every, rtol=1e-9)
         systems.append(s)
         log["problems"].append({"name": name, "dofs": mesh.n_dof, "pairs": n, "every": every})
         log["pairs"] += n
         print(f"{name}: {mesh.n_dof} DOFs, {n} pairs (every {every})", flush=True)
+    if args.base == "multigrid":
+        ts.rebase()   # the loss scores V(r) + network term against e
+    log["base"] = args.base
     t1 = time.perf_counter()
     m = D.Model(seed=args.seed)
     losses = []
@@ -92,11 +106,12 @@ def evaluate(args):
     from . import meshgen
     mesh, stress = meshgen.workload(args.workload, seed=args.seed)
     s = D.beam_system(mesh, stress=stress)
-    m = D.Model.load(args.weights)
+    m = D.Model.load(args.weights or (WEIGHTS_MULTIGRID if args.base == "multigrid" else WEIGHTS))
     s.set_model(m)
-    out = {"workload": args.workload, "seed": args.seed, "dofs": mesh.n_dof}
-    for name, pre in (("learned_fcg", D.PRECOND_LEARNED), ("block_jacobi_pcg", D.PRECOND_BLOCK_JACOBI),
-                      ("jacobi_pcg", D.PRECOND_JACOBI), ("amg_fcg", D.PRECOND_AMG)):
+    out = {"workload": args.workload, "seed": args.seed, "dofs": mesh.n_dof, "base": args.base}
+    learned = D.PRECOND_AMG_LEARNED if args.base == "multigrid" else D.PRECOND_LEARNED
+    for name, pre in (("learned_fcg", learned), ("amg_fcg", D.PRECOND_AMG),
+                      ("block_jacobi_pcg", D.PRECOND_BLOCK_JACOBI), ("jacobi_pcg", D.PRECOND_JACOBI)):
         try:
             s.solve(precond=pre, rtol=args.rtol, want_u=False)   # warm-up (graph capture, set-up)
             t = time.perf_counter()
@@ -120,15 +135,20 @@ def main():
     t.add_argument("--batch", type=int, default=8)
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=0)
-    t.add_argument("--out", default=WEIGHTS)
+    t.add_argument("--panels", type=int, default=8, help="holed panels added to the beams")
+    t.add_argument("--base", choices=("multigrid", "jacobi"), default="multigrid")
+    t.add_argument("--out", default=None)
     e = sub.add_parser("eval")
-    e.add_argument("--weights", default=WEIGHTS)
+    e.add_argument("--base", choices=("multigrid", "jacobi"), default="multigrid")
+    e.add_argument("--weights", default=None)
     e.add_argument("--workload", default="q1_2m_het")
     e.add_argument("--seed", type=int, default=7)
     e.add_argument("--rtol", type=float, default=1e-8)
     e.add_argument("--het", type=int, default=1, help="per-element E log-uniform (1) or uniform (0)")
     args = ap.parse_args()
     if args.cmd == "train":
+        if args.out is None:
+            args.out = WEIGHTS_MULTIGRID if args.base == "multigrid" else WEIGHTS
         train(args)
     else:
         evaluate(args)

@@ -17,8 +17,10 @@
 typedef struct {
 	int precond;     /* 0 point Jacobi, 1 2x2 block Jacobi, 2 FP32 multigrid V-cycle
 	                    inside FP64 flexible CG, 3 learned network (flexible CG),
-	                    4 MINRES with |diagonal| Jacobi; axisymmetric sims and
-	                    systems always use 4 (their K is indefinite) */
+	                    4 MINRES with |diagonal| Jacobi, 5 multigrid V-cycle plus
+	                    the learned network's correction (flexible CG);
+	                    axisymmetric sims and systems always use 4 (their K is
+	                    indefinite) */
 	int precision;   /* 0 FP64, 1 FP32 matrix inside FP64 refinement */
 	double rtol;     /* ||b - K u|| / ||b|| target, FP64 */
 	size_t max_iter; /* 0 = 20 x DOFs */
@@ -35,7 +37,7 @@ typedef struct {
 int bfm_sim_set_solver(bfm_sim_t* sim, bfm_solver_t const* solver);
 int bfm_sim_get_solver(bfm_sim_t* sim, bfm_solver_t* solver);
 
-/* Parameters of the learned preconditioner (precond 3), copied: the FP32
+/* Parameters of the learned preconditioner (precond 3 or 5), copied: the FP32
  * vector of bfmx_model_get_params (n_params must equal bfmx_model_n_params).
  * NULL clears them. */
 int bfm_sim_set_model(bfm_sim_t* sim, float const* params, size_t n_params);

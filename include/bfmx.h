@@ -69,6 +69,13 @@ enum {
 	                                  for symmetric indefinite systems (the
 	                                  axisymmetric formulation); precision is
 	                                  ignored */
+	BFMX_PRECOND_AMG_LEARNED = 5,  /* flexible CG (FP64) preconditioned by the FP32
+	                                  multigrid V-cycle plus the correction of the
+	                                  network attached with bfmx_system_set_model
+	                                  (trained on pairs rebased with
+	                                  bfmx_trainset_rebase): z = V(r) + the
+	                                  network's SPD term; falls back to
+	                                  block-Jacobi PCG as BFMX_PRECOND_LEARNED */
 };
 
 /* Arithmetic of the solve. */
@@ -247,7 +254,7 @@ int bfmx_trainset_create(bfmx_trainset_t** out);
 int bfmx_trainset_destroy(bfmx_trainset_t* ts);
 /* Solves the system's current problem with Jacobi PCG to rtol (x*), then
  * reruns it recording up to max_pairs pairs (r_k, x* - x_k), one every
- * `every` iterations from k = 0. The matrix values and node features are
+ * `every` iterations from k = 0 (every = 0: spread over the whole run). The matrix values and node features are
  * snapshotted; the system must outlive the set (its graph is shared). */
 int bfmx_trainset_add(bfmx_trainset_t* ts, bfmx_system_t* s, int max_pairs, int every, double rtol,
                       int* n_added);
@@ -255,6 +262,15 @@ int bfmx_trainset_info(bfmx_trainset_t* ts, int entry, size_t* n_nodes, int* n_p
 /* An entry's data in the internal node order: feat (8 per node), r and e
  * (n_pairs x 2 n_nodes), s_den (n_pairs scales s, then n_pairs e^T A e). */
 int bfmx_trainset_get(bfmx_trainset_t* ts, int entry, float* feat, float* r, float* e, float* s_den);
+
+/* Prepares every entry's pairs for a model used with
+ * BFMX_PRECOND_AMG_LEARNED: with V the FP32 multigrid V-cycle of the entry's
+ * system (built here if needed) and D^-1 its block-Jacobi inverse (FP32
+ * features), each error e becomes e - V(r) + D^-1 r, so that the loss below,
+ * which scores z = D^-1 r + (network term), scores V(r) + (network term)
+ * against the original e (the normalizers stay those of the original e).
+ * Call once per set, after its last bfmx_trainset_add. */
+int bfmx_trainset_rebase(bfmx_trainset_t* ts);
 
 /* Loss (mean over pairs p0 .. p0 + batch - 1 of ||z - e||_A^2 / ||e||_A^2)
  * and its gradient with respect to the parameters (FP32), without an update. */
