@@ -1096,18 +1096,26 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	BFMX_CHECK(dalloc(&T.apptr, n + 1));
 	k_ap_pattern<true><<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, cnt, nullptr,
 	                                                 nullptr, flags);
+	{
+		// the total first in 64 bits: the 32-bit row pointers must hold it
+		int64_t* ap64;
+		BFMX_CHECK(tmp.get(&ap64, static_cast<size_t>(n) + 1));
+		BFMX_CHECK(scan(st, cnt, ap64, n, tmp));
+		int64_t total = 0;
+		BFMX_CHECK(read_one(st, ap64 + n, &total));
+		BFMX_CHECK(read_one(st, flags, &bad));
+		if (bad) {
+			set_error("multigrid: a row of A P has more than %d entries for the device build", kMaxAP);
+			return BFMX_E_UNSUPPORTED;
+		}
+		if (total >= (int64_t(1) << 31)) {
+			set_error("multigrid: A P too large for 32-bit indices (%lld entries)", static_cast<long long>(total));
+			return BFMX_E_UNSUPPORTED;
+		}
+	}
 	BFMX_CHECK(scan(st, cnt, T.apptr, n, tmp));
 	int32_t nap = 0;
 	BFMX_CHECK(read_one(st, T.apptr + n, &nap));
-	BFMX_CHECK(read_one(st, flags, &bad));
-	if (bad) {
-		set_error("multigrid: a row of A P has more than %d entries for the device build", kMaxAP);
-		return BFMX_E_UNSUPPORTED;
-	}
-	if (static_cast<int64_t>(nap) >= (int64_t(1) << 31) / 9) {
-		set_error("multigrid: A P too large for 32-bit indices");
-		return BFMX_E_ARG;
-	}
 	T.nnz_ap = nap;
 	BFMX_CHECK(dalloc(&T.apcol, nap));
 	k_ap_pattern<false><<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, nullptr,

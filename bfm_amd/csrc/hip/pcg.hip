@@ -1441,13 +1441,32 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	int64_t const n = s->n_nodes;
 	int64_t const cap = max_iter ? static_cast<int64_t>(max_iter) : 20 * 2 * n;
 	hipStream_t const st = s->stream;
+	// BFMX_SOLVE_TIMES=1: the phases of this call on stderr (each phase
+	// waited for, so the total is slightly longer than without)
+	static bool const times = [] {
+		const char* e = std::getenv("BFMX_SOLVE_TIMES");
+		return e && e[0] == '1';
+	}();
+	auto tp = std::chrono::steady_clock::now();
+	auto phase = [&](const char* what) {
+		if (!times) {
+			return;
+		}
+		(void) hipStreamSynchronize(st);
+		auto const now = std::chrono::steady_clock::now();
+		std::fprintf(stderr, "solve phase %-28s %8.3f ms\n", what,
+		             std::chrono::duration<double, std::milli>(now - tp).count());
+		tp = now;
+	};
 	BFMX_CHECK(ensure_workspace(s));
+	phase("workspace");
 	if ((mixed || amg || learned) && !s->d_val32) {
 		BFMX_CHECK(dalloc(&s->d_val32, 4 * s->nnzb_pad));   // before any capture reads its address
 	}
 	if (amg) {
 		BFMX_CHECK(amg_build(s));   // topology, once per system
 		BFMX_CHECK(fcg_segments(s));
+		phase("multigrid topology");
 	}
 	if (learned) {
 		BFMX_CHECK(mpnn_prepare(s));
@@ -1501,8 +1520,10 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	if (mixed || amg || learned) {
 		BFMX_CHECK(ensure_f32(s));
 	}
+	phase("right-hand side, FP32 copy");
 	if (amg) {
 		BFMX_CHECK(amg_setup_numeric(s));
+		phase("multigrid numeric set-up");
 	}
 	if (learned) {
 		BFMX_CHECK(mpnn_setup_numeric(s));
@@ -1511,6 +1532,7 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	// flexible CG stops when its residual stops halving for long (a
 	// divergent or stalled preconditioner) rather than running to the cap
 	BFMX_CHECK(run_chunks(s, precond, mixed, rtol2, cap, false, learned || amg, &last));
+	phase("iterations (with captures)");
 	int64_t iters = last.iter;
 	int32_t fallback = 0;
 	int64_t fb_iters = 0;
