@@ -1780,7 +1780,7 @@ int build_sell(int n, const std::vector<int64_t>& ptr, const std::vector<int32_t
 
 template <typename T>
 int upload(T** d, const std::vector<T>& h, hipStream_t st) {
-	BFMX_CHECK(dalloc(d, h.size()));
+	BFMX_CHECK(palloc(d, h.size()));
 	if (!h.empty()) {
 		BFMX_HIP(hipMemcpyAsync(*d, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
 	}
@@ -1843,54 +1843,58 @@ int64_t dense_entries(int bs, const std::vector<int32_t>& seg) {
 }  // namespace
 
 void amg_release(bfmx_system* s) {
+	if (!s->amg.empty() || s->d_amg_desc || s->d_amg_pf) {
+		// the pool takes blocks back only from drained streams
+		(void) hipStreamSynchronize(s->stream);
+	}
 	for (size_t l = 0; l < s->amg.size(); l++) {
 		AmgLevel& L = s->amg[l];
 		if (l > 0) {
-			dfree(L.d_slice_ptr);
-			dfree(L.d_row_len);
-			dfree(L.d_col);
-			dfree(L.d_diag_blk);
-			dfree(L.d_val64);
-			dfree(L.d_val32);
+			pfree(L.d_slice_ptr);
+			pfree(L.d_row_len);
+			pfree(L.d_col);
+			pfree(L.d_diag_blk);
+			pfree(L.d_val64);
+			pfree(L.d_val32);
 		}
-		dfree(L.d_dinv);
-		dfree(L.d_scal);
-		dfree(L.d_B);
-		dfree(L.d_B0);
-		dfree(L.d_agg);
-		dfree(L.d_Pt);
-		dfree(L.d_agg_ptr);
-		dfree(L.d_agg_nodes);
-		dfree(L.d_pptr);
-		dfree(L.d_pcol);
-		dfree(L.d_P);
-		dfree(L.d_ppos);
-		dfree(L.d_rptr);
-		dfree(L.d_rent);
-		dfree(L.d_R);
-		dfree(L.d_rrow);
-		dfree(L.d_gal_ptr);
-		dfree(L.d_gal_mirror);
-		dfree(L.d_gal_blk);
-		dfree(L.d_gal_pi);
-		dfree(L.d_gal_pj);
-		dfree(L.d_apptr);
-		dfree(L.d_apcol);
-		dfree(L.d_apval);
-		dfree(L.d_dense);
-		dfree(L.d_seg_ptr);
-		dfree(L.d_seg_row);
-		dfree(L.d_seg_off);
-		dfree(L.d_b);
-		dfree(L.d_x[0]);
-		dfree(L.d_x[1]);
-		dfree(L.d_d);
-		dfree(L.d_r);
-		dfree(L.d_partial);
+		pfree(L.d_dinv);
+		pfree(L.d_scal);
+		pfree(L.d_B);
+		pfree(L.d_B0);
+		pfree(L.d_agg);
+		pfree(L.d_Pt);
+		pfree(L.d_agg_ptr);
+		pfree(L.d_agg_nodes);
+		pfree(L.d_pptr);
+		pfree(L.d_pcol);
+		pfree(L.d_P);
+		pfree(L.d_ppos);
+		pfree(L.d_rptr);
+		pfree(L.d_rent);
+		pfree(L.d_R);
+		pfree(L.d_rrow);
+		pfree(L.d_gal_ptr);
+		pfree(L.d_gal_mirror);
+		pfree(L.d_gal_blk);
+		pfree(L.d_gal_pi);
+		pfree(L.d_gal_pj);
+		pfree(L.d_apptr);
+		pfree(L.d_apcol);
+		pfree(L.d_apval);
+		pfree(L.d_dense);
+		pfree(L.d_seg_ptr);
+		pfree(L.d_seg_row);
+		pfree(L.d_seg_off);
+		pfree(L.d_b);
+		pfree(L.d_x[0]);
+		pfree(L.d_x[1]);
+		pfree(L.d_d);
+		pfree(L.d_r);
+		pfree(L.d_partial);
 	}
 	s->amg.clear();
-	dfree(s->d_amg_desc);
-	dfree(s->d_amg_pf);
+	pfree(s->d_amg_desc);
+	pfree(s->d_amg_pf);
 	s->amg_npf = 0;
 	s->amg_wg = -1;
 }
@@ -2179,16 +2183,16 @@ int build_levels_host(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg) {
 		F.sa = smooth;
 		F.nnzp = static_cast<int64_t>(pcol.size());
 		BFMX_CHECK(upload(&F.d_agg, agg, st));
-		BFMX_CHECK(dalloc(&F.d_Pt, static_cast<size_t>(F.bs) * 3 * nf));
+		BFMX_CHECK(palloc(&F.d_Pt, static_cast<size_t>(F.bs) * 3 * nf));
 		BFMX_CHECK(upload(&F.d_agg_ptr, aptr, st));
 		BFMX_CHECK(upload(&F.d_agg_nodes, anodes, st));
 		BFMX_CHECK(upload(&F.d_pptr, pptr, st));
 		BFMX_CHECK(upload(&F.d_pcol, pcol, st));
-		BFMX_CHECK(dalloc(&F.d_P, static_cast<size_t>(F.bs) * 3 * pcol.size()));
+		BFMX_CHECK(palloc(&F.d_P, static_cast<size_t>(F.bs) * 3 * pcol.size()));
 		BFMX_CHECK(upload(&F.d_ppos, ppos, st));
 		BFMX_CHECK(upload(&F.d_rptr, rptr, st));
 		BFMX_CHECK(upload(&F.d_rent, rent, st));
-		BFMX_CHECK(dalloc(&F.d_R, static_cast<size_t>(F.bs) * 3 * pcol.size()));
+		BFMX_CHECK(palloc(&F.d_R, static_cast<size_t>(F.bs) * 3 * pcol.size()));
 		BFMX_CHECK(upload(&F.d_rrow, rrow, st));
 		BFMX_CHECK(upload(&F.d_gal_ptr, gptr, st));
 		BFMX_CHECK(upload(&F.d_gal_mirror, mirror, st));
@@ -2205,8 +2209,8 @@ int build_levels_host(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg) {
 		BFMX_CHECK(upload(&C.d_slice_ptr, CS.slice_ptr, st));
 		BFMX_CHECK(upload(&C.d_col, CS.col, st));
 		BFMX_CHECK(upload(&C.d_diag_blk, CS.diag_blk, st));
-		BFMX_CHECK(dalloc(&C.d_val64, 9 * static_cast<size_t>(CS.nnzb_pad)));
-		BFMX_CHECK(dalloc(&C.d_val32, 9 * static_cast<size_t>(CS.nnzb_pad)));
+		BFMX_CHECK(palloc(&C.d_val64, 9 * static_cast<size_t>(CS.nnzb_pad)));
+		BFMX_CHECK(palloc(&C.d_val32, 9 * static_cast<size_t>(CS.nnzb_pad)));
 		BFMX_HIP(hipStreamSynchronize(st));   // the host sources above are temporaries
 		fptr.swap(cptr);
 		fadj.swap(cadj);
@@ -2234,7 +2238,9 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 	G.slice_ptr = s->d_slice_ptr;
 	G.col = s->d_col;
 	G.row_len = s->d_row_len;
+	PhaseTimer phase(st, "multigrid levels");
 	BFMX_CHECK(topo_segments(st, G, fseg));
+	phase.mark("segments");
 	while (static_cast<int>(lv.size()) < kMaxLevels && seg_largest(fseg) > par.coarsest) {
 		int const lf = static_cast<int>(lv.size()) - 1;
 		int const nf = lv[lf].nb;
@@ -2242,9 +2248,11 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 		std::vector<int32_t> cseg;
 		int32_t* agg = nullptr;
 		int const ra = topo_aggregate(st, G, kAggSalt + static_cast<uint32_t>(lf), par.seed_frac, par.seed_growth,
+		                              par.host_agg_rows,
 		                              fseg, &agg, &na, cseg);
 		lv[lf].d_agg = agg;   // released with the hierarchy
 		BFMX_CHECK(ra);
+		phase.mark("aggregation");
 		if (!std::is_sorted(cseg.begin(), cseg.end()) || cseg[0] != 0 || cseg.back() != na) {
 			set_error("multigrid: aggregates do not follow the level's segments");
 			return BFMX_E_ARG;
@@ -2259,7 +2267,7 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 			}
 		}
 		if (big_a * 10 > big_f * 8) {
-			dfree(lv[lf].d_agg);
+			pfree(lv[lf].d_agg);
 			break;   // coarsening stalled
 		}
 		TopoLevel T;
@@ -2268,6 +2276,7 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 			topo_release(T);
 			return rt;
 		}
+		phase.mark("transfer patterns");
 		{
 			AmgLevel& F = lv[lf];
 			F.sa = smooth;
@@ -2297,8 +2306,8 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 			C.d_col = T.c_col;
 			C.d_diag_blk = T.c_diag_blk;
 			C.d_row_len = T.c_row_len;
-			BFMX_CHECK(dalloc(&C.d_val64, 9 * static_cast<size_t>(C.nnzb_pad)));
-			BFMX_CHECK(dalloc(&C.d_val32, 9 * static_cast<size_t>(C.nnzb_pad)));
+			BFMX_CHECK(palloc(&C.d_val64, 9 * static_cast<size_t>(C.nnzb_pad)));
+			BFMX_CHECK(palloc(&C.d_val32, 9 * static_cast<size_t>(C.nnzb_pad)));
 			G.n = na;
 			G.n_slices = C.n_slices;
 			G.nnzb_pad = C.nnzb_pad;
@@ -2307,10 +2316,11 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 			G.row_len = C.d_row_len;
 		}
 		AmgLevel& F = lv[lf];
-		BFMX_CHECK(dalloc(&F.d_Pt, static_cast<size_t>(F.bs) * 3 * nf));
-		BFMX_CHECK(dalloc(&F.d_P, static_cast<size_t>(F.bs) * 3 * F.nnzp));
-		BFMX_CHECK(dalloc(&F.d_R, static_cast<size_t>(F.bs) * 3 * F.nnzp));
-		BFMX_CHECK(dalloc(&F.d_apval, static_cast<size_t>(F.bs) * 3 * F.nnz_ap));
+		BFMX_CHECK(palloc(&F.d_Pt, static_cast<size_t>(F.bs) * 3 * nf));
+		BFMX_CHECK(palloc(&F.d_P, static_cast<size_t>(F.bs) * 3 * F.nnzp));
+		BFMX_CHECK(palloc(&F.d_R, static_cast<size_t>(F.bs) * 3 * F.nnzp));
+		BFMX_CHECK(palloc(&F.d_apval, static_cast<size_t>(F.bs) * 3 * F.nnz_ap));
+		phase.mark("level values");
 		fseg.swap(cseg);
 	}
 	return BFMX_OK;
@@ -2335,6 +2345,7 @@ int amg_build_impl(bfmx_system* s, bool device) {
 	}
 	int const rc = [&]() -> int {
 		hipStream_t const st = s->stream;
+		PhaseTimer phase(st, "multigrid build");
 		std::vector<AmgLevel>& lv = s->amg;
 		int64_t const n0 = s->n_nodes;
 		lv.emplace_back();
@@ -2356,12 +2367,12 @@ int amg_build_impl(bfmx_system* s, bool device) {
 			// batch, whose coordinates are gathered on the host) on the host
 			if (s->d_coords) {
 				double* part = nullptr;
-				BFMX_CHECK(dalloc(&L0.d_B0, 6 * n0));
-				BFMX_CHECK(dalloc(&part, 4 * kBboxGrid));
+				BFMX_CHECK(palloc(&L0.d_B0, 6 * n0));
+				BFMX_CHECK(palloc(&part, 4 * kBboxGrid));
 				k_bbox<<<kBboxGrid, kBlock, 0, st>>>(n0, s->d_coords, part);
 				k_modes0<<<grid(n0), kBlock, 0, st>>>(n0, part, s->d_coords, L0.d_B0);
 				hipError_t const e = hipStreamSynchronize(st);
-				dfree(part);
+				pfree(part);
 				BFMX_HIP(e);
 				BFMX_HIP(hipGetLastError());
 			} else {
@@ -2387,6 +2398,7 @@ int amg_build_impl(bfmx_system* s, bool device) {
 				BFMX_HIP(hipStreamSynchronize(st));
 			}
 		}
+		phase.mark("rigid-body modes");
 		bool const smooth = amg_smoothing();
 		AmgParams& par = s->amg_par;
 		par = AmgParams{};
@@ -2414,6 +2426,9 @@ int amg_build_impl(bfmx_system* s, bool device) {
 		if (const char* e = std::getenv("BFMX_AMG_SEED_GROWTH")) {
 			par.seed_growth = std::max(1.01, std::atof(e));
 		}
+		if (const char* e = std::getenv("BFMX_AMG_HOST_AGG")) {
+			par.host_agg_rows = std::max(0, std::atoi(e));
+		}
 		if (const char* e = std::getenv("BFMX_AMG_POWER")) {
 			par.power = std::max(0, std::atoi(e));
 		}
@@ -2423,17 +2438,18 @@ int amg_build_impl(bfmx_system* s, bool device) {
 		}
 		std::vector<int32_t> fseg;
 		BFMX_CHECK(device ? build_levels_device(s, smooth, fseg) : build_levels_host(s, smooth, fseg));
+		phase.mark("levels");
 		for (auto& L : lv) {
 			size_t const m = static_cast<size_t>(L.bs) * L.nb;
-			BFMX_CHECK(dalloc(&L.d_b, m));
-			BFMX_CHECK(dalloc(&L.d_x[0], m));
-			BFMX_CHECK(dalloc(&L.d_x[1], m));
-			BFMX_CHECK(dalloc(&L.d_d, m));
-			BFMX_CHECK(dalloc(&L.d_r, m));
-			BFMX_CHECK(dalloc(&L.d_dinv, static_cast<size_t>(L.bs) * L.bs * L.nb));
-			BFMX_CHECK(dalloc(&L.d_scal, kScalSize));
-			BFMX_CHECK(dalloc(&L.d_partial, grid(L.nb) + grid(static_cast<int64_t>(L.bs) * L.nb)));
-			BFMX_CHECK(dalloc(&L.d_B, 3 * m));
+			BFMX_CHECK(palloc(&L.d_b, m));
+			BFMX_CHECK(palloc(&L.d_x[0], m));
+			BFMX_CHECK(palloc(&L.d_x[1], m));
+			BFMX_CHECK(palloc(&L.d_d, m));
+			BFMX_CHECK(palloc(&L.d_r, m));
+			BFMX_CHECK(palloc(&L.d_dinv, static_cast<size_t>(L.bs) * L.bs * L.nb));
+			BFMX_CHECK(palloc(&L.d_scal, kScalSize));
+			BFMX_CHECK(palloc(&L.d_partial, grid(L.nb) + grid(static_cast<int64_t>(L.bs) * L.nb)));
+			BFMX_CHECK(palloc(&L.d_B, 3 * m));
 		}
 		{
 			AmgLevel& Lc = lv.back();
@@ -2449,7 +2465,7 @@ int amg_build_impl(bfmx_system* s, bool device) {
 					int64_t const n = static_cast<int64_t>(Lc.bs) * (fseg[c + 1] - fseg[c]);
 					off += n * n;
 				}
-				BFMX_CHECK(dalloc(&Lc.d_dense, static_cast<size_t>(entries)));
+				BFMX_CHECK(palloc(&Lc.d_dense, static_cast<size_t>(entries)));
 				Lc.n_dense = entries;
 				BFMX_CHECK(upload(&Lc.d_seg_ptr, fseg, st));
 				BFMX_CHECK(upload(&Lc.d_seg_row, seg_row, st));
@@ -2457,6 +2473,7 @@ int amg_build_impl(bfmx_system* s, bool device) {
 				BFMX_HIP(hipStreamSynchronize(st));
 			}
 		}
+		phase.mark("level vectors, coarsest segments");
 		if (const char* e = std::getenv("BFMX_AMG_VERBOSE"); e && e[0] == '1') {
 			for (size_t l = 0; l < lv.size(); l++) {
 				std::fprintf(stderr, "multigrid level %zu: %d rows of %d DOFs, %lld blocks (%lld with the slice padding)\n",

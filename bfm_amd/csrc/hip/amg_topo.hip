@@ -880,7 +880,8 @@ __global__ __launch_bounds__(kT) void k_gather_i32(int m, const int32_t* __restr
 
 // ------------------------------------------------------------------ host helpers
 
-// Device buffers freed when the scope ends (after the stream has drained).
+// Device buffers from the block cache, returned when the scope ends (after
+// the stream has drained).
 struct Temps {
 	hipStream_t st;
 	std::vector<void*> p;
@@ -888,12 +889,12 @@ struct Temps {
 	~Temps() {
 		(void) hipStreamSynchronize(st);
 		for (void* q : p) {
-			(void) hipFree(q);
+			pool_free_bytes(q);
 		}
 	}
 	template <typename T>
 	int get(T** out, size_t n) {
-		BFMX_CHECK(dalloc(out, n));
+		BFMX_CHECK(palloc(out, n));
 		p.push_back(*out);
 		return BFMX_OK;
 	}
@@ -960,9 +961,72 @@ int topo_segments(hipStream_t st, TopoGraph const& G, std::vector<int32_t>& seg)
 	return BFMX_OK;
 }
 
+namespace {
+
+// Aggregates of a small level by one sequential sweep on the host (the
+// level's graph copied down): a row whose neighbourhood is still free roots
+// an aggregate of that neighbourhood, then every other row joins the
+// aggregate of its first neighbour that has one (the rule of the host
+// build). A small level costs a few dozen independent-set rounds on the
+// device, each a handful of launches, for a sweep of a few thousand rows.
+int aggregate_on_host(hipStream_t st, TopoGraph const& G, std::vector<int32_t> const& fseg, int32_t* agg, int* na_out,
+                      std::vector<int32_t>& cseg) {
+	int const n = G.n;
+	std::vector<int32_t> sp(static_cast<size_t>(G.n_slices) + 1), col(static_cast<size_t>(G.nnzb_pad)), len(n);
+	BFMX_HIP(hipMemcpyAsync(sp.data(), G.slice_ptr, sp.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipMemcpyAsync(col.data(), G.col, col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipMemcpyAsync(len.data(), G.row_len, len.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	auto nbr = [&](int i, int k) { return col[static_cast<size_t>(sp[i >> 6]) + static_cast<size_t>(k) * kSlice + (i & 63)]; };
+	std::vector<int32_t> a(n, -1);
+	int na = 0;
+	for (int i = 0; i < n; i++) {
+		if (a[i] >= 0) {
+			continue;
+		}
+		bool free_nbhd = true;
+		for (int k = 0; k < len[i] && free_nbhd; k++) {
+			free_nbhd = a[nbr(i, k)] < 0;
+		}
+		if (free_nbhd) {
+			for (int k = 0; k < len[i]; k++) {
+				a[nbr(i, k)] = na;
+			}
+			a[i] = na++;
+		}
+	}
+	// a row left over saw an aggregated neighbour when the sweep passed it
+	std::vector<int32_t> out(a);
+	for (int i = 0; i < n; i++) {
+		for (int k = 0; k < len[i] && out[i] < 0; k++) {
+			out[i] = a[nbr(i, k)];
+		}
+		if (out[i] < 0) {
+			set_error("multigrid: a row found no aggregate");
+			return BFMX_E_ARG;
+		}
+	}
+	BFMX_HIP(hipMemcpyAsync(agg, out.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	cseg.assign(fseg.size(), na);
+	for (size_t c = 0; c + 1 < fseg.size(); c++) {
+		cseg[c] = out[fseg[c]];
+	}
+	*na_out = na;
+	return BFMX_OK;
+}
+
+}  // namespace
+
 int topo_aggregate(hipStream_t st, TopoGraph const& G, uint32_t salt, double seed_frac, double seed_growth,
-                   std::vector<int32_t> const& fseg, int32_t** agg_out, int* na_out, std::vector<int32_t>& cseg) {
+                   int host_rows, std::vector<int32_t> const& fseg, int32_t** agg_out, int* na_out,
+                   std::vector<int32_t>& cseg) {
+	if (G.n <= host_rows) {
+		BFMX_CHECK(palloc(agg_out, G.n));   // owned by the caller from here on
+		return aggregate_on_host(st, G, fseg, *agg_out, na_out, cseg);
+	}
 	Temps tmp(st);
+	PhaseTimer phase(st, "aggregation");
 	int const n = G.n;
 	uint8_t *state, *s1;
 	uint64_t *t0, *t1, *t2;
@@ -976,11 +1040,12 @@ int topo_aggregate(hipStream_t st, TopoGraph const& G, uint32_t salt, double see
 	BFMX_CHECK(tmp.get(&flag, n));
 	BFMX_CHECK(tmp.get(&rootid, n + 1));
 	BFMX_CHECK(tmp.get(&agg1, n));
-	BFMX_CHECK(dalloc(&agg, n));
+	BFMX_CHECK(palloc(&agg, n));
 	*agg_out = agg;   // owned by the caller from here on
 	bad = left + 1;
 	k_fill<uint8_t><<<nblk(n), kT, 0, st>>>(n, state, 1);
 	BFMX_CHECK(zeros(st, left, 2));
+	phase.mark("buffers");
 	// rounds in groups of kMisGroup between checks (a round after the last
 	// undecided row is gone changes nothing)
 	double thr = seed_frac * 1073741824.0;
@@ -1000,6 +1065,9 @@ int topo_aggregate(hipStream_t st, TopoGraph const& G, uint32_t salt, double see
 		int32_t l = 0;
 		BFMX_CHECK(read_one(st, left, &l));
 		if (l == 0) {
+			if (phase_times_on()) {
+				std::fprintf(stderr, "aggregation: %d rows, %d independent-set rounds\n", n, rounds);
+			}
 			break;
 		}
 		if (rounds >= kMisMaxRounds) {
@@ -1007,6 +1075,7 @@ int topo_aggregate(hipStream_t st, TopoGraph const& G, uint32_t salt, double see
 			return BFMX_E_UNSUPPORTED;
 		}
 	}
+	phase.mark("independent-set rounds");
 	k_is_root<<<nblk(n), kT, 0, st>>>(n, state, flag);
 	BFMX_CHECK(scan(st, flag, rootid, n, tmp));
 	k_attach_roots<<<nblk(n), kT, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, state, rootid, agg1);
@@ -1030,12 +1099,14 @@ int topo_aggregate(hipStream_t st, TopoGraph const& G, uint32_t salt, double see
 		set_error("multigrid: a row found no aggregate");
 		return BFMX_E_ARG;
 	}
+	phase.mark("attach");
 	*na_out = na;
 	return BFMX_OK;
 }
 
 int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na, bool smooth, TopoLevel& T) {
 	Temps tmp(st);
+	PhaseTimer phase(st, "transfer");
 	int const n = G.n;
 	int32_t *cnt, *fill, *flags;
 	BFMX_CHECK(tmp.get(&cnt, std::max(n, na) + 1));
@@ -1043,16 +1114,17 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	BFMX_CHECK(tmp.get(&flags, 1));
 	BFMX_CHECK(zeros(st, flags, 1));
 	// aggregate members, sorted
-	BFMX_CHECK(dalloc(&T.agg_ptr, na + 1));
-	BFMX_CHECK(dalloc(&T.agg_nodes, n));
+	BFMX_CHECK(palloc(&T.agg_ptr, na + 1));
+	BFMX_CHECK(palloc(&T.agg_nodes, n));
 	BFMX_CHECK(zeros(st, cnt, na + 1));
 	k_count_keys<<<nblk(n), kT, 0, st>>>(n, agg, cnt);
 	BFMX_CHECK(scan(st, cnt, T.agg_ptr, na, tmp));
 	BFMX_CHECK(zeros(st, fill, na + 1));
 	k_place_members<<<nblk(n), kT, 0, st>>>(n, agg, T.agg_ptr, fill, T.agg_nodes);
 	k_sort_lists<<<nblk(na), kT, 0, st>>>(na, T.agg_ptr, T.agg_nodes, nullptr);
+	phase.mark("aggregate members");
 	// prolongator pattern
-	BFMX_CHECK(dalloc(&T.pptr, n + 1));
+	BFMX_CHECK(palloc(&T.pptr, n + 1));
 	k_p_count<<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, agg, smooth ? 1 : 0, cnt, flags);
 	BFMX_CHECK(scan(st, cnt, T.pptr, n, tmp));
 	int32_t nnzp = 0, bad = 0;
@@ -1069,13 +1141,14 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 		return BFMX_E_ARG;
 	}
 	T.nnzp = nnzp;
-	BFMX_CHECK(dalloc(&T.pcol, nnzp));
-	BFMX_CHECK(dalloc(&T.ppos, static_cast<size_t>(G.nnzb_pad)));
+	BFMX_CHECK(palloc(&T.pcol, nnzp));
+	BFMX_CHECK(palloc(&T.ppos, static_cast<size_t>(G.nnzb_pad)));
 	BFMX_CHECK(zeros(st, T.ppos, static_cast<size_t>(G.nnzb_pad)));   // the padding rows' slots
 	k_p_fill<<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, agg, smooth ? 1 : 0, T.pptr, T.pcol,
 	                                       T.ppos);
+	phase.mark("prolongator pattern");
 	// restriction lists: P's columns, rows sorted
-	BFMX_CHECK(dalloc(&T.rptr, na + 1));
+	BFMX_CHECK(palloc(&T.rptr, na + 1));
 	k_r_lists<true><<<na, 64, 0, st>>>(na, smooth ? 1 : 0, G.slice_ptr, G.col, G.row_len, T.agg_ptr, T.agg_nodes, T.pptr, T.pcol,
 	                                   cnt, nullptr, nullptr, nullptr, flags);
 	BFMX_CHECK(scan(st, cnt, T.rptr, na, tmp));
@@ -1088,12 +1161,13 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 		          kSetCap);
 		return bad ? BFMX_E_UNSUPPORTED : BFMX_E_ARG;
 	}
-	BFMX_CHECK(dalloc(&T.rent, nnzp));
-	BFMX_CHECK(dalloc(&T.rrow, nnzp));
+	BFMX_CHECK(palloc(&T.rent, nnzp));
+	BFMX_CHECK(palloc(&T.rrow, nnzp));
 	k_r_lists<false><<<na, 64, 0, st>>>(na, smooth ? 1 : 0, G.slice_ptr, G.col, G.row_len, T.agg_ptr, T.agg_nodes, T.pptr, T.pcol,
 	                                    nullptr, T.rptr, T.rrow, T.rent, flags);
+	phase.mark("restriction lists");
 	// pattern of A P
-	BFMX_CHECK(dalloc(&T.apptr, n + 1));
+	BFMX_CHECK(palloc(&T.apptr, n + 1));
 	k_ap_pattern<true><<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, cnt, nullptr,
 	                                                 nullptr, flags);
 	{
@@ -1117,9 +1191,10 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	int32_t nap = 0;
 	BFMX_CHECK(read_one(st, T.apptr + n, &nap));
 	T.nnz_ap = nap;
-	BFMX_CHECK(dalloc(&T.apcol, nap));
+	BFMX_CHECK(palloc(&T.apcol, nap));
 	k_ap_pattern<false><<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, nullptr,
 	                                                  T.apptr, T.apcol, flags);
+	phase.mark("A P pattern");
 	// coarse pattern
 	int64_t* cptr;
 	BFMX_CHECK(tmp.get(&cptr, na + 1));
@@ -1135,6 +1210,7 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	int32_t* cadj;
 	BFMX_CHECK(tmp.get(&cadj, static_cast<size_t>(cnnz)));
 	k_c_pattern<false><<<na, 64, 0, st>>>(na, T.rptr, T.rrow, T.apptr, T.apcol, nullptr, cptr, cadj, flags);
+	phase.mark("coarse pattern");
 	// coarse sliced ELL
 	T.c_slices = (na + kSlice - 1) / kSlice;
 	T.c_nnzb = cnnz;
@@ -1151,38 +1227,39 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 		return BFMX_E_ARG;
 	}
 	T.c_nnzb_pad = pad;
-	BFMX_CHECK(dalloc(&T.c_slice_ptr, static_cast<size_t>(T.c_slices) + 1));
+	BFMX_CHECK(palloc(&T.c_slice_ptr, static_cast<size_t>(T.c_slices) + 1));
 	BFMX_CHECK(scan(st, sblocks, T.c_slice_ptr, T.c_slices, tmp));
-	BFMX_CHECK(dalloc(&T.c_col, static_cast<size_t>(pad)));
-	BFMX_CHECK(dalloc(&T.c_diag_blk, na));
-	BFMX_CHECK(dalloc(&T.c_row_len, na));
+	BFMX_CHECK(palloc(&T.c_col, static_cast<size_t>(pad)));
+	BFMX_CHECK(palloc(&T.c_diag_blk, na));
+	BFMX_CHECK(palloc(&T.c_row_len, na));
 	k_c_sell<<<nblk(T.c_slices * kSlice), kT, 0, st>>>(T.c_slices * kSlice, na, cptr, cadj, T.c_slice_ptr, T.c_col,
 	                                                   T.c_diag_blk, T.c_row_len);
 	// mirrors of the coarse slots
-	BFMX_CHECK(dalloc(&T.mirror, static_cast<size_t>(pad)));
+	BFMX_CHECK(palloc(&T.mirror, static_cast<size_t>(pad)));
 	k_fill<int32_t><<<nblk(pad), kT, 0, st>>>(pad, T.mirror, -1);
 	k_mirror<<<nblk(na), kT, 0, st>>>(na, cptr, cadj, T.c_slice_ptr, T.mirror);
 	BFMX_HIP(hipGetLastError());
 	BFMX_HIP(hipStreamSynchronize(st));
+	phase.mark("coarse sliced ELL");
 	return BFMX_OK;
 }
 
 void topo_release(TopoLevel& T) {
-	dfree(T.agg_ptr);
-	dfree(T.agg_nodes);
-	dfree(T.pptr);
-	dfree(T.pcol);
-	dfree(T.ppos);
-	dfree(T.rptr);
-	dfree(T.rent);
-	dfree(T.rrow);
-	dfree(T.c_slice_ptr);
-	dfree(T.c_col);
-	dfree(T.c_diag_blk);
-	dfree(T.c_row_len);
-	dfree(T.apptr);
-	dfree(T.apcol);
-	dfree(T.mirror);
+	pfree(T.agg_ptr);
+	pfree(T.agg_nodes);
+	pfree(T.pptr);
+	pfree(T.pcol);
+	pfree(T.ppos);
+	pfree(T.rptr);
+	pfree(T.rent);
+	pfree(T.rrow);
+	pfree(T.c_slice_ptr);
+	pfree(T.c_col);
+	pfree(T.c_diag_blk);
+	pfree(T.c_row_len);
+	pfree(T.apptr);
+	pfree(T.apcol);
+	pfree(T.mirror);
 }
 
 }  // namespace bfmx

@@ -51,6 +51,17 @@ void set_error(const char* fmt, ...);
 		}                         \
 	} while (0)
 
+// BFMX_SOLVE_TIMES=1: the set-up and solve phases on stderr. Each mark
+// waits for the stream, so timed runs are slightly slower than plain ones.
+bool phase_times_on();
+struct PhaseTimer {
+	hipStream_t st;
+	const char* scope;
+	double t;   // host clock at the last mark, ms
+	PhaseTimer(hipStream_t stream, const char* what);
+	void mark(const char* what);
+};
+
 // Device-resident scalars of one CG run. Written by the one-workgroup reduce
 // kernels, read by later kernels (kernel boundaries order the accesses).
 struct alignas(128) Scalars {
@@ -231,6 +242,7 @@ struct AmgParams {
 	int power_coarse = 10;  // ... on the coarser levels
 	double seed_frac = 0.002;   // device aggregation: rows that may seed a region in the first round
 	double seed_growth = 1.25;  // ... and that fraction's growth per round
+	int host_agg_rows = 16384;  // device build: levels up to this many rows are aggregated on the host
 };
 
 struct Graph {
@@ -531,6 +543,28 @@ template <typename T>
 void dfree(T*& p) {
 	if (p) {
 		(void) hipFree(p);
+		p = nullptr;
+	}
+}
+
+// Device blocks from a per-device cache (pool.hip): pool_free_bytes takes a
+// block back once every stream that used it has drained; a pointer the pool
+// did not hand out goes to hipFree.
+int pool_alloc_bytes(void** p, size_t bytes);
+void pool_free_bytes(void* p);
+
+template <typename T>
+int palloc(T** p, size_t n) {
+	void* q = nullptr;
+	int const rc = pool_alloc_bytes(&q, (n ? n : 1) * sizeof(T));
+	*p = static_cast<T*>(q);
+	return rc;
+}
+
+template <typename T>
+void pfree(T*& p) {
+	if (p) {
+		pool_free_bytes(const_cast<void*>(static_cast<const void*>(p)));
 		p = nullptr;
 	}
 }

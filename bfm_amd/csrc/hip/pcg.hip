@@ -1244,6 +1244,39 @@ __global__ __launch_bounds__(kRed) void k_mr_reduce(int nwg, const double* __res
 	}
 }
 
+}  // namespace
+
+bool phase_times_on() {
+	static bool const on = [] {
+		const char* e = std::getenv("BFMX_SOLVE_TIMES");
+		return e && e[0] == '1';
+	}();
+	return on;
+}
+
+static double host_ms() {
+	return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+PhaseTimer::PhaseTimer(hipStream_t stream, const char* what) : st(stream), scope(what), t(0) {
+	if (phase_times_on()) {
+		(void) hipStreamSynchronize(st);
+		t = host_ms();
+	}
+}
+
+void PhaseTimer::mark(const char* what) {
+	if (!phase_times_on()) {
+		return;
+	}
+	(void) hipStreamSynchronize(st);
+	double const now = host_ms();
+	std::fprintf(stderr, "%s phase %-32s %8.3f ms\n", scope, what, now - t);
+	t = now;
+}
+
+namespace {
+
 // Totals of the NV partial sums of the last kernel (host copy, synchronous).
 template <int NV>
 int mr_sums(hipStream_t st, int nwg, const double* partial, double* d_out, double (&h)[NV]) {
@@ -1441,32 +1474,16 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	int64_t const n = s->n_nodes;
 	int64_t const cap = max_iter ? static_cast<int64_t>(max_iter) : 20 * 2 * n;
 	hipStream_t const st = s->stream;
-	// BFMX_SOLVE_TIMES=1: the phases of this call on stderr (each phase
-	// waited for, so the total is slightly longer than without)
-	static bool const times = [] {
-		const char* e = std::getenv("BFMX_SOLVE_TIMES");
-		return e && e[0] == '1';
-	}();
-	auto tp = std::chrono::steady_clock::now();
-	auto phase = [&](const char* what) {
-		if (!times) {
-			return;
-		}
-		(void) hipStreamSynchronize(st);
-		auto const now = std::chrono::steady_clock::now();
-		std::fprintf(stderr, "solve phase %-28s %8.3f ms\n", what,
-		             std::chrono::duration<double, std::milli>(now - tp).count());
-		tp = now;
-	};
+	PhaseTimer phase(st, "solve");
 	BFMX_CHECK(ensure_workspace(s));
-	phase("workspace");
+	phase.mark("workspace");
 	if ((mixed || amg || learned) && !s->d_val32) {
 		BFMX_CHECK(dalloc(&s->d_val32, 4 * s->nnzb_pad));   // before any capture reads its address
 	}
 	if (amg) {
 		BFMX_CHECK(amg_build(s));   // topology, once per system
 		BFMX_CHECK(fcg_segments(s));
-		phase("multigrid topology");
+		phase.mark("multigrid topology");
 	}
 	if (learned) {
 		BFMX_CHECK(mpnn_prepare(s));
@@ -1520,10 +1537,10 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	if (mixed || amg || learned) {
 		BFMX_CHECK(ensure_f32(s));
 	}
-	phase("right-hand side, FP32 copy");
+	phase.mark("right-hand side, FP32 copy");
 	if (amg) {
 		BFMX_CHECK(amg_setup_numeric(s));
-		phase("multigrid numeric set-up");
+		phase.mark("multigrid numeric set-up");
 	}
 	if (learned) {
 		BFMX_CHECK(mpnn_setup_numeric(s));
@@ -1532,7 +1549,7 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	// flexible CG stops when its residual stops halving for long (a
 	// divergent or stalled preconditioner) rather than running to the cap
 	BFMX_CHECK(run_chunks(s, precond, mixed, rtol2, cap, false, learned || amg, &last));
-	phase("iterations (with captures)");
+	phase.mark("iterations (with captures)");
 	int64_t iters = last.iter;
 	int32_t fallback = 0;
 	int64_t fb_iters = 0;
