@@ -242,7 +242,7 @@ struct AmgParams {
 	int power_coarse = 10;  // ... on the coarser levels
 	double seed_frac = 0.002;   // device aggregation: rows that may seed a region in the first round
 	double seed_growth = 1.25;  // ... and that fraction's growth per round
-	int host_agg_rows = 16384;  // device build: levels up to this many rows are aggregated on the host
+	int host_agg_rows = 131072;  // device build: levels up to this many rows are aggregated on the host
 };
 
 struct Graph {

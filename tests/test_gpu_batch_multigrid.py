@@ -138,12 +138,14 @@ def test_disconnected_mesh_one_dense_inverse_per_part():
 def test_multigrid_set_up_threads_give_the_same_hierarchy(monkeypatch):
     """The host build of the transfer operators (BFMX_AMG_BUILD=host) runs one
     thread per chunk of whole segments; one thread and sixteen give bitwise
-    the same solve. The device build of the same batch (the default) also
-    meets the tolerance on every panel, and twice gives bitwise the same
-    solve."""
+    the same solve. The device build of the same batch (the default, whose
+    finest level is swept on host threads the same way) also meets the
+    tolerance on every panel, and gives bitwise the same solve with one
+    thread and with sixteen."""
     meshes = [meshgen.panel(500 + k, 1000, 4000) for k in range(20)]
     dev = []
-    for _ in range(2):
+    for threads in ("1", "16"):   # the finest level's sweeps run on host threads
+        monkeypatch.setenv("BFMX_AMG_THREADS", threads)
         pb = D.Batch.from_instances([D.panel_instance(m, t) for m, t in meshes])
         st, _ = pb.solve(precond=D.PRECOND_AMG, rtol=1e-8)
         assert all(x["status"] == 0 for x in st)
@@ -167,11 +169,17 @@ def test_multigrid_set_up_threads_give_the_same_hierarchy(monkeypatch):
 def test_segmented_cg_takes_the_iterations_of_the_hardest_instance(monkeypatch):
     """Every instance runs its own CG scalars: the batch needs about the
     iterations of its hardest instance solved alone (shared scalars needed
-    more, growing with the batch: 30 at 16 panels, 51 at 4096). Every level
-    is aggregated on the device here, alone and together: a small level is
-    otherwise swept on the host, which gives an instance alone other (and
-    better) aggregates than the same instance within the batch's union."""
-    monkeypatch.setenv("BFMX_AMG_HOST_AGG", "0")
+    more, growing with the batch: 30 at 16 panels, 51 at 4096). Run twice:
+    with the default aggregation (host sweeps over whole segments: every
+    instance of the union gets the aggregates it gets alone) and
+    with MIS-2 aggregates on the GPU at every level (BFMX_AMG_HOST_AGG=0)."""
+    for host_agg in (None, "0"):
+        if host_agg is not None:
+            monkeypatch.setenv("BFMX_AMG_HOST_AGG", host_agg)
+        segmented_vs_alone()
+
+
+def segmented_vs_alone():
     meshes = [meshgen.panel(600 + k, 2000, 12000) for k in range(8)]
     solo = []
     for m, t in meshes:

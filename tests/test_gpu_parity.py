@@ -446,13 +446,17 @@ def test_amg_smoothed_against_unsmoothed_aggregation(monkeypatch):
 
 
 @pytest.mark.parametrize("kind", [4, 3])
-def test_amg_device_and_host_hierarchies(monkeypatch, kind):
-    """The hierarchy topology built on the GPU (the default: MIS-2
-    aggregates, the two-step Galerkin product) and on the host
-    (BFMX_AMG_BUILD=host: sequential aggregation, contribution lists) both
-    lead flexible CG to the oracle's solution; two fresh systems built on the
-    GPU give bitwise the same solution (the device build does not depend on
-    scheduling)."""
+@pytest.mark.parametrize("host_agg", ["0", None])
+def test_amg_device_and_host_hierarchies(monkeypatch, kind, host_agg):
+    """The hierarchy topology built on the GPU (the default: aggregates by
+    host sweeps on the finest and the small levels, the two-step Galerkin
+    product on the GPU; with BFMX_AMG_HOST_AGG=0 MIS-2 aggregates on the GPU
+    at every level) and on the host (BFMX_AMG_BUILD=host: sequential
+    aggregation, contribution lists) all lead flexible CG to the oracle's
+    solution; two fresh systems built on the GPU give bitwise the same
+    solution (the device build does not depend on scheduling)."""
+    if host_agg is not None:
+        monkeypatch.setenv("BFMX_AMG_HOST_AGG", host_agg)
     import scipy.sparse.linalg as spla
     mesh = meshgen.beam(240, 60, kind=kind, jitter=0.2, seed=13)
     mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=13)
@@ -466,11 +470,32 @@ def test_amg_device_and_host_hierarchies(monkeypatch, kind):
     monkeypatch.setenv("BFMX_AMG_BUILD", "host")
     host = D.beam_system(mesh)
     u3, st3 = host.solve(precond=D.PRECOND_AMG, rtol=1e-9)
-    print(f"kind {kind}: device build {st1['iterations']} iterations (error {rel_err(u1, u_ref):.2e}), "
+    print(f"kind {kind}, host_agg {host_agg}: device build {st1['iterations']} iterations "
+          f"(error {rel_err(u1, u_ref):.2e}), "
           f"host build {st3['iterations']} (error {rel_err(u3, u_ref):.2e})")
     assert st1["status"] == 0 and st3["status"] == 0
     assert st1["iterations"] == st2["iterations"] and np.array_equal(u1, u2)
     assert rel_err(u1, u_ref) <= U_TOL and rel_err(u3, u_ref) <= U_TOL
+
+
+def test_finest_sweeps_do_not_depend_on_threads(monkeypatch):
+    """A beam of 90,000 nodes: its finest level is aggregated by one
+    sequential sweep on the host; one host thread and sixteen give bitwise
+    the same solve, which meets the tolerance on the host's residual."""
+    mesh = meshgen.beam(599, 149, kind=4, jitter=0.2, seed=21)
+    assert mesh.n_nodes > 65536
+    out = []
+    for threads in ("1", "16"):
+        monkeypatch.setenv("BFMX_AMG_THREADS", threads)
+        s = D.beam_system(mesh)
+        u, st = s.solve(precond=D.PRECOND_AMG, rtol=1e-8)
+        assert st["status"] == 0
+        out.append((st["iterations"], u))
+        K, b = s.get_csr_scipy(), s.get_rhs()
+        assert np.linalg.norm(b - K @ u) <= 1e-8 * np.linalg.norm(b)
+        s.close()
+    print(f"{2 * mesh.n_nodes} DOFs: {out[0][0]} iterations")
+    assert out[0][0] == out[1][0] and np.array_equal(out[0][1], out[1][1])
 
 
 def test_amg_single_level_is_a_direct_solve():
