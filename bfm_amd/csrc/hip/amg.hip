@@ -626,8 +626,16 @@ __global__ __launch_bounds__(kBlock) void k_galerkin(int64_t nslots, const int32
 // The Galerkin product in two steps (levels built on the device; the
 // patterns come from amg_topo.hip). Step 1, A P: entry (i, J) of row i =
 // sum over the row's blocks A_ij (FP32 copy) with J in P's row j of
-// A_ij P_jJ, summed in FP64 (one thread per row, one entry at a time, so the
-// sums stay in registers), stored FP32.
+// A_ij P_jJ, summed in FP64 and stored FP32. One thread per row takes the
+// row's entries kApWindow at a time (their sums in registers): for each of
+// the row's blocks it scans P's row j once (columns sorted) and adds the
+// products of the columns that fall in the window, so a row's blocks and P
+// rows are read once per window instead of once per entry.
+#ifndef BFMX_AP_WINDOW
+#define BFMX_AP_WINDOW 4
+#endif
+constexpr int kApWindow = BFMX_AP_WINDOW;
+
 template <int BS>
 __global__ __launch_bounds__(kBlock) void k_ap(int nb, const int32_t* __restrict__ slice_ptr,
                                                const int32_t* __restrict__ col, const int32_t* __restrict__ row_len,
@@ -635,6 +643,7 @@ __global__ __launch_bounds__(kBlock) void k_ap(int nb, const int32_t* __restrict
                                                const int32_t* __restrict__ pcol, const float* __restrict__ P,
                                                const int32_t* __restrict__ apptr, const int32_t* __restrict__ apcol,
                                                float* __restrict__ apval) {
+	constexpr int W = kApWindow;
 	int const i = blockIdx.x * kBlock + threadIdx.x;
 	if (i >= nb) {
 		return;
@@ -642,45 +651,88 @@ __global__ __launch_bounds__(kBlock) void k_ap(int nb, const int32_t* __restrict
 	int const r = i & 63;
 	int32_t const sbeg = slice_ptr[i >> 6] + r;
 	int const len = row_len[i];
-	for (int32_t q = apptr[i]; q < apptr[i + 1]; q++) {
-		int32_t const J = apcol[q];
-		double acc[BS][3] = {};
-		for (int k = 0; k < len; k++) {
-			int32_t const blk = sbeg + k * kSlice;
-			int const j = col[blk];
-			int32_t f = -1;
-			for (int32_t g = pptr[j]; g < pptr[j + 1]; g++) {
-				int32_t const c = pcol[g];
-				if (c >= J) {
-					f = c == J ? g : -1;
-					break;
-				}
-			}
-			if (f < 0) {
-				continue;
-			}
-			float A[BS][BS];
-			ld_block<BS>(val32, blk, r, A);
-			const float* pj = P + BS * 3 * static_cast<int64_t>(f);
+	int32_t const q1 = apptr[i + 1];
+	for (int32_t qa = apptr[i]; qa < q1; qa += W) {
+		int const nw = min(W, q1 - qa);
+		int32_t Jw[W];
+#pragma unroll
+		for (int w = 0; w < W; w++) {
+			Jw[w] = w < nw ? apcol[qa + w] : INT32_MAX;
+		}
+		int32_t const jlo = Jw[0], jhi = Jw[nw - 1];
+		double acc[W][BS][3];
+#pragma unroll
+		for (int w = 0; w < W; w++) {
 #pragma unroll
 			for (int a = 0; a < BS; a++) {
 #pragma unroll
 				for (int c = 0; c < 3; c++) {
-					double t = 0;
-#pragma unroll
-					for (int m = 0; m < BS; m++) {
-						t += static_cast<double>(A[a][m]) * static_cast<double>(pj[3 * m + c]);
-					}
-					acc[a][c] += t;
+					acc[w][a][c] = 0;
 				}
 			}
 		}
-		float* out = apval + BS * 3 * static_cast<int64_t>(q);
+		for (int k = 0; k < len; k++) {
+			int32_t const blk = sbeg + k * kSlice;
+			int const j = col[blk];
+			int32_t const g1 = pptr[j + 1];
+			bool have_a = false;
+			float A[BS][BS];
+			for (int32_t g = pptr[j]; g < g1; g++) {
+				int32_t const cg = pcol[g];
+				if (cg > jhi) {
+					break;
+				}
+				if (cg < jlo) {
+					continue;
+				}
+				if (!have_a) {
+					ld_block<BS>(val32, blk, r, A);
+					have_a = true;
+				}
+				const float* pj = P + BS * 3 * static_cast<int64_t>(g);
+				float pv[BS * 3];
 #pragma unroll
-		for (int a = 0; a < BS; a++) {
+				for (int e = 0; e < BS * 3; e++) {
+					pv[e] = pj[e];
+				}
+				double tv[BS][3];
 #pragma unroll
-			for (int c = 0; c < 3; c++) {
-				out[3 * a + c] = static_cast<float>(acc[a][c]);
+				for (int a = 0; a < BS; a++) {
+#pragma unroll
+					for (int c = 0; c < 3; c++) {
+						double s = 0;
+#pragma unroll
+						for (int m = 0; m < BS; m++) {
+							s += static_cast<double>(A[a][m]) * static_cast<double>(pv[3 * m + c]);
+						}
+						tv[a][c] = s;
+					}
+				}
+#pragma unroll
+				for (int w = 0; w < W; w++) {
+					if (Jw[w] == cg) {
+#pragma unroll
+						for (int a = 0; a < BS; a++) {
+#pragma unroll
+							for (int c = 0; c < 3; c++) {
+								acc[w][a][c] += tv[a][c];
+							}
+						}
+					}
+				}
+			}
+		}
+#pragma unroll
+		for (int w = 0; w < W; w++) {
+			if (w < nw) {
+				float* out = apval + BS * 3 * static_cast<int64_t>(qa + w);
+#pragma unroll
+				for (int a = 0; a < BS; a++) {
+#pragma unroll
+					for (int c = 0; c < 3; c++) {
+						out[3 * a + c] = static_cast<float>(acc[w][a][c]);
+					}
+				}
 			}
 		}
 	}
