@@ -1985,37 +1985,59 @@ void amg_release(bfmx_system* s) {
 
 namespace {
 
-// The levels below level 0, built on the host from the node adjacency
-// (s->adj_ptr, s->adj), by up to kBuildThreads threads over chunks of whole
-// segments. fseg receives the coarsest level's segments.
-int build_levels_host(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg) {
-	hipStream_t const st = s->stream;
-	std::vector<AmgLevel>& lv = s->amg;
-	int64_t const n0 = s->n_nodes;
-	AmgParams const& par = s->amg_par;
-	// graph of the current finest-so-far level, the block slot of each
-	// entry and the level's slice pointers
+// The host build's state for the level being coarsened: its graph (fptr,
+// fadj: sorted rows with the diagonal), the block slot of each graph entry,
+// the level's slice pointers and its segments.
+struct HostLevelGraph {
 	std::vector<int64_t> fptr;
 	std::vector<int32_t> fadj;
-	std::vector<int32_t> fslot(s->nnzb);
-	std::vector<int32_t> fslice(s->slice_ptr);
+	std::vector<int32_t> fslot;
+	std::vector<int32_t> fslice;
+};
+
+int host_levels_from(bfmx_system* s, bool smooth, bool finest, HostLevelGraph& H, std::vector<int32_t>& fseg);
+
+// The levels below level 0, built on the host from the node adjacency
+// (s->adj_ptr, s->adj). fseg receives the coarsest level's segments.
+int build_levels_host(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg) {
+	int64_t const n0 = s->n_nodes;
+	HostLevelGraph H;
+	H.fslot.resize(s->nnzb);
+	H.fslice = s->slice_ptr;
 	for (int64_t i = 0; i < n0; i++) {
 		for (int64_t k = 0; k < s->adj_ptr[i + 1] - s->adj_ptr[i]; k++) {
-			fslot[s->adj_ptr[i] + k] = static_cast<int32_t>(s->slice_ptr[i >> 6] + k * kSlice + (i & 63));
+			H.fslot[s->adj_ptr[i] + k] = static_cast<int32_t>(s->slice_ptr[i >> 6] + k * kSlice + (i & 63));
 		}
 	}
-	// coarsen to at most par.coarsest block rows (the one-workgroup
-	// dense inverse of the coarsest operator costs O(n^3) per set-up:
-	// ~0.1 ms at 192 DOFs, ~40 ms at 500)
 	// segments of the current level: aggregates never span two, and the
 	// first row of a segment is an aggregate root (its neighbours are all
 	// in the segment and not yet aggregated), so the aggregates of a
 	// segment are numbered consecutively and give the next level's segments
 	fseg = segments(static_cast<int>(n0), s->adj_ptr.data(), s->adj.data());
+	return host_levels_from(s, smooth, true, H, fseg);
+}
+
+// The levels below the current coarsest one (s->amg.back()), built on the
+// host from its graph H (finest: the system's own graph s->adj_ptr / s->adj
+// instead of H's), by up to kBuildThreads threads over chunks of whole
+// segments; fseg: the level's segments, on return the coarsest level's.
+int host_levels_from(bfmx_system* s, bool smooth, bool finest, HostLevelGraph& H, std::vector<int32_t>& fseg) {
+	hipStream_t const st = s->stream;
+	std::vector<AmgLevel>& lv = s->amg;
+	AmgParams const& par = s->amg_par;
+	std::vector<int64_t>& fptr = H.fptr;
+	std::vector<int32_t>& fadj = H.fadj;
+	std::vector<int32_t>& fslot = H.fslot;
+	std::vector<int32_t>& fslice = H.fslice;
+	int const lf0 = static_cast<int>(lv.size()) - 1;
+	// coarsen to at most par.coarsest block rows (the one-workgroup
+	// dense inverse of the coarsest operator costs O(n^3) per set-up:
+	// ~0.1 ms at 192 DOFs, ~40 ms at 500)
 	while (static_cast<int>(lv.size()) < kMaxLevels && seg_largest(fseg) > par.coarsest) {
 		int const lf = static_cast<int>(lv.size()) - 1;
-		const int64_t* ptr = lf == 0 ? s->adj_ptr.data() : fptr.data();
-		const int32_t* adj = lf == 0 ? s->adj.data() : fadj.data();
+		bool const sys = finest && lf == lf0;
+		const int64_t* ptr = sys ? s->adj_ptr.data() : fptr.data();
+		const int32_t* adj = sys ? s->adj.data() : fadj.data();
 		int const nf = lv[lf].nb;
 		// the rows split at segment boundaries into chunks of similar size,
 		// one thread each: the aggregates, prolongator rows, restriction
@@ -2314,6 +2336,38 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 	while (static_cast<int>(lv.size()) < kMaxLevels && seg_largest(fseg) > par.coarsest) {
 		int const lf = static_cast<int>(lv.size()) - 1;
 		int const nf = lv[lf].nb;
+		if (nf <= par.host_level_rows) {
+			// a small level and those below it by the host build
+			if (lf == 0 && s->adj_ptr.size() == static_cast<size_t>(nf) + 1) {
+				return build_levels_host(s, smooth, fseg);
+			}
+			HostLevelGraph H;
+			std::vector<int32_t> len(nf);
+			H.fslice.resize(static_cast<size_t>(G.n_slices) + 1);
+			std::vector<int32_t> col(static_cast<size_t>(G.nnzb_pad));
+			BFMX_HIP(hipMemcpyAsync(H.fslice.data(), G.slice_ptr, H.fslice.size() * sizeof(int32_t),
+			                        hipMemcpyDeviceToHost, st));
+			BFMX_HIP(hipMemcpyAsync(col.data(), G.col, col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+			BFMX_HIP(hipMemcpyAsync(len.data(), G.row_len, len.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+			BFMX_HIP(hipStreamSynchronize(st));
+			H.fptr.assign(static_cast<size_t>(nf) + 1, 0);
+			for (int i = 0; i < nf; i++) {
+				H.fptr[i + 1] = H.fptr[i] + len[i];
+			}
+			H.fadj.resize(H.fptr[nf]);
+			H.fslot.resize(H.fptr[nf]);
+			for (int i = 0; i < nf; i++) {
+				for (int k = 0; k < len[i]; k++) {
+					int32_t const slot = H.fslice[i >> 6] + k * kSlice + (i & 63);
+					H.fadj[H.fptr[i] + k] = col[slot];
+					H.fslot[H.fptr[i] + k] = slot;
+				}
+			}
+			phase.mark("level graph to the host");
+			int const rc = host_levels_from(s, smooth, false, H, fseg);
+			phase.mark("host build of the small levels");
+			return rc;
+		}
 		int na = 0;
 		std::vector<int32_t> cseg;
 		int32_t* agg = nullptr;
@@ -2521,6 +2575,9 @@ int amg_build_impl(bfmx_system* s, bool device) {
 		}
 		if (const char* e = std::getenv("BFMX_AMG_HOST_AGG")) {
 			par.host_agg_rows = std::max(0, std::atoi(e));
+		}
+		if (const char* e = std::getenv("BFMX_AMG_HOST_LEVELS")) {
+			par.host_level_rows = std::max(0, std::atoi(e));
 		}
 		if (const char* e = std::getenv("BFMX_AMG_POWER")) {
 			par.power = std::max(0, std::atoi(e));

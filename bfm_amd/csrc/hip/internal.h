@@ -243,6 +243,11 @@ struct AmgParams {
 	double seed_frac = 0.002;   // device aggregation: rows that may seed a region in the first round
 	double seed_growth = 1.25;  // ... and that fraction's growth per round
 	int host_agg_rows = 131072;  // device build: levels up to this many rows are aggregated on the host
+	// device build: from a level of at most this many rows on, the host build
+	// (off: the host's Galerkin lists grow with the wide stencils of the
+	// coarse levels; 7112 rows of the 2M-DOF beam's level 2 took 11.9 ms
+	// there against about 2 ms for the device build of levels 2 to 4)
+	int host_level_rows = 0;
 };
 
 struct Graph {

@@ -446,19 +446,22 @@ def test_amg_smoothed_against_unsmoothed_aggregation(monkeypatch):
 
 
 @pytest.mark.parametrize("kind", [4, 3])
-@pytest.mark.parametrize("host_agg", ["0", None])
+@pytest.mark.parametrize("host_agg", ["0", "levels", None])
 def test_amg_device_and_host_hierarchies(monkeypatch, kind, host_agg):
     """The hierarchy topology built on the GPU (the default: aggregates by
-    host sweeps on the finest and the small levels, the two-step Galerkin
-    product on the GPU; with BFMX_AMG_HOST_AGG=0 MIS-2 aggregates on the GPU
-    at every level) and on the host (BFMX_AMG_BUILD=host: sequential
-    aggregation, contribution lists) all lead flexible CG to the oracle's
-    solution; two fresh systems built on the GPU give bitwise the same
-    solution (the device build does not depend on scheduling)."""
-    if host_agg is not None:
-        monkeypatch.setenv("BFMX_AMG_HOST_AGG", host_agg)
+    host sweeps up to 131072 rows, the transfer operators and the two-step
+    Galerkin product on the GPU; with BFMX_AMG_HOST_AGG=0 MIS-2 aggregates on
+    the GPU at every level; with BFMX_AMG_HOST_LEVELS=16384 the levels from
+    16384 rows down by the host build) and on the host (BFMX_AMG_BUILD=host:
+    sequential aggregation, contribution lists) all lead flexible CG to the
+    oracle's solution; two fresh systems built on the GPU give bitwise the
+    same solution (the device build does not depend on scheduling)."""
+    if host_agg == "0":
+        monkeypatch.setenv("BFMX_AMG_HOST_AGG", "0")
+    elif host_agg == "levels":
+        monkeypatch.setenv("BFMX_AMG_HOST_LEVELS", "16384")
     import scipy.sparse.linalg as spla
-    mesh = meshgen.beam(240, 60, kind=kind, jitter=0.2, seed=13)
+    mesh = meshgen.beam(300, 75, kind=kind, jitter=0.2, seed=13)
     mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=13)
     A, b = oracle_csr(mesh, True)
     u_ref = spla.spsolve(A.to_scipy().tocsc(), b)
