@@ -10,8 +10,13 @@
 // visit order is reversed. Node-level ordering keeps the 2x2 blocks intact.
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 #include "internal.h"
 
@@ -19,49 +24,139 @@ namespace bfmx {
 
 namespace {
 
+constexpr int kPrepThreads = 16;   // host threads of the preparation's parallel passes
+
+// f(lo, hi) over [0, n) split in contiguous ranges, one thread each (up to
+// kPrepThreads, fewer for small n); exceptions are rethrown here.
+template <typename F>
+void parallel_ranges(int64_t n, const F& f, int64_t grain = 16384) {
+	int const hw = std::max(1, static_cast<int>(std::thread::hardware_concurrency()));
+	int const T = static_cast<int>(std::min<int64_t>({kPrepThreads, hw, std::max<int64_t>(1, n / grain)}));
+	if (T <= 1) {
+		f(int64_t(0), n);
+		return;
+	}
+	std::vector<std::thread> th;
+	std::vector<std::exception_ptr> err(T);
+	for (int t = 0; t < T; t++) {
+		th.emplace_back([&, t] {
+			try {
+				f(n * t / T, n * (t + 1) / T);
+			} catch (...) {
+				err[t] = std::current_exception();
+			}
+		});
+	}
+	for (auto& x : th) {
+		x.join();
+	}
+	for (auto& e : err) {
+		if (e) {
+			std::rethrow_exception(e);
+		}
+	}
+}
+
+// Small rows: insertion sort (a row holds a few dozen entries at most).
+void sort_small(int32_t* a, int64_t n) {
+	for (int64_t i = 1; i < n; i++) {
+		int32_t const v = a[i];
+		int64_t j = i;
+		for (; j > 0 && a[j - 1] > v; j--) {
+			a[j] = a[j - 1];
+		}
+		a[j] = v;
+	}
+}
+
 // Sorted, de-duplicated node adjacency (including the node itself) of the
-// element graph, in the numbering of `elems`.
+// element graph, in the numbering of `elems`: the node-to-element incidence
+// first, then each node's row from its elements' nodes, the rows of each
+// thread's range into its own buffer, then copied into place.
 void node_adjacency(int64_t n, int kind, int64_t n_elems, const int32_t* elems, std::vector<int64_t>& ptr,
                     std::vector<int32_t>& adj) {
-	std::vector<int64_t> cnt(n + 1, 0);
+	std::vector<int64_t> iptr(n + 1, 0);
 	for (int64_t e = 0; e < n_elems; e++) {
 		for (int a = 0; a < kind; a++) {
-			cnt[elems[e * kind + a] + 1] += kind;
+			iptr[elems[e * kind + a] + 1]++;
 		}
 	}
 	for (int64_t i = 0; i < n; i++) {
-		cnt[i + 1] += cnt[i];
+		iptr[i + 1] += iptr[i];
 	}
-	std::vector<int32_t> cand(cnt[n]);
-	std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
-	for (int64_t e = 0; e < n_elems; e++) {
-		const int32_t* en = elems + e * kind;
-		for (int a = 0; a < kind; a++) {
-			for (int b = 0; b < kind; b++) {
-				cand[fill[en[a]]++] = en[b];
+	std::vector<int32_t> inc(iptr[n]);
+	{
+		std::vector<int64_t> fill(iptr.begin(), iptr.end() - 1);
+		for (int64_t e = 0; e < n_elems; e++) {
+			for (int a = 0; a < kind; a++) {
+				inc[fill[elems[e * kind + a]]++] = static_cast<int32_t>(e);
 			}
 		}
 	}
 	ptr.assign(n + 1, 0);
-	adj.clear();
-	adj.reserve(cnt[n] / 2 + n);
-	for (int64_t i = 0; i < n; i++) {
-		int32_t* c = cand.data() + cnt[i];
-		int64_t const len = cnt[i + 1] - cnt[i];
-		std::sort(c, c + len);
-		int64_t u = 0;
-		for (int64_t t = 0; t < len; t++) {
-			if (u == 0 || c[u - 1] != c[t]) {
-				c[u++] = c[t];
+	std::mutex m;
+	std::vector<std::pair<int64_t, std::vector<int32_t>>> parts;   // (first row, its rows' entries)
+	parallel_ranges(n, [&](int64_t lo, int64_t hi) {
+		std::vector<int32_t> buf;
+		buf.reserve(static_cast<size_t>(hi - lo) * 10);
+		for (int64_t i = lo; i < hi; i++) {
+			size_t const at = buf.size();
+			for (int64_t t = iptr[i]; t < iptr[i + 1]; t++) {
+				const int32_t* en = elems + static_cast<int64_t>(inc[t]) * kind;
+				buf.insert(buf.end(), en, en + kind);
 			}
+			if (buf.size() == at) {
+				buf.push_back(static_cast<int32_t>(i));   // an isolated node keeps its diagonal
+			}
+			int32_t* c = buf.data() + at;
+			int64_t const len = static_cast<int64_t>(buf.size() - at);
+			sort_small(c, len);
+			int64_t u = 1;
+			for (int64_t k = 1; k < len; k++) {
+				if (c[k] != c[u - 1]) {
+					c[u++] = c[k];
+				}
+			}
+			buf.resize(at + u);
+			ptr[i + 1] = u;
 		}
-		if (len == 0) {
-			adj.push_back(static_cast<int32_t>(i));   // isolated node: keep the diagonal
-		} else {
-			adj.insert(adj.end(), c, c + u);
-		}
-		ptr[i + 1] = static_cast<int64_t>(adj.size());
+		std::lock_guard<std::mutex> lock(m);
+		parts.emplace_back(lo, std::move(buf));
+	});
+	for (int64_t i = 0; i < n; i++) {
+		ptr[i + 1] += ptr[i];
 	}
+	adj.resize(ptr[n]);
+	parallel_ranges(
+	    static_cast<int64_t>(parts.size()),
+	    [&](int64_t lo, int64_t hi) {
+		    for (int64_t q = lo; q < hi; q++) {
+			    std::copy(parts[q].second.begin(), parts[q].second.end(), adj.begin() + ptr[parts[q].first]);
+		    }
+	    },
+	    1);
+}
+
+// The adjacency of the renumbered graph (node perm[i] takes node i's row,
+// its entries renumbered and sorted), rows in parallel.
+void renumber_adjacency(int64_t n, const std::vector<int64_t>& ptr0, const std::vector<int32_t>& adj0,
+                        const std::vector<int32_t>& perm, const std::vector<int32_t>& iperm,
+                        std::vector<int64_t>& ptr, std::vector<int32_t>& adj) {
+	ptr.assign(n + 1, 0);
+	for (int64_t r = 0; r < n; r++) {
+		ptr[r + 1] = ptr[r] + (ptr0[iperm[r] + 1] - ptr0[iperm[r]]);
+	}
+	adj.resize(ptr[n]);
+	parallel_ranges(n, [&](int64_t lo, int64_t hi) {
+		for (int64_t r = lo; r < hi; r++) {
+			int64_t const o = iperm[r];
+			int32_t* out = adj.data() + ptr[r];
+			for (int64_t t = ptr0[o]; t < ptr0[o + 1]; t++) {
+				*out++ = perm[adj0[t]];
+			}
+			sort_small(adj.data() + ptr[r], out - (adj.data() + ptr[r]));
+		}
+	});
 }
 
 void rcm(int64_t n, const std::vector<int64_t>& ptr, const std::vector<int32_t>& adj, std::vector<int32_t>& perm,
@@ -71,12 +166,23 @@ void rcm(int64_t n, const std::vector<int64_t>& ptr, const std::vector<int32_t>&
 		deg[i] = static_cast<int32_t>(ptr[i + 1] - ptr[i]);
 	}
 	// Candidate start nodes: ascending degree, descending index, so the first
-	// unvisited entry is the last node of minimum degree.
+	// unvisited entry is the last node of minimum degree (a counting sort by
+	// degree over the nodes in descending order).
+	int32_t dmax = 0;
+	for (int64_t i = 0; i < n; i++) {
+		dmax = std::max(dmax, deg[i]);
+	}
+	std::vector<int64_t> dstart(static_cast<size_t>(dmax) + 2, 0);
+	for (int64_t i = 0; i < n; i++) {
+		dstart[deg[i] + 1]++;
+	}
+	for (int32_t d = 0; d <= dmax; d++) {
+		dstart[d + 1] += dstart[d];
+	}
 	std::vector<int32_t> order(n);
-	std::iota(order.begin(), order.end(), 0);
-	std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-		return deg[a] != deg[b] ? deg[a] < deg[b] : a > b;
-	});
+	for (int64_t i = n - 1; i >= 0; i--) {
+		order[dstart[deg[i]]++] = static_cast<int32_t>(i);
+	}
 	std::vector<uint8_t> visited(n, 0);
 	std::vector<int32_t> queue(n);
 	std::vector<int32_t> nb;
@@ -98,7 +204,14 @@ void rcm(int64_t n, const std::vector<int64_t>& ptr, const std::vector<int32_t>&
 					nb.push_back(j);
 				}
 			}
-			std::stable_sort(nb.begin(), nb.end(), [&](int32_t a, int32_t b) { return deg[a] < deg[b]; });
+			for (size_t u = 1; u < nb.size(); u++) {   // stable insertion sort by degree
+				int32_t const v = nb[u];
+				size_t w = u;
+				for (; w > 0 && deg[nb[w - 1]] > deg[v]; w--) {
+					nb[w] = nb[w - 1];
+				}
+				nb[w] = v;
+			}
 			for (int32_t j : nb) {
 				queue[qe++] = j;
 			}
@@ -122,12 +235,18 @@ int sell_layout(bfmx_system* s, std::vector<int32_t>& col) {
 	s->slice_ptr.assign(s->n_slices + 1, 0);
 	s->row_len.assign(n, 0);
 	s->diag_blk.assign(n, 0);
-	for (int64_t sl = 0; sl < s->n_slices; sl++) {
-		int64_t kmax = 0;
-		for (int64_t r = sl * kSlice; r < std::min(n, (sl + 1) * kSlice); r++) {
-			kmax = std::max(kmax, s->adj_ptr[r + 1] - s->adj_ptr[r]);
+	std::vector<int64_t> width(s->n_slices);
+	parallel_ranges(s->n_slices, [&](int64_t lo, int64_t hi) {
+		for (int64_t sl = lo; sl < hi; sl++) {
+			int64_t kmax = 0;
+			for (int64_t r = sl * kSlice; r < std::min(n, (sl + 1) * kSlice); r++) {
+				kmax = std::max(kmax, s->adj_ptr[r + 1] - s->adj_ptr[r]);
+			}
+			width[sl] = kmax;
 		}
-		int64_t const next = s->slice_ptr[sl] + kmax * kSlice;
+	}, 1024);
+	for (int64_t sl = 0; sl < s->n_slices; sl++) {
+		int64_t const next = s->slice_ptr[sl] + width[sl] * kSlice;
 		if (next >= (int64_t(1) << 31) / 4) {
 			set_error("matrix too large for 32-bit block indices");
 			return BFMX_E_ARG;
@@ -136,29 +255,31 @@ int sell_layout(bfmx_system* s, std::vector<int32_t>& col) {
 	}
 	s->nnzb = s->adj_ptr[n];
 	s->nnzb_pad = s->slice_ptr[s->n_slices];
-	col.assign(s->nnzb_pad, 0);
-	for (int64_t sl = 0; sl < s->n_slices; sl++) {
-		int64_t const kmax = (s->slice_ptr[sl + 1] - s->slice_ptr[sl]) / kSlice;
-		for (int r = 0; r < kSlice; r++) {
-			int64_t const row = sl * kSlice + r;
-			int64_t const len = row < n ? s->adj_ptr[row + 1] - s->adj_ptr[row] : 0;
-			if (row < n) {
-				s->row_len[row] = static_cast<int32_t>(len);
-			}
-			for (int64_t k = 0; k < kmax; k++) {
-				int64_t const blk = s->slice_ptr[sl] + k * kSlice + r;
-				if (k < len) {
-					int32_t const c = s->adj[s->adj_ptr[row] + k];
-					col[blk] = c;
-					if (c == row) {
-						s->diag_blk[row] = static_cast<int32_t>(blk);
+	col.resize(s->nnzb_pad);
+	parallel_ranges(s->n_slices, [&](int64_t lo, int64_t hi) {
+		for (int64_t sl = lo; sl < hi; sl++) {
+			int64_t const kmax = width[sl];
+			for (int r = 0; r < kSlice; r++) {
+				int64_t const row = sl * kSlice + r;
+				int64_t const len = row < n ? s->adj_ptr[row + 1] - s->adj_ptr[row] : 0;
+				if (row < n) {
+					s->row_len[row] = static_cast<int32_t>(len);
+				}
+				for (int64_t k = 0; k < kmax; k++) {
+					int64_t const blk = s->slice_ptr[sl] + k * kSlice + r;
+					if (k < len) {
+						int32_t const c = s->adj[s->adj_ptr[row] + k];
+						col[blk] = c;
+						if (c == row) {
+							s->diag_blk[row] = static_cast<int32_t>(blk);
+						}
+					} else {
+						col[blk] = row < n ? static_cast<int32_t>(row) : 0;
 					}
-				} else {
-					col[blk] = row < n ? static_cast<int32_t>(row) : 0;
 				}
 			}
 		}
-	}
+	}, 1024);
 	return BFMX_OK;
 }
 
@@ -171,26 +292,62 @@ int prepare_topology(bfmx_system* s, const double* coords, const size_t* elems, 
 		set_error("mesh size out of range: %lld nodes, %lld elements", (long long) n, (long long) ne);
 		return BFMX_E_ARG;
 	}
+	// BFMX_SOLVE_TIMES=1: the preparation's phases on stderr
+	auto t_last = std::chrono::steady_clock::now();
+	auto mark = [&](const char* what) {
+		if (phase_times_on()) {
+			auto const now = std::chrono::steady_clock::now();
+			std::fprintf(stderr, "prepare phase %-32s %8.3f ms\n", what,
+			             std::chrono::duration<double, std::milli>(now - t_last).count());
+			t_last = now;
+		}
+	};
 	std::vector<int32_t> el(ne * kind);
-	for (int64_t t = 0; t < ne * kind; t++) {
-		if (elems[t] >= static_cast<size_t>(n)) {
-			set_error("element %lld references node %zu of %lld", (long long) (t / kind), elems[t], (long long) n);
-			return BFMX_E_ARG;
+	std::atomic<int64_t> bad_elem{-1}, bad_node{-1};
+	parallel_ranges(ne * kind, [&](int64_t lo, int64_t hi) {
+		for (int64_t t = lo; t < hi; t++) {
+			if (elems[t] >= static_cast<size_t>(n)) {
+				int64_t expect = -1;
+				bad_elem.compare_exchange_strong(expect, t);
+				return;
+			}
+			el[t] = static_cast<int32_t>(elems[t]);
 		}
-		el[t] = static_cast<int32_t>(elems[t]);
+	});
+	parallel_ranges(2 * n, [&](int64_t lo, int64_t hi) {
+		for (int64_t i = lo; i < hi; i++) {
+			if (!std::isfinite(coords[i])) {
+				int64_t expect = -1;
+				bad_node.compare_exchange_strong(expect, i / 2);
+				return;
+			}
+		}
+	});
+	if (bad_elem >= 0) {
+		int64_t t = 0;   // the first offending entry, as a sequential scan reports it
+		while (elems[t] < static_cast<size_t>(n)) {
+			t++;
+		}
+		set_error("element %lld references node %zu of %lld", (long long) (t / kind), elems[t], (long long) n);
+		return BFMX_E_ARG;
 	}
-	for (int64_t i = 0; i < 2 * n; i++) {
-		if (!std::isfinite(coords[i])) {
-			set_error("node %lld has a non-finite coordinate", (long long) (i / 2));
-			return BFMX_E_ARG;
+	if (bad_node >= 0) {
+		int64_t i = 0;
+		while (std::isfinite(coords[i])) {
+			i++;
 		}
+		set_error("node %lld has a non-finite coordinate", (long long) (i / 2));
+		return BFMX_E_ARG;
 	}
 
+	mark("validation");
 	std::vector<int64_t> ptr0;
 	std::vector<int32_t> adj0;
 	node_adjacency(n, kind, ne, el.data(), ptr0, adj0);
+	mark("node adjacency");
 	if (renumber) {
 		rcm(n, ptr0, adj0, s->perm, s->iperm);
+		mark("RCM");
 	} else {
 		s->perm.resize(n);
 		s->iperm.resize(n);
@@ -201,16 +358,27 @@ int prepare_topology(bfmx_system* s, const double* coords, const size_t* elems, 
 
 	// renumbered coordinates and elements, renumbered adjacency
 	s->coords.resize(2 * n);
-	for (int64_t i = 0; i < n; i++) {
-		s->coords[2 * perm[i] + 0] = coords[2 * i + 0];
-		s->coords[2 * perm[i] + 1] = coords[2 * i + 1];
+	parallel_ranges(n, [&](int64_t lo, int64_t hi) {
+		for (int64_t i = lo; i < hi; i++) {
+			s->coords[2 * perm[i] + 0] = coords[2 * i + 0];
+			s->coords[2 * perm[i] + 1] = coords[2 * i + 1];
+		}
+	});
+	parallel_ranges(ne * kind, [&](int64_t lo, int64_t hi) {
+		for (int64_t t = lo; t < hi; t++) {
+			el[t] = perm[el[t]];
+		}
+	});
+	if (renumber) {
+		renumber_adjacency(n, ptr0, adj0, perm, s->iperm, s->adj_ptr, s->adj);
+	} else {
+		s->adj_ptr.swap(ptr0);
+		s->adj.swap(adj0);
 	}
-	for (auto& v : el) {
-		v = perm[v];
-	}
-	node_adjacency(n, kind, ne, el.data(), s->adj_ptr, s->adj);
+	mark("renumbered coordinates, elements, adjacency");
 
 	BFMX_CHECK(sell_layout(s, col));
+	mark("sliced ELL layout");
 
 	// Greedy colouring in order of each element's lowest node, so a colour's
 	// consecutive elements touch nearby rows; 64 colours at most.
@@ -251,6 +419,7 @@ int prepare_topology(bfmx_system* s, const double* coords, const size_t* elems, 
 			mask[el[e * kind + a]] |= uint64_t(1) << c;
 		}
 	}
+	mark("element colouring");
 	s->n_colors = ncol;
 	s->color_ptr.assign(ncol + 1, 0);
 	for (int64_t e = 0; e < ne; e++) {
@@ -268,6 +437,7 @@ int prepare_topology(bfmx_system* s, const double* coords, const size_t* elems, 
 		s->elem_order[pos] = e;
 		std::memcpy(&el_col[pos * kind], &el[e * kind], kind * sizeof(int32_t));
 	}
+	mark("colour order");
 	return BFMX_OK;
 }
 
