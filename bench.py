@@ -7,7 +7,8 @@ nu = 0.3, left edge clamped, traction -1e6 N/m on the top edge, 2x2 Gauss.
 One step = the first solve, from u = 0, of a freshly assembled device-resident
 system until ||b - K u||_2 / ||b||_2 <= 1e-8 (true residual recomputed in
 FP64): every per-system cost of the solver is inside it (FP32 matrix copy,
-multigrid topology and numeric set-up, solver buffers, captured chunks), as
+multigrid topology and numeric set-up, solver buffers; a first solve launches
+its kernels directly, later solves of the system replay captured chunks), as
 in each bfm_sim_run. value = DOFs / mean step time of the headline solver,
 flexible CG in FP64 preconditioned by one FP32 smoothed-aggregation V-cycle.
 A re-solve on the same system (hierarchy cached) is reported beside it, and
@@ -184,7 +185,8 @@ def batched_leg(D, meshgen, count, rtol):
     # multigrid mode: the same batch solved as one block-diagonal system by
     # flexible CG + the multigrid V-cycle (each load scaled to unit norm, the
     # tolerance checks testing every panel's own residual); the first launch
-    # builds the union and its hierarchy on the host
+    # builds the union on the host and its hierarchy (finest aggregates by
+    # host sweeps over each panel, the rest on the GPU)
     t = time.perf_counter()
     b.solve(precond=D.PRECOND_AMG, rtol=rtol)
     t_mg_setup = time.perf_counter() - t
@@ -195,7 +197,7 @@ def batched_leg(D, meshgen, count, rtol):
     worst = max(x["rel_residual"] for x in st_mg)
     log(f"batched multigrid: {ms_mg:.1f} ms device ({wall_mg * 1e3:.1f} ms wall), {dofs / (ms_mg / 1e3):.3e} DOF/s, "
         f"{st_mg[0]['iterations']} iterations, worst instance residual {worst:.2e}, {ms / ms_mg:.2f}x the "
-        f"persistent launch; first launch (host hierarchy build) {t_mg_setup:.1f} s")
+        f"persistent launch; first launch (union and hierarchy build) {t_mg_setup:.1f} s")
     return {
         "workload": f"{count} holed panels (seeds 0-{count - 1}), 1k-60k DOFs, plane strain, block-Jacobi PCG "
                     f"to rtol {rtol:g}, assembled and solved in one graph launch",
@@ -360,7 +362,7 @@ def main():
     # headline: the first solve of a freshly assembled system, so every
     # per-system cost of the solver is inside the step (the FP32 matrix copy,
     # the multigrid hierarchy's topology and numeric set-up, the solver's
-    # buffers and its captured chunks), as each bfm_sim_run pays it; the
+    # buffers), as each bfm_sim_run pays it; the
     # system itself (upload, renumbering, pattern, assembly, conditions) is
     # built before the step's clock starts
     first_ms, first_st, setups = [], None, []
@@ -481,7 +483,7 @@ def main():
             "iterations": int(first_st["iterations"]),
             "rel_residual": first_st["rel_residual"],
             "step": "first solve of a freshly assembled system: FP32 matrix copy, multigrid topology and "
-                    "numeric set-up, solver buffers and captured chunks, and the iterations to the tolerance",
+                    "numeric set-up, solver buffers, and the iterations to the tolerance",
             "setup_s": t_setup,
             "end_to_end_s": t_setup + ms_first / 1e3,
             "resolve_cached_hierarchy": {"ms_per_step": ms, "value": mesh.n_dof / (ms / 1e3),

@@ -2547,6 +2547,7 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 			// MIS-2 aggregates, on the 2M-DOF Q1 beam)
 			std::vector<int32_t> hagg(nf);
 			aggregate_chunks(s->adj_ptr.data(), s->adj.data(), fseg, chunk_rows(fseg, nf), hagg, cseg);
+			phase.mark("finest sweeps (host)");
 			na = cseg.back();
 			ra = palloc(&agg, nf);
 			if (ra == BFMX_OK) {

@@ -922,6 +922,23 @@ int read_one(hipStream_t st, const T* d, T* h) {
 	return BFMX_OK;
 }
 
+// A count and the overflow flags in one round trip.
+template <typename T>
+int read_count_flags(hipStream_t st, const T* d, T* h, const int32_t* dflags, int32_t* hflags) {
+	BFMX_HIP(hipMemcpyAsync(h, d, sizeof(T), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipMemcpyAsync(hflags, dflags, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	return BFMX_OK;
+}
+
+// out[i] = in[i] (64-bit prefix sums known to fit in 32 bits).
+__global__ __launch_bounds__(kT) void k_narrow(int64_t n, const int64_t* __restrict__ in, int32_t* __restrict__ out) {
+	int64_t const i = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x;
+	if (i < n) {
+		out[i] = static_cast<int32_t>(in[i]);
+	}
+}
+
 template <typename T>
 int zeros(hipStream_t st, T* p, size_t n) {
 	if (n) {
@@ -1128,8 +1145,7 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	k_p_count<<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, agg, smooth ? 1 : 0, cnt, flags);
 	BFMX_CHECK(scan(st, cnt, T.pptr, n, tmp));
 	int32_t nnzp = 0, bad = 0;
-	BFMX_CHECK(read_one(st, T.pptr + n, &nnzp));
-	BFMX_CHECK(read_one(st, flags, &bad));
+	BFMX_CHECK(read_count_flags(st, T.pptr + n, &nnzp, flags, &bad));
 	if (bad) {
 		set_error(bad & 1 ? "multigrid: a level row has more than %d neighbours for the device build"
 		                  : "multigrid: a row couples to more than 255 aggregates",
@@ -1153,8 +1169,7 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	                                   cnt, nullptr, nullptr, nullptr, flags);
 	BFMX_CHECK(scan(st, cnt, T.rptr, na, tmp));
 	int32_t nr = 0;
-	BFMX_CHECK(read_one(st, T.rptr + na, &nr));
-	BFMX_CHECK(read_one(st, flags, &bad));
+	BFMX_CHECK(read_count_flags(st, T.rptr + na, &nr, flags, &bad));
 	if (bad || nr != nnzp) {
 		set_error(bad ? "multigrid: a prolongator column has more than %d rows for the device build"
 		              : "multigrid: restriction lists do not match the prolongator",
@@ -1170,14 +1185,14 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	BFMX_CHECK(palloc(&T.apptr, n + 1));
 	k_ap_pattern<true><<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, cnt, nullptr,
 	                                                 nullptr, flags);
+	int32_t nap = 0;
 	{
-		// the total first in 64 bits: the 32-bit row pointers must hold it
+		// the prefix sums in 64 bits: the 32-bit row pointers must hold the total
 		int64_t* ap64;
 		BFMX_CHECK(tmp.get(&ap64, static_cast<size_t>(n) + 1));
 		BFMX_CHECK(scan(st, cnt, ap64, n, tmp));
 		int64_t total = 0;
-		BFMX_CHECK(read_one(st, ap64 + n, &total));
-		BFMX_CHECK(read_one(st, flags, &bad));
+		BFMX_CHECK(read_count_flags(st, ap64 + n, &total, flags, &bad));
 		if (bad) {
 			set_error("multigrid: a row of A P has more than %d entries for the device build", kMaxAP);
 			return BFMX_E_UNSUPPORTED;
@@ -1186,10 +1201,9 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 			set_error("multigrid: A P too large for 32-bit indices (%lld entries)", static_cast<long long>(total));
 			return BFMX_E_UNSUPPORTED;
 		}
+		k_narrow<<<nblk(static_cast<int64_t>(n) + 1), kT, 0, st>>>(static_cast<int64_t>(n) + 1, ap64, T.apptr);
+		nap = static_cast<int32_t>(total);
 	}
-	BFMX_CHECK(scan(st, cnt, T.apptr, n, tmp));
-	int32_t nap = 0;
-	BFMX_CHECK(read_one(st, T.apptr + n, &nap));
 	T.nnz_ap = nap;
 	BFMX_CHECK(palloc(&T.apcol, nap));
 	k_ap_pattern<false><<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, nullptr,
@@ -1201,8 +1215,7 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	k_c_pattern<true><<<na, 64, 0, st>>>(na, T.rptr, T.rrow, T.apptr, T.apcol, cnt, nullptr, nullptr, flags);
 	BFMX_CHECK(scan(st, cnt, cptr, na, tmp));
 	int64_t cnnz = 0;
-	BFMX_CHECK(read_one(st, cptr + na, &cnnz));
-	BFMX_CHECK(read_one(st, flags, &bad));
+	BFMX_CHECK(read_count_flags(st, cptr + na, &cnnz, flags, &bad));
 	if (bad) {
 		set_error("multigrid: a coarse row couples to more than %d coarse rows for the device build", kMaxCoarse);
 		return BFMX_E_UNSUPPORTED;

@@ -333,6 +333,7 @@ struct bfmx_system {
 	bfmx::Scalars* h_scal = nullptr; // pinned host mirror ring (4 entries)
 	hipEvent_t ev[6] = {};
 	std::vector<bfmx::Graph> graphs;   // captured chunks, one per solver configuration
+	uint64_t solved_once = 0;          // bit p: a solve with preconditioner p has run (its chunks are captured from the next)
 	std::vector<bfmx::AmgLevel> amg;   // multigrid hierarchy (built on first use)
 	bfmx::AmgParams amg_par;           // its parameters
 	int amg_wg = -1;                   // first level run by the one-workgroup coarse cycle (-1: none)

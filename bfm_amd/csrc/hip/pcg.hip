@@ -968,7 +968,15 @@ int run_chunks(bfmx_system* s, int precond, bool mixed, double rtol2, int64_t ca
                Scalars* out) {
 	int64_t const n = s->n_nodes;
 	hipStream_t const st = s->stream;
-	bool const use_graph = graphs_enabled();
+	// Chunks go through captured graphs from a system's second solve with a
+	// preconditioner on: capturing and instantiating them costs about as much
+	// as a large system's whole launch overhead (2M-DOF beam, multigrid FCG:
+	// first solve 19.4 ms launching directly, 21.2 ms capturing; later solves
+	// 10.5 ms either way), while a small system's repeated solves gain from
+	// them.
+	uint64_t const bit = uint64_t(1) << (precond & 63);
+	bool const use_graph = graphs_enabled() && (s->solved_once & bit);
+	s->solved_once |= bit;
 	// the residual kernel's z (unused by flexible CG); the learned decoder reads
 	// the 2x2 block inverses
 	int const pre_res = precond == BFMX_PRECOND_AMG      ? BFMX_PRECOND_JACOBI

@@ -295,14 +295,15 @@ def test_sim_batch_below_the_rounding_floor(L):
 
 
 def test_sim_learned_solver(L, golden):
-    """precond 3 (learned flexible CG) through the libbfm API with the
-    packaged parameters; refused without parameters."""
-    from bfm_amd.learned import default_params
+    """precond 3 (learned flexible CG on the Jacobi base) through the libbfm
+    API with the packaged parameters trained for that base; refused without
+    parameters."""
+    from bfm_amd.learned import WEIGHTS, default_params
     mesh = meshgen.beam(40, 10, kind=3)
     sim, (inst,) = beam_sim(L, [mesh], F.bfm_solver_t(3, 0, 1e-10, 0))
     with pytest.raises(AssertionError, match="bfm_sim_set_model"):
         sim.run()
-    sim2, (inst2,) = beam_sim(L, [mesh], F.bfm_solver_t(3, 0, 1e-10, 0), params=default_params())
+    sim2, (inst2,) = beam_sim(L, [mesh], F.bfm_solver_t(3, 0, 1e-10, 0), params=default_params(WEIGHTS))
     sim2.run()
     st = F.bfm_solve_stats_t()
     assert L.bfm_sim_get_stats(C.byref(sim2.c_sim), 0, C.byref(st)) == 0

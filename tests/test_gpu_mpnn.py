@@ -209,16 +209,19 @@ def test_model_swap_replays_no_stale_graph():
     """Regression: captured learned chunks were looked up by the model's host
     address, so a model created at the address of a destroyed one replayed
     graphs that read the destroyed model's freed parameters. Models A, B, C
-    are attached in turn with a learned solve each, A is destroyed before C
-    exists; C's solve must equal C's solve on a fresh system (no cached
-    chunks) bitwise, and its preconditioner the oracle's with C's weights."""
+    are attached in turn with learned solves (two with A: chunks are captured
+    from a system's second solve with a preconditioner on), A is destroyed
+    before C exists; C's solve (through captured chunks) must equal C's solve
+    on a fresh system (launched directly) bitwise, and its preconditioner the
+    oracle's with C's weights."""
     mesh, s = beam(40, 10, seed=7, hetero=True)
     models = {}
-    for name, seed in (("A", 11), ("B", 12)):
+    for name, seed, solves in (("A", 11, 2), ("B", 12, 1)):
         models[name], _ = model_with_encoder(seed)
         s.set_model(models[name])
-        _, st = s.solve(precond=D.PRECOND_LEARNED, rtol=1e-9)
-        assert st["status"] == 0
+        for _ in range(solves):
+            _, st = s.solve(precond=D.PRECOND_LEARNED, rtol=1e-9)
+            assert st["status"] == 0
     models["A"].close()
     models["C"], p_c = model_with_encoder(13)
     s.set_model(models["C"])
