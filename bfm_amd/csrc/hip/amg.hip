@@ -2163,6 +2163,38 @@ struct HostLevelGraph {
 
 int host_levels_from(bfmx_system* s, bool smooth, bool finest, HostLevelGraph& H, std::vector<int32_t>& fseg);
 
+// A level's graph (sliced ELL on the device) as the host build's state: its
+// rows in CSR and the block slot of every entry (rows in parallel).
+int level_graph_to_host(hipStream_t st, TopoGraph const& G, HostLevelGraph& H) {
+	int const nf = G.n;
+	std::vector<int32_t> len(nf);
+	H.fslice.resize(static_cast<size_t>(G.n_slices) + 1);
+	std::vector<int32_t> col(static_cast<size_t>(G.nnzb_pad));
+	BFMX_HIP(hipMemcpyAsync(H.fslice.data(), G.slice_ptr, H.fslice.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+	                        st));
+	BFMX_HIP(hipMemcpyAsync(col.data(), G.col, col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipMemcpyAsync(len.data(), G.row_len, len.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	H.fptr.assign(static_cast<size_t>(nf) + 1, 0);
+	for (int i = 0; i < nf; i++) {
+		H.fptr[i + 1] = H.fptr[i] + len[i];
+	}
+	H.fadj.resize(H.fptr[nf]);
+	H.fslot.resize(H.fptr[nf]);
+	int const T = std::max(1, std::min(kBuildThreads, nf / 16384));
+	parallel_chunks(T, [&](int c) {
+		for (int i = static_cast<int>(static_cast<int64_t>(nf) * c / T);
+		     i < static_cast<int>(static_cast<int64_t>(nf) * (c + 1) / T); i++) {
+			for (int k = 0; k < len[i]; k++) {
+				int32_t const slot = H.fslice[i >> 6] + k * kSlice + (i & 63);
+				H.fadj[H.fptr[i] + k] = col[slot];
+				H.fslot[H.fptr[i] + k] = slot;
+			}
+		}
+	});
+	return BFMX_OK;
+}
+
 // The levels below level 0, built on the host from the node adjacency
 // (s->adj_ptr, s->adj). fseg receives the coarsest level's segments.
 int build_levels_host(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg) {
@@ -2508,27 +2540,7 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 				return build_levels_host(s, smooth, fseg);
 			}
 			HostLevelGraph H;
-			std::vector<int32_t> len(nf);
-			H.fslice.resize(static_cast<size_t>(G.n_slices) + 1);
-			std::vector<int32_t> col(static_cast<size_t>(G.nnzb_pad));
-			BFMX_HIP(hipMemcpyAsync(H.fslice.data(), G.slice_ptr, H.fslice.size() * sizeof(int32_t),
-			                        hipMemcpyDeviceToHost, st));
-			BFMX_HIP(hipMemcpyAsync(col.data(), G.col, col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-			BFMX_HIP(hipMemcpyAsync(len.data(), G.row_len, len.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-			BFMX_HIP(hipStreamSynchronize(st));
-			H.fptr.assign(static_cast<size_t>(nf) + 1, 0);
-			for (int i = 0; i < nf; i++) {
-				H.fptr[i + 1] = H.fptr[i] + len[i];
-			}
-			H.fadj.resize(H.fptr[nf]);
-			H.fslot.resize(H.fptr[nf]);
-			for (int i = 0; i < nf; i++) {
-				for (int k = 0; k < len[i]; k++) {
-					int32_t const slot = H.fslice[i >> 6] + k * kSlice + (i & 63);
-					H.fadj[H.fptr[i] + k] = col[slot];
-					H.fslot[H.fptr[i] + k] = slot;
-				}
-			}
+			BFMX_CHECK(level_graph_to_host(st, G, H));
 			phase.mark("level graph to the host");
 			int const rc = host_levels_from(s, smooth, false, H, fseg);
 			phase.mark("host build of the small levels");
@@ -2538,16 +2550,27 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 		std::vector<int32_t> cseg;
 		int32_t* agg = nullptr;
 		int ra = BFMX_OK;
-		if (lf == 0 && par.host_agg_rows > 0 && s->adj_ptr.size() == static_cast<size_t>(nf) + 1) {
-			// the finest level by the host build's sweeps over the host graph
-			// the system keeps (in parallel over chunks of whole segments):
-			// sequential sweeps pack the aggregates as a regular lattice on
-			// structured parts of a mesh, which the FCG iterations show (21
-			// against 26 to 35 for sweeps on chunks cut inside a segment or
-			// MIS-2 aggregates, on the 2M-DOF Q1 beam)
+		bool const sys_graph = lf == 0 && s->adj_ptr.size() == static_cast<size_t>(nf) + 1;
+		bool const sweeps = par.host_agg_rows > 0 &&
+		                    (sys_graph || nf <= par.host_agg_rows || seg_largest(fseg) <= par.host_agg_rows);
+		if (sweeps) {
+			// the host build's sequential sweeps, in parallel over chunks of
+			// whole segments: on the finest level over the host graph the system
+			// keeps, below it over the level's graph copied down, when the level
+			// is small or made of small segments (a batch's union). Sweeps pack
+			// the aggregates as a regular lattice on structured parts of a mesh,
+			// which the FCG iterations show (21 against 26 to 35 for sweeps on
+			// chunks cut inside a segment or MIS-2 aggregates, on the 2M-DOF Q1
+			// beam)
 			std::vector<int32_t> hagg(nf);
-			aggregate_chunks(s->adj_ptr.data(), s->adj.data(), fseg, chunk_rows(fseg, nf), hagg, cseg);
-			phase.mark("finest sweeps (host)");
+			if (sys_graph) {
+				aggregate_chunks(s->adj_ptr.data(), s->adj.data(), fseg, chunk_rows(fseg, nf), hagg, cseg);
+			} else {
+				HostLevelGraph H;
+				BFMX_CHECK(level_graph_to_host(st, G, H));
+				aggregate_chunks(H.fptr.data(), H.fadj.data(), fseg, chunk_rows(fseg, nf), hagg, cseg);
+			}
+			phase.mark("sweeps (host)");
 			na = cseg.back();
 			ra = palloc(&agg, nf);
 			if (ra == BFMX_OK) {
@@ -2556,13 +2579,13 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 					e = hipStreamSynchronize(st);
 				}
 				if (e != hipSuccess) {
-					set_error("upload of the finest aggregates: %s", hipGetErrorString(e));
+					set_error("upload of the aggregates: %s", hipGetErrorString(e));
 					ra = BFMX_E_HIP;
 				}
 			}
 		} else {
-			ra = topo_aggregate(st, G, kAggSalt + static_cast<uint32_t>(lf), par.seed_frac, par.seed_growth,
-			                    par.host_agg_rows, fseg, &agg, &na, cseg);
+			ra = topo_aggregate(st, G, kAggSalt + static_cast<uint32_t>(lf), par.seed_frac, par.seed_growth, fseg,
+			                    &agg, &na, cseg);
 		}
 		lv[lf].d_agg = agg;   // released with the hierarchy
 		BFMX_CHECK(ra);

@@ -54,12 +54,9 @@ int topo_segments(hipStream_t st, TopoGraph const& G, std::vector<int32_t>& seg)
 // owned by the caller, also on failure once set), their count, and cseg[c]
 // = the first aggregate of segment c (fseg: the level's segment offsets).
 // Roots grow from seeds: a fraction seed_frac of the rows may start a region
-// in the first round, that fraction growing by seed_growth per round. A
-// level of at most host_rows rows is aggregated by a sequential sweep on the
-// host instead (the host build's rule).
+// in the first round, that fraction growing by seed_growth per round.
 int topo_aggregate(hipStream_t st, TopoGraph const& G, uint32_t salt, double seed_frac, double seed_growth,
-                   int host_rows, std::vector<int32_t> const& fseg, int32_t** agg, int* na,
-                   std::vector<int32_t>& cseg);
+                   std::vector<int32_t> const& fseg, int32_t** agg, int* na, std::vector<int32_t>& cseg);
 
 // The transfer operators' patterns, the pattern of A P, the coarse graph and
 // its mirrors for the aggregates agg (na of them); smooth: P = (I - w D^-1

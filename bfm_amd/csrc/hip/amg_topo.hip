@@ -978,70 +978,8 @@ int topo_segments(hipStream_t st, TopoGraph const& G, std::vector<int32_t>& seg)
 	return BFMX_OK;
 }
 
-namespace {
-
-// Aggregates of a small level by one sequential sweep on the host (the
-// level's graph copied down): a row whose neighbourhood is still free roots
-// an aggregate of that neighbourhood, then every other row joins the
-// aggregate of its first neighbour that has one (the rule of the host
-// build). A small level costs a few dozen independent-set rounds on the
-// device, each a handful of launches, for a sweep of a few thousand rows.
-int aggregate_on_host(hipStream_t st, TopoGraph const& G, std::vector<int32_t> const& fseg, int32_t* agg, int* na_out,
-                      std::vector<int32_t>& cseg) {
-	int const n = G.n;
-	std::vector<int32_t> sp(static_cast<size_t>(G.n_slices) + 1), col(static_cast<size_t>(G.nnzb_pad)), len(n);
-	BFMX_HIP(hipMemcpyAsync(sp.data(), G.slice_ptr, sp.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-	BFMX_HIP(hipMemcpyAsync(col.data(), G.col, col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-	BFMX_HIP(hipMemcpyAsync(len.data(), G.row_len, len.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-	BFMX_HIP(hipStreamSynchronize(st));
-	auto nbr = [&](int i, int k) { return col[static_cast<size_t>(sp[i >> 6]) + static_cast<size_t>(k) * kSlice + (i & 63)]; };
-	std::vector<int32_t> a(n, -1);
-	int na = 0;
-	for (int i = 0; i < n; i++) {
-		if (a[i] >= 0) {
-			continue;
-		}
-		bool free_nbhd = true;
-		for (int k = 0; k < len[i] && free_nbhd; k++) {
-			free_nbhd = a[nbr(i, k)] < 0;
-		}
-		if (free_nbhd) {
-			for (int k = 0; k < len[i]; k++) {
-				a[nbr(i, k)] = na;
-			}
-			a[i] = na++;
-		}
-	}
-	// a row left over saw an aggregated neighbour when the sweep passed it
-	std::vector<int32_t> out(a);
-	for (int i = 0; i < n; i++) {
-		for (int k = 0; k < len[i] && out[i] < 0; k++) {
-			out[i] = a[nbr(i, k)];
-		}
-		if (out[i] < 0) {
-			set_error("multigrid: a row found no aggregate");
-			return BFMX_E_ARG;
-		}
-	}
-	BFMX_HIP(hipMemcpyAsync(agg, out.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
-	BFMX_HIP(hipStreamSynchronize(st));
-	cseg.assign(fseg.size(), na);
-	for (size_t c = 0; c + 1 < fseg.size(); c++) {
-		cseg[c] = out[fseg[c]];
-	}
-	*na_out = na;
-	return BFMX_OK;
-}
-
-}  // namespace
-
 int topo_aggregate(hipStream_t st, TopoGraph const& G, uint32_t salt, double seed_frac, double seed_growth,
-                   int host_rows, std::vector<int32_t> const& fseg, int32_t** agg_out, int* na_out,
-                   std::vector<int32_t>& cseg) {
-	if (G.n <= host_rows) {
-		BFMX_CHECK(palloc(agg_out, G.n));   // owned by the caller from here on
-		return aggregate_on_host(st, G, fseg, *agg_out, na_out, cseg);
-	}
+                   std::vector<int32_t> const& fseg, int32_t** agg_out, int* na_out, std::vector<int32_t>& cseg) {
 	Temps tmp(st);
 	PhaseTimer phase(st, "aggregation");
 	int const n = G.n;

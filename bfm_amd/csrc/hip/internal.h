@@ -242,7 +242,10 @@ struct AmgParams {
 	int power_coarse = 10;  // ... on the coarser levels
 	double seed_frac = 0.002;   // device aggregation: rows that may seed a region in the first round
 	double seed_growth = 1.25;  // ... and that fraction's growth per round
-	int host_agg_rows = 131072;  // device build: levels up to this many rows are aggregated on the host
+	// device build: levels of at most this many rows, or whose segments all
+	// are, are aggregated by host sweeps (the finest always, from the host
+	// graph; 0: MIS-2 on the GPU at every level)
+	int host_agg_rows = 131072;
 	// device build: from a level of at most this many rows on, the host build
 	// (off: the host's Galerkin lists grow with the wide stencils of the
 	// coarse levels; 7112 rows of the 2M-DOF beam's level 2 took 11.9 ms
