@@ -604,6 +604,7 @@ void union_release(bfmx_batch* b) {
 		return;
 	}
 	release_solver(U);
+	dfree(U->d_coords);
 	dfree(U->d_slice_ptr);
 	dfree(U->d_row_len);
 	dfree(U->d_diag_blk);
@@ -847,29 +848,54 @@ int union_build_parts(bfmx_batch* b) {
 	U->stream = b->stream;
 	U->kind = b->systems[0]->kind;
 	U->n_nodes = n;
-	U->coords.resize(2 * n);
-	U->adj_ptr.assign(n + 1, 0);
-	U->adj.resize(nnz);
+	hipStream_t const st = b->stream;
+	// the coordinates on the device (the multigrid's rigid-body modes are
+	// formed there): a batch built from meshes keeps its instances' on the
+	// device only
+	BFMX_CHECK(dalloc(&U->d_coords, 2 * n));
 	for (size_t k = 0; k < m; k++) {
 		bfmx_system const* s = b->systems[k];
-		int64_t const o = off[k], e0 = U->adj_ptr[o];
-		if (s->coords.size() == static_cast<size_t>(2 * s->n_nodes)) {
-			std::copy(s->coords.begin(), s->coords.end(), U->coords.begin() + 2 * o);
+		size_t const bytes = 2 * static_cast<size_t>(s->n_nodes) * sizeof(double);
+		if (s->d_coords) {
+			BFMX_HIP(hipMemcpyAsync(U->d_coords + 2 * off[k], s->d_coords, bytes, hipMemcpyDeviceToDevice, st));
 		} else {
-			// a batch built from meshes keeps its coordinates on the device only
-			BFMX_HIP(hipMemcpy(U->coords.data() + 2 * o, s->d_coords, 2 * s->n_nodes * sizeof(double),
-			                   hipMemcpyDeviceToHost));
+			BFMX_HIP(hipMemcpyAsync(U->d_coords + 2 * off[k], s->coords.data(), bytes, hipMemcpyHostToDevice, st));
 		}
-		for (int64_t i = 0; i < s->n_nodes; i++) {
-			U->adj_ptr[o + i + 1] = e0 + s->adj_ptr[i + 1];
+	}
+	// the instances' graphs one after another (instances spread over threads)
+	std::vector<int64_t> eoff(m + 1, 0);
+	for (size_t k = 0; k < m; k++) {
+		eoff[k + 1] = eoff[k] + b->systems[k]->adj_ptr[b->systems[k]->n_nodes];
+	}
+	U->adj_ptr.assign(n + 1, 0);
+	U->adj.resize(nnz);
+	{
+		unsigned const hw = std::max(1u, std::thread::hardware_concurrency());
+		size_t const nthreads = std::min<size_t>({m, 16, hw});
+		std::atomic<size_t> next{0};
+		auto work = [&]() {
+			for (size_t k; (k = next.fetch_add(1)) < m;) {
+				bfmx_system const* s = b->systems[k];
+				int64_t const o = off[k], e0 = eoff[k];
+				for (int64_t i = 0; i < s->n_nodes; i++) {
+					U->adj_ptr[o + i + 1] = e0 + s->adj_ptr[i + 1];
+				}
+				for (int64_t e = 0; e < s->adj_ptr[s->n_nodes]; e++) {
+					U->adj[e0 + e] = static_cast<int32_t>(s->adj[e] + o);
+				}
+			}
+		};
+		std::vector<std::thread> pool;
+		for (size_t t = 1; t < nthreads; t++) {
+			pool.emplace_back(work);
 		}
-		for (int64_t e = 0; e < s->adj_ptr[s->n_nodes]; e++) {
-			U->adj[e0 + e] = static_cast<int32_t>(s->adj[e] + o);
+		work();
+		for (auto& t : pool) {
+			t.join();
 		}
 	}
 	std::vector<int32_t> col;
 	BFMX_CHECK(sell_layout(U, col));
-	hipStream_t const st = b->stream;
 	BFMX_CHECK(dalloc(&U->d_slice_ptr, U->n_slices + 1));
 	BFMX_CHECK(dalloc(&U->d_row_len, n));
 	BFMX_CHECK(dalloc(&U->d_diag_blk, n));
@@ -1235,6 +1261,7 @@ int batch_create_planar(bfmx_batch** out, size_t n, const bfmx_instance_t* inst,
 		size_t const nthreads = std::min<size_t>({n, 16, hw});
 		std::atomic<size_t> next{0};
 		auto work = [&]() {
+			PrepThreads const one(1);   // the instances are the parallelism here
 			for (size_t k; (k = next.fetch_add(1)) < n;) {
 				Prepared& P = prep[k];
 				try {

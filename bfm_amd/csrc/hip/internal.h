@@ -429,6 +429,13 @@ namespace bfmx {
 int prepare_topology(bfmx_system* s, const double* coords, const size_t* elems, int renumber,
                      std::vector<int32_t>& col, std::vector<int32_t>& el_col);
 int build_topology(bfmx_system* s, const double* coords, const size_t* elems, int renumber);
+// While alive, the mesh preparation called on this thread uses at most
+// `limit` host threads (callers that already run it on many threads at once).
+struct PrepThreads {
+	int prev;
+	explicit PrepThreads(int limit);
+	~PrepThreads();
+};
 // Sliced-ELL pattern of s->adj_ptr / s->adj (slice_ptr, row_len, diag_blk,
 // nnzb, nnzb_pad; col: the block column of every slot).
 int sell_layout(bfmx_system* s, std::vector<int32_t>& col);

@@ -25,12 +25,16 @@ namespace bfmx {
 namespace {
 
 constexpr int kPrepThreads = 16;   // host threads of the preparation's parallel passes
+thread_local int tl_threads = 0;   // this thread's limit (0: none), see PrepThreads
 
 // f(lo, hi) over [0, n) split in contiguous ranges, one thread each (up to
 // kPrepThreads, fewer for small n); exceptions are rethrown here.
 template <typename F>
 void parallel_ranges(int64_t n, const F& f, int64_t grain = 16384) {
-	int const hw = std::max(1, static_cast<int>(std::thread::hardware_concurrency()));
+	int hw = std::max(1, static_cast<int>(std::thread::hardware_concurrency()));
+	if (tl_threads > 0) {
+		hw = std::min(hw, tl_threads);
+	}
 	int const T = static_cast<int>(std::min<int64_t>({kPrepThreads, hw, std::max<int64_t>(1, n / grain)}));
 	if (T <= 1) {
 		f(int64_t(0), n);
@@ -225,6 +229,14 @@ void rcm(int64_t n, const std::vector<int64_t>& ptr, const std::vector<int32_t>&
 }
 
 }  // namespace
+
+PrepThreads::PrepThreads(int limit) : prev(tl_threads) {
+	tl_threads = limit;
+}
+
+PrepThreads::~PrepThreads() {
+	tl_threads = prev;
+}
 
 // Sliced-ELL layout of the system's renumbered adjacency: slice pointers,
 // row lengths, diagonal blocks and the column of every block slot (padding
