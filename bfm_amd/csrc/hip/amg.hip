@@ -78,6 +78,8 @@ constexpr double kDropTol = 1e-10;      // relative remainder below which a mode
 constexpr int kGj = 1024;               // threads of the Gauss-Jordan workgroup (one segment)
 constexpr int kGjSeg = 256;             // ... (one workgroup per segment of many)
 constexpr int kBuildThreads = 16;       // host threads building a level's transfer operators
+constexpr double kSpreadHetero = 10.0;  // per-element E spread (max / min) above which ...
+constexpr int kDegSpread = 3;           // ... the finest level takes this many Chebyshev steps per side
 // Blocks of a row whose loads the smoothing and residual row loops keep in
 // flight together, for 2x2 (finest level) and 3x3 blocks. Measured (2M-DOF
 // Q1 beam solve / 4096-panel P1 batch): no unrolling 12.20 ms / 797 ms,
@@ -2739,6 +2741,14 @@ int amg_build_impl(bfmx_system* s, bool device) {
 		bool const smooth = amg_smoothing();
 		AmgParams& par = s->amg_par;
 		par = AmgParams{};
+		// per-element E spread over more than a decade: three finest-level
+		// steps (2M-DOF beam, E log-uniform over two decades: 51 / 32 / 26
+		// FCG iterations and a first solve of 31.5 / 27.1 / 25.3 ms with one /
+		// two / three steps; one step stays best with a single E, 20.0 ms
+		// against 21.0 / 22.1)
+		if (s->e_spread > kSpreadHetero) {
+			par.deg = kDegSpread;
+		}
 		if (const char* e = std::getenv("BFMX_AMG_DEG")) {
 			par.deg = std::min(kMaxDeg, std::max(1, std::atoi(e)));
 		}

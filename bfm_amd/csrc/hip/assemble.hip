@@ -135,11 +135,17 @@ int assemble(bfmx_system* s, int stress, double E, double nu, double rho, const 
 	hipStream_t const st = s->stream;
 	int64_t const n = s->n_nodes;
 	double* d_force = nullptr;
+	s->e_spread = 1.0;
 	if (elemE) {
 		std::vector<double> eE(s->n_elems);
+		double lo = 0, hi = 0;
 		for (int64_t p = 0; p < s->n_elems; p++) {
 			eE[p] = elemE[s->elem_order[p]];
+			double const a = std::fabs(eE[p]);
+			lo = p == 0 ? a : std::min(lo, a);
+			hi = p == 0 ? a : std::max(hi, a);
 		}
+		s->e_spread = lo > 0 ? hi / lo : std::numeric_limits<double>::infinity();
 		if (!s->d_elemE) {
 			BFMX_CHECK(dalloc(&s->d_elemE, s->n_elems));
 		}

@@ -232,7 +232,8 @@ struct AmgLevelDesc {
 // or the environment's BFMX_AMG_DEG, BFMX_AMG_DEG_COARSE, BFMX_AMG_DEG_DEEP, BFMX_AMG_RATIO, BFMX_AMG_COARSEST,
 // BFMX_AMG_OMEGA).
 struct AmgParams {
-	int deg = 1;            // Chebyshev steps before and after each coarse correction (level 0)
+	int deg = 1;            // Chebyshev steps before and after each coarse correction (level 0);
+	                        // kDegSpread when E spreads over more than kSpreadHetero
 	int deg_coarse = 2;     // ... on level 1
 	int deg_deep = 2;       // ... on the levels below it
 	double ratio = 30.0;    // Chebyshev interval [lambda_max / ratio, lambda_max]
@@ -308,6 +309,7 @@ struct bfmx_system {
 	int32_t* d_elem = nullptr;       // kind per element, colour order
 	int32_t* d_slots = nullptr;      // kind*kind block indices per element
 	double* d_elemE = nullptr;
+	double e_spread = 1.0;        // max / min of the per-element E of the last assembly (1: one E)
 	int32_t* d_slice_ptr = nullptr;
 	int32_t* d_row_len = nullptr;
 	int32_t* d_diag_blk = nullptr;
