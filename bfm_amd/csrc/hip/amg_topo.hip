@@ -49,6 +49,7 @@ constexpr int kHash = 256;       // hash-table entries per coarse row (LDS)
 constexpr int kHashStride = kHash + 1;
 constexpr int kMisMaxRounds = 200;
 constexpr int kMisGroup = 4;     // rounds launched between two checks for undecided rows
+constexpr int kStash = 32;       // entries per row a count pass keeps for its fill pass
 constexpr int kScanItems = 8;    // items per thread of a scan block
 constexpr int kScanBlock = kT * kScanItems;
 
@@ -640,6 +641,12 @@ __global__ __launch_bounds__(64) void k_r_lists(int na, int smooth, const int32_
 	}
 	int const m = set.sorted(scratch, rows);
 	int32_t const r0 = rptr[J];
+	if (m != rptr[J + 1] - r0) {   // the rows next to the aggregate must be P's column
+		if (threadIdx.x == 0) {
+			atomicOr(bad, 32);
+		}
+		return;
+	}
 	for (int e = threadIdx.x; e < m; e += 64) {
 		int const i = rows[e];
 		int32_t const p0 = pptr[i];
@@ -651,8 +658,9 @@ __global__ __launch_bounds__(64) void k_r_lists(int na, int smooth, const int32_
 // ------------------------------------------------------------------ A P pattern
 
 // Row i of A P: the aggregates in the prolongator rows of i's neighbours,
-// sorted and distinct (per-thread list in LDS). COUNT: their number; else
-// written from apptr[i].
+// sorted and distinct (per-thread list in LDS). COUNT: their number, and a
+// row of at most kStash entries into stash (kStash per row); else written
+// from apptr[i] (copied from stash when it holds the row).
 template <bool COUNT>
 __global__ __launch_bounds__(kTW) void k_ap_pattern(int n, const int32_t* __restrict__ slice_ptr,
                                                     const int32_t* __restrict__ col,
@@ -660,10 +668,20 @@ __global__ __launch_bounds__(kTW) void k_ap_pattern(int n, const int32_t* __rest
                                                     const int32_t* __restrict__ pptr,
                                                     const int32_t* __restrict__ pcol, int32_t* __restrict__ cnt,
                                                     const int32_t* __restrict__ apptr, int32_t* __restrict__ apcol,
-                                                    int32_t* __restrict__ bad) {
+                                                    int32_t* __restrict__ stash, int32_t* __restrict__ bad) {
 	__shared__ int32_t lds[kTW * (kMaxAP + 1)];
 	int const i = blockIdx.x * kTW + threadIdx.x;
 	if (i >= n) {
+		return;
+	}
+	if (!COUNT && stash && apptr[i + 1] - apptr[i] <= kStash) {
+		// the count pass kept the row: copy it
+		int const c = apptr[i + 1] - apptr[i];
+		const int32_t* src = stash + static_cast<int64_t>(i) * kStash;
+		int32_t* out = apcol + apptr[i];
+		for (int k = 0; k < c; k++) {
+			out[k] = src[k];
+		}
 		return;
 	}
 	int32_t* u = lds + threadIdx.x * (kMaxAP + 1);
@@ -696,6 +714,12 @@ __global__ __launch_bounds__(kTW) void k_ap_pattern(int n, const int32_t* __rest
 	}
 	if (COUNT) {
 		cnt[i] = c;
+		if (stash && c <= kStash) {
+			int32_t* dst = stash + static_cast<int64_t>(i) * kStash;
+			for (int k = 0; k < c; k++) {
+				dst[k] = u[k];
+			}
+		}
 		return;
 	}
 	int32_t* out = apcol + apptr[i];
@@ -708,17 +732,26 @@ __global__ __launch_bounds__(kTW) void k_ap_pattern(int n, const int32_t* __rest
 
 // Coarse row I of P^T (A P) (one workgroup of one wavefront per row): the
 // columns of the A P rows of P's column I, sorted and distinct. COUNT: their
-// number into ccnt[I]; else written from cptr[I].
+// number into ccnt[I] and the sorted row into stash (kMaxCoarse per row);
+// else the stashed row copied to cadj from cptr[I].
 template <bool COUNT>
 __global__ __launch_bounds__(64) void k_c_pattern(int na, const int32_t* __restrict__ rptr,
                                                   const int32_t* __restrict__ rrow,
                                                   const int32_t* __restrict__ apptr,
                                                   const int32_t* __restrict__ apcol, int32_t* __restrict__ ccnt,
                                                   const int64_t* __restrict__ cptr, int32_t* __restrict__ cadj,
-                                                  int32_t* __restrict__ bad) {
+                                                  int32_t* __restrict__ stash, int32_t* __restrict__ bad) {
+	int const I = blockIdx.x;
+	if (!COUNT) {   // the count pass kept the sorted row
+		int64_t const c0 = cptr[I];
+		int const m = static_cast<int>(cptr[I + 1] - c0);
+		for (int e = threadIdx.x; e < m; e += 64) {
+			cadj[c0 + e] = stash[static_cast<int64_t>(I) * kMaxCoarse + e];
+		}
+		return;
+	}
 	WAVE_SET(set);
 	__shared__ int32_t scratch[kSetCap + 64];
-	int const I = blockIdx.x;
 	set.clear();
 	for (int32_t t = rptr[I] + threadIdx.x; t < rptr[I + 1]; t += 64) {
 		int const i = rrow[t];
@@ -739,13 +772,10 @@ __global__ __launch_bounds__(64) void k_c_pattern(int na, const int32_t* __restr
 		}
 		return;
 	}
-	if (COUNT) {
-		if (threadIdx.x == 0) {
-			ccnt[I] = m;
-		}
-		return;
+	if (threadIdx.x == 0) {
+		ccnt[I] = m;
 	}
-	set.sorted(scratch, cadj + cptr[I]);
+	set.sorted(scratch, stash + static_cast<int64_t>(I) * kMaxCoarse);
 }
 
 // Widest row of each slice of 64 rows, times 64: the slice's blocks.
@@ -1101,19 +1131,14 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	k_p_fill<<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, agg, smooth ? 1 : 0, T.pptr, T.pcol,
 	                                       T.ppos);
 	phase.mark("prolongator pattern");
-	// restriction lists: P's columns, rows sorted
+	// restriction lists: P's columns, rows sorted; a column's length is the
+	// count of its aggregate among P's entries (the lists are checked against
+	// it as they are written; a mismatch or an overflow is reported with the
+	// flags read after the A P count below)
 	BFMX_CHECK(palloc(&T.rptr, na + 1));
-	k_r_lists<true><<<na, 64, 0, st>>>(na, smooth ? 1 : 0, G.slice_ptr, G.col, G.row_len, T.agg_ptr, T.agg_nodes, T.pptr, T.pcol,
-	                                   cnt, nullptr, nullptr, nullptr, flags);
+	BFMX_CHECK(zeros(st, cnt, na + 1));
+	k_count_keys<<<nblk(nnzp), kT, 0, st>>>(nnzp, T.pcol, cnt);
 	BFMX_CHECK(scan(st, cnt, T.rptr, na, tmp));
-	int32_t nr = 0;
-	BFMX_CHECK(read_count_flags(st, T.rptr + na, &nr, flags, &bad));
-	if (bad || nr != nnzp) {
-		set_error(bad ? "multigrid: a prolongator column has more than %d rows for the device build"
-		              : "multigrid: restriction lists do not match the prolongator",
-		          kSetCap);
-		return bad ? BFMX_E_UNSUPPORTED : BFMX_E_ARG;
-	}
 	BFMX_CHECK(palloc(&T.rent, nnzp));
 	BFMX_CHECK(palloc(&T.rrow, nnzp));
 	k_r_lists<false><<<na, 64, 0, st>>>(na, smooth ? 1 : 0, G.slice_ptr, G.col, G.row_len, T.agg_ptr, T.agg_nodes, T.pptr, T.pcol,
@@ -1121,8 +1146,10 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	phase.mark("restriction lists");
 	// pattern of A P
 	BFMX_CHECK(palloc(&T.apptr, n + 1));
+	int32_t* stash;   // each row's sorted list, when short, from the count pass to the fill pass
+	BFMX_CHECK(tmp.get(&stash, static_cast<size_t>(n) * kStash));
 	k_ap_pattern<true><<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, cnt, nullptr,
-	                                                 nullptr, flags);
+	                                                 nullptr, stash, flags);
 	int32_t nap = 0;
 	{
 		// the prefix sums in 64 bits: the 32-bit row pointers must hold the total
@@ -1131,6 +1158,14 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 		BFMX_CHECK(scan(st, cnt, ap64, n, tmp));
 		int64_t total = 0;
 		BFMX_CHECK(read_count_flags(st, ap64 + n, &total, flags, &bad));
+		if (bad & 32) {
+			set_error("multigrid: restriction lists do not match the prolongator");
+			return BFMX_E_ARG;
+		}
+		if (bad & 8) {
+			set_error("multigrid: a prolongator column has more than %d rows for the device build", kSetCap);
+			return BFMX_E_UNSUPPORTED;
+		}
 		if (bad) {
 			set_error("multigrid: a row of A P has more than %d entries for the device build", kMaxAP);
 			return BFMX_E_UNSUPPORTED;
@@ -1145,12 +1180,14 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	T.nnz_ap = nap;
 	BFMX_CHECK(palloc(&T.apcol, nap));
 	k_ap_pattern<false><<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, nullptr,
-	                                                  T.apptr, T.apcol, flags);
+	                                                  T.apptr, T.apcol, stash, flags);
 	phase.mark("A P pattern");
 	// coarse pattern
 	int64_t* cptr;
 	BFMX_CHECK(tmp.get(&cptr, na + 1));
-	k_c_pattern<true><<<na, 64, 0, st>>>(na, T.rptr, T.rrow, T.apptr, T.apcol, cnt, nullptr, nullptr, flags);
+	int32_t* cstash;   // each coarse row, sorted, from the count pass to the fill pass
+	BFMX_CHECK(tmp.get(&cstash, static_cast<size_t>(na) * kMaxCoarse));
+	k_c_pattern<true><<<na, 64, 0, st>>>(na, T.rptr, T.rrow, T.apptr, T.apcol, cnt, nullptr, nullptr, cstash, flags);
 	BFMX_CHECK(scan(st, cnt, cptr, na, tmp));
 	int64_t cnnz = 0;
 	BFMX_CHECK(read_count_flags(st, cptr + na, &cnnz, flags, &bad));
@@ -1160,7 +1197,7 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	}
 	int32_t* cadj;
 	BFMX_CHECK(tmp.get(&cadj, static_cast<size_t>(cnnz)));
-	k_c_pattern<false><<<na, 64, 0, st>>>(na, T.rptr, T.rrow, T.apptr, T.apcol, nullptr, cptr, cadj, flags);
+	k_c_pattern<false><<<na, 64, 0, st>>>(na, T.rptr, T.rrow, T.apptr, T.apcol, nullptr, cptr, cadj, cstash, flags);
 	phase.mark("coarse pattern");
 	// coarse sliced ELL
 	T.c_slices = (na + kSlice - 1) / kSlice;
