@@ -44,6 +44,9 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE x 1.9998 + WRITE_SIZE, the factors read off a 256 MiB
 # device-to-device copy in the same runs): profiles/hbm_traffic_q1_2m.md
 TRAFFIC_Q1_2M = {"k_cg_spmv": 389.3e6, "k_fcg_spmv": 397.4e6, "k_cheb": 254.9e6}
+# the same for the 4096-panel persistent launch's k_batch_pcg (one dispatch,
+# picks [4, 4, 4]): profiles/batched_panels_persistent_and_multigrid_kernel_stats.md
+TRAFFIC_BATCH_4096 = 68.47e12
 CPU_SAMPLE = (180, 45)   # Q1 beam cells for the reference CPU path (16,652 DOFs)
 
 
@@ -207,7 +210,8 @@ def batched_leg(D, meshgen, count, rtol):
                                                                               "batch_create": t_create,
                                                                               "autotune": t_tune},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                     "frac": gbs / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_BATCH_4096 if count == 4096 and picks.tolist() == [4, 4, 4] else None,
                      "kernel": "k_batch_pcg (block Jacobi, autotuned) + k_batch_assemble (whole graph launch)",
                      "algorithmic_bytes": byt},
         "multigrid": {
