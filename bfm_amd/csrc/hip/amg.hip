@@ -2039,6 +2039,8 @@ int upload(T** d, const std::vector<T>& h, hipStream_t st) {
 	return BFMX_OK;
 }
 
+void numeric_level_head(bfmx_system* s, size_t l, bool coarsest);
+
 // Smoothed aggregation unless the environment sets BFMX_AMG_SMOOTH=0
 // (unsmoothed aggregation, P = P_t: for comparisons).
 bool amg_smoothing() {
@@ -2149,6 +2151,7 @@ void amg_release(bfmx_system* s) {
 	pfree(s->d_amg_pf);
 	s->amg_npf = 0;
 	s->amg_wg = -1;
+	s->amg_head0 = false;
 }
 
 namespace {
@@ -2519,7 +2522,8 @@ constexpr uint32_t kAggSalt = 0x2545F491u;   // seeds the aggregation's row prio
 // ELL (amg_topo.hip): the same arrays as build_levels_host, from MIS-2
 // aggregates. BFMX_E_UNSUPPORTED when a row exceeds the device build's
 // bounds (the caller then builds on the host).
-int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg) {
+int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
+                        const std::function<int()>& before_finest) {
 	hipStream_t const st = s->stream;
 	std::vector<AmgLevel>& lv = s->amg;
 	AmgParams const& par = s->amg_par;
@@ -2555,6 +2559,9 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg)
 		bool const sys_graph = lf == 0 && s->adj_ptr.size() == static_cast<size_t>(nf) + 1;
 		bool const sweeps = par.host_agg_rows > 0 &&
 		                    (sys_graph || nf <= par.host_agg_rows || seg_largest(fseg) <= par.host_agg_rows);
+		if (lf == 0) {
+			BFMX_CHECK(before_finest());
+		}
 		if (sweeps) {
 			// the host build's sequential sweeps, in parallel over chunks of
 			// whole segments: on the finest level over the host graph the system
@@ -2786,10 +2793,10 @@ int amg_build_impl(bfmx_system* s, bool device) {
 		if (const char* e = std::getenv("BFMX_AMG_POWER_COARSE")) {
 			par.power_coarse = std::max(0, std::atoi(e));
 		}
-		std::vector<int32_t> fseg;
-		BFMX_CHECK(device ? build_levels_device(s, smooth, fseg) : build_levels_host(s, smooth, fseg));
-		phase.mark("levels");
-		for (auto& L : lv) {
+		auto vectors = [&](AmgLevel& L) -> int {
+			if (L.d_b) {
+				return BFMX_OK;
+			}
 			size_t const m = static_cast<size_t>(L.bs) * L.nb;
 			BFMX_CHECK(palloc(&L.d_b, m));
 			BFMX_CHECK(palloc(&L.d_x[0], m));
@@ -2800,6 +2807,27 @@ int amg_build_impl(bfmx_system* s, bool device) {
 			BFMX_CHECK(palloc(&L.d_scal, kScalSize));
 			BFMX_CHECK(palloc(&L.d_partial, grid(L.nb) + grid(static_cast<int64_t>(L.bs) * L.nb)));
 			BFMX_CHECK(palloc(&L.d_B, 3 * m));
+			return BFMX_OK;
+		};
+		// the finest level's own part of the numeric set-up (inverse
+		// diagonal, power iteration, modes) needs no hierarchy: the device
+		// build queues it just before the host aggregates the finest level, so
+		// the GPU runs it meanwhile (the FP32 copy of the matrix it reads is
+		// queued before the build)
+		auto head0 = [&]() -> int {
+			if (n0 > 2 * par.coarsest) {
+				BFMX_CHECK(vectors(lv[0]));
+				numeric_level_head(s, 0, false);
+				BFMX_HIP(hipGetLastError());
+				s->amg_head0 = true;
+			}
+			return BFMX_OK;
+		};
+		std::vector<int32_t> fseg;
+		BFMX_CHECK(device ? build_levels_device(s, smooth, fseg, head0) : build_levels_host(s, smooth, fseg));
+		phase.mark("levels");
+		for (auto& L : lv) {
+			BFMX_CHECK(vectors(L));
 		}
 		{
 			AmgLevel& Lc = lv.back();
@@ -2940,10 +2968,61 @@ int amg_build(bfmx_system* s) {
 // inverse, from the current FP64 matrix and Dirichlet set, level by level:
 // a level's smoother data (inverse diagonal blocks, lambda_max) come first,
 // since its prolongator smoothing uses them.
+namespace {
+
+// The part of a level's numeric set-up that needs only its own matrix: the
+// inverse diagonal blocks and Gershgorin bounds, the power iteration for
+// lambda_max(D^-1 A) and the Chebyshev coefficients (on the finest level
+// also its near-null-space modes with the fixed DOFs removed).
+void numeric_level_head(bfmx_system* s, size_t l, bool coarsest) {
+	hipStream_t const st = s->stream;
+	AmgLevel& L = s->amg[l];
+	if (l == 0) {
+		k_fine_modes<<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_B0, s->d_fixed, L.d_B);
+	}
+	int const g = grid(L.nb);
+	if (L.bs == 2) {
+		k_diag_gersh<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_diag_blk, L.d_val64, L.d_dinv, L.d_partial);
+	} else {
+		k_diag_gersh<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_diag_blk, L.d_val64, L.d_dinv, L.d_partial);
+	}
+	int const power = coarsest ? 0 : l == 0 ? s->amg_par.power : s->amg_par.power_coarse;
+	if (power > 0) {
+		// the Gershgorin maxima stay in d_partial's first g entries; the
+		// power sums use the entries after them
+		double* const pp = L.d_partial + g;
+		int64_t const m = static_cast<int64_t>(L.bs) * L.nb;
+		k_power_init<<<grid(m), kBlock, 0, st>>>(m, L.d_x[0], pp);
+		k_power_norm<<<1, kBlock, 0, st>>>(grid(m), pp, L.d_scal);
+		// the iterates are normalised only for the last step, whose input is
+		// then a unit vector: lambda_max(D^-1 A) lies between 1 and a few
+		// units, so ten unnormalised FP32 steps neither overflow nor underflow
+		for (int k = 0; k < power; k++) {
+			float* const xin = L.d_x[k & 1];
+			float* const yout = L.d_x[(k & 1) ^ 1];
+			if (L.bs == 2) {
+				k_power_step<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, L.d_dinv, xin,
+				                                      L.d_scal, yout, pp);
+			} else {
+				k_power_step<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, L.d_dinv, xin,
+				                                      L.d_scal, yout, pp);
+			}
+			if (k + 2 >= power) {
+				k_power_norm<<<1, kBlock, 0, st>>>(g, pp, L.d_scal);
+			}
+		}
+	}
+	k_cheb_coef<<<1, kBlock, 0, st>>>(g, L.d_partial, coarsest ? kCoarseRatio : s->amg_par.ratio,
+	                                  coarsest ? kCoarseDeg : level_deg(s->amg_par, l), power > 0, L.d_scal);
+}
+
+}  // namespace
+
 int amg_setup_numeric(bfmx_system* s) {
 	hipStream_t const st = s->stream;
 	std::vector<AmgLevel>& lv = s->amg;
-	k_fine_modes<<<grid(lv[0].nb), kBlock, 0, st>>>(lv[0].nb, lv[0].d_B0, s->d_fixed, lv[0].d_B);
+	bool const head0 = s->amg_head0;   // launched by the build, ahead of the finest level's aggregation
+	s->amg_head0 = false;
 	for (size_t l = 0; l < lv.size(); l++) {
 		AmgLevel& L = lv[l];
 		bool const coarsest = l + 1 == lv.size();
@@ -2963,40 +3042,9 @@ int amg_setup_numeric(bfmx_system* s) {
 			continue;
 		}
 		int const g = grid(L.nb);
-		if (L.bs == 2) {
-			k_diag_gersh<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_diag_blk, L.d_val64, L.d_dinv, L.d_partial);
-		} else {
-			k_diag_gersh<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_diag_blk, L.d_val64, L.d_dinv, L.d_partial);
+		if (!(l == 0 && head0)) {
+			numeric_level_head(s, l, coarsest);
 		}
-		int const power = coarsest ? 0 : l == 0 ? s->amg_par.power : s->amg_par.power_coarse;
-		if (power > 0) {
-			// the Gershgorin maxima stay in d_partial's first g entries; the
-			// power sums use the entries after them
-			double* const pp = L.d_partial + g;
-			int64_t const m = static_cast<int64_t>(L.bs) * L.nb;
-			k_power_init<<<grid(m), kBlock, 0, st>>>(m, L.d_x[0], pp);
-			k_power_norm<<<1, kBlock, 0, st>>>(grid(m), pp, L.d_scal);
-			// the iterates are normalised only for the last step, whose input is
-			// then a unit vector: lambda_max(D^-1 A) lies between 1 and a few
-			// units, so ten unnormalised FP32 steps neither overflow nor underflow
-			for (int k = 0; k < power; k++) {
-				float* const xin = L.d_x[k & 1];
-				float* const yout = L.d_x[(k & 1) ^ 1];
-				if (L.bs == 2) {
-					k_power_step<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, L.d_dinv, xin,
-					                                      L.d_scal, yout, pp);
-				} else {
-					k_power_step<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val32, L.d_dinv, xin,
-					                                      L.d_scal, yout, pp);
-				}
-				if (k + 2 >= power) {
-					k_power_norm<<<1, kBlock, 0, st>>>(g, pp, L.d_scal);
-				}
-			}
-		}
-		k_cheb_coef<<<1, kBlock, 0, st>>>(g, L.d_partial, coarsest ? kCoarseRatio : s->amg_par.ratio,
-		                                  coarsest ? kCoarseDeg : level_deg(s->amg_par, l),
-		                                  power > 0, L.d_scal);
 		if (coarsest) {
 			continue;
 		}

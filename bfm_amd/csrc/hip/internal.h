@@ -342,6 +342,7 @@ struct bfmx_system {
 	std::vector<bfmx::AmgLevel> amg;   // multigrid hierarchy (built on first use)
 	bfmx::AmgParams amg_par;           // its parameters
 	int amg_wg = -1;                   // first level run by the one-workgroup coarse cycle (-1: none)
+	bool amg_head0 = false;            // the finest level's numeric head was queued by the build
 	bfmx::AmgLevelDesc* d_amg_desc = nullptr;   // descriptors of the levels from amg_wg on
 	uint64_t* d_amg_pf = nullptr;      // addresses of the 128-byte lines those levels' arrays span
 	int amg_npf = 0;

@@ -411,8 +411,13 @@ __global__ __launch_bounds__(kRed) void k_red_fcg_beta(int nwg, const double* __
 // matrix, partials p_new . q and p_new . r.
 // SEG: a segmented flexible CG (beta of the row's segment, partial sums kept
 // apart per segment).
+#ifdef BFMX_FCG_WAVES
+#define BFMX_FCG_ATTR __attribute__((amdgpu_waves_per_eu(BFMX_FCG_WAVES, BFMX_FCG_WAVES)))
+#else
+#define BFMX_FCG_ATTR
+#endif
 template <typename ZT, bool SEG = false>
-__global__ __launch_bounds__(kBlock) void k_fcg_spmv(int nb, const int32_t* __restrict__ slice_ptr,
+__global__ __launch_bounds__(kBlock) BFMX_FCG_ATTR void k_fcg_spmv(int nb, const int32_t* __restrict__ slice_ptr,
                                                      const int32_t* __restrict__ col,
                                                      const double* __restrict__ val, const ZT* __restrict__ z,
                                                      double* __restrict__ p0, double* __restrict__ p1,
@@ -1485,8 +1490,9 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 	PhaseTimer phase(st, "solve");
 	BFMX_CHECK(ensure_workspace(s));
 	phase.mark("workspace");
-	if ((mixed || amg || learned) && !s->d_val32) {
-		BFMX_CHECK(dalloc(&s->d_val32, 4 * s->nnzb_pad));   // before any capture reads its address
+	if (mixed || amg || learned) {
+		// the FP32 copy first: the multigrid build queues kernels that read it
+		BFMX_CHECK(ensure_f32(s));
 	}
 	if (amg) {
 		BFMX_CHECK(amg_build(s));   // topology, once per system
@@ -1542,10 +1548,7 @@ int solve(bfmx_system* s, const double* b_in, int precond, int precision, double
 		}
 		return status;
 	}
-	if (mixed || amg || learned) {
-		BFMX_CHECK(ensure_f32(s));
-	}
-	phase.mark("right-hand side, FP32 copy");
+	phase.mark("right-hand side");
 	if (amg) {
 		BFMX_CHECK(amg_setup_numeric(s));
 		phase.mark("multigrid numeric set-up");
