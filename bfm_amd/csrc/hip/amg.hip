@@ -2152,6 +2152,7 @@ void amg_release(bfmx_system* s) {
 	s->amg_npf = 0;
 	s->amg_wg = -1;
 	s->amg_head0 = false;
+	s->amg_num_gen = 0;
 }
 
 namespace {
@@ -3019,6 +3020,9 @@ void numeric_level_head(bfmx_system* s, size_t l, bool coarsest) {
 }  // namespace
 
 int amg_setup_numeric(bfmx_system* s) {
+	if (!s->amg_head0 && s->amg_num_gen == s->val_gen) {
+		return BFMX_OK;   // made from the current matrix values
+	}
 	hipStream_t const st = s->stream;
 	std::vector<AmgLevel>& lv = s->amg;
 	bool const head0 = s->amg_head0;   // launched by the build, ahead of the finest level's aggregation
@@ -3098,6 +3102,7 @@ int amg_setup_numeric(bfmx_system* s) {
 		}
 	}
 	BFMX_HIP(hipGetLastError());
+	s->amg_num_gen = s->val_gen;
 	return BFMX_OK;
 }
 

@@ -135,6 +135,7 @@ int assemble(bfmx_system* s, int stress, double E, double nu, double rho, const 
 	hipStream_t const st = s->stream;
 	int64_t const n = s->n_nodes;
 	double* d_force = nullptr;
+	s->val_gen++;
 	s->e_spread = 1.0;
 	if (elemE) {
 		std::vector<double> eE(s->n_elems);
@@ -202,6 +203,7 @@ int apply_condition(bfmx_system* s, int kind, double value, const uint8_t* flags
 		set_error("unknown condition kind %d", kind);
 		return BFMX_E_ARG;
 	}
+	s->val_gen++;
 	bool const neumann = kind >= BFMX_NEUMANN_X && kind <= BFMX_NEUMANN_TANGENT;
 	bool const nt = kind == BFMX_DIRICHLET_NORMAL || kind == BFMX_DIRICHLET_TANGENT;
 	if (s->stress == 2 && (kind == BFMX_NEUMANN_NORMAL || kind == BFMX_NEUMANN_TANGENT)) {
@@ -262,6 +264,7 @@ int finalize_conditions(bfmx_system* s) {
 	if (s->finalized) {
 		return BFMX_OK;
 	}
+	s->val_gen++;
 	int64_t const n = s->n_nodes;
 	std::vector<uint8_t> touched(n, 0);
 	for (int64_t i = 0; i < n; i++) {

@@ -765,6 +765,9 @@ namespace {
 // Assembly of a batch built from meshes (largest systems first).
 void launch_assemble(bfmx_batch* b) {
 	if (b->planar) {
+		for (auto* s : b->systems) {
+			s->val_gen++;
+		}
 		k_batch_assemble<<<static_cast<int>(b->systems.size()), kAsmBlock, 0, b->stream>>>(b->d_inst,
 		                                                                                   b->d_order_all);
 	}
@@ -935,6 +938,7 @@ int batch_solve_amg(bfmx_batch* b, double rtol, int64_t max_iter, bfmx_stats_t* 
 	BFMX_HIP(hipEventRecord(b->ev[0], st));
 	launch_assemble(b);
 	k_batch_bnorm<<<m, kUniBlock, 0, st>>>(b->d_inst, b->d_ubb);
+	U->val_gen++;
 	k_union_gather<<<blocks_of(n), kBlock, 0, st>>>(n, m, b->d_uoff, b->d_inst, b->d_ubb, U->d_slice_ptr, U->d_val,
 	                                                U->d_b, U->d_fixed);
 	BFMX_HIP(hipGetLastError());

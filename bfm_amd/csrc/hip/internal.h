@@ -343,6 +343,12 @@ struct bfmx_system {
 	bfmx::AmgParams amg_par;           // its parameters
 	int amg_wg = -1;                   // first level run by the one-workgroup coarse cycle (-1: none)
 	bool amg_head0 = false;            // the finest level's numeric head was queued by the build
+	// generations of the matrix values: bumped by every change of d_val;
+	// the FP32 copy and the multigrid's numeric set-up remember the one they
+	// were made from and are redone only when it moved
+	uint64_t val_gen = 1;
+	uint64_t f32_gen = 0;
+	uint64_t amg_num_gen = 0;
 	bfmx::AmgLevelDesc* d_amg_desc = nullptr;   // descriptors of the levels from amg_wg on
 	uint64_t* d_amg_pf = nullptr;      // addresses of the 128-byte lines those levels' arrays span
 	int amg_npf = 0;

@@ -787,9 +787,12 @@ int ensure_workspace(bfmx_system* s) {
 int ensure_f32(bfmx_system* s) {
 	if (!s->d_val32) {
 		BFMX_CHECK(dalloc(&s->d_val32, 4 * s->nnzb_pad));
+	} else if (s->f32_gen == s->val_gen) {
+		return BFMX_OK;   // made from the current values
 	}
 	k_to_f32<<<grid(4 * s->nnzb_pad), kBlock, 0, s->stream>>>(4 * s->nnzb_pad, s->d_val, s->d_val32);
 	BFMX_HIP(hipGetLastError());
+	s->f32_gen = s->val_gen;
 	return BFMX_OK;
 }
 
