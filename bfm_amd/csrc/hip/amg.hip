@@ -2039,7 +2039,8 @@ int upload(T** d, const std::vector<T>& h, hipStream_t st) {
 	return BFMX_OK;
 }
 
-void numeric_level_head(bfmx_system* s, size_t l, bool coarsest);
+void numeric_level_head(bfmx_system* s, size_t l, bool coarsest, hipStream_t st);
+void numeric_level(bfmx_system* s, size_t l, hipStream_t st, bool skip_head);
 
 // Smoothed aggregation unless the environment sets BFMX_AMG_SMOOTH=0
 // (unsmoothed aggregation, P = P_t: for comparisons).
@@ -2100,6 +2101,15 @@ void amg_release(bfmx_system* s) {
 	if (!s->amg.empty() || s->d_amg_desc || s->d_amg_pf) {
 		// the pool takes blocks back only from drained streams
 		(void) hipStreamSynchronize(s->stream);
+	}
+	if (s->amg_stream) {
+		(void) hipStreamSynchronize(s->amg_stream);
+		(void) hipStreamDestroy(s->amg_stream);
+		s->amg_stream = nullptr;
+	}
+	if (s->amg_ev) {
+		(void) hipEventDestroy(s->amg_ev);
+		s->amg_ev = nullptr;
 	}
 	for (size_t l = 0; l < s->amg.size(); l++) {
 		AmgLevel& L = s->amg[l];
@@ -2524,7 +2534,7 @@ constexpr uint32_t kAggSalt = 0x2545F491u;   // seeds the aggregation's row prio
 // aggregates. BFMX_E_UNSUPPORTED when a row exceeds the device build's
 // bounds (the caller then builds on the host).
 int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
-                        const std::function<int()>& before_finest) {
+                        const std::function<int()>& before_finest, const std::function<int(int)>& after_level) {
 	hipStream_t const st = s->stream;
 	std::vector<AmgLevel>& lv = s->amg;
 	AmgParams const& par = s->amg_par;
@@ -2669,6 +2679,7 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 		BFMX_CHECK(palloc(&F.d_apval, static_cast<size_t>(F.bs) * 3 * F.nnz_ap));
 		phase.mark("level values");
 		fseg.swap(cseg);
+		BFMX_CHECK(after_level(lf));
 	}
 	return BFMX_OK;
 }
@@ -2818,14 +2829,49 @@ int amg_build_impl(bfmx_system* s, bool device) {
 		auto head0 = [&]() -> int {
 			if (n0 > 2 * par.coarsest) {
 				BFMX_CHECK(vectors(lv[0]));
-				numeric_level_head(s, 0, false);
+				numeric_level_head(s, 0, false, st);
 				BFMX_HIP(hipGetLastError());
 				s->amg_head0 = true;
 			}
 			return BFMX_OK;
 		};
+		// a level's numeric set-up (its prolongator and the next level's
+		// Galerkin operator) needs only its own topology and the levels above:
+		// it goes on a second stream as soon as that topology is complete, so
+		// the GPU runs it while the host builds the levels below (the set-up
+		// kernels of all levels are the same as on one stream; BFMX_AMG_OVERLAP=0
+		// queues them after the build, on the system's stream)
+		bool overlap = true;
+		if (const char* e = std::getenv("BFMX_AMG_OVERLAP"); e && e[0] == '0') {
+			overlap = false;
+		}
+		if (overlap && !s->amg_stream) {
+			BFMX_HIP(hipStreamCreateWithFlags(&s->amg_stream, hipStreamNonBlocking));
+		}
+		if (overlap && !s->amg_ev) {
+			BFMX_HIP(hipEventCreateWithFlags(&s->amg_ev, hipEventDisableTiming));
+		}
+		size_t queued = 0;   // levels whose numeric set-up is on the second stream
+		auto queue_numeric = [&](size_t upto) -> int {
+			BFMX_HIP(hipEventRecord(s->amg_ev, st));
+			BFMX_HIP(hipStreamWaitEvent(s->amg_stream, s->amg_ev, 0));
+			for (; queued < upto; queued++) {
+				numeric_level(s, queued, s->amg_stream, queued == 0 && s->amg_head0);
+			}
+			BFMX_HIP(hipGetLastError());
+			return BFMX_OK;
+		};
+		auto after_level = [&](int lf) -> int {
+			if (overlap) {
+				BFMX_CHECK(vectors(lv[lf]));
+				BFMX_CHECK(vectors(lv[lf + 1]));
+				BFMX_CHECK(queue_numeric(static_cast<size_t>(lf) + 1));
+			}
+			return BFMX_OK;
+		};
 		std::vector<int32_t> fseg;
-		BFMX_CHECK(device ? build_levels_device(s, smooth, fseg, head0) : build_levels_host(s, smooth, fseg));
+		BFMX_CHECK(device ? build_levels_device(s, smooth, fseg, head0, after_level)
+		                  : build_levels_host(s, smooth, fseg));
 		phase.mark("levels");
 		for (auto& L : lv) {
 			BFMX_CHECK(vectors(L));
@@ -2941,6 +2987,15 @@ int amg_build_impl(bfmx_system* s, bool device) {
 			s->amg_npf = static_cast<int>(pf.size());
 			BFMX_HIP(hipStreamSynchronize(st));
 		}
+		if (overlap) {
+			// the levels left (the coarsest, levels built on the host), then
+			// the system's stream waits for the second one
+			BFMX_CHECK(queue_numeric(lv.size()));
+			BFMX_HIP(hipEventRecord(s->amg_ev, s->amg_stream));
+			BFMX_HIP(hipStreamWaitEvent(st, s->amg_ev, 0));
+			s->amg_head0 = false;
+			s->amg_num_gen = s->val_gen;
+		}
 		return BFMX_OK;
 	}();
 	if (rc != BFMX_OK) {
@@ -2975,8 +3030,7 @@ namespace {
 // inverse diagonal blocks and Gershgorin bounds, the power iteration for
 // lambda_max(D^-1 A) and the Chebyshev coefficients (on the finest level
 // also its near-null-space modes with the fixed DOFs removed).
-void numeric_level_head(bfmx_system* s, size_t l, bool coarsest) {
-	hipStream_t const st = s->stream;
+void numeric_level_head(bfmx_system* s, size_t l, bool coarsest, hipStream_t st) {
 	AmgLevel& L = s->amg[l];
 	if (l == 0) {
 		k_fine_modes<<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_B0, s->d_fixed, L.d_B);
@@ -3017,89 +3071,96 @@ void numeric_level_head(bfmx_system* s, size_t l, bool coarsest) {
 	                                  coarsest ? kCoarseDeg : level_deg(s->amg_par, l), power > 0, L.d_scal);
 }
 
+// Level l's numeric set-up on stream st: its head (unless skip_head: the
+// build queued it already) and, below the coarsest level, its prolongator and
+// the next level's Galerkin operator; on a coarsest level with dense inverses,
+// those inverses instead.
+void numeric_level(bfmx_system* s, size_t l, hipStream_t st, bool skip_head) {
+	std::vector<AmgLevel>& lv = s->amg;
+	AmgLevel& L = lv[l];
+	bool const coarsest = l + 1 == lv.size();
+	if (coarsest && L.d_dense) {
+		if (L.bs == 2) {
+			k_dense<2><<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_seg_ptr,
+			                                          L.d_seg_row, L.d_seg_off, L.d_dense);
+		} else {
+			k_dense<3><<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_seg_ptr,
+			                                          L.d_seg_row, L.d_seg_off, L.d_dense);
+		}
+		if (L.n_seg == 1) {
+			k_gauss_jordan<kGj><<<1, kGj, 0, st>>>(L.bs, L.d_seg_ptr, L.d_seg_off, L.d_dense);
+		} else {
+			k_gauss_jordan<kGjSeg><<<L.n_seg, kGjSeg, 0, st>>>(L.bs, L.d_seg_ptr, L.d_seg_off, L.d_dense);
+		}
+		return;
+	}
+	int const g = grid(L.nb);
+	if (!skip_head) {
+		numeric_level_head(s, l, coarsest, st);
+	}
+	if (coarsest) {
+		return;
+	}
+	AmgLevel& C = lv[l + 1];
+	int64_t const ns = C.nnzb_pad;
+	if (L.bs == 2) {
+		k_tentative<2><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
+		k_smooth_p<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_val32, L.d_diag_blk,
+		                                    L.d_Pt, L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0,
+		                                    L.d_P);
+		k_transpose_p<2><<<grid(L.nnzp), kBlock, 0, st>>>(L.nnzp, L.d_rent, L.d_P, L.d_R);
+		if (L.d_apptr) {
+			// finest level: windows of 8 entries per thread for A P (332 us
+			// on the 2M-DOF beam against 402 / 383 us with 4 / 16 and 738 us
+			// with a lane per entry), a lane per coarse slot for P^T (A P)
+			// (351 us against 382 us with 16 lanes per slot)
+			k_ap<2, 8><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_row_len, L.d_val32, L.d_pptr,
+			                                 L.d_pcol, L.d_P, L.d_apptr, L.d_apcol, L.d_apval);
+			k_rap_lanes<2><<<grid(C.n_slices * kSlice * 16), kBlock, 0, st>>>(
+			    C.n_slices * kSlice, C.nb, C.d_slice_ptr, C.d_col, C.d_row_len, C.d_diag_blk, L.d_gal_mirror,
+			    L.d_rptr, L.d_rrow, L.d_R, L.d_apptr, L.d_apcol, L.d_apval, C.d_val64, C.d_val32);
+		} else {
+			k_galerkin<2, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
+			                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
+			                                                               L.d_val32, L.d_P, C.d_val64,
+			                                                               C.d_val32);
+		}
+	} else {
+		k_tentative<3><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
+		k_smooth_p<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_val32, L.d_diag_blk,
+		                                    L.d_Pt, L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0,
+		                                    L.d_P);
+		k_transpose_p<3><<<grid(L.nnzp), kBlock, 0, st>>>(L.nnzp, L.d_rent, L.d_P, L.d_R);
+		if (L.d_apptr) {
+			// coarser levels (wider rows, longer prolongator columns): a lane
+			// per entry for A P (39 us per level against 51-69 us with
+			// windows), 16 lanes per coarse slot for P^T (A P) (23 us against
+			// 52 us with a lane per slot)
+			k_ap_lanes<3><<<grid(static_cast<int64_t>(L.nb) * 16), kBlock, 0, st>>>(
+			    L.nb, L.d_slice_ptr, L.d_col, L.d_row_len, L.d_val32, L.d_pptr, L.d_pcol, L.d_P, L.d_apptr,
+			    L.d_apcol, L.d_apval);
+			k_rap<3><<<grid(C.n_slices * kSlice * 64), kBlock, 0, st>>>(
+			    C.n_slices * kSlice, C.nb, C.d_slice_ptr, C.d_col, C.d_row_len, C.d_diag_blk, L.d_gal_mirror,
+			    L.d_rptr, L.d_rrow, L.d_R, L.d_apptr, L.d_apcol, L.d_apval, C.d_val64, C.d_val32);
+		} else {
+			k_galerkin<3, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
+			                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
+			                                                               L.d_val32, L.d_P, C.d_val64,
+			                                                               C.d_val32);
+		}
+	}
+}
+
 }  // namespace
 
 int amg_setup_numeric(bfmx_system* s) {
 	if (!s->amg_head0 && s->amg_num_gen == s->val_gen) {
 		return BFMX_OK;   // made from the current matrix values
 	}
-	hipStream_t const st = s->stream;
-	std::vector<AmgLevel>& lv = s->amg;
 	bool const head0 = s->amg_head0;   // launched by the build, ahead of the finest level's aggregation
 	s->amg_head0 = false;
-	for (size_t l = 0; l < lv.size(); l++) {
-		AmgLevel& L = lv[l];
-		bool const coarsest = l + 1 == lv.size();
-		if (coarsest && L.d_dense) {
-			if (L.bs == 2) {
-				k_dense<2><<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_seg_ptr,
-				                                          L.d_seg_row, L.d_seg_off, L.d_dense);
-			} else {
-				k_dense<3><<<grid(L.nb), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_seg_ptr,
-				                                          L.d_seg_row, L.d_seg_off, L.d_dense);
-			}
-			if (L.n_seg == 1) {
-				k_gauss_jordan<kGj><<<1, kGj, 0, st>>>(L.bs, L.d_seg_ptr, L.d_seg_off, L.d_dense);
-			} else {
-				k_gauss_jordan<kGjSeg><<<L.n_seg, kGjSeg, 0, st>>>(L.bs, L.d_seg_ptr, L.d_seg_off, L.d_dense);
-			}
-			continue;
-		}
-		int const g = grid(L.nb);
-		if (!(l == 0 && head0)) {
-			numeric_level_head(s, l, coarsest);
-		}
-		if (coarsest) {
-			continue;
-		}
-		AmgLevel& C = lv[l + 1];
-		int64_t const ns = C.nnzb_pad;
-		if (L.bs == 2) {
-			k_tentative<2><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
-			k_smooth_p<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_val32, L.d_diag_blk,
-			                                    L.d_Pt, L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0,
-			                                    L.d_P);
-			k_transpose_p<2><<<grid(L.nnzp), kBlock, 0, st>>>(L.nnzp, L.d_rent, L.d_P, L.d_R);
-			if (L.d_apptr) {
-				// finest level: windows of 8 entries per thread for A P (332 us
-				// on the 2M-DOF beam against 402 / 383 us with 4 / 16 and 738 us
-				// with a lane per entry), a lane per coarse slot for P^T (A P)
-				// (351 us against 382 us with 16 lanes per slot)
-				k_ap<2, 8><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_row_len, L.d_val32, L.d_pptr,
-				                                 L.d_pcol, L.d_P, L.d_apptr, L.d_apcol, L.d_apval);
-				k_rap_lanes<2><<<grid(C.n_slices * kSlice * 16), kBlock, 0, st>>>(
-				    C.n_slices * kSlice, C.nb, C.d_slice_ptr, C.d_col, C.d_row_len, C.d_diag_blk, L.d_gal_mirror,
-				    L.d_rptr, L.d_rrow, L.d_R, L.d_apptr, L.d_apcol, L.d_apval, C.d_val64, C.d_val32);
-			} else {
-				k_galerkin<2, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
-				                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
-				                                                               L.d_val32, L.d_P, C.d_val64,
-				                                                               C.d_val32);
-			}
-		} else {
-			k_tentative<3><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
-			k_smooth_p<3><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_val32, L.d_diag_blk,
-			                                    L.d_Pt, L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0,
-			                                    L.d_P);
-			k_transpose_p<3><<<grid(L.nnzp), kBlock, 0, st>>>(L.nnzp, L.d_rent, L.d_P, L.d_R);
-			if (L.d_apptr) {
-				// coarser levels (wider rows, longer prolongator columns): a lane
-				// per entry for A P (39 us per level against 51-69 us with
-				// windows), 16 lanes per coarse slot for P^T (A P) (23 us against
-				// 52 us with a lane per slot)
-				k_ap_lanes<3><<<grid(static_cast<int64_t>(L.nb) * 16), kBlock, 0, st>>>(
-				    L.nb, L.d_slice_ptr, L.d_col, L.d_row_len, L.d_val32, L.d_pptr, L.d_pcol, L.d_P, L.d_apptr,
-				    L.d_apcol, L.d_apval);
-				k_rap<3><<<grid(C.n_slices * kSlice * 64), kBlock, 0, st>>>(
-				    C.n_slices * kSlice, C.nb, C.d_slice_ptr, C.d_col, C.d_row_len, C.d_diag_blk, L.d_gal_mirror,
-				    L.d_rptr, L.d_rrow, L.d_R, L.d_apptr, L.d_apcol, L.d_apval, C.d_val64, C.d_val32);
-			} else {
-				k_galerkin<3, float><<<grid(ns * kGalLanes), kBlock, 0, st>>>(ns, L.d_gal_mirror, L.d_gal_ptr,
-				                                                               L.d_gal_blk, L.d_gal_pi, L.d_gal_pj,
-				                                                               L.d_val32, L.d_P, C.d_val64,
-				                                                               C.d_val32);
-			}
-		}
+	for (size_t l = 0; l < s->amg.size(); l++) {
+		numeric_level(s, l, s->stream, l == 0 && head0);
 	}
 	BFMX_HIP(hipGetLastError());
 	s->amg_num_gen = s->val_gen;

@@ -343,6 +343,11 @@ struct bfmx_system {
 	bfmx::AmgParams amg_par;           // its parameters
 	int amg_wg = -1;                   // first level run by the one-workgroup coarse cycle (-1: none)
 	bool amg_head0 = false;            // the finest level's numeric head was queued by the build
+	// the hierarchy's first build queues each level's numeric set-up on this
+	// second stream as soon as the level's topology is complete, so the GPU
+	// runs it while the host builds the next level's topology
+	hipStream_t amg_stream = nullptr;
+	hipEvent_t amg_ev = nullptr;
 	// generations of the matrix values: bumped by every change of d_val;
 	// the FP32 copy and the multigrid's numeric set-up remember the one they
 	// were made from and are redone only when it moved

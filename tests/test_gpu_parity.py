@@ -481,6 +481,31 @@ def test_amg_device_and_host_hierarchies(monkeypatch, kind, host_agg):
     assert rel_err(u1, u_ref) <= U_TOL and rel_err(u3, u_ref) <= U_TOL
 
 
+def test_amg_numeric_overlap_matches_one_stream(monkeypatch):
+    """The first build queues each level's numeric set-up on a second stream
+    while the host builds the levels below; the same set-up queued after the
+    build on the system's stream (BFMX_AMG_OVERLAP=0) gives bitwise the same
+    solve, on a heterogeneous beam and after a reassembly with new E."""
+    mesh = meshgen.beam(300, 75, kind=4, jitter=0.2, seed=17)
+    mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=17)
+    out = []
+    for overlap in ("1", "0"):
+        monkeypatch.setenv("BFMX_AMG_OVERLAP", overlap)
+        s = D.beam_system(mesh)
+        u, st = s.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+        assert st["status"] == 0
+        mat = meshgen.BEAM_MATERIAL
+        s.assemble(1, mat["E"], mat["nu"], mat["rho"], elem_E=meshgen.loguniform_E(mesh.n_elems, seed=18))
+        s.apply_conditions(*meshgen.beam_conditions(mesh))
+        u2, st2 = s.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+        assert st2["status"] == 0
+        s.close()
+        out.append((u, st["iterations"], u2, st2["iterations"]))
+    print(f"overlap: {out[0][1]} / {out[0][3]} iterations, one stream: {out[1][1]} / {out[1][3]}")
+    assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0])
+    assert out[0][3] == out[1][3] and np.array_equal(out[0][2], out[1][2])
+
+
 def test_finest_sweeps_do_not_depend_on_threads(monkeypatch):
     """A beam of 90,000 nodes: its finest level is aggregated by one
     sequential sweep on the host; one host thread and sixteen give bitwise
