@@ -1677,6 +1677,85 @@ __global__ __launch_bounds__(kBlock) void k_prolong(int nb, const int32_t* __res
 	}
 }
 
+// The prolongation and the first post-smoothing step of a level in one
+// pass. r (the level's d_r) is the residual b - A x that the residual kernel
+// left before the restriction, x_c the coarse correction. With x' = x + P x_c
+// the step's residual is b - A x' = r - (A P) x_c, row-local given row i of P
+// and of A P (the Galerkin set-up's A P, kept per level): d = c2 D^-1 (r -
+// (A P) x_c), x_out = x' + d, as the first step of k_cheb takes it from x'.
+// It saves the prolongation's launch and its write and re-read of x, and
+// reads a row of A P (its coarse blocks) in place of the row of A.
+// kProlongLanes lanes share a row: lane l takes entries l, l + kProlongLanes,
+// ... of its rows of P and A P (consecutive lanes read consecutive entries),
+// combined by a fixed butterfly.
+constexpr int kProlongLanes = 8;
+
+template <int BSF>
+__global__ __launch_bounds__(kBlock) void k_prolong_cheb(
+    int nb, const int32_t* __restrict__ pptr, const int32_t* __restrict__ pcol, const float* __restrict__ P,
+    const int32_t* __restrict__ apptr, const int32_t* __restrict__ apcol, const float* __restrict__ apval,
+    const float* __restrict__ xc, const float* __restrict__ r, const float* __restrict__ dinv,
+    const double* __restrict__ scal, const float* __restrict__ xin, float* __restrict__ xout, float* __restrict__ d,
+    const Scalars* __restrict__ S) {
+	int64_t const t = static_cast<int64_t>(xcd_remap(blockIdx.x, gridDim.x)) * kBlock + threadIdx.x;
+	int64_t const i = t / kProlongLanes;
+	int const lane = static_cast<int>(t % kProlongLanes);
+	bool const live = i < nb;
+	int64_t const ic = live ? i : nb - 1;
+	int32_t const e0 = pptr[ic], e1 = pptr[ic + 1], f0 = apptr[ic], f1 = apptr[ic + 1];
+	if (done_after(S, e0, e1, f0, f1)) {
+		return;
+	}
+	float px[BSF] = {}, apx[BSF] = {};
+	for (int32_t e = e0 + lane; e < e1; e += kProlongLanes) {
+		int64_t const J = pcol[e];
+		float const c0 = xc[3 * J], c1 = xc[3 * J + 1], c2 = xc[3 * J + 2];
+		const float* p = P + BSF * 3 * static_cast<int64_t>(e);
+#pragma unroll
+		for (int k = 0; k < BSF; k++) {
+			px[k] += p[3 * k] * c0 + p[3 * k + 1] * c1 + p[3 * k + 2] * c2;
+		}
+	}
+	for (int32_t f = f0 + lane; f < f1; f += kProlongLanes) {
+		int64_t const J = apcol[f];
+		float const c0 = xc[3 * J], c1 = xc[3 * J + 1], c2 = xc[3 * J + 2];
+		const float* q = apval + BSF * 3 * static_cast<int64_t>(f);
+#pragma unroll
+		for (int a = 0; a < BSF; a++) {
+			apx[a] += q[3 * a] * c0 + q[3 * a + 1] * c1 + q[3 * a + 2] * c2;
+		}
+	}
+#pragma unroll
+	for (int a = 0; a < BSF; a++) {
+#pragma unroll
+		for (int off = kProlongLanes / 2; off > 0; off >>= 1) {
+			px[a] += __shfl_xor(px[a], off, 64);
+			apx[a] += __shfl_xor(apx[a], off, 64);
+		}
+	}
+	if (!live || lane != 0) {
+		return;
+	}
+	float res[BSF];
+#pragma unroll
+	for (int a = 0; a < BSF; a++) {
+		res[a] = r[BSF * i + a] - apx[a];
+	}
+	float const c2 = static_cast<float>(scal[2]);
+#pragma unroll
+	for (int a = 0; a < BSF; a++) {
+		float z = 0;
+#pragma unroll
+		for (int k = 0; k < BSF; k++) {
+			z += dinv[BSF * BSF * i + BSF * a + k] * res[k];
+		}
+		int64_t const o = BSF * i + a;
+		float const dn = c2 * z;
+		d[o] = dn;
+		xout[o] = xin[o] + px[a] + dn;
+	}
+}
+
 // Row `row` (a DOF) of x = A^-1 b with the dense inverse of its segment, by
 // one wavefront (lane `lane`); every lane returns the sum. The inverse is
 // symmetric, so the row is read as a column: coalesced across the lanes.
@@ -3249,11 +3328,23 @@ float* vcycle(bfmx_system* s, size_t l, const BT* b, const double* q, double* pa
 	    C.nb, L.d_rptr, L.d_rrow, L.d_R, L.d_r, C.d_b, fuse ? C.d_dinv : nullptr, C.d_scal, C.d_x[0],
 	    C.d_d, S);
 	float* xc = vcycle<3, float>(s, l + 1, C.d_b, nullptr, nullptr, S, fuse);
-	k_prolong<BS><<<g, kBlock, 0, st>>>(L.nb, L.d_pptr, L.d_pcol, L.d_P, xc, L.d_x[cur], S);
+	int k0 = 0;
+	if (BS == 3 && L.d_apptr && !q) {
+		// a coarse level: the prolongation with the first post-smoothing step
+		// (k_prolong_cheb; on the finest level a row of A P costs more than the
+		// row of A it replaces, and the x . q sums go with the last step)
+		k_prolong_cheb<BS><<<grid(static_cast<int64_t>(L.nb) * kProlongLanes), kBlock, 0, st>>>(
+		    L.nb, L.d_pptr, L.d_pcol, L.d_P, L.d_apptr, L.d_apcol, L.d_apval, xc, L.d_r, L.d_dinv, L.d_scal,
+		    L.d_x[cur], L.d_x[cur ^ 1], L.d_d, S);
+		cur ^= 1;
+		k0 = 1;
+	} else {
+		k_prolong<BS><<<g, kBlock, 0, st>>>(L.nb, L.d_pptr, L.d_pcol, L.d_P, xc, L.d_x[cur], S);
+	}
 	// the finest level's last step also writes the partial sums of x . q,
 	// split by segment in a segmented flexible CG
 	SegView const sv = l == 0 ? s->fcg_sv : SegView{};
-	for (int k = 0; k < deg; k++) {
+	for (int k = k0; k < deg; k++) {
 		bool const last = k + 1 == deg;
 		cheb<BS, BT>(st, L, b, false, k, L.d_x[cur], L.d_x[cur ^ 1], last ? q : nullptr, last ? partial : nullptr, S,
 		             last ? sv : SegView{});
