@@ -1640,19 +1640,47 @@ __device__ __forceinline__ ProlongHead<BSF> prolong_head(int i, const int32_t* _
 	return h;
 }
 
-// Row i of x += P x_c from its head.
+// Row i of x += P x_c from its head. The entries go in groups of
+// kProlongGroup whose column indices, values and coarse values are all loaded
+// before any is used (clamped to the row's last entry, the extra ones not
+// added), so a row's few entries cost one round of dependent loads instead
+// of one per entry; the sum keeps the entries' order. On the 2M-DOF beam's
+// finest level (2.3 entries per row): 24.6 us against 27.4 us one by one
+// (groups of 2: no gain).
+constexpr int kProlongGroup = 4;
+
 template <int BSF>
 __device__ __forceinline__ void prolong_row(int i, ProlongHead<BSF> h, const int32_t* __restrict__ pcol,
                                             const float* __restrict__ P, const float* __restrict__ xc,
                                             float* __restrict__ x) {
 	float (&acc)[BSF] = h.acc;
-	for (int32_t e = h.e0; e < h.e1; e++) {
-		int64_t const J = pcol[e];
-		float const e0 = xc[3 * J], e1 = xc[3 * J + 1], e2 = xc[3 * J + 2];
-		const float* p = P + BSF * 3 * static_cast<int64_t>(e);
+	for (int32_t e = h.e0; e < h.e1; e += kProlongGroup) {
+		int64_t J[kProlongGroup];
+		float pv[kProlongGroup][BSF * 3], xv[kProlongGroup][3];
 #pragma unroll
-		for (int k = 0; k < BSF; k++) {
-			acc[k] += p[3 * k] * e0 + p[3 * k + 1] * e1 + p[3 * k + 2] * e2;
+		for (int u = 0; u < kProlongGroup; u++) {
+			int32_t const eu = min(e + u, h.e1 - 1);
+			J[u] = pcol[eu];
+#pragma unroll
+			for (int c = 0; c < BSF * 3; c++) {
+				pv[u][c] = P[BSF * 3 * static_cast<int64_t>(eu) + c];
+			}
+		}
+#pragma unroll
+		for (int u = 0; u < kProlongGroup; u++) {
+#pragma unroll
+			for (int c = 0; c < 3; c++) {
+				xv[u][c] = xc[3 * J[u] + c];
+			}
+		}
+#pragma unroll
+		for (int u = 0; u < kProlongGroup; u++) {
+			if (e + u < h.e1) {
+#pragma unroll
+				for (int k = 0; k < BSF; k++) {
+					acc[k] += pv[u][3 * k] * xv[u][0] + pv[u][3 * k + 1] * xv[u][1] + pv[u][3 * k + 2] * xv[u][2];
+				}
+			}
 		}
 	}
 #pragma unroll
@@ -2610,8 +2638,10 @@ constexpr uint32_t kAggSalt = 0x2545F491u;   // seeds the aggregation's row prio
 
 // The levels below level 0, built on the GPU from the level graphs' sliced
 // ELL (amg_topo.hip): the same arrays as build_levels_host, from MIS-2
-// aggregates. BFMX_E_UNSUPPORTED when a row exceeds the device build's
-// bounds (the caller then builds on the host).
+// aggregates. before_finest() runs just before the finest level's
+// aggregation, after_level(l) once level l's transfer topology and the next
+// level's arrays exist. BFMX_E_UNSUPPORTED when a row exceeds the device
+// build's bounds (the caller then builds on the host).
 int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
                         const std::function<int()>& before_finest, const std::function<int(int)>& after_level) {
 	hipStream_t const st = s->stream;

@@ -65,7 +65,8 @@ __device__ __forceinline__ int32_t nbr_slot(const int32_t* __restrict__ slice_pt
 
 // Exclusive prefix sums of in[0 .. n) into out[0 .. n] (out[n] = total), in
 // three passes: per-block sums, one workgroup over the block sums, and the
-// block offsets added.
+// block offsets added; one pass when a single block holds all n (bsum null:
+// the block writes the total to out[n]).
 template <typename T, typename U>
 __global__ __launch_bounds__(kT) void k_scan_block(int64_t n, const T* __restrict__ in, U* __restrict__ out,
                                                    U* __restrict__ bsum) {
@@ -105,7 +106,11 @@ __global__ __launch_bounds__(kT) void k_scan_block(int64_t n, const T* __restric
 		run += v[k];
 	}
 	if (threadIdx.x == kT - 1) {
-		bsum[blockIdx.x] = wbase + inc;
+		if (bsum) {
+			bsum[blockIdx.x] = wbase + inc;
+		} else {
+			out[n] = wbase + inc;
+		}
 	}
 }
 
@@ -933,6 +938,10 @@ struct Temps {
 template <typename T, typename U>
 int scan(hipStream_t st, const T* in, U* out, int64_t n, Temps& tmp) {
 	int const nb = std::max(1, nblk(n, kScanBlock));
+	if (n > 0 && nb == 1) {
+		k_scan_block<T, U><<<1, kT, 0, st>>>(n, in, out, nullptr);
+		return BFMX_OK;
+	}
 	U* bsum = nullptr;
 	BFMX_CHECK(tmp.get(&bsum, static_cast<size_t>(nb) + 1));
 	if (n > 0) {
