@@ -1958,13 +1958,32 @@ __global__ __launch_bounds__(kWg) void k_coarse_cycle(const AmgLevelDesc* __rest
 
 // ------------------------------------------------------------------ host topology
 
-// Aggregates of rows r0 .. r1 - 1 of the graph (ptr, adj) (rows include
-// themselves), whose entries stay in that range: agg[i] receives the
-// aggregate's number counted from 0 within the range; returns their count.
-// Pass 1 makes a row whose neighbourhood is still free the root of a new
-// aggregate of that neighbourhood; pass 2 gives every other row the
-// aggregate of its first aggregated neighbour (every such row has one).
-int aggregate(int r0, int r1, const int64_t* ptr, const int32_t* adj, int32_t* agg) {
+// A level's graph on the host (rows include themselves, sorted): as CSR
+// (ptr, adj), or as the device's sliced ELL copied down (row i's k-th
+// neighbour at col[slice[i >> 6] + 64 k + (i & 63)], k < len[i]), which the
+// sweep reads as it is.
+struct CsrRows {
+	const int64_t* ptr;
+	const int32_t* adj;
+	int len(int i) const { return static_cast<int>(ptr[i + 1] - ptr[i]); }
+	int32_t at(int i, int k) const { return adj[ptr[i] + k]; }
+};
+struct SellRows {
+	const int32_t* slice;
+	const int32_t* col;
+	const int32_t* rlen;
+	int len(int i) const { return rlen[i]; }
+	int32_t at(int i, int k) const { return col[slice[i >> 6] + k * kSlice + (i & 63)]; }
+};
+
+// Aggregates of rows r0 .. r1 - 1 of the graph G, whose entries stay in
+// that range: agg[i] receives the aggregate's number counted from 0 within
+// the range; returns their count. Pass 1 makes a row whose neighbourhood is
+// still free the root of a new aggregate of that neighbourhood; pass 2 gives
+// every other row the aggregate of its first aggregated neighbour (every such
+// row has one).
+template <typename Rows>
+int aggregate(int r0, int r1, Rows const& G, int32_t* agg) {
 	std::fill(agg + r0, agg + r1, -1);
 	int na = 0;
 	std::vector<int32_t> rest;   // rows not yet aggregated when the sweep passed them
@@ -1972,13 +1991,14 @@ int aggregate(int r0, int r1, const int64_t* ptr, const int32_t* adj, int32_t* a
 		if (agg[i] >= 0) {
 			continue;
 		}
+		int const m = G.len(i);
 		bool free_nbhd = true;
-		for (int64_t t = ptr[i]; t < ptr[i + 1] && free_nbhd; t++) {
-			free_nbhd = agg[adj[t]] < 0;
+		for (int k = 0; k < m && free_nbhd; k++) {
+			free_nbhd = agg[G.at(i, k)] < 0;
 		}
 		if (free_nbhd) {
-			for (int64_t t = ptr[i]; t < ptr[i + 1]; t++) {
-				agg[adj[t]] = na;
+			for (int k = 0; k < m; k++) {
+				agg[G.at(i, k)] = na;
 			}
 			agg[i] = na++;
 		} else {
@@ -1992,8 +2012,9 @@ int aggregate(int r0, int r1, const int64_t* ptr, const int32_t* adj, int32_t* a
 		if (agg[i] >= 0) {
 			continue;   // taken by a later root
 		}
-		for (int64_t t = ptr[i]; t < ptr[i + 1] && join[r] < 0; t++) {
-			join[r] = agg[adj[t]];
+		int const m = G.len(i);
+		for (int k = 0; k < m && join[r] < 0; k++) {
+			join[r] = agg[G.at(i, k)];
 		}
 		if (join[r] < 0) {
 			join[r] = na++;   // not reached: the sweep saw an aggregated neighbour
@@ -2065,12 +2086,12 @@ void parallel_chunks(int T, const F& f) {
 // chunk. The first row of a segment is a root (its neighbours are all in the
 // segment and not yet aggregated), and the sweeps of whole segments do not
 // depend on how they are grouped.
-std::vector<int32_t> aggregate_chunks(const int64_t* ptr, const int32_t* adj, const std::vector<int32_t>& seg,
-                                      const std::vector<int32_t>& rb, std::vector<int32_t>& agg,
-                                      std::vector<int32_t>& cseg) {
+template <typename Rows>
+std::vector<int32_t> aggregate_chunks(Rows const& G, const std::vector<int32_t>& seg, const std::vector<int32_t>& rb,
+                                      std::vector<int32_t>& agg, std::vector<int32_t>& cseg) {
 	int const T = static_cast<int>(rb.size()) - 1;
 	std::vector<int32_t> ab(T + 1, 0);   // first aggregate of each chunk
-	parallel_chunks(T, [&](int c) { ab[c + 1] = aggregate(rb[c], rb[c + 1], ptr, adj, agg.data()); });
+	parallel_chunks(T, [&](int c) { ab[c + 1] = aggregate(rb[c], rb[c + 1], G, agg.data()); });
 	for (int c = 0; c < T; c++) {
 		ab[c + 1] += ab[c];
 	}
@@ -2286,18 +2307,26 @@ struct HostLevelGraph {
 
 int host_levels_from(bfmx_system* s, bool smooth, bool finest, HostLevelGraph& H, std::vector<int32_t>& fseg);
 
+// A level's sliced ELL copied from the device: slice pointers, block
+// columns and row lengths.
+int level_sell_to_host(hipStream_t st, TopoGraph const& G, std::vector<int32_t>& slice, std::vector<int32_t>& col,
+                       std::vector<int32_t>& len) {
+	slice.resize(static_cast<size_t>(G.n_slices) + 1);
+	col.resize(static_cast<size_t>(G.nnzb_pad));
+	len.resize(static_cast<size_t>(G.n));
+	BFMX_HIP(hipMemcpyAsync(slice.data(), G.slice_ptr, slice.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipMemcpyAsync(col.data(), G.col, col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipMemcpyAsync(len.data(), G.row_len, len.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	BFMX_HIP(hipStreamSynchronize(st));
+	return BFMX_OK;
+}
+
 // A level's graph (sliced ELL on the device) as the host build's state: its
 // rows in CSR and the block slot of every entry (rows in parallel).
 int level_graph_to_host(hipStream_t st, TopoGraph const& G, HostLevelGraph& H) {
 	int const nf = G.n;
-	std::vector<int32_t> len(nf);
-	H.fslice.resize(static_cast<size_t>(G.n_slices) + 1);
-	std::vector<int32_t> col(static_cast<size_t>(G.nnzb_pad));
-	BFMX_HIP(hipMemcpyAsync(H.fslice.data(), G.slice_ptr, H.fslice.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
-	                        st));
-	BFMX_HIP(hipMemcpyAsync(col.data(), G.col, col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-	BFMX_HIP(hipMemcpyAsync(len.data(), G.row_len, len.size() * sizeof(int32_t), hipMemcpyDeviceToHost, st));
-	BFMX_HIP(hipStreamSynchronize(st));
+	std::vector<int32_t> len, col;
+	BFMX_CHECK(level_sell_to_host(st, G, H.fslice, col, len));
 	H.fptr.assign(static_cast<size_t>(nf) + 1, 0);
 	for (int i = 0; i < nf; i++) {
 		H.fptr[i + 1] = H.fptr[i] + len[i];
@@ -2368,7 +2397,7 @@ int host_levels_from(bfmx_system* s, bool smooth, bool finest, HostLevelGraph& H
 		int const T = static_cast<int>(rb.size()) - 1;
 		std::atomic<int> bad{0};
 		std::vector<int32_t> agg(nf), cseg;
-		std::vector<int32_t> const ab = aggregate_chunks(ptr, adj, fseg, rb, agg, cseg);
+		std::vector<int32_t> const ab = aggregate_chunks(CsrRows{ptr, adj}, fseg, rb, agg, cseg);
 		int const na = cseg.back();
 		if (!std::is_sorted(cseg.begin(), cseg.end()) || cseg[0] != 0) {
 			set_error("multigrid: aggregates do not follow the level's segments");
@@ -2693,11 +2722,14 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 			// beam)
 			std::vector<int32_t> hagg(nf);
 			if (sys_graph) {
-				aggregate_chunks(s->adj_ptr.data(), s->adj.data(), fseg, chunk_rows(fseg, nf), hagg, cseg);
+				aggregate_chunks(CsrRows{s->adj_ptr.data(), s->adj.data()}, fseg, chunk_rows(fseg, nf), hagg, cseg);
 			} else {
-				HostLevelGraph H;
-				BFMX_CHECK(level_graph_to_host(st, G, H));
-				aggregate_chunks(H.fptr.data(), H.fadj.data(), fseg, chunk_rows(fseg, nf), hagg, cseg);
+				// the level's sliced ELL as it is (no CSR needed for the sweep)
+				std::vector<int32_t> slice, col, rlen;
+				BFMX_CHECK(level_sell_to_host(st, G, slice, col, rlen));
+				phase.mark("level graph to the host");
+				aggregate_chunks(SellRows{slice.data(), col.data(), rlen.data()}, fseg, chunk_rows(fseg, nf), hagg,
+				                 cseg);
 			}
 			phase.mark("sweeps (host)");
 			na = cseg.back();
