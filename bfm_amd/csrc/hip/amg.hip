@@ -2724,12 +2724,13 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 			if (sys_graph) {
 				aggregate_chunks(CsrRows{s->adj_ptr.data(), s->adj.data()}, fseg, chunk_rows(fseg, nf), hagg, cseg);
 			} else {
-				// the level's sliced ELL as it is (no CSR needed for the sweep)
-				std::vector<int32_t> slice, col, rlen;
-				BFMX_CHECK(level_sell_to_host(st, G, slice, col, rlen));
+				// the level's sliced ELL as it is (no CSR needed for the sweep),
+				// packed by one kernel into pinned host memory: one launch and
+				// one wait instead of three pageable copies
+				const int32_t *slice = nullptr, *rlen = nullptr, *col = nullptr;
+				BFMX_CHECK(topo_sell_to_host(st, G, &slice, &rlen, &col));
 				phase.mark("level graph to the host");
-				aggregate_chunks(SellRows{slice.data(), col.data(), rlen.data()}, fseg, chunk_rows(fseg, nf), hagg,
-				                 cseg);
+				aggregate_chunks(SellRows{slice, col, rlen}, fseg, chunk_rows(fseg, nf), hagg, cseg);
 			}
 			phase.mark("sweeps (host)");
 			na = cseg.back();

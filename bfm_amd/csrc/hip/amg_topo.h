@@ -49,6 +49,12 @@ struct TopoLevel {
 // Row ranges of the graph that no entry couples, as offsets 0, ..., n.
 int topo_segments(hipStream_t st, TopoGraph const& G, std::vector<int32_t>& seg);
 
+// The graph's sliced ELL on the host, packed by one kernel into the calling
+// thread's pinned staging buffer: *slice (n_slices + 1 entries), *len (n),
+// *col (nnzb_pad); valid until the thread's next use of that buffer.
+int topo_sell_to_host(hipStream_t st, TopoGraph const& G, const int32_t** slice, const int32_t** len,
+                      const int32_t** col);
+
 // Aggregates of the graph (MIS-2 roots with their neighbours; the other rows
 // join a neighbour's aggregate): *agg (device, n entries, allocated here and
 // owned by the caller, also on failure once set), their count, and cseg[c]

@@ -954,20 +954,38 @@ int scan(hipStream_t st, const T* in, U* out, int64_t n, Temps& tmp) {
 	return BFMX_OK;
 }
 
+// A count (and the overflow flags) read back in one round trip: a kernel
+// writes them into pinned host memory, then the stream is drained.
+__global__ void k_read_back(const int64_t* __restrict__ c64, const int32_t* __restrict__ c32,
+                            const int32_t* __restrict__ flags, int64_t* __restrict__ out) {
+	if (threadIdx.x == 0) {
+		out[0] = c64 ? c64[0] : c32[0];
+		out[1] = flags ? flags[0] : 0;
+	}
+}
+
 template <typename T>
-int read_one(hipStream_t st, const T* d, T* h) {
-	BFMX_HIP(hipMemcpyAsync(h, d, sizeof(T), hipMemcpyDeviceToHost, st));
+int read_count_flags(hipStream_t st, const T* d, T* h, const int32_t* dflags, int32_t* hflags) {
+	void* buf = nullptr;
+	BFMX_CHECK(pinned_staging(&buf, 2 * sizeof(int64_t), 1));
+	int64_t* const out = static_cast<int64_t*>(buf);
+	if constexpr (sizeof(T) == 8) {
+		k_read_back<<<1, 64, 0, st>>>(reinterpret_cast<const int64_t*>(d), nullptr, dflags, out);
+	} else {
+		k_read_back<<<1, 64, 0, st>>>(nullptr, reinterpret_cast<const int32_t*>(d), dflags, out);
+	}
+	BFMX_HIP(hipGetLastError());
 	BFMX_HIP(hipStreamSynchronize(st));
+	*h = static_cast<T>(out[0]);
+	if (hflags) {
+		*hflags = static_cast<int32_t>(out[1]);
+	}
 	return BFMX_OK;
 }
 
-// A count and the overflow flags in one round trip.
 template <typename T>
-int read_count_flags(hipStream_t st, const T* d, T* h, const int32_t* dflags, int32_t* hflags) {
-	BFMX_HIP(hipMemcpyAsync(h, d, sizeof(T), hipMemcpyDeviceToHost, st));
-	BFMX_HIP(hipMemcpyAsync(hflags, dflags, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-	BFMX_HIP(hipStreamSynchronize(st));
-	return BFMX_OK;
+int read_one(hipStream_t st, const T* d, T* h) {
+	return read_count_flags<T>(st, d, h, nullptr, nullptr);
 }
 
 // out[i] = in[i] (64-bit prefix sums known to fit in 32 bits).
@@ -986,7 +1004,37 @@ int zeros(hipStream_t st, T* p, size_t n) {
 	return BFMX_OK;
 }
 
+// Slice pointers, row lengths and block columns of a sliced ELL, one after
+// the other into out (a host buffer the device writes).
+__global__ __launch_bounds__(kT) void k_pack_sell(int64_t ns1, int64_t n, int64_t total,
+                                                  const int32_t* __restrict__ slice, const int32_t* __restrict__ len,
+                                                  const int32_t* __restrict__ col, int32_t* __restrict__ out) {
+	int64_t const t = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x;
+	if (t < ns1) {
+		out[t] = slice[t];
+	} else if (t < ns1 + n) {
+		out[t] = len[t - ns1];
+	} else if (t < total) {
+		out[t] = col[t - ns1 - n];
+	}
+}
+
 }  // namespace
+
+int topo_sell_to_host(hipStream_t st, TopoGraph const& G, const int32_t** slice, const int32_t** len,
+                      const int32_t** col) {
+	int64_t const ns1 = G.n_slices + 1, n = G.n, total = ns1 + n + G.nnzb_pad;
+	void* buf = nullptr;
+	BFMX_CHECK(pinned_staging(&buf, static_cast<size_t>(total) * sizeof(int32_t)));
+	int32_t* const out = static_cast<int32_t*>(buf);
+	k_pack_sell<<<nblk(total), kT, 0, st>>>(ns1, n, total, G.slice_ptr, G.row_len, G.col, out);
+	BFMX_HIP(hipGetLastError());
+	BFMX_HIP(hipStreamSynchronize(st));
+	*slice = out;
+	*len = out + ns1;
+	*col = out + ns1 + n;
+	return BFMX_OK;
+}
 
 int topo_segments(hipStream_t st, TopoGraph const& G, std::vector<int32_t>& seg) {
 	Temps tmp(st);

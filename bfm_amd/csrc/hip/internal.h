@@ -583,6 +583,11 @@ void dfree(T*& p) {
 int pool_alloc_bytes(void** p, size_t bytes);
 void pool_free_bytes(void* p);
 
+// A pinned (device-visible) host buffer of at least `bytes`, one per host
+// thread and slot (0 or 1), grown as needed and kept until the process ends;
+// valid until the thread's next call for the same slot.
+int pinned_staging(void** p, size_t bytes, int slot = 0);
+
 template <typename T>
 int palloc(T** p, size_t n) {
 	void* q = nullptr;

@@ -10,6 +10,7 @@
 // per device goes back to HIP; when hipMalloc runs out of memory, the
 // device's cached blocks are released and the allocation tried once more.
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -116,6 +117,33 @@ void pool_free_bytes(void* q) {
 	}
 	P.free[b.dev][b.cls].push_back(q);
 	P.cached[b.dev] += b.cls;
+}
+
+int pinned_staging(void** out, size_t bytes, int slot) {
+	struct Staging {
+		void* p = nullptr;
+		size_t cap = 0;
+	};
+	// not freed at thread exit: the HIP runtime may already be gone then
+	thread_local Staging* slots = new Staging[2];
+	Staging* const st = &slots[slot & 1];
+	if (st->cap < bytes) {
+		if (st->p) {
+			(void) hipHostFree(st->p);
+			st->p = nullptr;
+			st->cap = 0;
+		}
+		size_t const cap = std::max(bytes, size_t(4) << 10);
+		hipError_t const e = hipHostMalloc(&st->p, cap, hipHostMallocDefault);
+		if (e != hipSuccess) {
+			st->p = nullptr;
+			set_error("hipHostMalloc of %zu bytes failed: %s", cap, hipGetErrorString(e));
+			return BFMX_E_NOMEM;
+		}
+		st->cap = cap;
+	}
+	*out = st->p;
+	return BFMX_OK;
 }
 
 }  // namespace bfmx
