@@ -1986,7 +1986,10 @@ template <typename Rows>
 int aggregate(int r0, int r1, Rows const& G, int32_t* agg) {
 	std::fill(agg + r0, agg + r1, -1);
 	int na = 0;
-	std::vector<int32_t> rest;   // rows not yet aggregated when the sweep passed them
+	// rows not yet aggregated when the sweep passed them, and what they join:
+	// kept per thread, so later sweeps on the thread reuse their memory
+	thread_local std::vector<int32_t> rest, join;
+	rest.clear();
 	for (int i = r0; i < r1; i++) {
 		if (agg[i] >= 0) {
 			continue;
@@ -2006,7 +2009,7 @@ int aggregate(int r0, int r1, Rows const& G, int32_t* agg) {
 		}
 	}
 	// pass 2 reads pass 1's aggregates only: the joins are written afterwards
-	std::vector<int32_t> join(rest.size(), -1);
+	join.assign(rest.size(), -1);
 	for (size_t r = 0; r < rest.size(); r++) {
 		int const i = rest[r];
 		if (agg[i] >= 0) {
@@ -2088,10 +2091,10 @@ void parallel_chunks(int T, const F& f) {
 // depend on how they are grouped.
 template <typename Rows>
 std::vector<int32_t> aggregate_chunks(Rows const& G, const std::vector<int32_t>& seg, const std::vector<int32_t>& rb,
-                                      std::vector<int32_t>& agg, std::vector<int32_t>& cseg) {
+                                      int32_t* agg, std::vector<int32_t>& cseg) {
 	int const T = static_cast<int>(rb.size()) - 1;
 	std::vector<int32_t> ab(T + 1, 0);   // first aggregate of each chunk
-	parallel_chunks(T, [&](int c) { ab[c + 1] = aggregate(rb[c], rb[c + 1], G, agg.data()); });
+	parallel_chunks(T, [&](int c) { ab[c + 1] = aggregate(rb[c], rb[c + 1], G, agg); });
 	for (int c = 0; c < T; c++) {
 		ab[c + 1] += ab[c];
 	}
@@ -2397,7 +2400,7 @@ int host_levels_from(bfmx_system* s, bool smooth, bool finest, HostLevelGraph& H
 		int const T = static_cast<int>(rb.size()) - 1;
 		std::atomic<int> bad{0};
 		std::vector<int32_t> agg(nf), cseg;
-		std::vector<int32_t> const ab = aggregate_chunks(CsrRows{ptr, adj}, fseg, rb, agg, cseg);
+		std::vector<int32_t> const ab = aggregate_chunks(CsrRows{ptr, adj}, fseg, rb, agg.data(), cseg);
 		int const na = cseg.back();
 		if (!std::is_sorted(cseg.begin(), cseg.end()) || cseg[0] != 0) {
 			set_error("multigrid: aggregates do not follow the level's segments");
@@ -2720,7 +2723,10 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 			// which the FCG iterations show (21 against 26 to 35 for sweeps on
 			// chunks cut inside a segment or MIS-2 aggregates, on the 2M-DOF Q1
 			// beam)
-			std::vector<int32_t> hagg(nf);
+			// kept per thread between builds (no fresh pages to fault in)
+			thread_local std::vector<int32_t> hagg_buf;
+			hagg_buf.resize(std::max(hagg_buf.size(), static_cast<size_t>(nf)));
+			int32_t* const hagg = hagg_buf.data();
 			if (sys_graph) {
 				aggregate_chunks(CsrRows{s->adj_ptr.data(), s->adj.data()}, fseg, chunk_rows(fseg, nf), hagg, cseg);
 			} else {
@@ -2736,7 +2742,7 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 			na = cseg.back();
 			ra = palloc(&agg, nf);
 			if (ra == BFMX_OK) {
-				hipError_t e = hipMemcpyAsync(agg, hagg.data(), nf * sizeof(int32_t), hipMemcpyHostToDevice, st);
+				hipError_t e = hipMemcpyAsync(agg, hagg, nf * sizeof(int32_t), hipMemcpyHostToDevice, st);
 				if (e == hipSuccess) {
 					e = hipStreamSynchronize(st);
 				}

@@ -584,8 +584,10 @@ int pool_alloc_bytes(void** p, size_t bytes);
 void pool_free_bytes(void* p);
 
 // A pinned (device-visible) host buffer of at least `bytes`, one per host
-// thread and slot (0 or 1), grown as needed and kept until the process ends;
-// valid until the thread's next call for the same slot.
+// thread and slot (0 .. kStagingSlots - 1), grown as needed and kept until
+// the process ends; valid until the thread's next call for the same slot.
+// Slots of the multigrid build: 0 a level's graph, 1 counts read back.
+constexpr int kStagingSlots = 2;
 int pinned_staging(void** p, size_t bytes, int slot = 0);
 
 template <typename T>

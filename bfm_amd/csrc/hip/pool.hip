@@ -125,8 +125,12 @@ int pinned_staging(void** out, size_t bytes, int slot) {
 		size_t cap = 0;
 	};
 	// not freed at thread exit: the HIP runtime may already be gone then
-	thread_local Staging* slots = new Staging[2];
-	Staging* const st = &slots[slot & 1];
+	thread_local Staging* slots = new Staging[kStagingSlots];
+	if (slot < 0 || slot >= kStagingSlots) {
+		set_error("pinned staging slot %d out of range", slot);
+		return BFMX_E_ARG;
+	}
+	Staging* const st = &slots[slot];
 	if (st->cap < bytes) {
 		if (st->p) {
 			(void) hipHostFree(st->p);
