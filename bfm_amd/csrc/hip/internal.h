@@ -590,6 +590,11 @@ void pool_free_bytes(void* p);
 constexpr int kStagingSlots = 2;
 int pinned_staging(void** p, size_t bytes, int slot = 0);
 
+// Small pinned host blocks from a process-wide cache (pool.hip); a block goes
+// back with the size it was asked for, once no pending copy uses it.
+int pinned_small_alloc(void** p, size_t bytes);
+void pinned_small_free(void* p, size_t bytes);
+
 template <typename T>
 int palloc(T** p, size_t n) {
 	void* q = nullptr;

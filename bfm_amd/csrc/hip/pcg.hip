@@ -759,19 +759,21 @@ int ensure_workspace(bfmx_system* s) {
 	if (s->d_x) {
 		return BFMX_OK;
 	}
-	BFMX_CHECK(dalloc(&s->d_x, 2 * n));
-	BFMX_CHECK(dalloc(&s->d_r, 2 * n));
-	BFMX_CHECK(dalloc(&s->d_z, 2 * n));
-	BFMX_CHECK(dalloc(&s->d_q, 2 * n));
-	BFMX_CHECK(dalloc(&s->d_p[0], 2 * n));
-	BFMX_CHECK(dalloc(&s->d_p[1], 2 * n));
-	BFMX_CHECK(dalloc(&s->d_dinv, 4 * n));
+	// from the device block cache (a fresh system's first solve would
+	// otherwise spend ~0.4 ms in hipMalloc and hipHostMalloc here)
+	BFMX_CHECK(palloc(&s->d_x, 2 * n));
+	BFMX_CHECK(palloc(&s->d_r, 2 * n));
+	BFMX_CHECK(palloc(&s->d_z, 2 * n));
+	BFMX_CHECK(palloc(&s->d_q, 2 * n));
+	BFMX_CHECK(palloc(&s->d_p[0], 2 * n));
+	BFMX_CHECK(palloc(&s->d_p[1], 2 * n));
+	BFMX_CHECK(palloc(&s->d_dinv, 4 * n));
 	// per-workgroup partial sums: three per 256-row workgroup (the residual
 	// kernel), one per 64-row workgroup (the learned preconditioner's decoder)
 	s->partial_cap = std::max<int64_t>(3 * static_cast<int64_t>(grid(n)), (n + 63) / 64);
-	BFMX_CHECK(dalloc(&s->d_partial, static_cast<size_t>(s->partial_cap)));
-	BFMX_CHECK(dalloc(&s->d_scal, 1));
-	BFMX_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->h_scal), 4 * sizeof(Scalars), hipHostMallocDefault));
+	BFMX_CHECK(palloc(&s->d_partial, static_cast<size_t>(s->partial_cap)));
+	BFMX_CHECK(palloc(&s->d_scal, 1));
+	BFMX_CHECK(pinned_small_alloc(reinterpret_cast<void**>(&s->h_scal), 4 * sizeof(Scalars)));
 	for (auto& e : s->ev) {
 		BFMX_HIP(hipEventCreate(&e));
 	}
@@ -786,7 +788,7 @@ int ensure_workspace(bfmx_system* s) {
 // reads it, stays).
 int ensure_f32(bfmx_system* s) {
 	if (!s->d_val32) {
-		BFMX_CHECK(dalloc(&s->d_val32, 4 * s->nnzb_pad));
+		BFMX_CHECK(palloc(&s->d_val32, 4 * s->nnzb_pad));
 	} else if (s->f32_gen == s->val_gen) {
 		return BFMX_OK;   // made from the current values
 	}
@@ -1099,6 +1101,9 @@ void drop_graphs(bfmx_system* s, int precond) {
 }
 
 void release_solver(bfmx_system* s) {
+	if (s->stream) {
+		(void) hipStreamSynchronize(s->stream);   // the block cache takes back drained blocks only
+	}
 	amg_release(s);
 	dfree(s->d_fcg_seg);
 	dfree(s->d_fcg_split);
@@ -1110,20 +1115,18 @@ void release_solver(bfmx_system* s) {
 		(void) hipGraphExecDestroy(gr.exec);
 	}
 	s->graphs.clear();
-	dfree(s->d_x);
-	dfree(s->d_r);
-	dfree(s->d_z);
-	dfree(s->d_q);
-	dfree(s->d_p[0]);
-	dfree(s->d_p[1]);
-	dfree(s->d_dinv);
-	dfree(s->d_partial);
-	dfree(s->d_scal);
-	dfree(s->d_val32);
-	if (s->h_scal) {
-		(void) hipHostFree(s->h_scal);
-		s->h_scal = nullptr;
-	}
+	pfree(s->d_x);
+	pfree(s->d_r);
+	pfree(s->d_z);
+	pfree(s->d_q);
+	pfree(s->d_p[0]);
+	pfree(s->d_p[1]);
+	pfree(s->d_dinv);
+	pfree(s->d_partial);
+	pfree(s->d_scal);
+	pfree(s->d_val32);
+	pinned_small_free(s->h_scal, 4 * sizeof(Scalars));
+	s->h_scal = nullptr;
 	for (auto& e : s->ev) {
 		if (e) {
 			(void) hipEventDestroy(e);

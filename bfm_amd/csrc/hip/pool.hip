@@ -119,6 +119,50 @@ void pool_free_bytes(void* q) {
 	P.cached[b.dev] += b.cls;
 }
 
+// Small pinned host blocks (a system's scalar mirror) cached per size: a
+// hipHostMalloc costs tens of microseconds, a fresh system's first solve
+// needs one.
+namespace {
+struct PinnedCache {
+	std::mutex m;
+	std::map<size_t, std::vector<void*>> free;
+};
+PinnedCache& pinned_cache() {
+	static PinnedCache* c = new PinnedCache;   // never destroyed, as the pool
+	return *c;
+}
+}  // namespace
+
+int pinned_small_alloc(void** out, size_t bytes) {
+	*out = nullptr;
+	{
+		PinnedCache& C = pinned_cache();
+		std::lock_guard<std::mutex> lock(C.m);
+		auto it = C.free.find(bytes);
+		if (it != C.free.end() && !it->second.empty()) {
+			*out = it->second.back();
+			it->second.pop_back();
+			return BFMX_OK;
+		}
+	}
+	hipError_t const e = hipHostMalloc(out, bytes, hipHostMallocDefault);
+	if (e != hipSuccess) {
+		*out = nullptr;
+		set_error("hipHostMalloc of %zu bytes failed: %s", bytes, hipGetErrorString(e));
+		return BFMX_E_NOMEM;
+	}
+	return BFMX_OK;
+}
+
+void pinned_small_free(void* p, size_t bytes) {
+	if (!p) {
+		return;
+	}
+	PinnedCache& C = pinned_cache();
+	std::lock_guard<std::mutex> lock(C.m);
+	C.free[bytes].push_back(p);
+}
+
 int pinned_staging(void** out, size_t bytes, int slot) {
 	struct Staging {
 		void* p = nullptr;
