@@ -545,9 +545,10 @@ int orc_band_from_dense(size_t m, size_t k, const double* A, double* band) {
 	return 0;
 }
 
-/* Unpivoted band LU in place (matrix.c:253-302). */
-int orc_band_lu(size_t m, size_t k, double* band) {
-	for (size_t p = 0; p + 1 < m; p++) {
+/* The first `steps` pivot steps of the unpivoted band LU in place
+ * (matrix.c:253-302); all of them when steps >= m - 1. */
+int orc_band_lu_steps(size_t m, size_t k, double* band, size_t steps) {
+	for (size_t p = 0; p + 1 < m && p < steps; p++) {
 		double const pivot = *orc_band_at(band, k, p, p);
 		if (pivot != pivot || fabs(pivot) < 1e-20) {
 			return -1;
@@ -563,6 +564,11 @@ int orc_band_lu(size_t m, size_t k, double* band) {
 		}
 	}
 	return 0;
+}
+
+/* Unpivoted band LU in place (matrix.c:253-302). */
+int orc_band_lu(size_t m, size_t k, double* band) {
+	return orc_band_lu_steps(m, k, band, m);
 }
 
 /* Forward (unit L) and backward (U) substitution in place (matrix.c:351-404). */
@@ -925,4 +931,121 @@ int64_t orc_csr_pcg(size_t m, const int64_t* rowptr, const int32_t* col, const d
 done:
 	free(r), free(z), free(p), free(q), free(dinv);
 	return rv < 0 ? rv : it;
+}
+
+
+/* ------------------------------------------------------- sparse band path
+ * libbfm's direct path (RCM, band LU, substitution) with the matrix kept
+ * sparse until it is laid out as a band: the same algorithm at sizes whose
+ * dense m x m matrix cannot be held. Used as the direct CPU baseline at
+ * scale; the LU is timed over a window of its pivot steps (the band LU does
+ * the same arithmetic at every interior step, whatever the values). */
+
+/* orc_rcm over a CSR matrix (perm.c:117-331 rules): degree = entries whose
+ * value is non-zero (stored zeros, as left by Dirichlet elimination, do not
+ * count), start = the last unvisited node of minimum degree, neighbours in
+ * index order, enqueued sorted by degree, order reversed. perm[old] = new. */
+int orc_rcm_csr(size_t n, const int64_t* rowptr, const int32_t* col, const double* val, size_t* perm,
+                size_t* inv_perm) {
+	size_t* degs = calloc(n, sizeof *degs);
+	size_t* queue = malloc(n * sizeof *queue);
+	orc_rcm_node_t* to_sort = NULL;
+	uint8_t* visited = calloc(n, 1);
+	size_t max_row = 0;
+	for (size_t i = 0; i < n; i++) {
+		size_t const len = (size_t) (rowptr[i + 1] - rowptr[i]);
+		max_row = len > max_row ? len : max_row;
+	}
+	to_sort = malloc((max_row + 1) * sizeof *to_sort);
+	if (!degs || !queue || !to_sort || !visited) {
+		free(degs), free(queue), free(to_sort), free(visited);
+		return -1;
+	}
+	for (size_t i = 0; i < n; i++) {
+		for (int64_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+			degs[i] += val[e] != 0;
+		}
+	}
+	size_t qs = 0, qe = 0, p = 0, unvisited = n;
+	while (unvisited && p < n) {
+		size_t min_deg = (size_t) -1, start = 0;
+		for (size_t i = 0; i < n; i++) {
+			if (!visited[i] && degs[i] <= min_deg) {
+				min_deg = degs[i];
+				start = i;
+			}
+		}
+		queue[qe++ % n] = start;
+		while (qs != qe) {
+			size_t const cur = queue[qs++ % n];
+			if (!visited[cur]) {
+				visited[cur] = 1;
+				unvisited--;
+			}
+			size_t cnt = 0;
+			for (int64_t e = rowptr[cur]; e < rowptr[cur + 1]; e++) {
+				size_t const i = (size_t) col[e];
+				if (visited[i] || val[e] == 0) {
+					continue;
+				}
+				to_sort[cnt].i = i;
+				to_sort[cnt].deg = degs[i];
+				cnt++;
+				visited[i] = 1;
+			}
+			orc_sort_deg(to_sort, cnt);
+			for (size_t i = 0; i < cnt; i++) {
+				queue[qe++ % n] = to_sort[i].i;
+			}
+			inv_perm[n - p++ - 1] = cur;
+		}
+	}
+	for (size_t i = 0; i < n; i++) {
+		perm[inv_perm[i]] = i;
+	}
+	free(degs), free(queue), free(to_sort), free(visited);
+	return 0;
+}
+
+/* matrix_full_bandwidth (matrix.c:57-71) of P A P^T over the non-zero values. */
+size_t orc_bandwidth_csr(size_t n, const int64_t* rowptr, const int32_t* col, const double* val, const size_t* perm) {
+	size_t k = 0;
+	for (size_t i = 0; i < n; i++) {
+		for (int64_t e = rowptr[i]; e < rowptr[i + 1]; e++) {
+			if (val[e] != 0) {
+				size_t const a = perm[i], b = perm[col[e]];
+				size_t const d = a > b ? a - b : b - a;
+				k = d > k ? d : k;
+			}
+		}
+	}
+	return k;
+}
+
+/* Rows r0 .. r0 + nrows - 1 of P A P^T in libbfm's band storage of
+ * half-bandwidth k, row and column indices counted from r0 (entries whose
+ * column falls outside the window are left out); band has nrows (2k+1)
+ * entries. -1 when a non-zero lies outside the band. */
+int orc_band_window_csr(size_t n, const int64_t* rowptr, const int32_t* col, const double* val, const size_t* perm,
+                        const size_t* inv_perm, size_t k, size_t r0, size_t nrows, double* band) {
+	memset(band, 0, nrows * (2 * k + 1) * sizeof *band);
+	for (size_t li = 0; li < nrows && r0 + li < n; li++) {
+		size_t const o = inv_perm[r0 + li];
+		for (int64_t e = rowptr[o]; e < rowptr[o + 1]; e++) {
+			size_t const j = perm[col[e]];
+			size_t const i = r0 + li;
+			size_t const d = i > j ? i - j : j - i;
+			if (d > k) {
+				if (val[e] != 0) {
+					return -1;
+				}
+				continue;
+			}
+			if (j < r0 || j >= r0 + nrows) {
+				continue;
+			}
+			*orc_band_at(band, k, li, j - r0) = val[e];
+		}
+	}
+	return 0;
 }

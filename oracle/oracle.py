@@ -77,6 +77,14 @@ def lib():
         L.orc_csr_pcg.restype = C.c_int64
         L.orc_csr_pcg.argtypes = [sz, _i64, _i32, _dp, _dp, _dp, C.c_double, C.c_int64,
                                   C.POINTER(C.c_double)]
+        L.orc_rcm_csr.restype = C.c_int
+        L.orc_rcm_csr.argtypes = [sz, _i64, _i32, _dp, _up, _up]
+        L.orc_bandwidth_csr.restype = sz
+        L.orc_bandwidth_csr.argtypes = [sz, _i64, _i32, _dp, _up]
+        L.orc_band_window_csr.restype = C.c_int
+        L.orc_band_window_csr.argtypes = [sz, _i64, _i32, _dp, _up, _up, sz, sz, sz, _dp]
+        L.orc_band_lu_steps.restype = C.c_int
+        L.orc_band_lu_steps.argtypes = [sz, sz, _dp, sz]
         L.orc_num_threads.restype = C.c_int
         L.orc_set_num_threads.argtypes = [C.c_int]
         _lib = L
@@ -226,3 +234,30 @@ def rcm(A_dense):
     inv = np.zeros(m, dtype=np.uint64)
     assert lib().orc_rcm(m, np.ascontiguousarray(A_dense.reshape(-1), np.float64), perm, inv) == 0
     return perm, inv
+
+
+def rcm_csr(A):
+    """RCM of a CSR matrix by the reference's rules (orc_rcm_csr): perm, inv."""
+    perm = np.zeros(A.m, dtype=np.uint64)
+    inv = np.zeros(A.m, dtype=np.uint64)
+    assert lib().orc_rcm_csr(A.m, A.rowptr, A.col, A.val, perm, inv) == 0
+    return perm, inv
+
+
+def bandwidth_csr(A, perm):
+    return int(lib().orc_bandwidth_csr(A.m, A.rowptr, A.col, A.val, perm))
+
+
+def band_window(A, perm, inv, k, r0, nrows):
+    """Rows r0 .. r0 + nrows - 1 of P A P^T in band storage (window-local)."""
+    band = np.zeros(nrows * (2 * k + 1))
+    assert lib().orc_band_window_csr(A.m, A.rowptr, A.col, A.val, perm, inv, k, r0, nrows, band) == 0
+    return band
+
+
+def band_lu_steps(band, m, k, steps):
+    return int(lib().orc_band_lu_steps(m, k, band, steps))
+
+
+def band_lu_solve(band, m, k, y):
+    return int(lib().orc_band_lu_solve(m, k, band, y))

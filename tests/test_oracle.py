@@ -321,6 +321,39 @@ def test_csr_pcg_matches_direct():
     assert np.linalg.norm(r) / np.linalg.norm(b) <= 1e-10
 
 
+@pytest.mark.parametrize("kind,nx,ny", [(3, 40, 10), (4, 24, 6)])
+def test_sparse_band_path_equals_the_dense_path(kind, nx, ny):
+    """libbfm's direct path with the matrix kept sparse (RCM over the CSR
+    pattern, band storage built from it, band LU and substitution) is the
+    dense path bit for bit: the same permutation, half-bandwidth and u as
+    run_elasticity restated (itself bit-identical to the compiled reference
+    on the 40 x 10 P1 beam). A window of band rows away from the top holds
+    exactly those rows' band entries whose columns fall in the window."""
+    mesh = meshgen.beam(nx, ny, kind=kind, jitter=0.2 if kind == 4 else 0.0, seed=8)
+    kinds, values, flags = meshgen.beam_conditions(mesh)
+    u_d, _, _, perm_d, k_d, rc = O.solve_dense(mesh, True, MAT["E"], MAT["nu"], MAT["rho"], kinds, values, flags)
+    assert rc == 0
+    A, b = O.assemble_csr(mesh, 1, MAT["E"], MAT["nu"], MAT["rho"], kinds=kinds, values=values, flags=flags)
+    perm, inv = O.rcm_csr(A)
+    assert np.array_equal(perm, perm_d)
+    k = O.bandwidth_csr(A, perm)
+    assert k == k_d
+    m = A.m
+    band = O.band_window(A, perm, inv, k, 0, m)
+    assert O.band_lu_steps(band, m, k, m) == 0
+    y = np.zeros(m)
+    y[perm.astype(np.int64)] = b
+    assert O.band_lu_solve(band, m, k, y) == 0
+    u = y[perm.astype(np.int64)]
+    assert np.array_equal(u, u_d)
+    r0, nrows = m // 3, min(m - m // 3, 2 * k + 5)
+    win = O.band_window(A, perm, inv, k, r0, nrows)
+    Pd = A.to_scipy().toarray()[np.ix_(inv.astype(np.int64), inv.astype(np.int64))]
+    for li in range(nrows):
+        for lj in range(max(0, li - k), min(nrows, li + k + 1)):
+            assert win[lj + li * 2 * k] == Pd[r0 + li, r0 + lj]
+
+
 @pytest.mark.parametrize("kind", [3, 4])
 def test_per_element_E_is_linear_sum_of_unit_elements(kind):
     """The oracle's per-element-E branch (elem_E[e] in place of the material
