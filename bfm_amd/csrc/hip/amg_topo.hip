@@ -50,6 +50,14 @@ constexpr int kHashStride = kHash + 1;
 constexpr int kMisMaxRounds = 200;
 constexpr int kMisGroup = 4;     // rounds launched between two checks for undecided rows
 constexpr int kStash = 32;       // entries per row a count pass keeps for its fill pass
+// Per-thread LDS tables sized for the common rows: prolongator rows of at
+// most kSmallRow neighbours and rows of A P of at most kStash entries go
+// through launches whose small tables let many wavefronts share a CU (the
+// full-size tables, kMaxRow + 1 or kMaxAP + 1 entries per thread, allow four
+// per CU); the other rows are listed and taken by kBigBlocks workgroups with
+// the full tables.
+constexpr int kSmallRow = 16;
+constexpr int kBigBlocks = 128;
 constexpr int kScanItems = 8;    // items per thread of a scan block
 constexpr int kScanBlock = kT * kScanItems;
 
@@ -421,42 +429,66 @@ __device__ __forceinline__ int row_aggs(int i, const int32_t* __restrict__ slice
 	return cnt;
 }
 
-__global__ __launch_bounds__(kTW) void k_p_count(int n, const int32_t* __restrict__ slice_ptr,
-                                                 const int32_t* __restrict__ col, const int32_t* __restrict__ row_len,
-                                                 const int32_t* __restrict__ agg, int smooth,
-                                                 int32_t* __restrict__ pcnt, int32_t* __restrict__ bad) {
-	__shared__ int32_t lds[kTW * (kMaxRow + 1)];
-	int const i = blockIdx.x * kTW + threadIdx.x;
-	if (i >= n) {
-		return;
-	}
-	if (row_len[i] > kMaxRow) {
-		atomicOr(bad, 1);
-		pcnt[i] = 0;
-		return;
-	}
-	int const c = row_aggs(i, slice_ptr, col, row_len, agg, smooth != 0, lds + threadIdx.x * (kMaxRow + 1));
+// Row i's count of prolongator entries (u: a table of CAP + 1 entries).
+__device__ __forceinline__ void p_count_row(int i, const int32_t* __restrict__ slice_ptr,
+                                            const int32_t* __restrict__ col, const int32_t* __restrict__ row_len,
+                                            const int32_t* __restrict__ agg, bool smooth, int32_t* u,
+                                            int32_t* __restrict__ pcnt, int32_t* __restrict__ bad) {
+	int const c = row_aggs(i, slice_ptr, col, row_len, agg, smooth, u);
 	if (c > 255) {
 		atomicOr(bad, 2);
 	}
 	pcnt[i] = c;
 }
 
-// P's columns of row i, and for every block slot of the row (padding
-// included) the entry of its column's aggregate within the row (padding
-// slots and the unsmoothed case: the row's own aggregate).
-__global__ __launch_bounds__(kTW) void k_p_fill(int n, const int32_t* __restrict__ slice_ptr,
-                                                const int32_t* __restrict__ col, const int32_t* __restrict__ row_len,
-                                                const int32_t* __restrict__ agg, int smooth,
-                                                const int32_t* __restrict__ pptr, int32_t* __restrict__ pcol,
-                                                uint8_t* __restrict__ ppos) {
-	__shared__ int32_t lds[kTW * (kMaxRow + 1)];
+// Rows of at most kSmallRow neighbours; the others are listed in big
+// (*nbig of them) for k_p_count_big.
+__global__ __launch_bounds__(kTW) void k_p_count(int n, const int32_t* __restrict__ slice_ptr,
+                                                 const int32_t* __restrict__ col, const int32_t* __restrict__ row_len,
+                                                 const int32_t* __restrict__ agg, int smooth,
+                                                 int32_t* __restrict__ pcnt, int32_t* __restrict__ bad,
+                                                 int32_t* __restrict__ big, int32_t* __restrict__ nbig) {
+	__shared__ int32_t lds[kTW * (kSmallRow + 1)];
 	int const i = blockIdx.x * kTW + threadIdx.x;
 	if (i >= n) {
 		return;
 	}
-	int32_t* u = lds + threadIdx.x * (kMaxRow + 1);
-	int const c = row_aggs(i, slice_ptr, col, row_len, agg, smooth != 0, u);
+	if (row_len[i] > kSmallRow) {
+		big[atomicAdd(nbig, 1)] = i;
+		return;
+	}
+	p_count_row(i, slice_ptr, col, row_len, agg, smooth != 0, lds + threadIdx.x * (kSmallRow + 1), pcnt, bad);
+}
+
+// The listed rows with full-size tables (a row above kMaxRow is flagged).
+__global__ __launch_bounds__(kTW) void k_p_count_big(const int32_t* __restrict__ slice_ptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const int32_t* __restrict__ row_len,
+                                                     const int32_t* __restrict__ agg, int smooth,
+                                                     int32_t* __restrict__ pcnt, int32_t* __restrict__ bad,
+                                                     const int32_t* __restrict__ big, const int32_t* __restrict__ nbig) {
+	__shared__ int32_t lds[kTW * (kMaxRow + 1)];
+	int const m = *nbig;
+	for (int t = blockIdx.x * kTW + threadIdx.x; t < m; t += gridDim.x * kTW) {
+		int const i = big[t];
+		if (row_len[i] > kMaxRow) {
+			atomicOr(bad, 1);
+			pcnt[i] = 0;
+			continue;
+		}
+		p_count_row(i, slice_ptr, col, row_len, agg, smooth != 0, lds + threadIdx.x * (kMaxRow + 1), pcnt, bad);
+	}
+}
+
+// P's columns of row i, and for every block slot of the row (padding
+// included) the entry of its column's aggregate within the row (padding
+// slots and the unsmoothed case: the row's own aggregate).
+__device__ __forceinline__ void p_fill_row(int i, const int32_t* __restrict__ slice_ptr,
+                                           const int32_t* __restrict__ col, const int32_t* __restrict__ row_len,
+                                           const int32_t* __restrict__ agg, bool smooth, int32_t* u,
+                                           const int32_t* __restrict__ pptr, int32_t* __restrict__ pcol,
+                                           uint8_t* __restrict__ ppos) {
+	int const c = row_aggs(i, slice_ptr, col, row_len, agg, smooth, u);
 	int32_t const p0 = pptr[i];
 	for (int k = 0; k < c; k++) {
 		pcol[p0 + k] = u[k];
@@ -480,6 +512,39 @@ __global__ __launch_bounds__(kTW) void k_p_fill(int n, const int32_t* __restrict
 	for (int k = 0; k < width; k++) {
 		int32_t const sl = nbr_slot(slice_ptr, i, k);
 		ppos[sl] = smooth && k < len ? static_cast<uint8_t>(pos(agg[col[sl]])) : own;
+	}
+}
+
+// Rows of at most kSmallRow neighbours (the others: k_p_fill_big over the
+// count pass's list).
+__global__ __launch_bounds__(kTW) void k_p_fill(int n, const int32_t* __restrict__ slice_ptr,
+                                                const int32_t* __restrict__ col, const int32_t* __restrict__ row_len,
+                                                const int32_t* __restrict__ agg, int smooth,
+                                                const int32_t* __restrict__ pptr, int32_t* __restrict__ pcol,
+                                                uint8_t* __restrict__ ppos) {
+	__shared__ int32_t lds[kTW * (kSmallRow + 1)];
+	int const i = blockIdx.x * kTW + threadIdx.x;
+	if (i >= n || row_len[i] > kSmallRow) {
+		return;
+	}
+	p_fill_row(i, slice_ptr, col, row_len, agg, smooth != 0, lds + threadIdx.x * (kSmallRow + 1), pptr, pcol, ppos);
+}
+
+__global__ __launch_bounds__(kTW) void k_p_fill_big(const int32_t* __restrict__ slice_ptr,
+                                                    const int32_t* __restrict__ col,
+                                                    const int32_t* __restrict__ row_len,
+                                                    const int32_t* __restrict__ agg, int smooth,
+                                                    const int32_t* __restrict__ pptr, int32_t* __restrict__ pcol,
+                                                    uint8_t* __restrict__ ppos, const int32_t* __restrict__ big,
+                                                    const int32_t* __restrict__ nbig) {
+	__shared__ int32_t lds[kTW * (kMaxRow + 1)];
+	int const m = *nbig;
+	for (int t = blockIdx.x * kTW + threadIdx.x; t < m; t += gridDim.x * kTW) {
+		int const i = big[t];
+		if (row_len[i] <= kMaxRow) {
+			p_fill_row(i, slice_ptr, col, row_len, agg, smooth != 0, lds + threadIdx.x * (kMaxRow + 1), pptr, pcol,
+			           ppos);
+		}
 	}
 }
 
@@ -580,6 +645,15 @@ struct WaveSet {
 	__shared__ int32_t name##_c[2];             \
 	WaveSet name{name##_h, name##_c, name##_c + 1}
 
+// The largest v over the wavefront's lanes.
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+	for (int off = 32; off > 0; off >>= 1) {
+		v = max(v, __shfl_xor(v, off, 64));
+	}
+	return v;
+}
+
 // Position of J in the sorted list a[0 .. len) (J must be present).
 __device__ __forceinline__ int find_sorted(const int32_t* a, int len, int32_t J) {
 	int lo = 0, hi = len - 1;
@@ -615,15 +689,24 @@ __global__ __launch_bounds__(64) void k_r_lists(int na, int smooth, const int32_
 	__shared__ int32_t rows[kSetCap];
 	int const J = blockIdx.x;
 	set.clear();
-	for (int32_t t = agg_ptr[J] + threadIdx.x; t < agg_ptr[J + 1]; t += 64) {
-		int const m = agg_nodes[t];
-		if (!smooth) {
-			set.insert(m);
-			continue;
+	int32_t const a0 = agg_ptr[J];
+	int const nm = agg_ptr[J + 1] - a0;
+	if (!smooth) {
+		for (int t = threadIdx.x; t < nm; t += 64) {
+			set.insert(agg_nodes[a0 + t]);
 		}
-		int const len = row_len[m];
-		for (int k = 0; k < len; k++) {
-			if (!set.insert(col[nbr_slot(slice_ptr, m, k)])) {
+	} else {
+		// the (member, neighbour slot) pairs spread over the lanes, w slots
+		// per member (w: the widest member row)
+		int w = 0;
+		for (int t = threadIdx.x; t < nm; t += 64) {
+			w = max(w, row_len[agg_nodes[a0 + t]]);
+		}
+		w = wave_max(w);
+		for (int p = threadIdx.x; p < nm * w; p += 64) {
+			int const mi = p / w, k = p - mi * w;
+			int const m = agg_nodes[a0 + mi];
+			if (k < row_len[m] && !set.insert(col[nbr_slot(slice_ptr, m, k)])) {
 				break;
 			}
 		}
@@ -662,34 +745,13 @@ __global__ __launch_bounds__(64) void k_r_lists(int na, int smooth, const int32_
 
 // ------------------------------------------------------------------ A P pattern
 
-// Row i of A P: the aggregates in the prolongator rows of i's neighbours,
-// sorted and distinct (per-thread list in LDS). COUNT: their number, and a
-// row of at most kStash entries into stash (kStash per row); else written
-// from apptr[i] (copied from stash when it holds the row).
-template <bool COUNT>
-__global__ __launch_bounds__(kTW) void k_ap_pattern(int n, const int32_t* __restrict__ slice_ptr,
-                                                    const int32_t* __restrict__ col,
-                                                    const int32_t* __restrict__ row_len,
-                                                    const int32_t* __restrict__ pptr,
-                                                    const int32_t* __restrict__ pcol, int32_t* __restrict__ cnt,
-                                                    const int32_t* __restrict__ apptr, int32_t* __restrict__ apcol,
-                                                    int32_t* __restrict__ stash, int32_t* __restrict__ bad) {
-	__shared__ int32_t lds[kTW * (kMaxAP + 1)];
-	int const i = blockIdx.x * kTW + threadIdx.x;
-	if (i >= n) {
-		return;
-	}
-	if (!COUNT && stash && apptr[i + 1] - apptr[i] <= kStash) {
-		// the count pass kept the row: copy it
-		int const c = apptr[i + 1] - apptr[i];
-		const int32_t* src = stash + static_cast<int64_t>(i) * kStash;
-		int32_t* out = apcol + apptr[i];
-		for (int k = 0; k < c; k++) {
-			out[k] = src[k];
-		}
-		return;
-	}
-	int32_t* u = lds + threadIdx.x * (kMaxAP + 1);
+// Row i of A P into u (a table of CAP + 1 entries): the aggregates in the
+// prolongator rows of i's neighbours, sorted and distinct; returns their
+// count, or -1 when there are more than CAP.
+template <int CAP>
+__device__ __forceinline__ int ap_row(int i, const int32_t* __restrict__ slice_ptr, const int32_t* __restrict__ col,
+                                      const int32_t* __restrict__ row_len, const int32_t* __restrict__ pptr,
+                                      const int32_t* __restrict__ pcol, int32_t* u) {
 	int const len = row_len[i];
 	int c = 0;
 	for (int k = 0; k < len; k++) {
@@ -703,12 +765,8 @@ __global__ __launch_bounds__(kTW) void k_ap_pattern(int n, const int32_t* __rest
 			if (y >= 0 && u[y] == a) {
 				continue;
 			}
-			if (c == kMaxAP) {
-				atomicOr(bad, 16);
-				if (COUNT) {
-					cnt[i] = 0;
-				}
-				return;
+			if (c == CAP) {
+				return -1;
 			}
 			for (int z = c; z > y + 1; z--) {
 				u[z] = u[z - 1];
@@ -717,19 +775,96 @@ __global__ __launch_bounds__(kTW) void k_ap_pattern(int n, const int32_t* __rest
 			c++;
 		}
 	}
-	if (COUNT) {
-		cnt[i] = c;
-		if (stash && c <= kStash) {
-			int32_t* dst = stash + static_cast<int64_t>(i) * kStash;
-			for (int k = 0; k < c; k++) {
-				dst[k] = u[k];
-			}
-		}
+	return c;
+}
+
+// Count pass: a row of at most kStash entries has its count in cnt[i] and
+// its sorted entries in stash (kStash per row); a longer one is listed in
+// big (*nbig of them) for k_ap_count_big.
+__global__ __launch_bounds__(kTW) void k_ap_count(int n, const int32_t* __restrict__ slice_ptr,
+                                                  const int32_t* __restrict__ col,
+                                                  const int32_t* __restrict__ row_len,
+                                                  const int32_t* __restrict__ pptr,
+                                                  const int32_t* __restrict__ pcol, int32_t* __restrict__ cnt,
+                                                  int32_t* __restrict__ stash, int32_t* __restrict__ big,
+                                                  int32_t* __restrict__ nbig) {
+	__shared__ int32_t lds[kTW * (kStash + 1)];
+	int const i = blockIdx.x * kTW + threadIdx.x;
+	if (i >= n) {
 		return;
 	}
-	int32_t* out = apcol + apptr[i];
+	int32_t* u = lds + threadIdx.x * (kStash + 1);
+	int const c = ap_row<kStash>(i, slice_ptr, col, row_len, pptr, pcol, u);
+	if (c < 0) {
+		big[atomicAdd(nbig, 1)] = i;
+		return;
+	}
+	cnt[i] = c;
+	int32_t* dst = stash + static_cast<int64_t>(i) * kStash;
 	for (int k = 0; k < c; k++) {
-		out[k] = u[k];
+		dst[k] = u[k];
+	}
+}
+
+// The listed rows with full-size tables (a row above kMaxAP is flagged).
+__global__ __launch_bounds__(kTW) void k_ap_count_big(const int32_t* __restrict__ slice_ptr,
+                                                      const int32_t* __restrict__ col,
+                                                      const int32_t* __restrict__ row_len,
+                                                      const int32_t* __restrict__ pptr,
+                                                      const int32_t* __restrict__ pcol, int32_t* __restrict__ cnt,
+                                                      const int32_t* __restrict__ big,
+                                                      const int32_t* __restrict__ nbig, int32_t* __restrict__ bad) {
+	__shared__ int32_t lds[kTW * (kMaxAP + 1)];
+	int const m = *nbig;
+	for (int t = blockIdx.x * kTW + threadIdx.x; t < m; t += gridDim.x * kTW) {
+		int const i = big[t];
+		int const c = ap_row<kMaxAP>(i, slice_ptr, col, row_len, pptr, pcol, lds + threadIdx.x * (kMaxAP + 1));
+		if (c < 0) {
+			atomicOr(bad, 16);
+		}
+		cnt[i] = c < 0 ? 0 : c;
+	}
+}
+
+// Fill pass: the rows the count pass kept, copied from stash to apcol from
+// apptr[i], kFillLanes lanes a row (consecutive lanes, consecutive entries).
+constexpr int kFillLanes = 8;
+
+__global__ __launch_bounds__(kT) void k_ap_fill(int n, const int32_t* __restrict__ apptr,
+                                                const int32_t* __restrict__ stash, int32_t* __restrict__ apcol) {
+	int64_t const t = static_cast<int64_t>(blockIdx.x) * kT + threadIdx.x;
+	int64_t const i = t / kFillLanes;
+	if (i >= n) {
+		return;
+	}
+	int32_t const a0 = apptr[i], c = apptr[i + 1] - a0;
+	if (c > kStash) {
+		return;   // a listed row: k_ap_fill_big
+	}
+	const int32_t* src = stash + i * kStash;
+	for (int k = static_cast<int>(t % kFillLanes); k < c; k += kFillLanes) {
+		apcol[a0 + k] = src[k];
+	}
+}
+
+// The listed rows, formed again with full-size tables.
+__global__ __launch_bounds__(kTW) void k_ap_fill_big(const int32_t* __restrict__ slice_ptr,
+                                                     const int32_t* __restrict__ col,
+                                                     const int32_t* __restrict__ row_len,
+                                                     const int32_t* __restrict__ pptr,
+                                                     const int32_t* __restrict__ pcol,
+                                                     const int32_t* __restrict__ apptr, int32_t* __restrict__ apcol,
+                                                     const int32_t* __restrict__ big, const int32_t* __restrict__ nbig) {
+	__shared__ int32_t lds[kTW * (kMaxAP + 1)];
+	int const m = *nbig;
+	for (int t = blockIdx.x * kTW + threadIdx.x; t < m; t += gridDim.x * kTW) {
+		int const i = big[t];
+		int32_t* u = lds + threadIdx.x * (kMaxAP + 1);
+		int const c = ap_row<kMaxAP>(i, slice_ptr, col, row_len, pptr, pcol, u);
+		int32_t* out = apcol + apptr[i];
+		for (int k = 0; k < c; k++) {
+			out[k] = u[k];
+		}
 	}
 }
 
@@ -758,12 +893,22 @@ __global__ __launch_bounds__(64) void k_c_pattern(int na, const int32_t* __restr
 	WAVE_SET(set);
 	__shared__ int32_t scratch[kSetCap + 64];
 	set.clear();
-	for (int32_t t = rptr[I] + threadIdx.x; t < rptr[I + 1]; t += 64) {
-		int const i = rrow[t];
-		for (int32_t e = apptr[i]; e < apptr[i + 1]; e++) {
-			if (!set.insert(apcol[e])) {
-				break;
-			}
+	// the (row of P's column, entry of its A P row) pairs spread over the
+	// lanes, w slots per row (w: the longest of those A P rows)
+	int32_t const r0 = rptr[I];
+	int const nr = rptr[I + 1] - r0;
+	int w = 0;
+	for (int t = threadIdx.x; t < nr; t += 64) {
+		int const i = rrow[r0 + t];
+		w = max(w, apptr[i + 1] - apptr[i]);
+	}
+	w = wave_max(w);
+	for (int p = threadIdx.x; p < nr * w; p += 64) {
+		int const ri = p / w, k = p - ri * w;
+		int const i = rrow[r0 + ri];
+		int32_t const e = apptr[i] + k;
+		if (e < apptr[i + 1] && !set.insert(apcol[e])) {
+			break;
 		}
 	}
 	__syncthreads();
@@ -1153,8 +1298,10 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	int32_t *cnt, *fill, *flags;
 	BFMX_CHECK(tmp.get(&cnt, std::max(n, na) + 1));
 	BFMX_CHECK(tmp.get(&fill, std::max(n, na) + 1));
-	BFMX_CHECK(tmp.get(&flags, 1));
-	BFMX_CHECK(zeros(st, flags, 1));
+	BFMX_CHECK(tmp.get(&flags, 3));   // overflow bits, then the counts of the two row lists
+	BFMX_CHECK(zeros(st, flags, 3));
+	int32_t* big;   // rows for the full-table kernels
+	BFMX_CHECK(tmp.get(&big, n));
 	// aggregate members, sorted
 	BFMX_CHECK(palloc(&T.agg_ptr, na + 1));
 	BFMX_CHECK(palloc(&T.agg_nodes, n));
@@ -1167,7 +1314,10 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	phase.mark("aggregate members");
 	// prolongator pattern
 	BFMX_CHECK(palloc(&T.pptr, n + 1));
-	k_p_count<<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, agg, smooth ? 1 : 0, cnt, flags);
+	k_p_count<<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, agg, smooth ? 1 : 0, cnt, flags, big,
+	                                        flags + 1);
+	k_p_count_big<<<kBigBlocks, kTW, 0, st>>>(G.slice_ptr, G.col, G.row_len, agg, smooth ? 1 : 0, cnt, flags, big,
+	                                          flags + 1);
 	BFMX_CHECK(scan(st, cnt, T.pptr, n, tmp));
 	int32_t nnzp = 0, bad = 0;
 	BFMX_CHECK(read_count_flags(st, T.pptr + n, &nnzp, flags, &bad));
@@ -1187,6 +1337,8 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	BFMX_CHECK(zeros(st, T.ppos, static_cast<size_t>(G.nnzb_pad)));   // the padding rows' slots
 	k_p_fill<<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, agg, smooth ? 1 : 0, T.pptr, T.pcol,
 	                                       T.ppos);
+	k_p_fill_big<<<kBigBlocks, kTW, 0, st>>>(G.slice_ptr, G.col, G.row_len, agg, smooth ? 1 : 0, T.pptr, T.pcol,
+	                                         T.ppos, big, flags + 1);
 	phase.mark("prolongator pattern");
 	// restriction lists: P's columns, rows sorted; a column's length is the
 	// count of its aggregate among P's entries (the lists are checked against
@@ -1205,8 +1357,10 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	BFMX_CHECK(palloc(&T.apptr, n + 1));
 	int32_t* stash;   // each row's sorted list, when short, from the count pass to the fill pass
 	BFMX_CHECK(tmp.get(&stash, static_cast<size_t>(n) * kStash));
-	k_ap_pattern<true><<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, cnt, nullptr,
-	                                                 nullptr, stash, flags);
+	k_ap_count<<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, cnt, stash, big,
+	                                         flags + 2);
+	k_ap_count_big<<<kBigBlocks, kTW, 0, st>>>(G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, cnt, big, flags + 2,
+	                                           flags);
 	int32_t nap = 0;
 	{
 		// the prefix sums in 64 bits: the 32-bit row pointers must hold the total
@@ -1236,8 +1390,9 @@ int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na
 	}
 	T.nnz_ap = nap;
 	BFMX_CHECK(palloc(&T.apcol, nap));
-	k_ap_pattern<false><<<nblk(n, kTW), kTW, 0, st>>>(n, G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, nullptr,
-	                                                  T.apptr, T.apcol, stash, flags);
+	k_ap_fill<<<nblk(static_cast<int64_t>(n) * kFillLanes), kT, 0, st>>>(n, T.apptr, stash, T.apcol);
+	k_ap_fill_big<<<kBigBlocks, kTW, 0, st>>>(G.slice_ptr, G.col, G.row_len, T.pptr, T.pcol, T.apptr, T.apcol, big,
+	                                          flags + 2);
 	phase.mark("A P pattern");
 	// coarse pattern
 	int64_t* cptr;
