@@ -56,7 +56,7 @@ constexpr int kStash = 32;       // entries per row a count pass keeps for its f
 // full-size tables, kMaxRow + 1 or kMaxAP + 1 entries per thread, allow four
 // per CU); the other rows are listed and taken by kBigBlocks workgroups with
 // the full tables.
-constexpr int kSmallRow = 16;
+constexpr int kSmallRow = 32;
 constexpr int kBigBlocks = 128;
 constexpr int kScanItems = 8;    // items per thread of a scan block
 constexpr int kScanBlock = kT * kScanItems;
