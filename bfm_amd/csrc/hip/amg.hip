@@ -2666,6 +2666,223 @@ int host_levels_from(bfmx_system* s, bool smooth, bool finest, HostLevelGraph& H
 	return BFMX_OK;
 }
 
+// The transfer topology of a small level, built on the host from its graph
+// G (n rows, n_slices slices, as copied down) and its aggregates agg (na of
+// them): the arrays topo_transfer builds on the GPU, with the same contents,
+// order and conventions and the same bounds, uploaded into T's device arrays
+// in one copy (topo_upload). BFMX_E_UNSUPPORTED where the device build
+// reports a row beyond its bounds.
+int host_transfer(hipStream_t st, SellRows const& G, int n, int64_t n_slices, const int32_t* agg, int na,
+                  bool smooth, TopoLevel& T) {
+	PhaseTimer phase(st, "host transfer");
+	int64_t const nnzb_pad = G.slice[n_slices];
+	// aggregate members in row order
+	std::vector<int32_t> agg_ptr(na + 1, 0), agg_nodes(n);
+	for (int i = 0; i < n; i++) {
+		agg_ptr[agg[i] + 1]++;
+	}
+	for (int a = 0; a < na; a++) {
+		agg_ptr[a + 1] += agg_ptr[a];
+	}
+	{
+		std::vector<int32_t> fill(agg_ptr.begin(), agg_ptr.end() - 1);
+		for (int i = 0; i < n; i++) {
+			agg_nodes[fill[agg[i]]++] = i;
+		}
+	}
+	// prolongator rows: the sorted distinct aggregates of the neighbours
+	// (smoothed) or the own aggregate; per block slot the entry of its
+	// column's aggregate (padding slots and the unsmoothed case: the own)
+	std::vector<int32_t> pptr(n + 1, 0), pcol;
+	std::vector<uint8_t> ppos(static_cast<size_t>(nnzb_pad), 0);
+	std::vector<int32_t> row;
+	for (int i = 0; i < n; i++) {
+		int const len = G.len(i);
+		if (len > kMaxRow) {
+			set_error("multigrid: a level row has more than %d neighbours for the device build", kMaxRow);
+			return BFMX_E_UNSUPPORTED;
+		}
+		row.clear();
+		if (smooth) {
+			for (int k = 0; k < len; k++) {
+				row.push_back(agg[G.at(i, k)]);
+			}
+			std::sort(row.begin(), row.end());
+			row.erase(std::unique(row.begin(), row.end()), row.end());
+		} else {
+			row.push_back(agg[i]);
+		}
+		if (row.size() > 255) {
+			set_error("multigrid: a row couples to more than 255 aggregates");
+			return BFMX_E_UNSUPPORTED;
+		}
+		auto const pos = [&](int32_t a) {
+			return static_cast<uint8_t>(std::lower_bound(row.begin(), row.end(), a) - row.begin());
+		};
+		uint8_t const own = pos(agg[i]);
+		int32_t const s0 = G.slice[i >> 6];
+		int const width = (G.slice[(i >> 6) + 1] - s0) / kSlice;
+		for (int k = 0; k < width; k++) {
+			ppos[s0 + k * kSlice + (i & 63)] = smooth && k < len ? pos(agg[G.at(i, k)]) : own;
+		}
+		pcol.insert(pcol.end(), row.begin(), row.end());
+		pptr[i + 1] = static_cast<int32_t>(pcol.size());
+	}
+	int64_t const nnzp = static_cast<int64_t>(pcol.size());
+	if (nnzp >= (int64_t(1) << 31) / 9) {
+		set_error("multigrid prolongator too large for 32-bit indices");
+		return BFMX_E_ARG;
+	}
+	// P's columns: rows in order and their entries
+	std::vector<int32_t> rptr(na + 1, 0), rent(nnzp), rrow(nnzp);
+	for (int64_t e = 0; e < nnzp; e++) {
+		rptr[pcol[e] + 1]++;
+	}
+	for (int a = 0; a < na; a++) {
+		if (rptr[a + 1] > kMaxPCol) {
+			set_error("multigrid: a prolongator column has more than %d rows for the device build", kMaxPCol);
+			return BFMX_E_UNSUPPORTED;
+		}
+		rptr[a + 1] += rptr[a];
+	}
+	{
+		std::vector<int32_t> fill(rptr.begin(), rptr.end() - 1);
+		for (int i = 0; i < n; i++) {
+			for (int32_t e = pptr[i]; e < pptr[i + 1]; e++) {
+				int32_t const at = fill[pcol[e]]++;
+				rrow[at] = i;
+				rent[at] = e;
+			}
+		}
+	}
+	// A P: the sorted distinct aggregates of the neighbours' prolongator rows
+	// (a stamp per aggregate keeps each once, so only the distinct ones are
+	// sorted)
+	std::vector<int32_t> apptr(n + 1, 0), apcol;
+	std::vector<int32_t> stamp(na, -1);
+	for (int i = 0; i < n; i++) {
+		row.clear();
+		int const len = G.len(i);
+		for (int k = 0; k < len; k++) {
+			int32_t const j = G.at(i, k);
+			for (int32_t e = pptr[j]; e < pptr[j + 1]; e++) {
+				if (stamp[pcol[e]] != i) {
+					stamp[pcol[e]] = i;
+					row.push_back(pcol[e]);
+				}
+			}
+		}
+		std::sort(row.begin(), row.end());
+		if (row.size() > static_cast<size_t>(kMaxAP)) {
+			set_error("multigrid: a row of A P has more than %d entries for the device build", kMaxAP);
+			return BFMX_E_UNSUPPORTED;
+		}
+		apcol.insert(apcol.end(), row.begin(), row.end());
+		apptr[i + 1] = static_cast<int32_t>(apcol.size());
+	}
+	// coarse rows: the sorted distinct columns of the A P rows of P's column
+	std::vector<int64_t> cptr(na + 1, 0);
+	std::vector<int32_t> cadj;
+	std::fill(stamp.begin(), stamp.end(), -1);
+	for (int I = 0; I < na; I++) {
+		row.clear();
+		for (int32_t t = rptr[I]; t < rptr[I + 1]; t++) {
+			int32_t const i = rrow[t];
+			for (int32_t e = apptr[i]; e < apptr[i + 1]; e++) {
+				if (stamp[apcol[e]] != I) {
+					stamp[apcol[e]] = I;
+					row.push_back(apcol[e]);
+				}
+			}
+		}
+		std::sort(row.begin(), row.end());
+		if (row.size() > static_cast<size_t>(kMaxCoarse)) {
+			set_error("multigrid: a coarse row couples to more than %d coarse rows for the device build", kMaxCoarse);
+			return BFMX_E_UNSUPPORTED;
+		}
+		cadj.insert(cadj.end(), row.begin(), row.end());
+		cptr[I + 1] = static_cast<int64_t>(cadj.size());
+	}
+	// the coarse sliced ELL (padding repeats the row's own column, rows past
+	// na use column 0) and the mirror of every coarse slot
+	int64_t const cs = (na + kSlice - 1) / kSlice;
+	std::vector<int32_t> c_slice(cs + 1, 0);
+	for (int64_t sl = 0; sl < cs; sl++) {
+		int64_t w = 0;
+		for (int64_t r = sl * kSlice; r < std::min<int64_t>(na, (sl + 1) * kSlice); r++) {
+			w = std::max(w, cptr[r + 1] - cptr[r]);
+		}
+		int64_t const next = c_slice[sl] + w * kSlice;
+		if (next * 9 >= (int64_t(1) << 31)) {
+			set_error("multigrid level too large for 32-bit block indices");
+			return BFMX_E_ARG;
+		}
+		c_slice[sl + 1] = static_cast<int32_t>(next);
+	}
+	int64_t const pad = c_slice[cs];
+	std::vector<int32_t> c_col(pad, 0), c_diag(na, 0), c_len(na, 0), mirror(pad, -1);
+	for (int64_t I = 0; I < cs * kSlice; I++) {
+		int64_t const sl = I >> 6;
+		int const width = (c_slice[sl + 1] - c_slice[sl]) / kSlice;
+		int const len = I < na ? static_cast<int>(cptr[I + 1] - cptr[I]) : 0;
+		for (int k = 0; k < width; k++) {
+			int32_t const slot = c_slice[sl] + k * kSlice + static_cast<int32_t>(I & 63);
+			int32_t c = I < na ? static_cast<int32_t>(I) : 0;
+			if (k < len) {
+				c = cadj[cptr[I] + k];
+				if (c == I) {
+					c_diag[I] = slot;
+				}
+				int32_t m = -2;
+				if (c == I) {
+					m = slot;
+				} else if (c > I) {
+					auto const b = cadj.begin() + cptr[c], e = cadj.begin() + cptr[c + 1];
+					int32_t const p = static_cast<int32_t>(std::lower_bound(b, e, static_cast<int32_t>(I)) - b);
+					m = c_slice[c >> 6] + p * kSlice + (c & 63);
+				}
+				mirror[slot] = m;
+			}
+			c_col[slot] = c;
+		}
+		if (I < na) {
+			c_len[I] = len;
+		}
+	}
+	phase.mark("arrays");
+	// device arrays, filled by one upload
+	T.nnzp = nnzp;
+	T.nnz_ap = static_cast<int64_t>(apcol.size());
+	T.c_slices = cs;
+	T.c_nnzb = cptr[na];
+	T.c_nnzb_pad = pad;
+	BFMX_CHECK(palloc(&T.agg_ptr, na + 1));
+	BFMX_CHECK(palloc(&T.agg_nodes, n));
+	BFMX_CHECK(palloc(&T.pptr, n + 1));
+	BFMX_CHECK(palloc(&T.pcol, nnzp));
+	BFMX_CHECK(palloc(&T.ppos, static_cast<size_t>(nnzb_pad)));
+	BFMX_CHECK(palloc(&T.rptr, na + 1));
+	BFMX_CHECK(palloc(&T.rent, nnzp));
+	BFMX_CHECK(palloc(&T.rrow, nnzp));
+	BFMX_CHECK(palloc(&T.apptr, n + 1));
+	BFMX_CHECK(palloc(&T.apcol, apcol.size()));
+	BFMX_CHECK(palloc(&T.c_slice_ptr, c_slice.size()));
+	BFMX_CHECK(palloc(&T.c_col, c_col.size()));
+	BFMX_CHECK(palloc(&T.c_diag_blk, na));
+	BFMX_CHECK(palloc(&T.c_row_len, na));
+	BFMX_CHECK(palloc(&T.mirror, mirror.size()));
+	auto item = [](auto* d, auto const& v) {
+		return UploadItem{d, v.data(), v.size() * sizeof(v[0])};
+	};
+	int const rc = topo_upload(st, {item(T.agg_ptr, agg_ptr), item(T.agg_nodes, agg_nodes), item(T.pptr, pptr),
+	                                item(T.pcol, pcol), item(T.ppos, ppos), item(T.rptr, rptr), item(T.rent, rent),
+	                                item(T.rrow, rrow), item(T.apptr, apptr), item(T.apcol, apcol),
+	                                item(T.c_slice_ptr, c_slice), item(T.c_col, c_col), item(T.c_diag_blk, c_diag),
+	                                item(T.c_row_len, c_len), item(T.mirror, mirror)});
+	phase.mark("upload");
+	return rc;
+}
+
 constexpr uint32_t kAggSalt = 0x2545F491u;   // seeds the aggregation's row priorities (per level)
 
 // The levels below level 0, built on the GPU from the level graphs' sliced
@@ -2711,6 +2928,10 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 		bool const sys_graph = lf == 0 && s->adj_ptr.size() == static_cast<size_t>(nf) + 1;
 		bool const sweeps = par.host_agg_rows > 0 &&
 		                    (sys_graph || nf <= par.host_agg_rows || seg_largest(fseg) <= par.host_agg_rows);
+		// a small level swept on the host gets its transfer topology there too
+		bool const host_xfer = sweeps && !sys_graph && nf <= par.host_transfer_rows;
+		const int32_t *gslice = nullptr, *grlen = nullptr, *gcol = nullptr;   // the level's graph, copied down
+		int32_t* hagg = nullptr;
 		if (lf == 0) {
 			BFMX_CHECK(before_finest());
 		}
@@ -2726,22 +2947,21 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 			// kept per thread between builds (no fresh pages to fault in)
 			thread_local std::vector<int32_t> hagg_buf;
 			hagg_buf.resize(std::max(hagg_buf.size(), static_cast<size_t>(nf)));
-			int32_t* const hagg = hagg_buf.data();
+			hagg = hagg_buf.data();
 			if (sys_graph) {
 				aggregate_chunks(CsrRows{s->adj_ptr.data(), s->adj.data()}, fseg, chunk_rows(fseg, nf), hagg, cseg);
 			} else {
 				// the level's sliced ELL as it is (no CSR needed for the sweep),
 				// packed by one kernel into pinned host memory: one launch and
 				// one wait instead of three pageable copies
-				const int32_t *slice = nullptr, *rlen = nullptr, *col = nullptr;
-				BFMX_CHECK(topo_sell_to_host(st, G, &slice, &rlen, &col));
+				BFMX_CHECK(topo_sell_to_host(st, G, &gslice, &grlen, &gcol));
 				phase.mark("level graph to the host");
-				aggregate_chunks(SellRows{slice, col, rlen}, fseg, chunk_rows(fseg, nf), hagg, cseg);
+				aggregate_chunks(SellRows{gslice, gcol, grlen}, fseg, chunk_rows(fseg, nf), hagg, cseg);
 			}
 			phase.mark("sweeps (host)");
 			na = cseg.back();
-			ra = palloc(&agg, nf);
-			if (ra == BFMX_OK) {
+			ra = host_xfer ? BFMX_OK : palloc(&agg, nf);   // the host transfer needs no device copy
+			if (ra == BFMX_OK && !host_xfer) {
 				hipError_t e = hipMemcpyAsync(agg, hagg, nf * sizeof(int32_t), hipMemcpyHostToDevice, st);
 				if (e == hipSuccess) {
 					e = hipStreamSynchronize(st);
@@ -2776,7 +2996,8 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 			break;   // coarsening stalled
 		}
 		TopoLevel T;
-		int const rt = topo_transfer(st, G, agg, na, smooth, T);
+		int const rt = host_xfer ? host_transfer(st, SellRows{gslice, gcol, grlen}, nf, G.n_slices, hagg, na, smooth, T)
+		                         : topo_transfer(st, G, agg, na, smooth, T);
 		if (rt != BFMX_OK) {
 			topo_release(T);
 			return rt;
@@ -2942,6 +3163,9 @@ int amg_build_impl(bfmx_system* s, bool device) {
 		}
 		if (const char* e = std::getenv("BFMX_AMG_HOST_AGG")) {
 			par.host_agg_rows = std::max(0, std::atoi(e));
+		}
+		if (const char* e = std::getenv("BFMX_AMG_HOST_TRANSFER")) {
+			par.host_transfer_rows = std::max(0, std::atoi(e));
 		}
 		if (const char* e = std::getenv("BFMX_AMG_HOST_LEVELS")) {
 			par.host_level_rows = std::max(0, std::atoi(e));

@@ -10,6 +10,7 @@ namespace bfmx {
 constexpr int kMaxRow = 128;     // neighbours of a level row the device build handles
 constexpr int kMaxCoarse = 128;  // columns of a coarse row the device build handles
 constexpr int kMaxAP = 128;      // entries of a row of A P the device build handles
+constexpr int kMaxPCol = 256;    // rows of a prolongator column the device build handles
 
 // A level's graph on the device, as its sliced ELL: row i's k-th neighbour
 // (k < row_len[i], sorted, the row itself included) is
@@ -72,5 +73,16 @@ int topo_aggregate(hipStream_t st, TopoGraph const& G, uint32_t salt, double see
 int topo_transfer(hipStream_t st, TopoGraph const& G, const int32_t* agg, int na, bool smooth, TopoLevel& T);
 
 void topo_release(TopoLevel& T);
+
+// Host arrays copied to device arrays with one copy and one kernel: the
+// items are packed into the calling thread's pinned staging buffer, copied
+// to one device block and scattered from there; returns once they are in
+// place.
+struct UploadItem {
+	void* dst;          // device array of at least `bytes`
+	const void* src;    // host data
+	size_t bytes;
+};
+int topo_upload(hipStream_t st, std::vector<UploadItem> const& items);
 
 }  // namespace bfmx

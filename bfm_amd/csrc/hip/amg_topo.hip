@@ -35,6 +35,7 @@
 // instead.
 
 #include <algorithm>
+#include <cstring>
 #include <vector>
 
 #include "amg_topo.h"
@@ -558,7 +559,7 @@ __global__ __launch_bounds__(kTW) void k_p_fill_big(const int32_t* __restrict__ 
 // plus a wavefront, never fills).
 constexpr int kSetBits = 9;
 constexpr int kSet = 1 << kSetBits;   // table entries
-constexpr int kSetCap = 256;          // values kept
+constexpr int kSetCap = kMaxPCol;     // values kept
 static_assert(kSetCap + 64 < kSet, "the set's table must never fill");
 
 struct WaveSet {
@@ -1164,7 +1165,67 @@ __global__ __launch_bounds__(kT) void k_pack_sell(int64_t ns1, int64_t n, int64_
 	}
 }
 
+// One copy item of topo_upload: `bytes` from offset `off` of the packed
+// block to dst.
+struct UnpackDesc {
+	void* dst;
+	int64_t off;
+	int64_t bytes;
+};
+
+// Workgroup k copies item k (words, then the last bytes).
+__global__ __launch_bounds__(kT) void k_unpack(const UnpackDesc* __restrict__ desc, const char* __restrict__ src) {
+	UnpackDesc const d = desc[blockIdx.x];
+	const char* s = src + d.off;
+	char* o = static_cast<char*>(d.dst);
+	int64_t const words = d.bytes / 4;
+	for (int64_t w = threadIdx.x; w < words; w += kT) {
+		reinterpret_cast<int32_t*>(o)[w] = reinterpret_cast<const int32_t*>(s)[w];
+	}
+	for (int64_t b = words * 4 + threadIdx.x; b < d.bytes; b += kT) {
+		o[b] = s[b];
+	}
+}
+
 }  // namespace
+
+int topo_upload(hipStream_t st, std::vector<UploadItem> const& items) {
+	if (items.empty()) {
+		return BFMX_OK;
+	}
+	auto const align = [](size_t b) { return (b + 15) & ~size_t(15); };
+	size_t const ndesc = items.size();
+	size_t total = align(ndesc * sizeof(UnpackDesc));
+	for (auto const& it : items) {
+		total += align(it.bytes);
+	}
+	void* hbuf = nullptr;
+	BFMX_CHECK(pinned_staging(&hbuf, total, 0));
+	char* const h = static_cast<char*>(hbuf);
+	UnpackDesc* desc = reinterpret_cast<UnpackDesc*>(h);
+	size_t off = align(ndesc * sizeof(UnpackDesc));
+	for (size_t k = 0; k < ndesc; k++) {
+		desc[k] = UnpackDesc{items[k].dst, static_cast<int64_t>(off), static_cast<int64_t>(items[k].bytes)};
+		if (items[k].bytes) {
+			std::memcpy(h + off, items[k].src, items[k].bytes);
+		}
+		off += align(items[k].bytes);
+	}
+	char* dbuf = nullptr;
+	BFMX_CHECK(palloc(&dbuf, total));
+	hipError_t e = hipMemcpyAsync(dbuf, h, total, hipMemcpyHostToDevice, st);
+	if (e == hipSuccess) {
+		k_unpack<<<static_cast<unsigned>(ndesc), kT, 0, st>>>(reinterpret_cast<const UnpackDesc*>(dbuf), dbuf);
+		e = hipGetLastError();
+	}
+	hipError_t const e2 = hipStreamSynchronize(st);
+	pfree(dbuf);
+	if (e != hipSuccess || e2 != hipSuccess) {
+		set_error("upload of a level's transfer arrays: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+		return BFMX_E_HIP;
+	}
+	return BFMX_OK;
+}
 
 int topo_sell_to_host(hipStream_t st, TopoGraph const& G, const int32_t** slice, const int32_t** len,
                       const int32_t** col) {

@@ -247,6 +247,11 @@ struct AmgParams {
 	// are, are aggregated by host sweeps (the finest always, from the host
 	// graph; 0: MIS-2 on the GPU at every level)
 	int host_agg_rows = 131072;
+	// device build: levels of at most this many rows that are swept on the
+	// host get their transfer topology from the host as well (one upload
+	// instead of the device build's chain of launches and read-backs, ~0.3 ms
+	// a level; 0: the device build everywhere)
+	int host_transfer_rows = 4096;
 	// device build: from a level of at most this many rows on, the host build
 	// (off: the host's Galerkin lists grow with the wide stencils of the
 	// coarse levels; 7112 rows of the 2M-DOF beam's level 2 took 11.9 ms

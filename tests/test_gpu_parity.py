@@ -506,6 +506,26 @@ def test_amg_numeric_overlap_matches_one_stream(monkeypatch):
     assert out[0][3] == out[1][3] and np.array_equal(out[0][2], out[1][2])
 
 
+@pytest.mark.parametrize("kind", [3, 4])
+def test_small_levels_host_transfer_matches_the_device_build(monkeypatch, kind):
+    """Levels of at most 4096 rows swept on the host get their transfer
+    topology (prolongator, restriction, A P and coarse patterns, coarse
+    sliced ELL, mirrors) from the host in one upload; the device build of the
+    same arrays (BFMX_AMG_HOST_TRANSFER=0) gives bitwise the same solve."""
+    mesh = meshgen.beam(300, 75, kind=kind, jitter=0.2, seed=19)
+    mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=19)
+    out = []
+    for rows in ("4096", "0"):
+        monkeypatch.setenv("BFMX_AMG_HOST_TRANSFER", rows)
+        s = D.beam_system(mesh)
+        u, st = s.solve(precond=D.PRECOND_AMG, rtol=1e-9)
+        assert st["status"] == 0
+        s.close()
+        out.append((u, st["iterations"]))
+    print(f"kind {kind}: host transfer {out[0][1]} iterations, device {out[1][1]}")
+    assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0])
+
+
 def test_finest_sweeps_do_not_depend_on_threads(monkeypatch):
     """A beam of 90,000 nodes: its finest level is aggregated by one
     sequential sweep on the host; one host thread and sixteen give bitwise
