@@ -1986,10 +1986,7 @@ template <typename Rows>
 int aggregate(int r0, int r1, Rows const& G, int32_t* agg) {
 	std::fill(agg + r0, agg + r1, -1);
 	int na = 0;
-	// rows not yet aggregated when the sweep passed them, and what they join:
-	// kept per thread, so later sweeps on the thread reuse their memory
-	thread_local std::vector<int32_t> rest, join;
-	rest.clear();
+	std::vector<int32_t> rest;   // rows not yet aggregated when the sweep passed them
 	for (int i = r0; i < r1; i++) {
 		if (agg[i] >= 0) {
 			continue;
@@ -2009,7 +2006,7 @@ int aggregate(int r0, int r1, Rows const& G, int32_t* agg) {
 		}
 	}
 	// pass 2 reads pass 1's aggregates only: the joins are written afterwards
-	join.assign(rest.size(), -1);
+	std::vector<int32_t> join(rest.size(), -1);
 	for (size_t r = 0; r < rest.size(); r++) {
 		int const i = rest[r];
 		if (agg[i] >= 0) {
@@ -2931,6 +2928,8 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 		// a small level swept on the host gets its transfer topology there too
 		bool const host_xfer = sweeps && !sys_graph && nf <= par.host_transfer_rows;
 		const int32_t *gslice = nullptr, *grlen = nullptr, *gcol = nullptr;   // the level's graph, copied down
+		std::vector<int32_t> gspill;   // ... when too large for the pinned staging buffer
+		std::vector<int32_t> hagg_buf;
 		int32_t* hagg = nullptr;
 		if (lf == 0) {
 			BFMX_CHECK(before_finest());
@@ -2944,9 +2943,7 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 			// which the FCG iterations show (21 against 26 to 35 for sweeps on
 			// chunks cut inside a segment or MIS-2 aggregates, on the 2M-DOF Q1
 			// beam)
-			// kept per thread between builds (no fresh pages to fault in)
-			thread_local std::vector<int32_t> hagg_buf;
-			hagg_buf.resize(std::max(hagg_buf.size(), static_cast<size_t>(nf)));
+			hagg_buf.resize(nf);
 			hagg = hagg_buf.data();
 			if (sys_graph) {
 				aggregate_chunks(CsrRows{s->adj_ptr.data(), s->adj.data()}, fseg, chunk_rows(fseg, nf), hagg, cseg);
@@ -2954,7 +2951,7 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 				// the level's sliced ELL as it is (no CSR needed for the sweep),
 				// packed by one kernel into pinned host memory: one launch and
 				// one wait instead of three pageable copies
-				BFMX_CHECK(topo_sell_to_host(st, G, &gslice, &grlen, &gcol));
+				BFMX_CHECK(topo_sell_to_host(st, G, &gslice, &grlen, &gcol, gspill));
 				phase.mark("level graph to the host");
 				aggregate_chunks(SellRows{gslice, gcol, grlen}, fseg, chunk_rows(fseg, nf), hagg, cseg);
 			}

@@ -50,11 +50,14 @@ struct TopoLevel {
 // Row ranges of the graph that no entry couples, as offsets 0, ..., n.
 int topo_segments(hipStream_t st, TopoGraph const& G, std::vector<int32_t>& seg);
 
-// The graph's sliced ELL on the host, packed by one kernel into the calling
-// thread's pinned staging buffer: *slice (n_slices + 1 entries), *len (n),
-// *col (nnzb_pad); valid until the thread's next use of that buffer.
+// The graph's sliced ELL on the host: *slice (n_slices + 1 entries), *len
+// (n), *col (nnzb_pad). Up to kPinnedGraphBytes it is packed by one kernel
+// into the calling thread's pinned staging buffer (valid until the thread's
+// next use of it); a larger graph is copied into spill (pageable, released
+// with it), so the staging buffer kept for the process stays small.
+constexpr size_t kPinnedGraphBytes = size_t(32) << 20;
 int topo_sell_to_host(hipStream_t st, TopoGraph const& G, const int32_t** slice, const int32_t** len,
-                      const int32_t** col);
+                      const int32_t** col, std::vector<int32_t>& spill);
 
 // Aggregates of the graph (MIS-2 roots with their neighbours; the other rows
 // join a neighbour's aggregate): *agg (device, n entries, allocated here and

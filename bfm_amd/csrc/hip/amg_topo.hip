@@ -1228,13 +1228,22 @@ int topo_upload(hipStream_t st, std::vector<UploadItem> const& items) {
 }
 
 int topo_sell_to_host(hipStream_t st, TopoGraph const& G, const int32_t** slice, const int32_t** len,
-                      const int32_t** col) {
+                      const int32_t** col, std::vector<int32_t>& spill) {
 	int64_t const ns1 = G.n_slices + 1, n = G.n, total = ns1 + n + G.nnzb_pad;
-	void* buf = nullptr;
-	BFMX_CHECK(pinned_staging(&buf, static_cast<size_t>(total) * sizeof(int32_t)));
-	int32_t* const out = static_cast<int32_t*>(buf);
-	k_pack_sell<<<nblk(total), kT, 0, st>>>(ns1, n, total, G.slice_ptr, G.row_len, G.col, out);
-	BFMX_HIP(hipGetLastError());
+	int32_t* out = nullptr;
+	if (static_cast<size_t>(total) * sizeof(int32_t) > kPinnedGraphBytes) {
+		spill.resize(static_cast<size_t>(total));
+		out = spill.data();
+		BFMX_HIP(hipMemcpyAsync(out, G.slice_ptr, ns1 * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+		BFMX_HIP(hipMemcpyAsync(out + ns1, G.row_len, n * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+		BFMX_HIP(hipMemcpyAsync(out + ns1 + n, G.col, G.nnzb_pad * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	} else {
+		void* buf = nullptr;
+		BFMX_CHECK(pinned_staging(&buf, static_cast<size_t>(total) * sizeof(int32_t)));
+		out = static_cast<int32_t*>(buf);
+		k_pack_sell<<<nblk(total), kT, 0, st>>>(ns1, n, total, G.slice_ptr, G.row_len, G.col, out);
+		BFMX_HIP(hipGetLastError());
+	}
 	BFMX_HIP(hipStreamSynchronize(st));
 	*slice = out;
 	*len = out + ns1;
