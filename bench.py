@@ -22,9 +22,12 @@ multigrid mode); the roofline of the headline solver's largest kernel (the
 flexible-CG FP64 SpMV), of its finest-level post-smoothing step (FP32 SpMV +
 block-Jacobi Chebyshev step) and of the FP64 CG SpMV (algorithmic bytes /
 mean launch time from HIP events on the solver's stream), the end-to-end time (upload + renumbering + assembly + conditions +
-solve), and a CPU baseline: the reference libbfm compiled from its own C
+solve), and CPU baselines: the reference libbfm compiled from its own C
 sources (oracle/_ref), single-threaded as libbfm is, on a bounded sample (its
-dense path cannot hold the 2M-DOF matrix: 32 TB).
+dense path cannot hold the 2M-DOF matrix: 32 TB); the oracle's OpenMP CSR
+Jacobi-PCG on the full system (a bounded number of iterations, scaled); and
+libbfm's RCM + band LU with the matrix kept sparse, on the full system
+(interior pivot steps timed and scaled).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload q1_2m]
 """
@@ -245,8 +248,9 @@ def batched_leg(D, meshgen, count, rtol):
     # multigrid mode: the same batch solved as one block-diagonal system by
     # flexible CG + the multigrid V-cycle (each load scaled to unit norm, the
     # tolerance checks testing every panel's own residual); the first launch
-    # builds the union on the host and its hierarchy (finest aggregates by
-    # host sweeps over each panel, the rest on the GPU)
+    # builds the union (on the device from the instances' arrays) and its
+    # hierarchy (finest and level-1 aggregates by host sweeps over whole
+    # panels, the patterns on the GPU)
     t = time.perf_counter()
     b.solve(precond=D.PRECOND_AMG, rtol=rtol)
     t_mg_setup = time.perf_counter() - t
