@@ -248,7 +248,8 @@ def batched_leg(D, meshgen, count, rtol):
     # multigrid mode: the same batch solved as one block-diagonal system by
     # flexible CG + the multigrid V-cycle (each load scaled to unit norm, the
     # tolerance checks testing every panel's own residual); the first launch
-    # builds the union (on the device from the instances' arrays) and its
+    # builds the union (its graph concatenated on host threads, its values
+    # and coordinates gathered on the device) and its
     # hierarchy (finest and level-1 aggregates by host sweeps over whole
     # panels, the patterns on the GPU)
     t = time.perf_counter()
