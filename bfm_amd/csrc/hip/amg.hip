@@ -1965,64 +1965,78 @@ __global__ __launch_bounds__(kWg) void k_coarse_cycle(const AmgLevelDesc* __rest
 struct CsrRows {
 	const int64_t* ptr;
 	const int32_t* adj;
+	static constexpr int kStride = 1;   // between a row's entries in row()
 	int len(int i) const { return static_cast<int>(ptr[i + 1] - ptr[i]); }
+	const int32_t* row(int i) const { return adj + ptr[i]; }
 	int32_t at(int i, int k) const { return adj[ptr[i] + k]; }
 };
 struct SellRows {
 	const int32_t* slice;
 	const int32_t* col;
 	const int32_t* rlen;
+	static constexpr int kStride = kSlice;
 	int len(int i) const { return rlen[i]; }
+	const int32_t* row(int i) const { return col + slice[i >> 6] + (i & 63); }
 	int32_t at(int i, int k) const { return col[slice[i >> 6] + k * kSlice + (i & 63)]; }
 };
 
 // Aggregates of rows r0 .. r1 - 1 of the graph G, whose entries stay in
-// that range: agg[i] receives the aggregate's number counted from 0 within
-// the range; returns their count. Pass 1 makes a row whose neighbourhood is
-// still free the root of a new aggregate of that neighbourhood; pass 2 gives
-// every other row the aggregate of its first aggregated neighbour (every such
-// row has one).
+// that range, numbered from 0 within the range; returns their count. Pass 1
+// makes a row whose neighbourhood is still free the root of a new aggregate
+// of that neighbourhood; pass 2 gives every other row the aggregate of its
+// first neighbour aggregated by pass 1 (every such row has one). Pass 2
+// writes aggregate a of a row as -2 - a, which its later rows still read as
+// not aggregated (the same joins as writing them all after the pass):
+// agg[i] < 0 holds -2 - agg_i on return, decoded by the caller. Pass 1
+// notes the rows it passes over in a bit mask (one 64-bit word per 64 rows,
+// written once per word), so pass 2 visits only those rows.
 template <typename Rows>
 int aggregate(int r0, int r1, Rows const& G, int32_t* agg) {
+	constexpr int S = Rows::kStride;
 	std::fill(agg + r0, agg + r1, -1);
 	int na = 0;
-	std::vector<int32_t> rest;   // rows not yet aggregated when the sweep passed them
-	for (int i = r0; i < r1; i++) {
-		if (agg[i] >= 0) {
-			continue;
-		}
-		int const m = G.len(i);
-		bool free_nbhd = true;
-		for (int k = 0; k < m && free_nbhd; k++) {
-			free_nbhd = agg[G.at(i, k)] < 0;
-		}
-		if (free_nbhd) {
-			for (int k = 0; k < m; k++) {
-				agg[G.at(i, k)] = na;
+	int const nw = (r1 - r0 + 63) / 64;
+	std::vector<uint64_t> passed(nw);
+	for (int w = 0; w < nw; w++) {
+		uint64_t bits = 0;
+		int const i1 = std::min(r1, r0 + 64 * w + 64);
+		for (int i = r0 + 64 * w; i < i1; i++) {
+			if (agg[i] >= 0) {
+				continue;
 			}
-			agg[i] = na++;
-		} else {
-			rest.push_back(i);
+			const int32_t* const p = G.row(i);
+			int const m = G.len(i);
+			bool free_nbhd = true;
+			for (int k = 0; k < m && free_nbhd; k++) {
+				free_nbhd = agg[p[k * S]] < 0;
+			}
+			if (free_nbhd) {
+				for (int k = 0; k < m; k++) {
+					agg[p[k * S]] = na;
+				}
+				agg[i] = na++;
+			} else {
+				bits |= uint64_t(1) << ((i - r0) & 63);
+			}
 		}
+		passed[w] = bits;
 	}
-	// pass 2 reads pass 1's aggregates only: the joins are written afterwards
-	std::vector<int32_t> join(rest.size(), -1);
-	for (size_t r = 0; r < rest.size(); r++) {
-		int const i = rest[r];
-		if (agg[i] >= 0) {
-			continue;   // taken by a later root
-		}
-		int const m = G.len(i);
-		for (int k = 0; k < m && join[r] < 0; k++) {
-			join[r] = agg[G.at(i, k)];
-		}
-		if (join[r] < 0) {
-			join[r] = na++;   // not reached: the sweep saw an aggregated neighbour
-		}
-	}
-	for (size_t r = 0; r < rest.size(); r++) {
-		if (join[r] >= 0) {
-			agg[rest[r]] = join[r];
+	for (int w = 0; w < nw; w++) {
+		for (uint64_t bits = passed[w]; bits; bits &= bits - 1) {
+			int const i = r0 + 64 * w + __builtin_ctzll(bits);
+			if (agg[i] >= 0) {
+				continue;   // taken by a later root
+			}
+			const int32_t* const p = G.row(i);
+			int const m = G.len(i);
+			int32_t a = -1;
+			for (int k = 0; k < m && a < 0; k++) {
+				a = std::max(agg[p[k * S]], -1);   // pass 2's own joins read as -1
+			}
+			if (a < 0) {
+				a = na++;   // not reached: the sweep saw an aggregated neighbour
+			}
+			agg[i] = -2 - a;
 		}
 	}
 	return na;
@@ -2097,7 +2111,8 @@ std::vector<int32_t> aggregate_chunks(Rows const& G, const std::vector<int32_t>&
 	}
 	parallel_chunks(T, [&](int c) {
 		for (int i = rb[c]; i < rb[c + 1]; i++) {
-			agg[i] += ab[c];
+			int32_t const a = agg[i];
+			agg[i] = (a < 0 ? -2 - a : a) + ab[c];   // pass 2's joins decoded
 		}
 	});
 	cseg.assign(seg.size(), ab[T]);
