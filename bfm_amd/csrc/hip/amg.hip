@@ -392,106 +392,111 @@ __global__ __launch_bounds__(kBlock) void k_smooth_p(int nb, const int32_t* __re
                                                      const float* __restrict__ Pt, const int32_t* __restrict__ pptr,
                                                      const uint8_t* __restrict__ ppos, const double* __restrict__ scal,
                                                      double omega, float* __restrict__ P) {
-	int const i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= nb) {
-		return;
-	}
-	int const r = i & 63;
-	double const w = omega * 4.0 / (3.0 * scal[kScalLambda]);
-	int32_t const dblk = diag_blk[i];
-	double Di[BS][BS];
-	diag_inverse<BS>(val, dblk, r, Di);
-	int const self = ppos[dblk];
-	int32_t const p0 = pptr[i], np = pptr[i + 1] - p0;
-	int32_t const sbeg = slice_ptr[i >> 6] + r, send = slice_ptr[(i >> 6) + 1];
-	// writes entry k of the row from its sum acc = sum A_ij Pt_j
-	auto store = [&](int k, const double (&acc)[BS][3]) {
-		float* out = P + BS * 3 * static_cast<int64_t>(p0 + k);
-#pragma unroll
-		for (int a = 0; a < BS; a++) {
-#pragma unroll
-			for (int c = 0; c < 3; c++) {
-				double t = 0;
-#pragma unroll
-				for (int m = 0; m < BS; m++) {
-					t += Di[a][m] * acc[m][c];
+	auto row = [&](int const i) {
+		int const r = i & 63;
+		double const w = omega * 4.0 / (3.0 * scal[kScalLambda]);
+		int32_t const dblk = diag_blk[i];
+		double Di[BS][BS];
+		diag_inverse<BS>(val, dblk, r, Di);
+		int const self = ppos[dblk];
+		int32_t const p0 = pptr[i], np = pptr[i + 1] - p0;
+		int32_t const sbeg = slice_ptr[i >> 6] + r, send = slice_ptr[(i >> 6) + 1];
+		// writes entry k of the row from its sum acc = sum A_ij Pt_j
+		auto store = [&](int k, const double (&acc)[BS][3]) {
+			float* out = P + BS * 3 * static_cast<int64_t>(p0 + k);
+	#pragma unroll
+			for (int a = 0; a < BS; a++) {
+	#pragma unroll
+				for (int c = 0; c < 3; c++) {
+					double t = 0;
+	#pragma unroll
+					for (int m = 0; m < BS; m++) {
+						t += Di[a][m] * acc[m][c];
+					}
+					double v = -w * t;
+					if (k == self) {
+						v += static_cast<double>(Pt[BS * 3 * static_cast<int64_t>(i) + 3 * a + c]);
+					}
+					out[3 * a + c] = static_cast<float>(v);
 				}
-				double v = -w * t;
-				if (k == self) {
-					v += static_cast<double>(Pt[BS * 3 * static_cast<int64_t>(i) + 3 * a + c]);
-				}
-				out[3 * a + c] = static_cast<float>(v);
 			}
-		}
-	};
-	// A_ij Pt_j for block blk (FP32 values, FP64 products)
-	auto term = [&](int32_t blk, double (&T)[BS][3]) {
-		int64_t const j = col[blk];
-		float A[BS][BS];
-		ld_block<BS>(val32, blk, r, A);
-#pragma unroll
-		for (int a = 0; a < BS; a++) {
-#pragma unroll
-			for (int c = 0; c < 3; c++) {
-				double t = 0;
-#pragma unroll
-				for (int m = 0; m < BS; m++) {
-					t += static_cast<double>(A[a][m]) * static_cast<double>(Pt[BS * 3 * j + 3 * m + c]);
+		};
+		// A_ij Pt_j for block blk (FP32 values, FP64 products)
+		auto term = [&](int32_t blk, double (&T)[BS][3]) {
+			int64_t const j = col[blk];
+			float A[BS][BS];
+			ld_block<BS>(val32, blk, r, A);
+	#pragma unroll
+			for (int a = 0; a < BS; a++) {
+	#pragma unroll
+				for (int c = 0; c < 3; c++) {
+					double t = 0;
+	#pragma unroll
+					for (int m = 0; m < BS; m++) {
+						t += static_cast<double>(A[a][m]) * static_cast<double>(Pt[BS * 3 * j + 3 * m + c]);
+					}
+					T[a][c] = t;
 				}
-				T[a][c] = t;
 			}
-		}
-	};
-	if (np <= kSmoothRegs) {
-		// one pass over the row, each block's term added to its entry's sum
-		// (the entry is selected, not indexed, so the sums stay in registers)
-		double acc[kSmoothRegs][BS][3] = {};
-		if (w != 0) {
-#pragma unroll 2
-			for (int32_t blk = sbeg; blk < send; blk += kSlice) {
-				int const p = ppos[blk];
-				double T[BS][3];
-				term(blk, T);
-#pragma unroll
-				for (int k = 0; k < kSmoothRegs; k++) {
-#pragma unroll
-					for (int a = 0; a < BS; a++) {
-#pragma unroll
-						for (int c = 0; c < 3; c++) {
-							acc[k][a][c] += p == k ? T[a][c] : 0.0;
+		};
+		if (np <= kSmoothRegs) {
+			// one pass over the row, each block's term added to its entry's sum
+			// (the entry is selected, not indexed, so the sums stay in registers)
+			double acc[kSmoothRegs][BS][3] = {};
+			if (w != 0) {
+	#pragma unroll 2
+				for (int32_t blk = sbeg; blk < send; blk += kSlice) {
+					int const p = ppos[blk];
+					double T[BS][3];
+					term(blk, T);
+	#pragma unroll
+					for (int k = 0; k < kSmoothRegs; k++) {
+	#pragma unroll
+						for (int a = 0; a < BS; a++) {
+	#pragma unroll
+							for (int c = 0; c < 3; c++) {
+								acc[k][a][c] += p == k ? T[a][c] : 0.0;
+							}
 						}
 					}
 				}
 			}
-		}
-#pragma unroll
-		for (int k = 0; k < kSmoothRegs; k++) {
-			if (k < np) {
-				store(k, acc[k]);
-			}
-		}
-		return;
-	}
-	// rows coupled to more aggregates: one pass per entry
-	for (int k = 0; k < np; k++) {
-		double acc[BS][3] = {};
-		if (w != 0) {
-			for (int32_t blk = sbeg; blk < send; blk += kSlice) {
-				if (ppos[blk] != k) {
-					continue;
+	#pragma unroll
+			for (int k = 0; k < kSmoothRegs; k++) {
+				if (k < np) {
+					store(k, acc[k]);
 				}
-				double T[BS][3];
-				term(blk, T);
-#pragma unroll
-				for (int a = 0; a < BS; a++) {
-#pragma unroll
-					for (int c = 0; c < 3; c++) {
-						acc[a][c] += T[a][c];
+			}
+			return;
+		}
+		// rows coupled to more aggregates: one pass per entry
+		for (int k = 0; k < np; k++) {
+			double acc[BS][3] = {};
+			if (w != 0) {
+				for (int32_t blk = sbeg; blk < send; blk += kSlice) {
+					if (ppos[blk] != k) {
+						continue;
+					}
+					double T[BS][3];
+					term(blk, T);
+	#pragma unroll
+					for (int a = 0; a < BS; a++) {
+	#pragma unroll
+						for (int c = 0; c < 3; c++) {
+							acc[a][c] += T[a][c];
+						}
 					}
 				}
 			}
+			store(k, acc);
 		}
-		store(k, acc);
+	};
+	// a grid-stride loop over the rows: the grid may be capped, leaving room for other work
+	for (int64_t vb = blockIdx.x; vb * kBlock < nb; vb += gridDim.x) {
+		int64_t const i = vb * kBlock + threadIdx.x;
+		if (i < nb) {
+			row(static_cast<int>(i));
+		}
 	}
 }
 
@@ -640,96 +645,101 @@ __global__ __launch_bounds__(kBlock) void k_ap(int nb, const int32_t* __restrict
                                                const int32_t* __restrict__ pcol, const float* __restrict__ P,
                                                const int32_t* __restrict__ apptr, const int32_t* __restrict__ apcol,
                                                float* __restrict__ apval) {
-	int const i = blockIdx.x * kBlock + threadIdx.x;
-	if (i >= nb) {
-		return;
-	}
-	int const r = i & 63;
-	int32_t const sbeg = slice_ptr[i >> 6] + r;
-	int const len = row_len[i];
-	int32_t const q1 = apptr[i + 1];
-	for (int32_t qa = apptr[i]; qa < q1; qa += W) {
-		int const nw = min(W, q1 - qa);
-		int32_t Jw[W];
-#pragma unroll
-		for (int w = 0; w < W; w++) {
-			Jw[w] = w < nw ? apcol[qa + w] : INT32_MAX;
-		}
-		int32_t const jlo = Jw[0], jhi = Jw[nw - 1];
-		double acc[W][BS][3];
-#pragma unroll
-		for (int w = 0; w < W; w++) {
-#pragma unroll
-			for (int a = 0; a < BS; a++) {
-#pragma unroll
-				for (int c = 0; c < 3; c++) {
-					acc[w][a][c] = 0;
-				}
+	auto row = [&](int const i) {
+		int const r = i & 63;
+		int32_t const sbeg = slice_ptr[i >> 6] + r;
+		int const len = row_len[i];
+		int32_t const q1 = apptr[i + 1];
+		for (int32_t qa = apptr[i]; qa < q1; qa += W) {
+			int const nw = min(W, q1 - qa);
+			int32_t Jw[W];
+	#pragma unroll
+			for (int w = 0; w < W; w++) {
+				Jw[w] = w < nw ? apcol[qa + w] : INT32_MAX;
 			}
-		}
-		for (int k = 0; k < len; k++) {
-			int32_t const blk = sbeg + k * kSlice;
-			int const j = col[blk];
-			int32_t const g1 = pptr[j + 1];
-			bool have_a = false;
-			float A[BS][BS];
-			for (int32_t g = pptr[j]; g < g1; g++) {
-				int32_t const cg = pcol[g];
-				if (cg > jhi) {
-					break;
-				}
-				if (cg < jlo) {
-					continue;
-				}
-				if (!have_a) {
-					ld_block<BS>(val32, blk, r, A);
-					have_a = true;
-				}
-				const float* pj = P + BS * 3 * static_cast<int64_t>(g);
-				float pv[BS * 3];
-#pragma unroll
-				for (int e = 0; e < BS * 3; e++) {
-					pv[e] = pj[e];
-				}
-				double tv[BS][3];
-#pragma unroll
+			int32_t const jlo = Jw[0], jhi = Jw[nw - 1];
+			double acc[W][BS][3];
+	#pragma unroll
+			for (int w = 0; w < W; w++) {
+	#pragma unroll
 				for (int a = 0; a < BS; a++) {
-#pragma unroll
+	#pragma unroll
 					for (int c = 0; c < 3; c++) {
-						double s = 0;
-#pragma unroll
-						for (int m = 0; m < BS; m++) {
-							s += static_cast<double>(A[a][m]) * static_cast<double>(pv[3 * m + c]);
-						}
-						tv[a][c] = s;
+						acc[w][a][c] = 0;
 					}
 				}
-#pragma unroll
-				for (int w = 0; w < W; w++) {
-					if (Jw[w] == cg) {
-#pragma unroll
-						for (int a = 0; a < BS; a++) {
-#pragma unroll
-							for (int c = 0; c < 3; c++) {
-								acc[w][a][c] += tv[a][c];
+			}
+			for (int k = 0; k < len; k++) {
+				int32_t const blk = sbeg + k * kSlice;
+				int const j = col[blk];
+				int32_t const g1 = pptr[j + 1];
+				bool have_a = false;
+				float A[BS][BS];
+				for (int32_t g = pptr[j]; g < g1; g++) {
+					int32_t const cg = pcol[g];
+					if (cg > jhi) {
+						break;
+					}
+					if (cg < jlo) {
+						continue;
+					}
+					if (!have_a) {
+						ld_block<BS>(val32, blk, r, A);
+						have_a = true;
+					}
+					const float* pj = P + BS * 3 * static_cast<int64_t>(g);
+					float pv[BS * 3];
+	#pragma unroll
+					for (int e = 0; e < BS * 3; e++) {
+						pv[e] = pj[e];
+					}
+					double tv[BS][3];
+	#pragma unroll
+					for (int a = 0; a < BS; a++) {
+	#pragma unroll
+						for (int c = 0; c < 3; c++) {
+							double s = 0;
+	#pragma unroll
+							for (int m = 0; m < BS; m++) {
+								s += static_cast<double>(A[a][m]) * static_cast<double>(pv[3 * m + c]);
+							}
+							tv[a][c] = s;
+						}
+					}
+	#pragma unroll
+					for (int w = 0; w < W; w++) {
+						if (Jw[w] == cg) {
+	#pragma unroll
+							for (int a = 0; a < BS; a++) {
+	#pragma unroll
+								for (int c = 0; c < 3; c++) {
+									acc[w][a][c] += tv[a][c];
+								}
 							}
 						}
 					}
 				}
 			}
-		}
-#pragma unroll
-		for (int w = 0; w < W; w++) {
-			if (w < nw) {
-				float* out = apval + BS * 3 * static_cast<int64_t>(qa + w);
-#pragma unroll
-				for (int a = 0; a < BS; a++) {
-#pragma unroll
-					for (int c = 0; c < 3; c++) {
-						out[3 * a + c] = static_cast<float>(acc[w][a][c]);
+	#pragma unroll
+			for (int w = 0; w < W; w++) {
+				if (w < nw) {
+					float* out = apval + BS * 3 * static_cast<int64_t>(qa + w);
+	#pragma unroll
+					for (int a = 0; a < BS; a++) {
+	#pragma unroll
+						for (int c = 0; c < 3; c++) {
+							out[3 * a + c] = static_cast<float>(acc[w][a][c]);
+						}
 					}
 				}
 			}
+		}
+	};
+	// a grid-stride loop over the rows: the grid may be capped, leaving room for other work
+	for (int64_t vb = blockIdx.x; vb * kBlock < nb; vb += gridDim.x) {
+		int64_t const i = vb * kBlock + threadIdx.x;
+		if (i < nb) {
+			row(static_cast<int>(i));
 		}
 	}
 }
@@ -823,85 +833,90 @@ __global__ __launch_bounds__(kBlock) void k_rap_lanes(int64_t nrows_pad, int nc,
                                                       const int32_t* __restrict__ apcol,
                                                       const float* __restrict__ apval, double* __restrict__ cval64,
                                                       float* __restrict__ cval32) {
-	int64_t const I = (static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x) >> 4;
-	if (I >= nrows_pad) {
-		return;
-	}
-	int const l = threadIdx.x & 15;
-	int64_t const s = I >> 6;
-	int const r = static_cast<int>(I & 63);
-	int32_t const sp = c_slice_ptr[s];
-	int const width = (c_slice_ptr[s + 1] - sp) / kSlice;
-	int const len = I < nc ? c_row_len[I] : 0;
-	for (int q = len + l; q < width; q += 16) {   // padding slots
-		int64_t const slot = sp + static_cast<int64_t>(q) * kSlice + r;
-#pragma unroll
-		for (int a = 0; a < 3; a++) {
-#pragma unroll
-			for (int c = 0; c < 3; c++) {
-				int64_t const k = Blk<3>::at(slot, r, a, c);
-				cval64[k] = 0;
-				cval32[k] = 0;
-			}
-		}
-	}
-	if (I >= nc) {
-		return;
-	}
-	int const q0 = (c_diag_blk[I] - sp - r) / kSlice;
-	int32_t const t0 = rptr[I], t1 = rptr[I + 1];
-	for (int qb = q0; qb < len; qb += 16) {
-		int const q = qb + l;
-		bool const act = q < len;
-		int64_t const slot = sp + static_cast<int64_t>(act ? q : q0) * kSlice + r;
-		int32_t const J = act ? c_col[slot] : -1;
-		double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-		for (int32_t t = t0; t < t1; t++) {
-			int const i = rrow[t];
-			int32_t lo = apptr[i], hi = apptr[i + 1] - 1;
-			while (lo < hi) {
-				int32_t const mid = (lo + hi) >> 1;
-				if (apcol[mid] < J) {
-					lo = mid + 1;
-				} else {
-					hi = mid;
-				}
-			}
-			if (!act || lo > hi || apcol[lo] != J) {
-				continue;
-			}
-			const float* rt = R + BSF * 3 * static_cast<int64_t>(t);
-			const float* ap = apval + BSF * 3 * static_cast<int64_t>(lo);
-#pragma unroll
+	auto slot_lanes = [&](int64_t const I) {
+		int const l = threadIdx.x & 15;
+		int64_t const s = I >> 6;
+		int const r = static_cast<int>(I & 63);
+		int32_t const sp = c_slice_ptr[s];
+		int const width = (c_slice_ptr[s + 1] - sp) / kSlice;
+		int const len = I < nc ? c_row_len[I] : 0;
+		for (int q = len + l; q < width; q += 16) {   // padding slots
+			int64_t const slot = sp + static_cast<int64_t>(q) * kSlice + r;
+	#pragma unroll
 			for (int a = 0; a < 3; a++) {
-#pragma unroll
-				for (int c = 0; c < 3; c++) {
-					double v = 0;
-#pragma unroll
-					for (int k = 0; k < BSF; k++) {
-						v += static_cast<double>(rt[3 * k + a]) * static_cast<double>(ap[3 * k + c]);
-					}
-					C[a][c] += v;
-				}
-			}
-		}
-		if (act) {
-			int32_t const mb = mirror[slot];
-			int const rm = mb & 63;
-#pragma unroll
-			for (int a = 0; a < 3; a++) {
-#pragma unroll
+	#pragma unroll
 				for (int c = 0; c < 3; c++) {
 					int64_t const k = Blk<3>::at(slot, r, a, c);
-					cval64[k] = C[a][c];
-					cval32[k] = static_cast<float>(C[a][c]);
-					if (mb >= 0 && mb != slot) {
-						int64_t const km = Blk<3>::at(mb, rm, c, a);
-						cval64[km] = C[a][c];
-						cval32[km] = static_cast<float>(C[a][c]);
+					cval64[k] = 0;
+					cval32[k] = 0;
+				}
+			}
+		}
+		if (I >= nc) {
+			return;
+		}
+		int const q0 = (c_diag_blk[I] - sp - r) / kSlice;
+		int32_t const t0 = rptr[I], t1 = rptr[I + 1];
+		for (int qb = q0; qb < len; qb += 16) {
+			int const q = qb + l;
+			bool const act = q < len;
+			int64_t const slot = sp + static_cast<int64_t>(act ? q : q0) * kSlice + r;
+			int32_t const J = act ? c_col[slot] : -1;
+			double C[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+			for (int32_t t = t0; t < t1; t++) {
+				int const i = rrow[t];
+				int32_t lo = apptr[i], hi = apptr[i + 1] - 1;
+				while (lo < hi) {
+					int32_t const mid = (lo + hi) >> 1;
+					if (apcol[mid] < J) {
+						lo = mid + 1;
+					} else {
+						hi = mid;
+					}
+				}
+				if (!act || lo > hi || apcol[lo] != J) {
+					continue;
+				}
+				const float* rt = R + BSF * 3 * static_cast<int64_t>(t);
+				const float* ap = apval + BSF * 3 * static_cast<int64_t>(lo);
+	#pragma unroll
+				for (int a = 0; a < 3; a++) {
+	#pragma unroll
+					for (int c = 0; c < 3; c++) {
+						double v = 0;
+	#pragma unroll
+						for (int k = 0; k < BSF; k++) {
+							v += static_cast<double>(rt[3 * k + a]) * static_cast<double>(ap[3 * k + c]);
+						}
+						C[a][c] += v;
 					}
 				}
 			}
+			if (act) {
+				int32_t const mb = mirror[slot];
+				int const rm = mb & 63;
+	#pragma unroll
+				for (int a = 0; a < 3; a++) {
+	#pragma unroll
+					for (int c = 0; c < 3; c++) {
+						int64_t const k = Blk<3>::at(slot, r, a, c);
+						cval64[k] = C[a][c];
+						cval32[k] = static_cast<float>(C[a][c]);
+						if (mb >= 0 && mb != slot) {
+							int64_t const km = Blk<3>::at(mb, rm, c, a);
+							cval64[km] = C[a][c];
+							cval32[km] = static_cast<float>(C[a][c]);
+						}
+					}
+				}
+			}
+		}
+	};
+	// a grid-stride loop over the rows: the grid may be capped, leaving room for other work
+	for (int64_t vb = blockIdx.x; vb * kBlock < 16 * nrows_pad; vb += gridDim.x) {
+		int64_t const I = (vb * kBlock + threadIdx.x) >> 4;
+		if (I < nrows_pad) {
+			slot_lanes(I);
 		}
 	}
 }
@@ -3179,6 +3194,9 @@ int amg_build_impl(bfmx_system* s, bool device) {
 		if (const char* e = std::getenv("BFMX_AMG_HOST_TRANSFER")) {
 			par.host_transfer_rows = std::max(0, std::atoi(e));
 		}
+		if (const char* e = std::getenv("BFMX_AMG_NUMERIC_WGS")) {
+			par.numeric_wgs = std::max(0, std::atoi(e));
+		}
 		if (const char* e = std::getenv("BFMX_AMG_HOST_LEVELS")) {
 			par.host_level_rows = std::max(0, std::atoi(e));
 		}
@@ -3479,6 +3497,9 @@ void numeric_level(bfmx_system* s, size_t l, hipStream_t st, bool skip_head) {
 		return;
 	}
 	int const g = grid(L.nb);
+	// on the second stream of a first build, the largest kernels' grids capped
+	int const cap = st == s->amg_stream ? s->amg_par.numeric_wgs : 0;
+	auto capped = [cap](int natural) { return cap > 0 ? std::min(natural, cap) : natural; };
 	if (!skip_head) {
 		numeric_level_head(s, l, coarsest, st);
 	}
@@ -3489,7 +3510,7 @@ void numeric_level(bfmx_system* s, size_t l, hipStream_t st, bool skip_head) {
 	int64_t const ns = C.nnzb_pad;
 	if (L.bs == 2) {
 		k_tentative<2><<<grid(C.nb), kBlock, 0, st>>>(C.nb, L.d_agg_ptr, L.d_agg_nodes, L.d_B, L.d_Pt, C.d_B);
-		k_smooth_p<2><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_val32, L.d_diag_blk,
+		k_smooth_p<2><<<capped(g), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_val64, L.d_val32, L.d_diag_blk,
 		                                    L.d_Pt, L.d_pptr, L.d_ppos, L.d_scal, L.sa ? s->amg_par.omega : 0.0,
 		                                    L.d_P);
 		k_transpose_p<2><<<grid(L.nnzp), kBlock, 0, st>>>(L.nnzp, L.d_rent, L.d_P, L.d_R);
@@ -3498,9 +3519,9 @@ void numeric_level(bfmx_system* s, size_t l, hipStream_t st, bool skip_head) {
 			// on the 2M-DOF beam against 402 / 383 us with 4 / 16 and 738 us
 			// with a lane per entry), a lane per coarse slot for P^T (A P)
 			// (351 us against 382 us with 16 lanes per slot)
-			k_ap<2, 8><<<g, kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_row_len, L.d_val32, L.d_pptr,
+			k_ap<2, 8><<<capped(g), kBlock, 0, st>>>(L.nb, L.d_slice_ptr, L.d_col, L.d_row_len, L.d_val32, L.d_pptr,
 			                                 L.d_pcol, L.d_P, L.d_apptr, L.d_apcol, L.d_apval);
-			k_rap_lanes<2><<<grid(C.n_slices * kSlice * 16), kBlock, 0, st>>>(
+			k_rap_lanes<2><<<capped(grid(C.n_slices * kSlice * 16)), kBlock, 0, st>>>(
 			    C.n_slices * kSlice, C.nb, C.d_slice_ptr, C.d_col, C.d_row_len, C.d_diag_blk, L.d_gal_mirror,
 			    L.d_rptr, L.d_rrow, L.d_R, L.d_apptr, L.d_apcol, L.d_apval, C.d_val64, C.d_val32);
 		} else {

@@ -481,13 +481,18 @@ def test_amg_device_and_host_hierarchies(monkeypatch, kind, host_agg):
     assert rel_err(u1, u_ref) <= U_TOL and rel_err(u3, u_ref) <= U_TOL
 
 
-def test_amg_numeric_overlap_matches_one_stream(monkeypatch):
+@pytest.mark.parametrize("wgs", ["512", "0", "3"])
+def test_amg_numeric_overlap_matches_one_stream(monkeypatch, wgs):
     """The first build queues each level's numeric set-up on a second stream
-    while the host builds the levels below; the same set-up queued after the
-    build on the system's stream (BFMX_AMG_OVERLAP=0) gives bitwise the same
-    solve, on a heterogeneous beam and after a reassembly with new E."""
+    while the host builds the levels below, the finest level's largest
+    kernels held to BFMX_AMG_NUMERIC_WGS workgroups (grid-stride loops: the
+    default 512, the natural grids, and 3, which makes each workgroup take
+    many rows); the same set-up queued after the build on the system's
+    stream (BFMX_AMG_OVERLAP=0, natural grids) gives bitwise the same solve,
+    on a heterogeneous beam and after a reassembly with new E."""
     mesh = meshgen.beam(300, 75, kind=4, jitter=0.2, seed=17)
     mesh.elem_E = meshgen.loguniform_E(mesh.n_elems, seed=17)
+    monkeypatch.setenv("BFMX_AMG_NUMERIC_WGS", wgs)
     out = []
     for overlap in ("1", "0"):
         monkeypatch.setenv("BFMX_AMG_OVERLAP", overlap)
@@ -501,7 +506,7 @@ def test_amg_numeric_overlap_matches_one_stream(monkeypatch):
         assert st2["status"] == 0
         s.close()
         out.append((u, st["iterations"], u2, st2["iterations"]))
-    print(f"overlap: {out[0][1]} / {out[0][3]} iterations, one stream: {out[1][1]} / {out[1][3]}")
+    print(f"overlap ({wgs} workgroups): {out[0][1]} / {out[0][3]} iterations, one stream: {out[1][1]} / {out[1][3]}")
     assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0])
     assert out[0][3] == out[1][3] and np.array_equal(out[0][2], out[1][2])
 
