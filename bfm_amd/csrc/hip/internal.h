@@ -255,9 +255,10 @@ struct AmgParams {
 	// first build: workgroups at most of the finest level's prolongator
 	// smoothing, A P and P^T (A P) kernels, which run on the second stream
 	// beside the coarser levels' topology; a quarter of what the GPU holds
-	// leaves the topology's small kernels room (2M-DOF beam, first solve:
-	// 15.5-15.7 ms uncapped, 15.35 at 512, 15.6-15.8 at 1024, 16.1 at 256;
-	// 0: the natural grid)
+	// leaves the topology's small kernels room (2M-DOF beam, first solve,
+	// on one box: 15.5-15.7 ms uncapped, 15.35 at 512, 15.6-15.8 at 1024,
+	// 16.1 at 256; on others within the spread between runs; 0: the natural
+	// grid)
 	int numeric_wgs = 512;
 	// device build: from a level of at most this many rows on, the host build
 	// (off: the host's Galerkin lists grow with the wide stencils of the
