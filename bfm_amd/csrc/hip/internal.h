@@ -256,7 +256,7 @@ struct AmgParams {
 	// smoothing, A P and P^T (A P) kernels, which run on the second stream
 	// beside the coarser levels' topology; a quarter of what the GPU holds
 	// leaves the topology's small kernels room (2M-DOF beam, first solve,
-	// on one box: 15.5-15.7 ms uncapped, 15.35 at 512, 15.6-15.8 at 1024,
+	// on one MI355X: 15.5-15.7 ms uncapped, 15.35 at 512, 15.6-15.8 at 1024,
 	// 16.1 at 256; on others within the spread between runs; 0: the natural
 	// grid)
 	int numeric_wgs = 512;
