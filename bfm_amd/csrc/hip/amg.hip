@@ -2004,15 +2004,19 @@ struct SellRows {
 // not aggregated (the same joins as writing them all after the pass):
 // agg[i] < 0 holds -2 - agg_i on return, decoded by the caller. Pass 1
 // notes the rows it passes over in a bit mask (one 64-bit word per 64 rows,
-// written once per word), so pass 2 visits only those rows.
+// written once per word), so pass 2 visits only those rows. A set *stop
+// ends the sweep early (returns -1; checked every 4096 rows).
 template <typename Rows>
-int aggregate(int r0, int r1, Rows const& G, int32_t* agg) {
+int aggregate(int r0, int r1, Rows const& G, int32_t* agg, const std::atomic<bool>* stop = nullptr) {
 	constexpr int S = Rows::kStride;
 	std::fill(agg + r0, agg + r1, -1);
 	int na = 0;
 	int const nw = (r1 - r0 + 63) / 64;
 	std::vector<uint64_t> passed(nw);
 	for (int w = 0; w < nw; w++) {
+		if (stop && (w & 63) == 0 && stop->load(std::memory_order_relaxed)) {
+			return -1;
+		}
 		uint64_t bits = 0;
 		int const i1 = std::min(r1, r0 + 64 * w + 64);
 		for (int i = r0 + 64 * w; i < i1; i++) {
@@ -2918,6 +2922,40 @@ constexpr uint32_t kAggSalt = 0x2545F491u;   // seeds the aggregation's row prio
 // aggregation, after_level(l) once level l's transfer topology and the next
 // level's arrays exist. BFMX_E_UNSUPPORTED when a row exceeds the device
 // build's bounds (the caller then builds on the host).
+// The finest level's sweep over the host graph the system keeps, on a host
+// thread: aggregates of all rows as one chunk (whole segments sweep alike
+// however they are grouped), decoded; na = their count, or -1 when stopped.
+// Destroying it stops and joins the thread.
+struct EarlySweep {
+	std::vector<int32_t> agg;
+	int na = -1;
+	std::atomic<bool> stop{false};
+	std::thread th;
+	void start(bfmx_system* s, int n) {
+		agg.resize(n);
+		th = std::thread([this, s, n] {
+			try {
+				int const a = aggregate(0, n, CsrRows{s->adj_ptr.data(), s->adj.data()}, agg.data(), &stop);
+				for (int i = 0; a >= 0 && i < n; i++) {
+					agg[i] = agg[i] < 0 ? -2 - agg[i] : agg[i];   // pass 2's joins
+				}
+				na = a;
+			} catch (...) {
+				na = -1;
+			}
+		});
+	}
+	void finish(bool cancel) {
+		if (th.joinable()) {
+			if (cancel) {
+				stop = true;
+			}
+			th.join();
+		}
+	}
+	~EarlySweep() { finish(true); }
+};
+
 int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
                         const std::function<int()>& before_finest, const std::function<int(int)>& after_level) {
 	hipStream_t const st = s->stream;
@@ -2931,6 +2969,14 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 	G.col = s->d_col;
 	G.row_len = s->d_row_len;
 	PhaseTimer phase(st, "multigrid levels");
+	// the finest level's sweep starts at once, beside the segments' kernels and
+	// the finest level's numeric head; it is kept when the level's segments
+	// make one chunk (else the chunks are swept in parallel below)
+	EarlySweep early;
+	if (par.host_agg_rows > 0 && G.n > par.host_level_rows &&
+	    s->adj_ptr.size() == static_cast<size_t>(G.n) + 1) {
+		early.start(s, G.n);
+	}
 	BFMX_CHECK(topo_segments(st, G, fseg));
 	phase.mark("segments");
 	while (static_cast<int>(lv.size()) < kMaxLevels && seg_largest(fseg) > par.coarsest) {
@@ -2973,11 +3019,23 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 			// which the FCG iterations show (21 against 26 to 35 for sweeps on
 			// chunks cut inside a segment or MIS-2 aggregates, on the 2M-DOF Q1
 			// beam)
-			hagg_buf.resize(nf);
-			hagg = hagg_buf.data();
 			if (sys_graph) {
-				aggregate_chunks(CsrRows{s->adj_ptr.data(), s->adj.data()}, fseg, chunk_rows(fseg, nf), hagg, cseg);
+				std::vector<int32_t> const rb = chunk_rows(fseg, nf);
+				early.finish(rb.size() > 2);
+				if (rb.size() == 2 && early.na >= 0 && static_cast<int>(early.agg.size()) == nf) {
+					hagg = early.agg.data();
+					cseg.assign(fseg.size(), early.na);
+					for (size_t c = 0; c + 1 < fseg.size(); c++) {
+						cseg[c] = hagg[fseg[c]];
+					}
+				} else {
+					hagg_buf.resize(nf);
+					hagg = hagg_buf.data();
+					aggregate_chunks(CsrRows{s->adj_ptr.data(), s->adj.data()}, fseg, rb, hagg, cseg);
+				}
 			} else {
+				hagg_buf.resize(nf);
+				hagg = hagg_buf.data();
 				// the level's sliced ELL as it is (no CSR needed for the sweep),
 				// packed by one kernel into pinned host memory: one launch and
 				// one wait instead of three pageable copies
