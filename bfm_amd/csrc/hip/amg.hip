@@ -2932,9 +2932,9 @@ struct EarlySweep {
 	std::atomic<bool> stop{false};
 	std::thread th;
 	void start(bfmx_system* s, int n) {
-		agg.resize(n);
 		th = std::thread([this, s, n] {
 			try {
+				agg.resize(n);   // here: its first touch off the build's thread
 				int const a = aggregate(0, n, CsrRows{s->adj_ptr.data(), s->adj.data()}, agg.data(), &stop);
 				for (int i = 0; a >= 0 && i < n; i++) {
 					agg[i] = agg[i] < 0 ? -2 - agg[i] : agg[i];   // pass 2's joins
@@ -2973,8 +2973,8 @@ int build_levels_device(bfmx_system* s, bool smooth, std::vector<int32_t>& fseg,
 	// the finest level's numeric head; it is kept when the level's segments
 	// make one chunk (else the chunks are swept in parallel below)
 	EarlySweep early;
-	if (par.host_agg_rows > 0 && G.n > par.host_level_rows &&
-	    s->adj_ptr.size() == static_cast<size_t>(G.n) + 1) {
+	if (par.host_agg_rows > 0 && G.n > par.host_level_rows && s->n_conv_seg <= 1 &&
+	    s->adj_ptr.size() == static_cast<size_t>(G.n) + 1) {   // not a batch's union (many segments)
 		early.start(s, G.n);
 	}
 	BFMX_CHECK(topo_segments(st, G, fseg));
